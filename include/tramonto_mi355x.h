@@ -1,0 +1,192 @@
+/*
+ * tramonto_mi355x.h -- C ABI of the MI355X nonlocal-density fill and
+ * hard-sphere Schur operator library (libtramonto_mi355x.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Field-level entry points (tmi_*).  They replace the per-entry fill of
+ *     Tramonto's fill_resid_and_matrix_control (src/dft_fill_control.c:39)
+ *     and the matrix-based Schur apply (src/dft_Schur_Epetra_Operator.cpp:86)
+ *     with whole-field device kernels.  Arrays named *_d are device pointers
+ *     holding unknowns in the reference's x[iunk][inode_box] order, one
+ *     contiguous row of nnodes doubles per unknown (src/dft_main.c:323-331).
+ *     `stream` is a hipStream_t (NULL = default stream).
+ *
+ *  2. The reference's LinProbMgr ABI (dft_linprobmgr_* and
+ *     dft_hardsphere_lin_prob_mgr_*, src/dft_basic_lin_prob_mgr_wrapper.h:44-92,
+ *     src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58) and the fill entry
+ *     point fill_resid_and_matrix_control with the reference's signature, so
+ *     the reference's C Newton driver (src/dft_newton.c:131) and NOX group
+ *     (src/NOXLOCA_Tramonto_Group.cpp:176-313) link against this library.
+ *
+ * Every function returns 0 on success and a negative TMI_E* code on error
+ * (tmi_last_error() gives the message), except where stated.
+ */
+#ifndef TRAMONTO_MI355X_H
+#define TRAMONTO_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMI_NCOMP_MAX 4
+
+#define TMI_OK 0
+#define TMI_EINVAL (-1)   /* bad argument */
+#define TMI_EUNSUP (-2)   /* feature outside the implemented set */
+#define TMI_EHIP (-3)     /* HIP runtime error */
+#define TMI_ESTATE (-4)   /* call sequence error (e.g. solve before fill) */
+
+/* boundary conditions, Type_bc (src/dft_globals_const.h:496-499) */
+#define TMI_IN_WALL (-1)
+#define TMI_IN_BULK 0
+#define TMI_PERIODIC 1
+#define TMI_REFLECT 2
+
+/* functionals, Type_func */
+#define TMI_FMT1 0
+#define TMI_FMT3 2
+
+/* stencil types, src/dft_globals_const.h:154-156 */
+#define TMI_DELTA 0
+#define TMI_THETA 1
+#define TMI_UATT 2
+
+/* Problem description: the inputs the reference fill reads from its globals
+ * (Ndim, Nodes_x, Esize_x, Type_bc, Type_func, Type_attr, Nwall, Lhard_surf,
+ * VEXT_MAX, HS_diam, Sigma_ff, Eps_ff/kT, Cut_ff, Rho_b). */
+typedef struct tmi_params {
+    int ndim, ncomp;
+    int nx[3];
+    double dx[3];
+    int bc[3][2];
+    int type_func;
+    int type_attr;      /* -1 none; 0 strict mean field with u(r<r_min)=u(r_min) */
+    int nwall;          /* > 0: external field present */
+    int lhard_surf;     /* must be 0 */
+    double vext_max;
+    double hs_diam[TMI_NCOMP_MAX];
+    double sigma_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
+    double eps_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
+    double cut_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
+    double rho_b[TMI_NCOMP_MAX];
+} tmi_params;
+
+typedef struct tmi_info {
+    int nunk, nnodes;
+    int iu_rho, iu_rb, n_rb, iu_mf, n_mf;
+    int sten_len[3][TMI_NCOMP_MAX * TMI_NCOMP_MAX];
+} tmi_info;
+
+/* precision of the inner Krylov products in tmi_newton_solve */
+#define TMI_PREC_FP64 0
+#define TMI_PREC_MIXED 1   /* FP32 stencil sums and JVPs, FP64 residual/update/norm */
+
+typedef struct tmi_newton_opts {
+    int max_iter;
+    double nl_rel_tol, nl_abs_tol, update_scaling;   /* update_solution damping, src/dft_newton.c:266 */
+    double resid_tol;        /* converged when ||F||_2 < resid_tol */
+    int gmres_restart, gmres_max_iter;
+    double gmres_tol;        /* relative tolerance of each Schur solve */
+    int precision;           /* TMI_PREC_* */
+    int verbose;
+} tmi_newton_opts;
+
+typedef struct tmi_newton_stats {
+    int iters, converged, gmres_iters_total;
+    double resid_norm;
+    double seconds;
+} tmi_newton_stats;
+
+const char *tmi_last_error(void);
+int tmi_version(void);
+
+/* Builds stencils (src/dft_stencil.c:40 calc_stencils), bulk constants
+ * (src/dft_thermo_hs.c:42, src/dft_thermo_att.c:44) and uploads tables.
+ * vext and zero_tf are host arrays [nnodes][ncomp] (Vext[loc_inode][icomp],
+ * Zero_density_TF[inode_box][icomp]); either may be NULL. */
+int tmi_problem_create(const tmi_params *p, const double *vext, const unsigned char *zero_tf, void **handle);
+int tmi_problem_destroy(void *handle);
+int tmi_problem_info(void *handle, tmi_info *info);
+/* copies stencil offsets [L][3] and weights [L]; *len receives L */
+int tmi_stencil_get(void *handle, int isten, int icomp, int jcomp, int *len, int *off, double *w);
+/* bulk constants, same packing as the oracle (60 doubles) */
+int tmi_bulk_get(void *handle, double *out60);
+
+/* Host-only variants (no GPU needed): the stencil table and the bulk
+ * constants a problem description yields.  Same outputs as tmi_stencil_get /
+ * tmi_bulk_get on a created problem. */
+int tmi_host_stencil(const tmi_params *p, int isten, int icomp, int jcomp, int *len, int *off, double *w);
+int tmi_host_bulk(const tmi_params *p, double *out60);
+
+/* F(x): the residual of every equation (the reference inserts -F into the
+ * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
+ * fill_resid_and_matrix_control returns. */
+int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sumsq, void *stream);
+/* Jv = J(x) v, the assembled-Jacobian product, src/dft_Schur_Epetra_Operator.cpp:149 (ApplyGlobal) */
+int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
+/* same product with FP32 stencil sums and FP32 accumulation (FP64 pointwise
+ * free-energy derivatives); the inner product of the mixed-precision solve */
+int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
+/* JTw = J(x)^T w, adjoint (transpose-Jacobian-vector) product */
+int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *w_d, double *JTw_d, void *stream);
+/* rho-bar and mean-field unknowns from the densities, src/dft_guess_HSFMT.c:70, src/dft_guess_MFATT.c:66 */
+int tmi_init_fields(void *handle, double *x_d, void *stream);
+/* Newton-Krylov solve in place on the device (Schur GMRES with A22 preconditioning) */
+int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi_newton_stats *stats, void *stream);
+/* one Schur-complement solve J dx = -F at x (dx full length, device) */
+int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *dx_d, double rtol, int restart,
+                    int maxit, int precision, int *iters, void *stream);
+
+/* --------------------------------------------------------------------------
+ * Reference ABI.  The handle is the LinProbMgr; the fill entry point uses
+ * the problem registered with tmi_set_active_problem (the reference reads
+ * its problem from globals).  Arrays are host double** in box layout,
+ * exactly as in the reference.
+ * ------------------------------------------------------------------------ */
+int tmi_set_active_problem(void *problem_handle, void *linprobmgr_handle);
+
+/* src/dft_fill_control.c:39 */
+double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag);
+
+/* src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58 */
+void *dft_hardsphere_lin_prob_mgr_create(int numUnks, void *ParameterList, void *comm);
+int dft_hardsphere_lin_prob_mgr_setindnonlocalequationids(void *linprobmgr, int numIndNonLocalEquations,
+                                                          int *indNonLocalEquationIDs);
+int dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids(void *linprobmgr, int numDepNonLocalEquations,
+                                                          int *depNonLocalEquationIDs);
+int dft_hardsphere_lin_prob_mgr_setdensityequationids(void *linprobmgr, int numDensityEquations,
+                                                      int *densityEquationIDs);
+int dft_hardsphere_lin_prob_mgr_seta22blockisdiagonal(void *linprobmgr, int isA22Diagonal);
+
+/* src/dft_basic_lin_prob_mgr_wrapper.h:44-92 */
+int dft_linprobmgr_destruct(void *linprobmgr);
+int dft_linprobmgr_setnodalrowmap(void *linprobmgr, int numgids, int *gids);
+int dft_linprobmgr_setnodalcolmap(void *linprobmgr, int numgids, int *gids);
+int dft_linprobmgr_setcoarsenednodeslist(void *linprobmgr, int numgids, int *gids);
+int dft_linprobmgr_finalizeblockstructure(void *linprobmgr);
+int dft_linprobmgr_initializeproblemvalues(void *linprobmgr);
+int dft_linprobmgr_insertrhsvalue(void *linprobmgr, int iunk, int inode, double value);
+int dft_linprobmgr_insertonematrixvalue(void *linprobmgr, int iunk, int ownednode, int boxphysicsID, int boxnode,
+                                        double value);
+int dft_linprobmgr_insertmultinodematrixvalues(void *linprobmgr, int iunk, int ownednode, int boxphysicsID,
+                                               int *boxnodes, double *values, int numEntries);
+int dft_linprobmgr_insertmultiphysicsmatrixvalues(void *linprobmgr, int iunk, int ownednode, int *boxphysicsIDs,
+                                                  int boxnode, double *values, int numEntries);
+int dft_linprobmgr_finalizeproblemvalues(void *linprobmgr);
+int dft_linprobmgr_setrhs(void *linprobmgr, double **b);
+int dft_linprobmgr_getrhs(void *linprobmgr, double **b);
+int dft_linprobmgr_getlhs(void *linprobmgr, double **x);
+int dft_linprobmgr_setupsolver(void *linprobmgr);
+int dft_linprobmgr_solve(void *linprobmgr);
+int dft_linprobmgr_applymatrix(void *linprobmgr, double **x, double **b);
+int dft_linprobmgr_importr2c(void *linprobmgr, double **xOwned, double **xBox);
+int dft_linprobmgr_importnodalr2c(void *linprobmgr, double *xOwned, double *xBox);
+int dft_linprobmgr_importsingleunknownr2c(void *linprobmgr, double *xOwned, double *xBox);
+double dft_linprobmgr_getmatrixvalue(void *linprobmgr, int iunk, int ownednode, int junk, int boxnode);
+int dft_linprobmgr_writeMatrix(void *linprobmgr, char *filename, char *matrixName, char *matrixDescription);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

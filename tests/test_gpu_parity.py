@@ -1,0 +1,185 @@
+"""GPU parity of the HIP fill against the CPU oracle (run on an MI355X).
+
+Tolerances: residual and Jacobian products of the FP64 path match the oracle
+to 1e-12 relative to the vector's max-norm (the oracle follows the reference's
+operation order; the GPU sums in a different order and contracts FMAs).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sps
+
+torch = pytest.importorskip("torch")
+
+from oracle.oracle import Oracle  # noqa: E402
+from tramonto_amd.problem import small_case  # noqa: E402
+
+CASES = ["1d_hw", "1d_reflect_wca", "1d_pore_mix", "2d_periodic", "2d_wall_reflect", "3d_periodic_small",
+         "3d_periodic_wb", "3d_mix_bulk"]
+# White Bear under a REFLECT boundary: the reference's delta-stencil Hessian
+# (src/dft_physics_FMT3.c:129-160) omits the mirror sign on the vector
+# densities, so its assembled entries differ from the exact derivative there.
+EXACT_ONLY = {"3d_mix_bulk"}
+
+RTOL = 1e-12
+
+
+def generic_state(spec, o, seed):
+    x = o.init_fields(spec.initial_x())
+    rng = np.random.default_rng(seed)
+    x[spec.ncomp:] *= 1.0 + 0.01 * rng.uniform(-1.0, 1.0, x[spec.ncomp:].shape)
+    return x
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+@pytest.fixture(scope="module")
+def device_problem():
+    from tramonto_amd.device import DeviceProblem
+    cache = {}
+
+    def get(case):
+        if case not in cache:
+            spec = small_case(case)
+            cache[case] = (spec, Oracle.from_spec(spec), DeviceProblem(spec))
+        return cache[case]
+    return get
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_stencils_on_device_problem(case, device_problem):
+    spec, o, dp = device_problem(case)
+    for t in range(3 if spec.type_attr >= 0 else 2):
+        for i in range(spec.ncomp):
+            for j in range(spec.ncomp if t == 2 else 1):
+                og, wg = dp.stencil(t, i, j)
+                oo, wo = o.stencil(t, i, j)
+                assert np.array_equal(og, oo)
+                assert np.array_equal(wg, wo)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_init_fields_parity(case, device_problem):
+    spec, o, dp = device_problem(case)
+    xo = o.init_fields(spec.initial_x())
+    xg = dp.init_fields(dev(spec.initial_x())).cpu().numpy()
+    err = np.abs(xg - xo).max() / np.abs(xo).max()
+    assert err < RTOL, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_residual_parity(case, device_problem):
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 11)
+    Fo = o.residual(x)
+    Fg, sumsq = dp.fill_residual(dev(x))
+    Fg = Fg.cpu().numpy()
+    err = np.abs(Fg - Fo).max() / np.abs(Fo).max()
+    assert err < RTOL, err
+    assert abs(sumsq - float((Fo ** 2).sum())) <= 1e-10 * float((Fo ** 2).sum())
+    # per block (densities, rho-bars, mean field) against the block's own scale
+    blocks = [(0, spec.ncomp), (spec.ncomp, spec.ncomp + spec.nrb), (spec.ncomp + spec.nrb, spec.nunk)]
+    for a, b in blocks:
+        if b > a:
+            scale = max(np.abs(Fo[a:b]).max(), 1.0)
+            assert np.abs(Fg[a:b] - Fo[a:b]).max() < RTOL * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [c for c in CASES if c not in EXACT_ONLY])
+def test_jvp_matches_assembled_jacobian(case, device_problem):
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 12)
+    r, c, v = o.jacobian_coo(x)
+    n = spec.nunk * spec.nnodes
+    J = sps.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        vec = rng.uniform(-1.0, 1.0, n)
+        Jv_o = (J @ vec).reshape(spec.nunk, spec.nnodes)
+        Jv_g = dp.apply_jacobian(dev(x), dev(vec.reshape(spec.nunk, spec.nnodes))).cpu().numpy()
+        err = np.abs(Jv_g - Jv_o).max() / np.abs(Jv_o).max()
+        assert err < RTOL, (trial, err)
+
+
+@pytest.mark.gpu
+def test_jacobian_entries_1d(device_problem):
+    """Entry-level parity: every column of J from unit-vector products."""
+    spec, o, dp = device_problem("1d_reflect_wca")
+    x = generic_state(spec, o, 13)
+    Jo = o.jacobian_dense(x)
+    n = Jo.shape[0]
+    xd = dev(x)
+    Jg = np.zeros_like(Jo)
+    e = torch.zeros((spec.nunk, spec.nnodes), dtype=torch.float64, device="cuda")
+    for j in range(n):
+        e.view(-1)[j] = 1.0
+        Jg[:, j] = dp.apply_jacobian(xd, e).cpu().numpy().reshape(-1)
+        e.view(-1)[j] = 0.0
+    scale = np.abs(Jo).max()
+    assert np.abs(Jg - Jo).max() < RTOL * scale
+    # sparsity pattern: exact zeros where the reference inserts nothing
+    assert np.all((Jg != 0) <= (Jo != 0) | (np.abs(Jg) < 1e-14 * scale))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_jvp_central_differences(case, device_problem):
+    """J v against (F(x+hv) - F(x-hv)) / 2h on the device (h = 1e-6)."""
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 14)
+    rng = np.random.default_rng(6)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    v[: spec.ncomp] *= x[: spec.ncomp] * 0.1   # keep densities positive
+    h = 1e-6
+    Fp, _ = dp.fill_residual(dev(x + h * v))
+    Fm, _ = dp.fill_residual(dev(x - h * v))
+    fd = ((Fp - Fm) / (2 * h)).cpu().numpy()
+    Jv = dp.apply_jacobian(dev(x), dev(v)).cpu().numpy()
+    err = np.abs(Jv - fd).max() / np.abs(fd).max()
+    assert err < 1e-6, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["1d_hw", "1d_pore_mix", "2d_wall_reflect", "3d_periodic_small"])
+def test_fp32_jvp_accuracy(case, device_problem):
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 15)
+    rng = np.random.default_rng(7)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    J64 = dp.apply_jacobian(dev(x), dev(v)).cpu().numpy()
+    J32 = dp.apply_jacobian(dev(x), dev(v), fp32=True).cpu().numpy()
+    err = np.abs(J32 - J64).max() / np.abs(J64).max()
+    assert err < 1e-5, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["1d_hw", "1d_pore_mix", "3d_periodic_small"])
+def test_newton_matches_oracle(case, device_problem):
+    spec, o, dp = device_problem(case)
+    x0 = o.init_fields(spec.initial_x())
+    xo, sto = o.newton(x0, resid_tol=1e-10)
+    assert sto["converged"]
+    xg = dev(x0)
+    st = dp.newton(xg, resid_tol=1e-10, gmres_tol=1e-8, max_iter=60)
+    assert st["converged"], st
+    xg = xg.cpu().numpy()
+    assert np.abs(xg[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+    Fo = o.residual(xg)
+    assert np.linalg.norm(Fo) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_newton_mixed_precision_converges(precision, device_problem):
+    spec, o, dp = device_problem("3d_periodic_small")
+    x0 = o.init_fields(spec.initial_x())
+    xg = dev(x0)
+    st = dp.newton(xg, resid_tol=1e-8, gmres_tol=1e-6, max_iter=60, precision=precision)
+    assert st["converged"], st
+    xo, _ = o.newton(x0, resid_tol=1e-10)
+    assert np.abs(xg.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8

@@ -1,0 +1,623 @@
+// C ABI of libtramonto_mi355x: problem lifecycle and the field-level entry
+// points (fill, Jacobian products, Schur solve, Newton).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tmi_internal.hpp"
+
+namespace tmi {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+template <typename F>
+int guarded(F &&fn)
+{
+    try {
+        return fn();
+    } catch (const HipFailure &) {
+        return TMI_EHIP;
+    } catch (const std::invalid_argument &e) {
+        set_error(e.what());
+        return TMI_EINVAL;
+    } catch (const std::domain_error &e) {
+        set_error(e.what());
+        return TMI_EUNSUP;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return TMI_EINVAL;
+    }
+}
+
+template <typename T>
+T *dev_alloc(size_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    TMI_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    return (T *)p;
+}
+
+template <typename T>
+void upload(T *dst, const T *src, size_t n)
+{
+    if (n) TMI_HIP_CHECK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void validate(const tmi_params &p)
+{
+    if (p.ndim < 1 || p.ndim > 3) throw std::invalid_argument("ndim must be 1, 2 or 3");
+    if (p.ncomp < 1 || p.ncomp > TMI_NCOMP_MAX) throw std::invalid_argument("ncomp must be 1..4");
+    if (p.type_func != TMI_FMT1 && p.type_func != TMI_FMT3)
+        throw std::domain_error("type_func: only Rosenfeld (0) and White Bear (2) are implemented");
+    if (p.type_attr != -1 && p.type_attr != 0) throw std::domain_error("type_attr: only -1 and 0 are implemented");
+    if (p.lhard_surf) throw std::domain_error("lhard_surf: hard-surface boundary weights are not implemented");
+    for (int d = 0; d < p.ndim; ++d) {
+        if (p.nx[d] < 1) throw std::invalid_argument("nx must be positive");
+        if (!(p.dx[d] > 0.0)) throw std::invalid_argument("dx must be positive");
+        for (int s = 0; s < 2; ++s) {
+            int b = p.bc[d][s];
+            if (b != TMI_IN_WALL && b != TMI_IN_BULK && b != TMI_PERIODIC && b != TMI_REFLECT)
+                throw std::invalid_argument("bc code");
+        }
+        if ((p.bc[d][0] == TMI_PERIODIC) != (p.bc[d][1] == TMI_PERIODIC))
+            throw std::invalid_argument("periodic boundaries must be paired");
+    }
+    for (int i = 0; i < p.ncomp; ++i) {
+        if (!(p.hs_diam[i] > 0.0)) throw std::invalid_argument("hs_diam must be positive");
+        if (!(p.rho_b[i] > 0.0)) throw std::invalid_argument("rho_b must be positive");
+    }
+}
+
+void set_geometry(Problem &P, PadGeom &pg, const int *halo)
+{
+    pg.n = 1;
+    for (int d = 0; d < 3; ++d) {
+        int n = (d < P.p.ndim) ? P.p.nx[d] : 1;
+        pg.h[d] = (d < P.p.ndim) ? halo[d] : 0;
+        if (d < P.p.ndim && P.p.bc[d][0] == TMI_PERIODIC && pg.h[d] > n)
+            throw std::domain_error("stencil reach exceeds the periodic domain");
+        pg.P[d] = n + 2 * pg.h[d];
+        pg.n *= pg.P[d];
+    }
+}
+
+template <typename T>
+void upload_stencil(Problem &P, int type, int idx, int comp, DevStencilT<T> &ds)
+{
+    const HostStencil &hs = P.hs[type][idx];
+    const int L = hs.L, nd = P.p.ndim;
+    ds.L = L;
+    std::vector<int> lo(2 * (size_t)L);
+    std::vector<T> w(L), wv((size_t)L * nd);
+    for (int k = 0; k < L; ++k) {
+        const int *o = &hs.off[3 * (size_t)k];
+        lo[k] = o[0] + o[1] * P.pad_u.P[0] + o[2] * P.pad_u.P[0] * P.pad_u.P[1];
+        lo[L + k] = o[0] + o[1] * P.pad_f.P[0] + o[2] * P.pad_f.P[0] * P.pad_f.P[1];
+        w[k] = (T)hs.w[k];
+        for (int d = 0; d < nd; ++d) wv[(size_t)k * nd + d] = (T)(hs.w[k] * ((double)o[d] * P.p.dx[d] * P.bulk.inv_rad[comp]));
+    }
+    ds.lo = dev_alloc<int>(lo.size());
+    ds.w = dev_alloc<T>(w.size());
+    ds.wv = dev_alloc<T>(wv.size());
+    upload(ds.lo, lo.data(), lo.size());
+    upload(ds.w, w.data(), w.size());
+    upload(ds.wv, wv.data(), wv.size());
+}
+
+template <typename T>
+void free_stencil(DevStencilT<T> &ds)
+{
+    (void)hipFree(ds.lo);
+    (void)hipFree(ds.w);
+    (void)hipFree(ds.wv);
+    ds = DevStencilT<T>();
+}
+
+Problem *create_problem(const tmi_params &p, const double *vext, const unsigned char *zero_tf)
+{
+    validate(p);
+    auto P = new Problem();
+    P->p = p;
+    const int C = p.ncomp, nd = p.ndim;
+    P->N = 1;
+    for (int d = 0; d < nd; ++d) P->N *= p.nx[d];
+    P->nrb = 4 + 2 * nd;
+    P->nmf = (p.type_attr >= 0) ? C : 0;
+    P->iu_rho = 0;
+    P->iu_rb = C;
+    P->iu_mf = C + P->nrb;
+    P->nunk = C + P->nrb + P->nmf;
+    P->nadj = C * (1 + nd) + 1;
+    compute_bulk_constants(p, P->bulk);
+    if (p.type_attr >= 0)
+        for (int i = 0; i < C; ++i)
+            for (int j = 0; j < C; ++j)
+                P->bulk.rmin[i][j] = lj_potential_minimum(p.sigma_ff[i][j], p.eps_ff[i][j], p.cut_ff[i][j], p.dx[0]);
+    build_all_stencils(p, P->bulk, P->hs);
+    if (p.type_attr >= 0) compute_attraction_constants(p, P->bulk, P->hs);
+
+    int hu[3] = {0, 0, 0}, hf[3] = {0, 0, 0};
+    for (int t = 0; t < 3; ++t)
+        for (int q = 0; q < C * C; ++q) {
+            const HostStencil &s = P->hs[t][q];
+            for (int k = 0; k < s.L; ++k)
+                for (int d = 0; d < nd; ++d) {
+                    int a = std::abs(s.off[3 * (size_t)k + d]);
+                    hu[d] = std::max(hu[d], a);
+                    if (t != TMI_UATT) hf[d] = std::max(hf[d], a);
+                }
+        }
+    set_geometry(*P, P->pad_u, hu);
+    set_geometry(*P, P->pad_f, hf);
+
+    // node-major inputs -> component-major device rows
+    const size_t N = P->N;
+    P->h_zero.assign((size_t)C * N, 0);
+    P->h_elzero.assign((size_t)C * N, 0);
+    P->h_vext.assign((size_t)C * N, 0.0);
+    for (size_t i = 0; i < N; ++i)
+        for (int c = 0; c < C; ++c) {
+            unsigned char z = zero_tf ? zero_tf[i * C + c] : 0;
+            double v = vext ? vext[i * C + c] : 0.0;
+            P->h_zero[c * N + i] = z ? 1 : 0;
+            P->h_vext[c * N + i] = v;
+            P->h_elzero[c * N + i] = (z || (p.nwall > 0 && v == p.vext_max)) ? 1 : 0;
+        }
+    try {
+        P->d_zero = dev_alloc<unsigned char>(P->h_zero.size());
+        P->d_elzero = dev_alloc<unsigned char>(P->h_elzero.size());
+        P->d_vext = dev_alloc<double>(P->h_vext.size());
+        upload(P->d_zero, P->h_zero.data(), P->h_zero.size());
+        upload(P->d_elzero, P->h_elzero.data(), P->h_elzero.size());
+        upload(P->d_vext, P->h_vext.data(), P->h_vext.size());
+        for (int i = 0; i < C; ++i) {
+            upload_stencil(*P, TMI_DELTA, i, i, P->dsd[TMI_DELTA][i]);
+            upload_stencil(*P, TMI_THETA, i, i, P->dsd[TMI_THETA][i]);
+            upload_stencil(*P, TMI_DELTA, i, i, P->dsf[TMI_DELTA][i]);
+            upload_stencil(*P, TMI_THETA, i, i, P->dsf[TMI_THETA][i]);
+        }
+        if (p.type_attr >= 0)
+            for (int q = 0; q < C * C; ++q) {
+                upload_stencil(*P, TMI_UATT, q, q % C, P->dsd[TMI_UATT][q]);
+                upload_stencil(*P, TMI_UATT, q, q % C, P->dsf[TMI_UATT][q]);
+            }
+        P->wsd.rho_pad = dev_alloc<double>((size_t)C * P->pad_u.n);
+        P->wsd.adj_pad = dev_alloc<double>((size_t)P->nadj * P->pad_f.n);
+        P->wsf.rho_pad = dev_alloc<float>((size_t)C * P->pad_u.n);
+        P->wsf.adj_pad = dev_alloc<float>((size_t)P->nadj * P->pad_f.n);
+        P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
+        TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
+    } catch (...) {
+        delete P;
+        throw;
+    }
+    return P;
+}
+
+void destroy_problem(Problem *P)
+{
+    for (int t = 0; t < 3; ++t)
+        for (int q = 0; q < kNC * kNC; ++q) {
+            free_stencil(P->dsd[t][q]);
+            free_stencil(P->dsf[t][q]);
+        }
+    (void)hipFree(P->d_zero);
+    (void)hipFree(P->d_elzero);
+    (void)hipFree(P->d_vext);
+    (void)hipFree(P->wsd.rho_pad);
+    (void)hipFree(P->wsd.adj_pad);
+    (void)hipFree(P->wsf.rho_pad);
+    (void)hipFree(P->wsf.adj_pad);
+    (void)hipFree(P->d_red);
+    (void)hipFree(P->d_scratch);
+    if (P->h_red) (void)hipHostFree(P->h_red);
+    delete P;
+}
+
+Problem *as_problem(void *h)
+{
+    if (!h) throw std::invalid_argument("null problem handle");
+    return (Problem *)h;
+}
+
+double *scratch(Problem &P, size_t doubles)
+{
+    size_t bytes = doubles * sizeof(double);
+    if (bytes > P.scratch_bytes) {
+        (void)hipFree(P.d_scratch);
+        P.d_scratch = nullptr;
+        P.scratch_bytes = 0;
+        TMI_HIP_CHECK(hipMalloc((void **)&P.d_scratch, bytes));
+        P.scratch_bytes = bytes;
+    }
+    return P.d_scratch;
+}
+
+// right-preconditioned restarted GMRES on the Schur complement
+// S = A22 - A21 A11^{-1} A12 (dft_HardSphereLinProbMgr.cpp:345-428);
+// classical Gram-Schmidt with one reorthogonalisation pass.
+int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, double rtol, int m, int maxit, int prec,
+                double *work, hipStream_t s)
+{
+    const long long n = (long long)P.p.ncomp * P.N;
+    m = std::max(1, std::min(m, 500));
+    double *V = work;                                  // (m+1) n
+    double *z = V + (size_t)(m + 1) * n;               // n
+    double *w = z + n;                                 // n
+    double *dinv = w + n;                              // n
+    double *t1 = dinv + n;                             // (nrb + nmf) N
+    dev_a22_inverse_diag(P, x, dinv, s);
+    TMI_HIP_CHECK(hipMemsetAsync(x2, 0, sizeof(double) * n, s));
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    const double bn0 = std::sqrt(dev_dot(P, bS, bS, n, s));
+    const double bn = bn0 > 0.0 ? bn0 : 1.0;
+    int its = 0;
+    bool first = true;
+    while (its < maxit) {
+        // r = bS - S x2
+        if (first) {
+            TMI_HIP_CHECK(hipMemcpyAsync(V, bS, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        } else {
+            if (prec == TMI_PREC_MIXED)
+                launch_schur_apply<float>(P, x, x2, t1, w, s);
+            else
+                launch_schur_apply<double>(P, x, x2, t1, w, s);
+            dev_scale_copy(1.0, bS, V, n, s);
+            dev_axpy(-1.0, w, V, n, s);
+        }
+        first = false;
+        const double beta = std::sqrt(dev_dot(P, V, V, n, s));
+        if (beta / bn <= rtol || beta == 0.0) break;
+        dev_scale_copy(1.0 / beta, V, V, n, s);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        bool done = false;
+        for (; k < m && its < maxit; ++k) {
+            dev_mul(dinv, V + (size_t)k * n, z, n, s);
+            if (prec == TMI_PREC_MIXED)
+                launch_schur_apply<float>(P, x, z, t1, w, s);
+            else
+                launch_schur_apply<double>(P, x, z, t1, w, s);
+            dev_multi_dot(P, V, n, k + 1, w, n, h.data(), s);
+            TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, h.data(), sizeof(double) * (k + 1),
+                                         hipMemcpyHostToDevice, s));
+            dev_gemv_sub(V, n, k + 1, P.d_red + 1024 * 1024 / 2, w, n, s);
+            dev_multi_dot(P, V, n, k + 1, w, n, h2.data(), s);
+            TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, h2.data(), sizeof(double) * (k + 1),
+                                         hipMemcpyHostToDevice, s));
+            dev_gemv_sub(V, n, k + 1, P.d_red + 1024 * 1024 / 2, w, n, s);
+            for (int j = 0; j <= k; ++j) h[j] += h2[j];
+            const double hn = std::sqrt(dev_dot(P, w, w, n, s));
+            for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = h[j];
+            H[(size_t)(k + 1) * m + k] = hn;
+            if (hn > 0.0) dev_scale_copy(1.0 / hn, w, V + (size_t)(k + 1) * n, n, s);
+            for (int j = 0; j < k; ++j) {
+                double a = H[(size_t)j * m + k], b = H[(size_t)(j + 1) * m + k];
+                H[(size_t)j * m + k] = cs[j] * a + sn[j] * b;
+                H[(size_t)(j + 1) * m + k] = -sn[j] * a + cs[j] * b;
+            }
+            double a = H[(size_t)k * m + k], b = H[(size_t)(k + 1) * m + k];
+            double den = std::sqrt(a * a + b * b);
+            cs[k] = den > 0 ? a / den : 1.0;
+            sn[k] = den > 0 ? b / den : 0.0;
+            H[(size_t)k * m + k] = den;
+            H[(size_t)(k + 1) * m + k] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            ++its;
+            if (std::fabs(g[k + 1]) / bn <= rtol || hn == 0.0) {
+                ++k;
+                done = true;
+                break;
+            }
+        }
+        for (int i = k - 1; i >= 0; --i) {
+            double sum = g[i];
+            for (int j = i + 1; j < k; ++j) sum -= H[(size_t)i * m + j] * y[j];
+            y[i] = sum / H[(size_t)i * m + i];
+        }
+        TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        TMI_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * n, s));
+        dev_gemv_add(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
+        dev_mul(dinv, z, w, n, s);
+        dev_axpy(1.0, w, x2, n, s);
+        TMI_HIP_CHECK(hipStreamSynchronize(s));
+        if (done) break;
+    }
+    return its;
+}
+
+size_t gmres_work_doubles(const Problem &P, int m)
+{
+    const size_t n = (size_t)P.p.ncomp * P.N;
+    return (size_t)(m + 1) * n + 3 * n + (size_t)(P.nrb + P.nmf) * P.N;
+}
+
+int schur_solve(Problem &P, const double *x, const double *F, double *dx, double rtol, int restart, int maxit, int prec,
+                int *iters, hipStream_t s)
+{
+    const size_t N = P.N, ntot = (size_t)P.nunk * N, n2 = (size_t)P.p.ncomp * N;
+    restart = std::max(1, std::min(restart, 500));
+    double *work = scratch(P, 2 * ntot + n2 + gmres_work_doubles(P, restart));
+    double *b = work, *bS = b + ntot, *gw = bS + n2;
+    dev_scale_copy(-1.0, F, b, (long long)ntot, s);
+    launch_schur_rhs(P, x, b, bS, s);
+    int its = gmres_schur(P, x, bS, dx, rtol, restart, maxit, prec, gw, s);
+    launch_schur_backsub(P, x, b, dx, s);
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    if (iters) *iters = its;
+    return 0;
+}
+
+}  // namespace
+
+// exported for the LinProbMgr layer
+int schur_solve_export(Problem &P, const double *x, const double *F, double *dx, double rtol, int restart, int maxit,
+                       int prec, int *iters, hipStream_t s)
+{
+    return schur_solve(P, x, F, dx, rtol, restart, maxit, prec, iters, s);
+}
+
+int newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_stats &st, hipStream_t s)
+{
+    const size_t N = P.N, ntot = (size_t)P.nunk * N;
+    std::vector<double *> bufs;
+    double *F = nullptr, *dx = nullptr;
+    TMI_HIP_CHECK(hipMalloc((void **)&F, sizeof(double) * ntot));
+    try {
+        TMI_HIP_CHECK(hipMalloc((void **)&dx, sizeof(double) * ntot));
+    } catch (...) {
+        (void)hipFree(F);
+        throw;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    st = tmi_newton_stats{};
+    try {
+        int iter = 0;
+        while (true) {
+            launch_residual(P, x, F, s);
+            const double rn = std::sqrt(dev_dot(P, F, F, (long long)ntot, s));
+            st.resid_norm = rn;
+            if (o.verbose) std::fprintf(stderr, "[tmi newton] iter %d ||F|| = %.6e\n", iter, rn);
+            if (!(rn == rn)) throw std::runtime_error("residual is NaN");
+            if (rn < o.resid_tol) {
+                st.converged = 1;
+                break;
+            }
+            if (iter >= o.max_iter) break;
+            int its = 0;
+            schur_solve(P, x, F, dx, o.gmres_tol, o.gmres_restart, o.gmres_max_iter, o.precision, &its, s);
+            st.gmres_iters_total += its;
+            double frac = 1.0, unorm = 0.0;
+            dev_newton_update(P, x, dx, o, &frac, &unorm, s);
+            if (o.verbose) std::fprintf(stderr, "[tmi newton]   gmres %d its, frac %.3g, weighted update %.3e\n", its, frac, unorm);
+            ++iter;
+        }
+        st.iters = iter;
+    } catch (...) {
+        (void)hipFree(F);
+        (void)hipFree(dx);
+        throw;
+    }
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    (void)hipFree(F);
+    (void)hipFree(dx);
+    return 0;
+}
+
+Problem *problem_from_handle(void *h) { return as_problem(h); }
+
+}  // namespace tmi
+
+using namespace tmi;
+
+extern "C" {
+
+const char *tmi_last_error(void) { return g_last_error.c_str(); }
+int tmi_version(void) { return 1; }
+
+int tmi_problem_create(const tmi_params *p, const double *vext, const unsigned char *zero_tf, void **handle)
+{
+    return guarded([&] {
+        if (!p || !handle) throw std::invalid_argument("null argument");
+        *handle = create_problem(*p, vext, zero_tf);
+        return 0;
+    });
+}
+
+int tmi_problem_destroy(void *handle)
+{
+    return guarded([&] {
+        if (handle) destroy_problem((Problem *)handle);
+        return 0;
+    });
+}
+
+int tmi_problem_info(void *handle, tmi_info *info)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        std::memset(info, 0, sizeof(*info));
+        info->nunk = P->nunk;
+        info->nnodes = P->N;
+        info->iu_rho = P->iu_rho;
+        info->iu_rb = P->iu_rb;
+        info->n_rb = P->nrb;
+        info->iu_mf = P->iu_mf;
+        info->n_mf = P->nmf;
+        for (int t = 0; t < 3; ++t)
+            for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
+        return 0;
+    });
+}
+
+int tmi_stencil_get(void *handle, int isten, int icomp, int jcomp, int *len, int *off, double *w)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (isten < 0 || isten > 2 || icomp < 0 || icomp >= P->p.ncomp || jcomp < 0 || jcomp >= P->p.ncomp)
+            throw std::invalid_argument("stencil index");
+        const HostStencil &s = P->hs[isten][icomp + P->p.ncomp * jcomp];
+        if (len) *len = s.L;
+        if (off) std::memcpy(off, s.off.data(), sizeof(int) * s.off.size());
+        if (w) std::memcpy(w, s.w.data(), sizeof(double) * s.w.size());
+        return 0;
+    });
+}
+
+static void pack_bulk(const tmi_params &p, const BulkConstants &b, double *out)
+{
+    std::memset(out, 0, 60 * sizeof(double));
+    for (int q = 0; q < 10; ++q) out[q] = b.rhobar_b[q];
+    for (int q = 0; q < 10; ++q) out[10 + q] = b.dphi_b[q];
+    for (int i = 0; i < p.ncomp; ++i) {
+        out[20 + i] = b.betamu_hs_ex[i];
+        out[24 + i] = b.betamu_att[i];
+        for (int j = 0; j < p.ncomp; ++j) {
+            out[28 + 4 * i + j] = b.avdw[i][j];
+            out[44 + 4 * i + j] = b.rmin[i][j];
+        }
+    }
+}
+
+// host-side tables without touching the GPU
+static void host_tables(const tmi_params &p, BulkConstants &b, HostStencil (*hs)[kNC * kNC])
+{
+    validate(p);
+    compute_bulk_constants(p, b);
+    if (p.type_attr >= 0)
+        for (int i = 0; i < p.ncomp; ++i)
+            for (int j = 0; j < p.ncomp; ++j)
+                b.rmin[i][j] = lj_potential_minimum(p.sigma_ff[i][j], p.eps_ff[i][j], p.cut_ff[i][j], p.dx[0]);
+    build_all_stencils(p, b, hs);
+    if (p.type_attr >= 0) compute_attraction_constants(p, b, hs);
+}
+
+int tmi_host_stencil(const tmi_params *p, int isten, int icomp, int jcomp, int *len, int *off, double *w)
+{
+    return guarded([&] {
+        if (!p) throw std::invalid_argument("null params");
+        if (isten < 0 || isten > 2 || icomp < 0 || icomp >= p->ncomp || jcomp < 0 || jcomp >= p->ncomp)
+            throw std::invalid_argument("stencil index");
+        BulkConstants b;
+        static thread_local HostStencil hs[3][kNC * kNC];
+        for (auto &row : hs)
+            for (auto &s : row) s = HostStencil();
+        host_tables(*p, b, hs);
+        const HostStencil &s = hs[isten][icomp + p->ncomp * jcomp];
+        if (len) *len = s.L;
+        if (off) std::memcpy(off, s.off.data(), sizeof(int) * s.off.size());
+        if (w) std::memcpy(w, s.w.data(), sizeof(double) * s.w.size());
+        return 0;
+    });
+}
+
+int tmi_host_bulk(const tmi_params *p, double *out)
+{
+    return guarded([&] {
+        if (!p || !out) throw std::invalid_argument("null argument");
+        BulkConstants b;
+        static thread_local HostStencil hs[3][kNC * kNC];
+        for (auto &row : hs)
+            for (auto &s : row) s = HostStencil();
+        host_tables(*p, b, hs);
+        pack_bulk(*p, b, out);
+        return 0;
+    });
+}
+
+int tmi_bulk_get(void *handle, double *out)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        pack_bulk(P->p, P->bulk, out);
+        return 0;
+    });
+}
+
+int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sumsq, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !F_d) throw std::invalid_argument("null device pointer");
+        hipStream_t s = (hipStream_t)stream;
+        launch_residual(*P, x_d, F_d, s);
+        if (sumsq) *sumsq = dev_dot(*P, F_d, F_d, (long long)P->nunk * P->N, s);
+        return 0;
+    });
+}
+
+int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !v_d || !Jv_d) throw std::invalid_argument("null device pointer");
+        launch_jvp<double>(*P, x_d, v_d, Jv_d, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !v_d || !Jv_d) throw std::invalid_argument("null device pointer");
+        launch_jvp<float>(*P, x_d, v_d, Jv_d, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *w_d, double *JTw_d, void *stream)
+{
+    return guarded([&] {
+        (void)handle;
+        (void)x_d;
+        (void)w_d;
+        (void)JTw_d;
+        (void)stream;
+        throw std::domain_error("transpose product not built yet");
+        return 0;
+    });
+}
+
+int tmi_init_fields(void *handle, double *x_d, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        launch_init_fields(*P, x_d, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *dx_d, double rtol, int restart,
+                    int maxit, int precision, int *iters, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        return schur_solve(*P, x_d, F_d, dx_d, rtol, restart, maxit, precision, iters, (hipStream_t)stream);
+    });
+}
+
+int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi_newton_stats *stats, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!opts || !stats) throw std::invalid_argument("null argument");
+        return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Pointwise fundamental-measure free-energy derivatives, shared by host and
+// device code.  Densities are in "n order": n0 n1 n2 n3 nv1[D] nv2[D].
+//
+// Both functionals are written as
+//   Phi = -n0 ln(1-n3) + (n1 n2 - nv1.nv2)/(1-n3) + (n2^3 - 3 n2 nv2.nv2) f(n3) / (36 pi)
+// with f = 1.5/(1-n3)^2 for Rosenfeld (FMT1, src/dft_physics_FMT1.c:39-57) and
+// f = [n3 + (1-n3)^2 ln(1-n3)] / (n3^2 (1-n3)^2) for White Bear
+// (FMT3, src/dft_physics_FMT3.c:39-59).  The first derivatives reproduce
+// FMT1_1stderiv / FMT3_1stderiv (src/dft_physics_FMT1.c:59-75,
+// src/dft_physics_FMT3.c:61-96), including White Bear's n3 <= 1e-10 cut-off.
+// The Jacobian-vector product uses the exact Hessian of Phi; contracted with a
+// stencil kernel vector it equals the reference's d2phi_drb2_{delta,theta}_rb
+// values (src/dft_physics_FMT1.c:79-168, src/dft_physics_FMT3.c:101-266),
+// which are zero for White Bear when n3 <= 1e-10.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace tmi {
+
+struct FmtScalars {
+    double a;    // 1/(1-n3)
+    double f0, f1, f2;   // f(n3), f'(n3), f''(n3)
+    double lnA;  // ln(1/(1-n3))
+    bool cut;    // White Bear below the n3 cut-off
+};
+
+__host__ __device__ inline FmtScalars fmt_scalars(int func, double n3)
+{
+    FmtScalars s;
+    const double one_m = 1.0 - n3;
+    s.a = 1.0 / one_m;
+    s.lnA = log(s.a);
+    s.cut = false;
+    const double a = s.a, a2 = a * a, a3 = a2 * a, a4 = a3 * a;
+    if (func == 0) {
+        s.f0 = 1.5 * a2;
+        s.f1 = 3.0 * a3;
+        s.f2 = 9.0 * a4;
+        return s;
+    }
+    if (!(n3 > 1.e-10)) {
+        s.cut = true;
+        s.f0 = s.f1 = s.f2 = 0.0;
+        return s;
+    }
+    const double L = log(one_m);
+    const double g = n3 + one_m * one_m * L;   // n3 + (1-n3)^2 ln(1-n3)
+    const double g1 = n3 - 2.0 * one_m * L;    // g'
+    const double g2 = 3.0 + 2.0 * L;           // g''
+    const double i2 = 1.0 / (n3 * n3), i3 = i2 / n3, i4 = i3 / n3;
+    s.f0 = g * a2 * i2;
+    s.f1 = g1 * a2 * i2 + 2.0 * g * a3 * i2 - 2.0 * g * a2 * i3;
+    s.f2 = g2 * a2 * i2 + 4.0 * g1 * a3 * i2 - 4.0 * g1 * a2 * i3 + 6.0 * g * a4 * i2 - 8.0 * g * a3 * i3 +
+           6.0 * g * a2 * i4;
+    return s;
+}
+
+// dPhi/dn in n order
+template <int ND>
+__host__ __device__ inline void fmt_dphi(int func, const double *n, double *out)
+{
+    constexpr double inv_pi = 0.318309886183790671537767526745028724;
+    const FmtScalars s = fmt_scalars(func, n[3]);
+    double d12 = 0.0, d22 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        d12 += n[4 + d] * n[4 + ND + d];
+        d22 += n[4 + ND + d] * n[4 + ND + d];
+    }
+    const double a = s.a, a2 = a * a;
+    out[0] = s.lnA;
+    out[1] = n[2] * a;
+    if (s.cut) {
+        out[2] = 0.0;
+        out[3] = 0.0;
+    } else {
+        out[2] = n[1] * a + (n[2] * n[2] - d22) * s.f0 * (inv_pi / 12.0);
+        out[3] = n[0] * a + (n[1] * n[2] - d12) * a2 + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f1 * (inv_pi / 36.0);
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        out[4 + d] = -n[4 + ND + d] * a;
+        out[4 + ND + d] = s.cut ? 0.0 : (-n[4 + d] * a - n[2] * n[4 + ND + d] * s.f0 * (inv_pi / 6.0));
+    }
+}
+
+// out = H(n) dn (exact Hessian of Phi; zero below the White Bear cut-off)
+template <int ND>
+__host__ __device__ inline void fmt_hvp(int func, const double *n, const double *dn, double *out)
+{
+    constexpr double inv_pi = 0.318309886183790671537767526745028724;
+    const FmtScalars s = fmt_scalars(func, n[3]);
+    if (s.cut) {
+#pragma unroll
+        for (int q = 0; q < 4 + 2 * ND; ++q) out[q] = 0.0;
+        return;
+    }
+    double d12 = 0.0, d22 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        d12 += n[4 + d] * n[4 + ND + d];
+        d22 += n[4 + ND + d] * n[4 + ND + d];
+    }
+    const double a = s.a, a2 = a * a, a3 = a2 * a;
+    const double n0 = n[0], n1 = n[1], n2 = n[2];
+    const double h22 = n2 * s.f0 * (inv_pi / 6.0);
+    const double h23 = n1 * a2 + (n2 * n2 - d22) * s.f1 * (inv_pi / 12.0);
+    const double h33 = n0 * a2 + 2.0 * (n1 * n2 - d12) * a3 + (n2 * n2 * n2 - 3.0 * n2 * d22) * s.f2 * (inv_pi / 36.0);
+    const double c2v = s.f0 * (inv_pi / 6.0);   // -H(2,v2d) / nv2d and -H(v2d,v2d)/n2
+    const double c3v = s.f1 * (inv_pi / 6.0);
+    double o2 = a * dn[1] + h22 * dn[2] + h23 * dn[3];
+    double o3 = a * dn[0] + n2 * a2 * dn[1] + h23 * dn[2] + h33 * dn[3];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double v1 = n[4 + d], v2 = n[4 + ND + d];
+        const double dv1 = dn[4 + d], dv2 = dn[4 + ND + d];
+        o2 += -v2 * c2v * dv2;
+        o3 += -v2 * a2 * dv1 + (-v1 * a2 - n2 * v2 * c3v) * dv2;
+        out[4 + d] = -v2 * a2 * dn[3] - a * dv2;
+        out[4 + ND + d] = -v2 * c2v * dn[2] + (-v1 * a2 - n2 * v2 * c3v) * dn[3] - a * dv1 - n2 * c2v * dv2;
+    }
+    out[0] = a * dn[3];
+    out[1] = a * dn[2] + n2 * a2 * dn[3];
+    out[2] = o2;
+    out[3] = o3;
+}
+
+}  // namespace tmi

@@ -1,0 +1,146 @@
+// Internal declarations of libtramonto_mi355x.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tramonto_mi355x.h"
+
+namespace tmi {
+
+constexpr double kPi = 3.141592653589793238462643383279502884197169399375;
+constexpr int kNC = TMI_NCOMP_MAX;
+
+// ---------------------------------------------------------------- host setup
+struct HostStencil {
+    int L = 0;
+    std::vector<int> off;     // [L][3]
+    std::vector<double> w;    // [L]
+};
+
+struct BulkConstants {
+    double inv_4pi = 0.0;
+    double inv_rad[kNC] = {}, inv_4pir[kNC] = {}, inv_4pirsq[kNC] = {};
+    double rhobar_b[10] = {};   // solution order
+    double dphi_b[10] = {};     // n order
+    double betamu_hs_ex[kNC] = {}, betamu_att[kNC] = {};
+    double avdw[kNC][kNC] = {}, rmin[kNC][kNC] = {};
+};
+
+// stencils[type][icomp + ncomp*jcomp]
+void build_all_stencils(const tmi_params &p, const BulkConstants &bc, HostStencil (*stencils)[kNC * kNC]);
+void compute_bulk_constants(const tmi_params &p, BulkConstants &bc);
+void compute_attraction_constants(const tmi_params &p, BulkConstants &bc, HostStencil (*stencils)[kNC * kNC]);
+double lj_potential_minimum(double sigma, double eps, double rcut, double dx0);
+
+// ---------------------------------------------------------------- device data
+// Geometry of a halo-padded copy of the grid.  A node (x,y,z) sits at padded
+// position (x+h0) + (y+h1)*P0 + (z+h2)*P0*P1; halo positions carry the value
+// the reference's offset_to_node_box assigns to an out-of-box stencil point.
+struct PadGeom {
+    int h[3] = {0, 0, 0};
+    int P[3] = {1, 1, 1};
+    long long n = 0;   // P0*P1*P2
+};
+
+template <typename T>
+struct DevStencilT {
+    int L = 0;
+    int *lo = nullptr;   // padded linear offsets
+    T *w = nullptr;      // weights
+    T *wv = nullptr;     // [L][ndim] w * offset_d*dx_d/R (delta stencils only)
+};
+
+struct Grid {
+    int nd;
+    int nx[3];
+    int bc[3][2];
+    int N;
+};
+
+struct Problem;
+
+// kernel-side launch helpers (tmi_kernels.hip)
+template <typename T>
+struct Workspace {
+    T *rho_pad = nullptr;    // [C][pad_u.n]   density (or direction) with halo
+    T *adj_pad = nullptr;    // [nadj][pad_f.n] adjoint fields A_c, B_c,d, S3 with halo
+};
+
+struct Problem {
+    tmi_params p{};
+    Grid g{};
+    int N = 0, nunk = 0, nrb = 0, nmf = 0, iu_rho = 0, iu_rb = 0, iu_mf = 0;
+    int nadj = 0;   // number of adjoint fields = C*(1+D) + 1
+    BulkConstants bulk;
+    HostStencil hs[3][kNC * kNC];
+    PadGeom pad_u, pad_f;   // density halo (max reach of all stencils), FMT halo
+    // device tables
+    DevStencilT<double> dsd[3][kNC * kNC];
+    DevStencilT<float> dsf[3][kNC * kNC];
+    unsigned char *d_zero = nullptr;     // [C][N] Zero_density_TF
+    unsigned char *d_elzero = nullptr;   // [C][N] EL row replaced by x (zero density or V_ext = VEXT_MAX)
+    double *d_vext = nullptr;            // [C][N]
+    std::vector<unsigned char> h_zero, h_elzero;
+    std::vector<double> h_vext;
+    Workspace<double> wsd;
+    Workspace<float> wsf;
+    // solver scratch
+    double *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double *d_red = nullptr;     // reduction buffer
+    double *h_red = nullptr;     // pinned host mirror
+};
+
+// ---------------------------------------------------------------- kernels
+enum Mode { kResidual = 0, kLinear = 1 };
+
+// F = residual (all rows); sumsq optional (host)
+void launch_residual(Problem &P, const double *x, double *F, hipStream_t s);
+// Jv for all rows, precision T for the stencil sums
+template <typename T>
+void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s);
+template <typename T>
+void launch_vjp(Problem &P, const double *x, const double *w, double *JTw, hipStream_t s);
+void launch_init_fields(Problem &P, double *x, hipStream_t s);
+// Schur pieces (T precision for stencil sums):
+//   t1 = A12 x2 (block-1 rows of J [x2;0]);  y2 = A22 x2 + A21 t1
+template <typename T>
+void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t1, double *y2, hipStream_t s);
+// bS = b2 + A21 b1 (b full length)
+void launch_schur_rhs(Problem &P, const double *x, const double *b, double *bS, hipStream_t s);
+// x1 = A12 x2 - b1 written into dx (full length), x2 already in dx[0:C*N]
+void launch_schur_backsub(Problem &P, const double *x, const double *b, double *dx, hipStream_t s);
+
+// vector helpers
+double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s);
+void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
+                   hipStream_t s);
+void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);
+void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s);
+void dev_mul(const double *a, const double *b, double *y, long long n, hipStream_t s);
+void dev_gemv_sub(const double *V, long long ld, int k, const double *h, double *w, long long n, hipStream_t s);
+void dev_gemv_add(const double *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s);
+void dev_a22_inverse_diag(Problem &P, const double *x, double *dinv, hipStream_t s);
+void dev_newton_update(Problem &P, double *x, const double *dx, const tmi_newton_opts &o, double *frac_out,
+                       double *unorm_out, hipStream_t s);
+
+// errors
+void set_error(const std::string &msg);
+#define TMI_HIP_CHECK(expr)                                                                            \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            ::tmi::set_error(std::string("HIP error ") + hipGetErrorString(_e) + " at " __FILE__ ":" + \
+                             std::to_string(__LINE__));                                                \
+            throw ::tmi::HipFailure();                                                                 \
+        }                                                                                              \
+    } while (0)
+
+struct HipFailure {};
+
+}  // namespace tmi

@@ -1,0 +1,840 @@
+// Device kernels of libtramonto_mi355x (gfx950).
+//
+// Data layout in HBM: every field is structure-of-arrays, one contiguous row
+// of N doubles per unknown (the reference's x[iunk][inode_box]).  Stencil
+// sums read halo-padded copies of their inputs: the halo holds exactly what
+// offset_to_node_box (src/dft_mesh_lib.c:203-297) returns for out-of-box
+// points -- the periodic image, the mirror image (REFLECT, with vector
+// components negated), the bulk constant (IN_BULK) or zero (IN_WALL) -- so the
+// sum kernels are branch-free: out[i] = sum_k w_k in_pad[p(i) + lo_k].
+//
+//   pad_rho    : densities rho_c (zero where Zero_density_TF), halo of the
+//                widest stencil; read by the forward (rho-bar) and mean-field
+//                (u) sums.
+//   fmt_point  : dPhi/dn (residual) or H(n) dn (Jacobian-vector product) at
+//                every node, folded per component into the adjoint fields
+//                A_c = S0/4piR^2 + S1/4piR + S2, B_c,d = V1_d/4piR + V2_d, S3.
+//   pad_adj    : halo of the adjoint fields.
+//   rb_rows    : rho-bar definition rows  (load_rho_bar_s/_v, src/dft_fill_FMT.c:195-347)
+//   mf_rows    : mean-field rows          (load_mean_field, src/dft_fill_mf.c:38-63)
+//   el_rows    : Euler-Lagrange rows      (load_euler_lagrange, src/dft_fill_EL.c:38-197,
+//                adjoint stencil load_nonlocal_hs_rosen_rb, src/dft_fill_FMT.c:41-189)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "tmi_fmt.hpp"
+#include "tmi_internal.hpp"
+
+namespace tmi {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Geo {
+    int nd, C, N;
+    int nx[3];
+    int bc[3][2];
+    int h[3], P[3];
+    long long np;
+};
+
+Geo make_geo(const Problem &P, const PadGeom &pg)
+{
+    Geo g;
+    g.nd = P.p.ndim;
+    g.C = P.p.ncomp;
+    g.N = P.N;
+    for (int d = 0; d < 3; ++d) {
+        g.nx[d] = (d < g.nd) ? P.p.nx[d] : 1;
+        g.bc[d][0] = P.p.bc[d][0];
+        g.bc[d][1] = P.p.bc[d][1];
+        g.h[d] = pg.h[d];
+        g.P[d] = pg.P[d];
+    }
+    g.np = pg.n;
+    return g;
+}
+
+struct CompConsts {
+    double i4pir[kNC], i4pirsq[kNC], rho_b[kNC], mu[kNC];
+    double Ab[kNC], S3b;    // bulk adjoint-field values
+};
+
+__device__ inline long long pad_pos(const Geo &g, int i)
+{
+    int x = i % g.nx[0];
+    int r = i / g.nx[0];
+    int y = r % g.nx[1];
+    int z = r / g.nx[1];
+    return (long long)(x + g.h[0]) + (long long)(y + g.h[1]) * g.P[0] + (long long)(z + g.h[2]) * g.P[0] * g.P[1];
+}
+
+// map a padded position to (code, source node, reflect mask); code 0: node,
+// -1: bulk, -2: wall.  Dimension order and codes as offset_to_node_box.
+__device__ inline int map_pad(const Geo &g, long long pos, int &src, int &refl)
+{
+    int c[3];
+    c[0] = (int)(pos % g.P[0]);
+    long long r = pos / g.P[0];
+    c[1] = (int)(r % g.P[1]);
+    c[2] = (int)(r / g.P[1]);
+    refl = 0;
+    int s[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = c[d] - g.h[d];
+        const int n = g.nx[d];
+        if (d < g.nd) {
+            if (v < 0) {
+                const int b = g.bc[d][0];
+                if (b == TMI_IN_BULK) return -1;
+                if (b == TMI_IN_WALL) return -2;
+                if (b == TMI_PERIODIC) {
+                    v += n;
+                } else {
+                    refl |= 1 << d;
+                    v = -v;
+                    if (v >= n) {
+                        if (g.bc[d][1] == TMI_IN_BULK) return -1;
+                        return -2;
+                    }
+                }
+            } else if (v >= n) {
+                const int b = g.bc[d][1];
+                if (b == TMI_IN_BULK) return -1;
+                if (b == TMI_IN_WALL) return -2;
+                if (b == TMI_PERIODIC) {
+                    v -= n;
+                } else {
+                    refl |= 1 << d;
+                    v = 2 * (n - 1) - v;
+                    if (v < 0) {
+                        if (g.bc[d][0] == TMI_IN_BULK) return -1;
+                        return -2;
+                    }
+                }
+            }
+        }
+        s[d] = v;
+    }
+    src = s[0] + s[1] * g.nx[0] + s[2] * g.nx[0] * g.nx[1];
+    return 0;
+}
+
+// ---------------------------------------------------------------- padding
+// rho_pad[c][pos] from in[c*N + node]; bulk value rho_b (residual) or 0 (linear)
+template <typename T>
+__global__ void k_pad_rho(Geo g, const double *__restrict__ in, const unsigned char *__restrict__ zero, CompConsts cc,
+                          int linear, T *__restrict__ out)
+{
+    long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= g.np) return;
+    int src = 0, refl = 0;
+    int code = map_pad(g, pos, src, refl);
+    for (int c = 0; c < g.C; ++c) {
+        double v;
+        if (code == 0)
+            v = zero[(size_t)c * g.N + src] ? 0.0 : in[(size_t)c * g.N + src];
+        else if (code == -1 && !linear)
+            v = cc.rho_b[c];
+        else
+            v = 0.0;
+        out[(size_t)c * g.np + pos] = (T)v;
+    }
+}
+
+// pointwise FMT: residual (dphi) or linear (H dn); writes the adjoint fields
+// at the interior positions of the FMT-halo geometry
+template <typename T, int ND>
+__global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, const double *__restrict__ dnrb, int linear,
+                            CompConsts cc, T *__restrict__ adj)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    constexpr int NR = 4 + 2 * ND;
+    double rb[NR], n[NR], out[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) rb[q] = xrb[(size_t)q * g.N + i];
+    // solution order (n3 n2 n1 n0 nv2 nv1) -> n order, src/dft_utils.c:205
+    n[0] = rb[3];
+    n[1] = rb[2];
+    n[2] = rb[1];
+    n[3] = rb[0];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        n[4 + d] = rb[4 + ND + d];
+        n[4 + ND + d] = rb[4 + d];
+    }
+    if (!linear) {
+        fmt_dphi<ND>(func, n, out);
+    } else {
+        double dr[NR], dn[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) dr[q] = dnrb[(size_t)q * g.N + i];
+        dn[0] = dr[3];
+        dn[1] = dr[2];
+        dn[2] = dr[1];
+        dn[3] = dr[0];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            dn[4 + d] = dr[4 + ND + d];
+            dn[4 + ND + d] = dr[4 + d];
+        }
+        fmt_hvp<ND>(func, n, dn, out);
+    }
+    const long long p = pad_pos(g, i);
+    const int C = g.C;
+    for (int c = 0; c < C; ++c) {
+        adj[(size_t)(c * (1 + ND)) * g.np + p] = (T)(out[0] * cc.i4pirsq[c] + out[1] * cc.i4pir[c] + out[2]);
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+            adj[(size_t)(c * (1 + ND) + 1 + d) * g.np + p] = (T)(out[4 + d] * cc.i4pir[c] + out[4 + ND + d]);
+    }
+    adj[(size_t)(C * (1 + ND)) * g.np + p] = (T)out[3];
+}
+
+template <typename T, int ND>
+__global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj)
+{
+    long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= g.np) return;
+    int src = 0, refl = 0;
+    int code = map_pad(g, pos, src, refl);
+    const int C = g.C;
+    if (code == 0) {
+        const long long ps = pad_pos(g, src);
+        if (ps == pos) return;   // interior position
+        for (int c = 0; c < C; ++c) {
+            const size_t b = (size_t)(c * (1 + ND));
+            adj[b * g.np + pos] = adj[b * g.np + ps];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                T v = adj[(b + 1 + d) * g.np + ps];
+                adj[(b + 1 + d) * g.np + pos] = ((refl >> d) & 1) ? -v : v;
+            }
+        }
+        adj[(size_t)(C * (1 + ND)) * g.np + pos] = adj[(size_t)(C * (1 + ND)) * g.np + ps];
+        return;
+    }
+    const bool bulk = (code == -1) && !linear;
+    for (int c = 0; c < C; ++c) {
+        const size_t b = (size_t)(c * (1 + ND));
+        adj[b * g.np + pos] = bulk ? (T)cc.Ab[c] : (T)0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) adj[(b + 1 + d) * g.np + pos] = (T)0;
+    }
+    adj[(size_t)(C * (1 + ND)) * g.np + pos] = bulk ? (T)cc.S3b : (T)0;
+}
+
+// ---------------------------------------------------------------- stencil rows
+struct StenPtrs {
+    const int *lo;
+    const void *w;
+    const void *wv;
+    int L;
+};
+
+struct RbArgs {
+    StenPtrs th[kNC], de[kNC];
+};
+
+// rho-bar rows: out_rb[q][i] = sum - in_rb[q][i] - sub[q][i]
+template <typename T, int ND>
+__global__ void k_rb_rows(Geo g, RbArgs sa, const T *__restrict__ rho_pad, CompConsts cc, const double *__restrict__ in_rb,
+                          const double *__restrict__ sub, double *__restrict__ out_rb)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    const long long p = pad_pos(g, i);
+    T n3 = 0, n2 = 0, n1 = 0, n0 = 0, v2[ND], v1[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v2[d] = v1[d] = 0;
+    for (int c = 0; c < g.C; ++c) {
+        const T *src = rho_pad + (size_t)c * g.np + p;
+        T st = 0;
+        {
+            const int *lo = sa.th[c].lo;
+            const T *w = (const T *)sa.th[c].w;
+            const int L = sa.th[c].L;
+            for (int k = 0; k < L; ++k) st += w[k] * src[lo[k]];
+        }
+        T sd = 0, vd[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) vd[d] = 0;
+        {
+            const int *lo = sa.de[c].lo;
+            const T *w = (const T *)sa.de[c].w;
+            const T *wv = (const T *)sa.de[c].wv;
+            const int L = sa.de[c].L;
+            for (int k = 0; k < L; ++k) {
+                const T r = src[lo[k]];
+                sd += w[k] * r;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) vd[d] += wv[k * ND + d] * r;
+            }
+        }
+        n3 += st;
+        n2 += sd;
+        n1 += (T)cc.i4pir[c] * sd;
+        n0 += (T)cc.i4pirsq[c] * sd;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            v2[d] += vd[d];
+            v1[d] += (T)cc.i4pir[c] * vd[d];
+        }
+    }
+    double vals[4 + 2 * ND];
+    vals[0] = n3;
+    vals[1] = n2;
+    vals[2] = n1;
+    vals[3] = n0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        vals[4 + d] = v2[d];
+        vals[4 + ND + d] = v1[d];
+    }
+#pragma unroll
+    for (int q = 0; q < 4 + 2 * ND; ++q) {
+        double r = vals[q];
+        if (in_rb) r -= in_rb[(size_t)q * g.N + i];
+        if (sub) r -= sub[(size_t)q * g.N + i];
+        out_rb[(size_t)q * g.N + i] = r;
+    }
+}
+
+struct UArgs {
+    StenPtrs u[kNC * kNC];
+};
+
+// mean-field rows: out[c][i] = sum_c' sum_k u_cc'(k) rho_pad[c'] - in_mf - sub
+template <typename T>
+__global__ void k_mf_rows(Geo g, UArgs ua, const T *__restrict__ rho_pad, const double *__restrict__ in_mf,
+                          const double *__restrict__ sub, double *__restrict__ out_mf)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= g.N) return;
+    const long long p = pad_pos(g, i);
+    T acc = 0;
+    for (int cj = 0; cj < g.C; ++cj) {
+        const StenPtrs s = ua.u[c + g.C * cj];
+        const T *src = rho_pad + (size_t)cj * g.np + p;
+        const T *w = (const T *)s.w;
+        for (int k = 0; k < s.L; ++k) acc += w[k] * src[s.lo[k]];
+    }
+    double r = acc;
+    if (in_mf) r -= in_mf[(size_t)c * g.N + i];
+    if (sub) r -= sub[(size_t)c * g.N + i];
+    out_mf[(size_t)c * g.N + i] = r;
+}
+
+struct ElArgs {
+    StenPtrs th[kNC], de[kNC];
+};
+
+// Euler-Lagrange rows.
+//  residual: F = ln x + mu_c + Vext + adjoint sums + x_MF ; zero rows: x
+//  linear  : J v = v/x + adjoint sums + v_MF           ; zero rows: v
+// in_rho may be NULL (treated as 0); add (may be NULL) is added to the result.
+template <typename T, int ND>
+__global__ void k_el_rows(Geo g, ElArgs ea, const T *__restrict__ adj, CompConsts cc, int linear, int nwall,
+                          const double *__restrict__ x_rho, const double *__restrict__ in_rho,
+                          const double *__restrict__ in_mf, const double *__restrict__ vext,
+                          const unsigned char *__restrict__ elzero, const double *__restrict__ add,
+                          double *__restrict__ out_el)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= g.N) return;
+    const size_t row = (size_t)c * g.N + i;
+    double r;
+    if (elzero[row]) {
+        r = in_rho ? in_rho[row] : 0.0;
+    } else {
+        const long long p = pad_pos(g, i);
+        const T *A = adj + (size_t)(c * (1 + ND)) * g.np + p;
+        T acc = 0;
+        {
+            const int *lo = ea.de[c].lo;
+            const T *w = (const T *)ea.de[c].w;
+            const T *wv = (const T *)ea.de[c].wv;
+            const int L = ea.de[c].L;
+            for (int k = 0; k < L; ++k) {
+                const long long q = lo[k];
+                T t = w[k] * A[q];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) t -= wv[k * ND + d] * A[(size_t)(1 + d) * g.np + q];
+                acc += t;
+            }
+        }
+        {
+            const T *S3 = adj + (size_t)(g.C * (1 + ND)) * g.np + p;
+            const int *lo = ea.th[c].lo;
+            const T *w = (const T *)ea.th[c].w;
+            const int L = ea.th[c].L;
+            for (int k = 0; k < L; ++k) acc += w[k] * S3[lo[k]];
+        }
+        r = (double)acc;
+        const double xv = x_rho[row];
+        if (!linear) {
+            r += log(xv) + cc.mu[c];
+            if (nwall > 0) r += vext[row];
+            if (in_mf) r += in_mf[row];
+        } else {
+            if (in_rho) r += in_rho[row] / xv;
+            if (in_mf) r += in_mf[row];
+        }
+    }
+    if (add) r += add[row];
+    out_el[row] = r;
+}
+
+// ---------------------------------------------------------------- init fields
+// calc_init_rho_bar (src/dft_guess_HSFMT.c:70-83): rho-bar = forward sums of the
+// densities with bulk halo, n3 capped at 0.98 when above 1
+__global__ void k_cap_n3(double *n3, int N)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N && n3[i] > 1.0) n3[i] = 0.98;
+}
+
+// ---------------------------------------------------------------- vector ops
+template <int B>
+__device__ inline double block_sum(double v)
+{
+    __shared__ double sh[B / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        v = (lane < B / 64) ? sh[lane] : 0.0;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    }
+    __syncthreads();
+    return v;
+}
+
+// partial[j*gridDim.x + blk] = sum over the block's range of V_j . w
+__global__ void k_multidot(const double *__restrict__ V, long long ld, int k, const double *__restrict__ w, long long n,
+                           double *__restrict__ partial)
+{
+    for (int j = 0; j < k; ++j) {
+        double s = 0.0;
+        for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+            s += V[(size_t)j * ld + i] * w[i];
+        s = block_sum<kBlock>(s);
+        if (threadIdx.x == 0) partial[(size_t)j * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k, double *__restrict__ out)
+{
+    int j = blockIdx.x;
+    if (j >= k) return;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[(size_t)j * nb + b];
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+__global__ void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+__global__ void k_scale_copy(double a, const double *__restrict__ x, double *__restrict__ y, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = a * x[i];
+}
+
+__global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ y, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = a[i] * b[i];
+}
+
+// w -= V h (k columns)
+__global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, const double *__restrict__ h,
+                           double *__restrict__ w, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * h[j];
+    w[i] -= s;
+}
+
+__global__ void k_gemv_add(const double *__restrict__ V, long long ld, int k, const double *__restrict__ y,
+                           double *__restrict__ x, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * y[j];
+    x[i] += s;
+}
+
+// inverse of the A22 diagonal: x (EL rows: dF/drho = 1/rho) or 1 on zero rows
+__global__ void k_a22_dinv(const double *__restrict__ x_rho, const unsigned char *__restrict__ elzero,
+                           double *__restrict__ dinv, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dinv[i] = elzero[i] ? 1.0 : x_rho[i];
+}
+
+// update_solution (src/dft_newton.c:266-380): fraction limited by density positivity
+__global__ void k_frac_min(const double *__restrict__ x, const double *__restrict__ dx, long long n, double scaling,
+                           double *__restrict__ partial)
+{
+    double m = 1.0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const double xv = x[i], d = dx[i];
+        if (xv + d < 0.0) {
+            double f = fmin(1.0, xv / (-d));
+            f = fmax(f, scaling);
+            m = fmin(m, f);
+        }
+    }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_update_norm(const double *__restrict__ x, const double *__restrict__ dx, long long n, double frac,
+                              double rel, double abs_tol, double *__restrict__ partial)
+{
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const double t = (frac * dx[i]) / (rel * x[i] + abs_tol);
+        s += t * t;
+    }
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_apply_update(double *__restrict__ x, const double *__restrict__ dx, long long n, long long n_rho,
+                               int N, int iu_n3, double frac)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double xv = x[i], d = dx[i];
+    const bool dens = i < n_rho;
+    const bool is_n3 = (i / N) == iu_n3;
+    if (xv + frac * d < 1.e-99 && dens)
+        x[i] = 0.1 * xv;
+    else if (is_n3 && xv + frac * d > 1.0)
+        x[i] = xv + 0.5 * (1.0 - xv);
+    else
+        x[i] = xv + frac * d;
+}
+
+inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+CompConsts make_consts(const Problem &P)
+{
+    CompConsts cc{};
+    const auto &b = P.bulk;
+    for (int c = 0; c < P.p.ncomp; ++c) {
+        cc.i4pir[c] = b.inv_4pir[c];
+        cc.i4pirsq[c] = b.inv_4pirsq[c];
+        cc.rho_b[c] = P.p.rho_b[c];
+        double mu = 0.0;
+        mu -= std::log(P.p.rho_b[c]);
+        mu -= b.betamu_hs_ex[c];
+        if (P.p.type_attr >= 0) mu -= b.betamu_att[c];
+        cc.mu[c] = mu;
+        cc.Ab[c] = b.dphi_b[0] * b.inv_4pirsq[c] + b.dphi_b[1] * b.inv_4pir[c] + b.dphi_b[2];
+    }
+    cc.S3b = b.dphi_b[3];
+    return cc;
+}
+
+template <typename T>
+const DevStencilT<T> &dev_sten(const Problem &P, int type, int idx);
+template <>
+const DevStencilT<double> &dev_sten<double>(const Problem &P, int type, int idx)
+{
+    return P.dsd[type][idx];
+}
+template <>
+const DevStencilT<float> &dev_sten<float>(const Problem &P, int type, int idx)
+{
+    return P.dsf[type][idx];
+}
+
+template <typename T>
+Workspace<T> &workspace(Problem &P);
+template <>
+Workspace<double> &workspace<double>(Problem &P)
+{
+    return P.wsd;
+}
+template <>
+Workspace<float> &workspace<float>(Problem &P)
+{
+    return P.wsf;
+}
+
+// lo tables: dev stencil arrays hold lo for the density geometry at lo[0..L)
+// and for the FMT geometry at lo[L..2L)
+template <typename T>
+StenPtrs sten_ptrs(const Problem &P, int type, int idx, bool fmt_geom)
+{
+    const DevStencilT<T> &s = dev_sten<T>(P, type, idx);
+    StenPtrs sp;
+    sp.L = s.L;
+    sp.lo = s.lo + (fmt_geom ? s.L : 0);
+    sp.w = s.w;
+    sp.wv = s.wv;
+    return sp;
+}
+
+template <typename T>
+void pad_rho(Problem &P, const double *in, bool linear, hipStream_t s)
+{
+    Geo gu = make_geo(P, P.pad_u);
+    hipLaunchKernelGGL(k_pad_rho<T>, dim3(grid_for(gu.np)), dim3(kBlock), 0, s, gu, in, P.d_zero, make_consts(P),
+                       linear ? 1 : 0, workspace<T>(P).rho_pad);
+}
+
+template <typename T>
+void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb, hipStream_t s)
+{
+    Geo gu = make_geo(P, P.pad_u);
+    RbArgs ra{};
+    for (int c = 0; c < P.p.ncomp; ++c) {
+        ra.th[c] = sten_ptrs<T>(P, TMI_THETA, c, false);
+        ra.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, false);
+    }
+    const CompConsts cc = make_consts(P);
+    const T *rp = workspace<T>(P).rho_pad;
+    switch (P.p.ndim) {
+    case 1: hipLaunchKernelGGL((k_rb_rows<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
+    case 2: hipLaunchKernelGGL((k_rb_rows<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
+    default: hipLaunchKernelGGL((k_rb_rows<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
+    }
+}
+
+template <typename T>
+void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
+{
+    if (P.nmf == 0) return;
+    Geo gu = make_geo(P, P.pad_u);
+    UArgs ua{};
+    for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ua.u[q] = sten_ptrs<T>(P, TMI_UATT, q, false);
+    hipLaunchKernelGGL(k_mf_rows<T>, dim3(grid_for(P.N), P.p.ncomp), dim3(kBlock), 0, s, gu, ua,
+                       workspace<T>(P).rho_pad, in_mf, sub, out_mf);
+}
+
+template <typename T>
+void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear, hipStream_t s)
+{
+    Geo gf = make_geo(P, P.pad_f);
+    const CompConsts cc = make_consts(P);
+    T *adj = workspace<T>(P).adj_pad;
+    const int lin = linear ? 1 : 0;
+    const int func = P.p.type_func;
+    switch (P.p.ndim) {
+    case 1:
+        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_pad_adj<T, 1>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_pad_adj<T, 2>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        break;
+    default:
+        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_pad_adj<T, 3>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        break;
+    }
+}
+
+template <typename T>
+void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
+             double *out_el, hipStream_t s)
+{
+    Geo gf = make_geo(P, P.pad_f);
+    ElArgs ea{};
+    for (int c = 0; c < P.p.ncomp; ++c) {
+        ea.th[c] = sten_ptrs<T>(P, TMI_THETA, c, true);
+        ea.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, true);
+    }
+    const CompConsts cc = make_consts(P);
+    const T *adj = workspace<T>(P).adj_pad;
+    dim3 grid(grid_for(P.N), P.p.ncomp);
+    const int lin = linear ? 1 : 0;
+    switch (P.p.ndim) {
+    case 1:
+        hipLaunchKernelGGL((k_el_rows<T, 1>), grid, dim3(kBlock), 0, s, gf, ea, adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf,
+                           P.d_vext, P.d_elzero, add, out_el);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_el_rows<T, 2>), grid, dim3(kBlock), 0, s, gf, ea, adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf,
+                           P.d_vext, P.d_elzero, add, out_el);
+        break;
+    default:
+        hipLaunchKernelGGL((k_el_rows<T, 3>), grid, dim3(kBlock), 0, s, gf, ea, adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf,
+                           P.d_vext, P.d_elzero, add, out_el);
+        break;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- public launchers
+void launch_residual(Problem &P, const double *x, double *F, hipStream_t s)
+{
+    const size_t N = P.N;
+    pad_rho<double>(P, x + P.iu_rho * N, false, s);
+    rb_rows<double>(P, x + P.iu_rb * N, nullptr, F + P.iu_rb * N, s);
+    mf_rows<double>(P, P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s);
+    fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
+    el_rows<double>(P, false, x + P.iu_rho * N, x + P.iu_rho * N, P.nmf ? x + P.iu_mf * N : nullptr, nullptr,
+                    F + P.iu_rho * N, s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s)
+{
+    const size_t N = P.N;
+    pad_rho<T>(P, v + P.iu_rho * N, true, s);
+    rb_rows<T>(P, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N, s);
+    mf_rows<T>(P, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_mf * N, s);
+    fmt_point<T>(P, x + P.iu_rb * N, v + P.iu_rb * N, true, s);
+    el_rows<T>(P, true, x + P.iu_rho * N, v + P.iu_rho * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_rho * N,
+               s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+template void launch_jvp<double>(Problem &, const double *, const double *, double *, hipStream_t);
+template void launch_jvp<float>(Problem &, const double *, const double *, double *, hipStream_t);
+
+void launch_init_fields(Problem &P, double *x, hipStream_t s)
+{
+    const size_t N = P.N;
+    pad_rho<double>(P, x + P.iu_rho * N, false, s);
+    rb_rows<double>(P, nullptr, nullptr, x + P.iu_rb * N, s);
+    hipLaunchKernelGGL(k_cap_n3, dim3(grid_for(P.N)), dim3(kBlock), 0, s, x + P.iu_rb * N, P.N);
+    mf_rows<double>(P, nullptr, nullptr, x + P.iu_mf * N, s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
+// t1 = A12 x2 (rho-bar and mean-field rows of J [x2; 0]); y2 = A22 x2 + A21 t1
+template <typename T>
+void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t1, double *y2, hipStream_t s)
+{
+    const size_t N = P.N;
+    pad_rho<T>(P, x2, true, s);
+    rb_rows<T>(P, nullptr, nullptr, t1, s);
+    mf_rows<T>(P, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
+    fmt_point<T>(P, x + P.iu_rb * N, t1, true, s);
+    el_rows<T>(P, true, x + P.iu_rho * N, x2, P.nmf ? t1 + (size_t)P.nrb * N : nullptr, nullptr, y2, s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+template void launch_schur_apply<double>(Problem &, const double *, const double *, double *, double *, hipStream_t);
+template void launch_schur_apply<float>(Problem &, const double *, const double *, double *, double *, hipStream_t);
+
+// bS = b2 + A21 b1  (A11 = -I, so -A21 A11^{-1} b1 = A21 b1)
+void launch_schur_rhs(Problem &P, const double *x, const double *b, double *bS, hipStream_t s)
+{
+    const size_t N = P.N;
+    fmt_point<double>(P, x + P.iu_rb * N, b + P.iu_rb * N, true, s);
+    el_rows<double>(P, true, x + P.iu_rho * N, nullptr, P.nmf ? b + P.iu_mf * N : nullptr, b + P.iu_rho * N, bS, s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
+// x1 = A11^{-1}(b1 - A12 x2) = A12 x2 - b1
+void launch_schur_backsub(Problem &P, const double *x, const double *b, double *dx, hipStream_t s)
+{
+    (void)x;
+    const size_t N = P.N;
+    pad_rho<double>(P, dx + P.iu_rho * N, true, s);
+    rb_rows<double>(P, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, s);
+    if (P.nmf) mf_rows<double>(P, nullptr, b + P.iu_mf * N, dx + P.iu_mf * N, s);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- vector ops
+static unsigned red_blocks(long long n) { return (unsigned)std::min<long long>(1024, (n + kBlock - 1) / kBlock); }
+
+void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
+                   hipStream_t s)
+{
+    unsigned nb = red_blocks(n);
+    double *partial = P.d_red;            // [k][nb]
+    double *res = P.d_red + (size_t)k * 1024;
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, partial);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, partial, (int)nb, k, res);
+    TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, res, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    for (int j = 0; j < k; ++j) out[j] = P.h_red[j];
+}
+
+double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s)
+{
+    double r;
+    dev_multi_dot(P, a, 0, 1, b, n, &r, s);
+    return r;
+}
+
+void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+void dev_mul(const double *a, const double *b, double *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, y, n);
+}
+void dev_gemv_sub(const double *V, long long ld, int k, const double *h, double *w, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemv_sub, dim3(grid_for(n)), dim3(kBlock), 0, s, V, ld, k, h, w, n);
+}
+void dev_gemv_add(const double *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemv_add, dim3(grid_for(n)), dim3(kBlock), 0, s, V, ld, k, y, x, n);
+}
+void dev_a22_inverse_diag(Problem &P, const double *x, double *dinv, hipStream_t s)
+{
+    long long n = (long long)P.p.ncomp * P.N;
+    hipLaunchKernelGGL(k_a22_dinv, dim3(grid_for(n)), dim3(kBlock), 0, s, x + (size_t)P.iu_rho * P.N, P.d_elzero, dinv, n);
+}
+
+void dev_newton_update(Problem &P, double *x, const double *dx, const tmi_newton_opts &o, double *frac_out,
+                       double *unorm_out, hipStream_t s)
+{
+    const long long n = (long long)P.nunk * P.N, n_rho = (long long)P.p.ncomp * P.N;
+    unsigned nb = red_blocks(n_rho);
+    hipLaunchKernelGGL(k_frac_min, dim3(nb), dim3(kBlock), 0, s, x + (size_t)P.iu_rho * P.N, dx + (size_t)P.iu_rho * P.N,
+                       n_rho, o.update_scaling, P.d_red);
+    TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, P.d_red, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    double frac = 1.0;
+    for (unsigned b = 0; b < nb; ++b) frac = std::min(frac, P.h_red[b]);
+    unsigned nb2 = red_blocks(n);
+    hipLaunchKernelGGL(k_update_norm, dim3(nb2), dim3(kBlock), 0, s, x, dx, n, frac, o.nl_rel_tol, o.nl_abs_tol, P.d_red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb2, 1, P.d_red + 1024);
+    TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, P.d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    const double unorm = std::sqrt(P.h_red[0]);
+    hipLaunchKernelGGL(k_apply_update, dim3(grid_for(n)), dim3(kBlock), 0, s, x, dx, n, n_rho, P.N, P.iu_rb, frac);
+    TMI_HIP_CHECK(hipGetLastError());
+    *frac_out = frac;
+    *unorm_out = unorm;
+}
+
+}  // namespace tmi

@@ -1,0 +1,133 @@
+"""Host-side handle of a device problem: the Python mirror of the C ABI.
+
+Method names follow the reference interface they stand in for:
+
+* ``fill_residual``   -- fill_resid_and_matrix_control(x, iter, TRUE)   (src/dft_fill_control.c:39)
+* ``apply_jacobian``  -- dft_linprobmgr_applymatrix / Schur ApplyGlobal    (src/dft_Schur_Epetra_Operator.cpp:149)
+* ``init_fields``     -- calc_init_rho_bar + calc_init_mf_attract          (src/dft_guess_HSFMT.c:70, src/dft_guess_MFATT.c:66)
+* ``newton``          -- newton_solver                                     (src/dft_newton.c:131)
+
+Device memory and streams come from PyTorch; the arithmetic is the HIP
+library's.  Tensors are float64, shape [nunk, nnodes], on cuda:0.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import ProblemSpec
+
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+def _stream() -> ct.c_void_p:
+    return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ct.c_void_p:
+    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    return ct.c_void_p(t.data_ptr())
+
+
+class DeviceProblem:
+    def __init__(self, spec: ProblemSpec):
+        L = _lib.lib()
+        self.spec = spec
+        self.params = _lib.fill_params(_lib.TmiParams(), spec.as_params())
+        C, N = spec.ncomp, spec.nnodes
+        vext = None if spec.vext is None else np.ascontiguousarray(spec.vext, dtype=np.float64).reshape(N, C)
+        zero = None if spec.zero_tf is None else np.ascontiguousarray(spec.zero_tf, dtype=np.uint8).reshape(N, C)
+        self._keep = (vext, zero)
+        h = ct.c_void_p()
+        _lib.check(L.tmi_problem_create(ct.byref(self.params),
+                                        None if vext is None else vext.ctypes.data_as(ct.c_void_p),
+                                        None if zero is None else zero.ctypes.data_as(ct.c_void_p),
+                                        ct.byref(h)), "tmi_problem_create")
+        self.handle = h
+        info = _lib.TmiInfo()
+        _lib.check(L.tmi_problem_info(h, ct.byref(info)), "tmi_problem_info")
+        self.info = info
+        self.nunk, self.nnodes = info.nunk, info.nnodes
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().tmi_problem_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ tables
+    def stencil(self, isten: int, icomp: int = 0, jcomp: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+        L = _lib.lib()
+        n = ct.c_int()
+        _lib.check(L.tmi_stencil_get(self.handle, isten, icomp, jcomp, ct.byref(n), None, None), "tmi_stencil_get")
+        off = np.zeros((n.value, 3), dtype=np.int32)
+        w = np.zeros(n.value)
+        _lib.check(L.tmi_stencil_get(self.handle, isten, icomp, jcomp, ct.byref(n), off.ctypes.data_as(ct.c_void_p),
+                                     w.ctypes.data_as(ct.c_void_p)), "tmi_stencil_get")
+        return off, w
+
+    def bulk(self) -> np.ndarray:
+        out = np.zeros(60)
+        _lib.check(_lib.lib().tmi_bulk_get(self.handle, out.ctypes.data_as(ct.c_void_p)), "tmi_bulk_get")
+        return out
+
+    # ------------------------------------------------------------ device ops
+    def zeros(self) -> torch.Tensor:
+        return torch.zeros((self.nunk, self.nnodes), dtype=torch.float64, device="cuda")
+
+    def fill_residual(self, x: torch.Tensor, F: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, float]:
+        F = self.zeros() if F is None else F
+        s = ct.c_double()
+        _lib.check(_lib.lib().tmi_fill_residual(self.handle, _ptr(x), _ptr(F), ct.byref(s), _stream()),
+                   "tmi_fill_residual")
+        return F, s.value
+
+    def apply_jacobian(self, x: torch.Tensor, v: torch.Tensor, out: Optional[torch.Tensor] = None,
+                       fp32: bool = False) -> torch.Tensor:
+        out = self.zeros() if out is None else out
+        fn = _lib.lib().tmi_apply_jacobian_fp32 if fp32 else _lib.lib().tmi_apply_jacobian
+        _lib.check(fn(self.handle, _ptr(x), _ptr(v), _ptr(out), _stream()), "tmi_apply_jacobian")
+        return out
+
+    def apply_jacobian_transpose(self, x: torch.Tensor, w: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        out = self.zeros() if out is None else out
+        _lib.check(_lib.lib().tmi_apply_jacobian_transpose(self.handle, _ptr(x), _ptr(w), _ptr(out), _stream()),
+                   "tmi_apply_jacobian_transpose")
+        return out
+
+    def init_fields(self, x: torch.Tensor) -> torch.Tensor:
+        _lib.check(_lib.lib().tmi_init_fields(self.handle, _ptr(x), _stream()), "tmi_init_fields")
+        return x
+
+    def schur_solve(self, x: torch.Tensor, F: torch.Tensor, rtol=1e-10, restart=100, maxit=1000,
+                    precision=PREC_FP64) -> Tuple[torch.Tensor, int]:
+        dx = self.zeros()
+        its = ct.c_int()
+        _lib.check(_lib.lib().tmi_schur_solve(self.handle, _ptr(x), _ptr(F), _ptr(dx), rtol, restart, maxit, precision,
+                                              ct.byref(its), _stream()), "tmi_schur_solve")
+        return dx, its.value
+
+    def newton(self, x: torch.Tensor, max_iter=50, resid_tol=1e-8, nl_rel_tol=1e-3, nl_abs_tol=1e-8,
+               update_scaling=0.1, gmres_restart=60, gmres_max_iter=600, gmres_tol=1e-6,
+               precision=PREC_FP64, verbose=False) -> Dict:
+        o = _lib.TmiNewtonOpts(max_iter, nl_rel_tol, nl_abs_tol, update_scaling, resid_tol, gmres_restart,
+                               gmres_max_iter, gmres_tol, precision, 1 if verbose else 0)
+        st = _lib.TmiNewtonStats()
+        _lib.check(_lib.lib().tmi_newton_solve(self.handle, _ptr(x), ct.byref(o), ct.byref(st), _stream()),
+                   "tmi_newton_solve")
+        return dict(iters=st.iters, converged=bool(st.converged), gmres_iters=st.gmres_iters_total,
+                    resid_norm=st.resid_norm, seconds=st.seconds)
+
+    def initial_state(self) -> torch.Tensor:
+        x = torch.from_numpy(self.spec.initial_x()).to("cuda")
+        return self.init_fields(x.contiguous())

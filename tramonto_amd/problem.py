@@ -1,0 +1,290 @@
+"""Problem descriptions and the synthetic state points the benchmarks and tests use.
+
+A :class:`ProblemSpec` carries exactly the inputs the reference fill reads from
+its globals (grid, boundary conditions, functional switches, fluid parameters,
+``Vext`` and ``Zero_density_TF``) plus a seeded density field.  The builders
+below construct the five state points of the hot-path workload:
+
+* ``hard_wall_1d``     -- 1D Rosenfeld hard spheres against a hard planar wall
+* ``periodic_3d``      -- 3D periodic Rosenfeld + WCA mean-field fluid
+* ``slit_pores_1d``    -- binary mixture in LJ 9-3 slit pores (one pore each)
+* ``rough_wall_2d``    -- 2D fluid at a rough LJ 9-3 wall
+* ``solute_3d``        -- 3D White Bear mixture around a spherical solute
+
+Random inputs come from SplitMix64 so that every harness (C, HIP, numpy)
+reproduces them bit for bit.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional
+
+import numpy as np
+
+IN_WALL, IN_BULK, PERIODIC, REFLECT = -1, 0, 1, 2
+FMT1, FMT3 = 0, 2
+NCOMP_MAX = 4
+
+
+# --------------------------------------------------------------------------- RNG
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): 64-bit state, golden-gamma increment."""
+
+    GAMMA = np.uint64(0x9E3779B97F4A7C15)
+    M1 = np.uint64(0xBF58476D1CE4E5B9)
+    M2 = np.uint64(0x94D049BB133111EB)
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def next_u64(self, n: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            steps = np.arange(1, n + 1, dtype=np.uint64)
+            z = self.state + steps * self.GAMMA
+            self.state = self.state + np.uint64(n) * self.GAMMA
+            z = (z ^ (z >> np.uint64(30))) * self.M1
+            z = (z ^ (z >> np.uint64(27))) * self.M2
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        u = (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+
+# --------------------------------------------------------------------------- spec
+@dataclasses.dataclass
+class ProblemSpec:
+    name: str
+    ndim: int
+    ncomp: int
+    nx: List[int]
+    dx: List[float]
+    bc: List[List[int]]
+    type_func: int = FMT1
+    type_attr: int = -1
+    nwall: int = 0
+    vext_max: float = 10.0
+    hs_diam: List[float] = dataclasses.field(default_factory=lambda: [1.0])
+    sigma_ff: Optional[np.ndarray] = None
+    eps_ff: Optional[np.ndarray] = None     # epsilon / kT
+    cut_ff: Optional[np.ndarray] = None
+    rho_b: List[float] = dataclasses.field(default_factory=lambda: [0.5])
+    vext: Optional[np.ndarray] = None       # [N, C]
+    zero_tf: Optional[np.ndarray] = None    # [N, C] uint8
+    rho0: Optional[np.ndarray] = None       # [C, N] initial densities
+
+    @property
+    def nnodes(self) -> int:
+        return int(np.prod(self.nx[: self.ndim]))
+
+    @property
+    def nrb(self) -> int:
+        return 4 + 2 * self.ndim
+
+    @property
+    def nmf(self) -> int:
+        return self.ncomp if self.type_attr >= 0 else 0
+
+    @property
+    def nunk(self) -> int:
+        return self.ncomp + self.nrb + self.nmf
+
+    def as_params(self) -> Dict:
+        C = self.ncomp
+        pad = lambda v: list(v) + [0.0] * (NCOMP_MAX - len(v))
+        mat = lambda m: [[float(m[i][j]) if (i < C and j < C) else 0.0 for j in range(NCOMP_MAX)] for i in range(NCOMP_MAX)]
+        sig = self.sigma_ff if self.sigma_ff is not None else np.ones((C, C))
+        eps = self.eps_ff if self.eps_ff is not None else np.zeros((C, C))
+        cut = self.cut_ff if self.cut_ff is not None else np.zeros((C, C))
+        return dict(
+            ndim=self.ndim, ncomp=C,
+            nx=list(self.nx) + [1] * (3 - len(self.nx)),
+            dx=list(self.dx) + [1.0] * (3 - len(self.dx)),
+            bc=[list(b) for b in self.bc] + [[IN_BULK, IN_BULK]] * (3 - len(self.bc)),
+            type_func=self.type_func, type_attr=self.type_attr, nwall=self.nwall, lhard_surf=0,
+            vext_max=self.vext_max, hs_diam=pad(self.hs_diam), sigma_ff=mat(sig), eps_ff=mat(eps),
+            cut_ff=mat(cut), rho_b=pad(self.rho_b),
+        )
+
+    def initial_x(self) -> np.ndarray:
+        """x[nunk, N] with the seeded densities; rho-bar / mean-field rows zero
+        (fill them with the library's init_fields)."""
+        x = np.zeros((self.nunk, self.nnodes))
+        x[: self.ncomp] = self.rho0
+        return x
+
+    def positions(self) -> List[np.ndarray]:
+        return [np.arange(self.nx[d]) * self.dx[d] for d in range(self.ndim)]
+
+
+def lorentz_berthelot(sig, eps, cut):
+    """Mixing rules of pot_parameters (src/dft_potential_params.c:61-63)."""
+    C = len(sig)
+    S = np.zeros((C, C)); E = np.zeros((C, C)); R = np.zeros((C, C))
+    for i in range(C):
+        for j in range(C):
+            S[i, j] = 0.5 * (sig[i] + sig[j])
+            E[i, j] = math.sqrt(eps[i] * eps[j])
+            R[i, j] = 0.5 * (cut[i] + cut[j])
+    return S, E, R
+
+
+def nodes_along(size: float, dx: float, periodic: bool) -> int:
+    """Nodes_x = round(Size_x/Esize_x) (+1 unless periodic)."""
+    n = int(math.floor(size / dx + 0.5))
+    return n if periodic else n + 1
+
+
+def lj93_wall(z: np.ndarray, eps_w: float, sigma_w: float, cut: float) -> np.ndarray:
+    """Cut-and-shifted LJ 9-3 wall potential (src/dft_vext_1D.c:42-56 form:
+    eps_w [ (2/15)(s/z)^9 - (s/z)^3 ] shifted to zero at the cut-off)."""
+    def u(zz):
+        r = sigma_w / zz
+        return eps_w * ((2.0 / 15.0) * r ** 9 - r ** 3)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        out = np.where(z <= cut, u(np.maximum(z, 1e-300)) - u(cut), 0.0)
+    return out
+
+
+# --------------------------------------------------------------------------- builders
+def hard_wall_1d(seed: int = 1, size: float = 20.0, dx: float = 0.05, rho_b: float = 0.7) -> ProblemSpec:
+    """1D Rosenfeld hard spheres (sigma = 1) against a hard planar wall at x = 0,
+    bulk on the right.  Centres are excluded within sigma/2 of the wall:
+    V_ext = VEXT_MAX and Zero_density_TF there; seeded densities
+    rho_i = rho_b (1 + 0.1 U(-1,1)) beyond 0.5 sigma."""
+    nx = nodes_along(size, dx, False)
+    x = np.arange(nx) * dx
+    excl = x < 0.5 - 1e-12
+    vext = np.where(excl, 10.0, 0.0)[:, None]
+    zero = excl.astype(np.uint8)[:, None]
+    rng = SplitMix64(seed)
+    rho = rho_b * (1.0 + 0.1 * rng.uniform(nx, -1.0, 1.0))
+    rho[excl] = 0.0
+    return ProblemSpec(name="hard_wall_1d", ndim=1, ncomp=1, nx=[nx], dx=[dx], bc=[[IN_WALL, IN_BULK]],
+                       type_func=FMT1, type_attr=-1, nwall=1, vext_max=10.0, hs_diam=[1.0],
+                       sigma_ff=np.ones((1, 1)), rho_b=[rho_b], vext=vext, zero_tf=zero, rho0=rho[None, :])
+
+
+def periodic_3d(seed: int = 2, n: int = 64, dx: float = 0.1, rho_b: float = 0.6, cut: float = 2.5,
+                eps: float = 1.0, kT: float = 1.0, func: int = FMT1, modes: int = 8) -> ProblemSpec:
+    """3D periodic Rosenfeld + WCA mean-field fluid (LJ 12-6, eps, cut-off `cut`).
+    Seeded field rho = clip(rho_b + 0.2 f, 0.05, 0.9) with
+    f = (1/modes) sum_m cos(k_m . r + phi_m); each k_m = 2 pi n_m / L is a lattice
+    vector of the periodic box with wavelength in [2, 6] sigma."""
+    L = n * dx
+    rng = SplitMix64(seed)
+    ks = []
+    while len(ks) < modes:
+        v = np.round(rng.uniform(3, -4.0, 4.0)).astype(int)
+        nrm = float(np.linalg.norm(v))
+        if nrm == 0:
+            continue
+        lam = L / nrm
+        if 2.0 <= lam <= 6.0:
+            ks.append(v)
+    phis = rng.uniform(modes, 0.0, 2 * math.pi)
+    g = np.arange(n) * dx
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    f = np.zeros_like(X)
+    for v, ph in zip(ks, phis):
+        k = 2 * math.pi * np.asarray(v, dtype=float) / L
+        f += np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)
+    f /= modes
+    rho = np.clip(rho_b + 0.2 * f, 0.05, 0.9)
+    # node index = i + j nx + k nx ny: x fastest
+    rho = np.transpose(rho, (2, 1, 0)).reshape(-1)
+    attr = 0 if cut > 0 else -1
+    return ProblemSpec(name="periodic_3d", ndim=3, ncomp=1, nx=[n, n, n], dx=[dx, dx, dx],
+                       bc=[[PERIODIC, PERIODIC]] * 3, type_func=func, type_attr=attr, nwall=0,
+                       hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.full((1, 1), eps / kT),
+                       cut_ff=np.full((1, 1), cut), rho_b=[rho_b], vext=None, zero_tf=None,
+                       rho0=rho[None, :])
+
+
+def slit_pore_1d(H: float, dx: float, sig=(1.0, 0.8), rho_b=(0.2, 0.2), eps_w: float = 1.0,
+                 cut_fac: float = 2.5, seed: int = 3, vext_max: float = 10.0) -> ProblemSpec:
+    """Binary Rosenfeld + WCA fluid in a slit of width H between two LJ 9-3
+    walls (eps_w, sigma_wf = sigma_i, cut-off 2.5 sigma_i), IN_WALL outside.
+    Nodes with V_ext >= VEXT_MAX are zero-density nodes."""
+    C = len(sig)
+    nx = nodes_along(H, dx, False)
+    z = np.arange(nx) * dx
+    vext = np.zeros((nx, C))
+    for c in range(C):
+        v = lj93_wall(z, eps_w, sig[c], 2.5 * sig[c] + 100.0) + lj93_wall(H - z, eps_w, sig[c], 2.5 * sig[c] + 100.0)
+        v = np.where(np.isfinite(v), v, vext_max)
+        v = np.where(v >= vext_max, vext_max, v)
+        vext[:, c] = v
+    zero = (vext >= vext_max).astype(np.uint8)
+    S, E, R = lorentz_berthelot(list(sig), [1.0] * C, [cut_fac * s for s in sig])
+    rng = SplitMix64(seed)
+    rho0 = np.zeros((C, nx))
+    for c in range(C):
+        rho0[c] = rho_b[c] * np.exp(-np.minimum(vext[:, c], 50.0)) * (1.0 + 0.05 * rng.uniform(nx, -1, 1))
+        rho0[c][zero[:, c] == 1] = 0.0
+    return ProblemSpec(name="slit_pore_1d", ndim=1, ncomp=C, nx=[nx], dx=[dx], bc=[[IN_WALL, IN_WALL]],
+                       type_func=FMT1, type_attr=0, nwall=2, vext_max=vext_max, hs_diam=list(sig),
+                       sigma_ff=S, eps_ff=E, cut_ff=R, rho_b=list(rho_b), vext=vext, zero_tf=zero, rho0=rho0)
+
+
+def small_case(kind: str, seed: int = 7) -> ProblemSpec:
+    """Small state points the oracle evaluates in seconds (parity tests)."""
+    if kind == "1d_hw":
+        return hard_wall_1d(seed=seed, size=6.0, dx=0.05)
+    if kind == "1d_hw_full":
+        return hard_wall_1d(seed=seed)
+    if kind == "1d_reflect_wca":
+        sp = hard_wall_1d(seed=seed, size=5.0, dx=0.05, rho_b=0.6)
+        sp.bc = [[REFLECT, IN_BULK]]
+        sp.nwall = 0
+        sp.vext = None
+        sp.zero_tf = None
+        rng = SplitMix64(seed + 11)
+        sp.rho0 = (0.6 * (1 + 0.1 * rng.uniform(sp.nnodes, -1, 1)))[None, :]
+        sp.type_attr = 0
+        sp.eps_ff = np.full((1, 1), 1.0)
+        sp.cut_ff = np.full((1, 1), 2.5)
+        return sp
+    if kind == "1d_pore_mix":
+        return slit_pore_1d(H=4.0, dx=0.02, seed=seed)
+    if kind == "2d_periodic":
+        n = 32
+        sp = periodic_3d(seed=seed, n=n, dx=0.1, cut=1.5)
+        # reduce to 2D: take the z = 0 plane
+        rho = sp.rho0.reshape(n, n, n)[0].reshape(-1)
+        return ProblemSpec(name="2d_periodic", ndim=2, ncomp=1, nx=[n, n], dx=[0.1, 0.1],
+                           bc=[[PERIODIC, PERIODIC]] * 2, type_func=FMT1, type_attr=0,
+                           hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.ones((1, 1)),
+                           cut_ff=np.full((1, 1), 1.5), rho_b=[0.6], rho0=rho[None, :])
+    if kind == "2d_wall_reflect":
+        nxx, nyy = 20, 16
+        dx = 0.1
+        rng = SplitMix64(seed)
+        x = np.arange(nxx) * dx
+        excl = np.repeat((x < 0.5 - 1e-12)[None, :], nyy, axis=0).reshape(-1)
+        rho = 0.55 * (1 + 0.1 * rng.uniform(nxx * nyy, -1, 1))
+        rho[excl] = 0.0
+        vext = np.where(excl, 10.0, 0.0)[:, None]
+        return ProblemSpec(name="2d_wall_reflect", ndim=2, ncomp=1, nx=[nxx, nyy], dx=[dx, dx],
+                           bc=[[IN_WALL, IN_BULK], [REFLECT, REFLECT]], type_func=FMT1, type_attr=0,
+                           nwall=1, hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.full((1, 1), 0.8),
+                           cut_ff=np.full((1, 1), 1.5), rho_b=[0.55], vext=vext,
+                           zero_tf=excl.astype(np.uint8)[:, None], rho0=rho[None, :])
+    if kind == "3d_periodic_small":
+        return periodic_3d(seed=seed, n=12, dx=0.25, cut=1.2)
+    if kind == "3d_periodic_wb":
+        return periodic_3d(seed=seed, n=12, dx=0.25, cut=1.2, func=FMT3)
+    if kind == "3d_mix_bulk":
+        n = 14
+        sp = periodic_3d(seed=seed, n=n, dx=0.2, cut=1.2, func=FMT3)
+        sig = [1.0, 0.8]
+        S, E, R = lorentz_berthelot(sig, [1.0, 1.0], [1.2, 1.0])
+        rng = SplitMix64(seed + 5)
+        rho0 = np.stack([0.3 * (1 + 0.2 * rng.uniform(sp.nnodes, -1, 1)), 0.25 * (1 + 0.2 * rng.uniform(sp.nnodes, -1, 1))])
+        return ProblemSpec(name="3d_mix_bulk", ndim=3, ncomp=2, nx=[n, n, n], dx=[0.2] * 3,
+                           bc=[[IN_BULK, IN_BULK], [PERIODIC, PERIODIC], [REFLECT, IN_WALL]],
+                           type_func=FMT3, type_attr=0, nwall=0, hs_diam=sig, sigma_ff=S, eps_ff=E,
+                           cut_ff=R, rho_b=[0.3, 0.25], rho0=rho0)
+    raise KeyError(kind)
