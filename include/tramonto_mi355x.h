@@ -123,6 +123,10 @@ int tmi_host_bulk(const tmi_params *p, double *out60);
  * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
  * fill_resid_and_matrix_control returns. */
 int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sumsq, void *stream);
+/* The residual fill with HIP events recorded on `stream` between its stages;
+ * stage_ms[0..4] receive the mean over `reps` fills of: density halo,
+ * rho-bar rows, mean-field rows, pointwise FMT + adjoint halo, EL rows. */
+int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int reps, double *stage_ms, void *stream);
 /* Jv = J(x) v, the assembled-Jacobian product, src/dft_Schur_Epetra_Operator.cpp:149 (ApplyGlobal) */
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
 /* same product with FP32 stencil sums and FP32 accumulation (FP64 pointwise

@@ -122,8 +122,23 @@ static void set_gauss(int n, double *gp, double *gw)
             gw[h + i] = ws[i] / 2.0;
         }
     } else if (n == 20 || n == 40) {
-        int h = n / 2;
-        gauss_exact(n, xp, wp);
+        /* computed once per rule (the tables are pure functions of n) */
+        static double cx[2][20], cw[2][20];
+        static int ready[2] = {0, 0};
+        int slot = (n == 40), h = n / 2;
+        if (!ready[slot]) {
+#pragma omp critical(tmo_gauss)
+            {
+                if (!ready[slot]) {
+                    gauss_exact(n, cx[slot], cw[slot]);
+                    ready[slot] = 1;
+                }
+            }
+        }
+        for (int i = 0; i < h; i++) {
+            xp[i] = cx[slot][i];
+            wp[i] = cw[slot][i];
+        }
         for (int i = 0; i < h; i++) {
             gp[i] = (xp[i] + 1.0) / 2.0;
             gw[i] = wp[i] / 2.0;

@@ -68,7 +68,7 @@ _lib: Optional[ct.CDLL] = None
 # exported symbols declared in include/tramonto_mi355x.h
 DECLARED_SYMBOLS = [
     "tmi_last_error", "tmi_version", "tmi_problem_create", "tmi_problem_destroy", "tmi_problem_info",
-    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
+    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "fill_resid_and_matrix_control",
     "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
@@ -103,6 +103,7 @@ def lib() -> ct.CDLL:
     L.tmi_host_bulk.argtypes = [ct.POINTER(TmiParams), vp]
     L.tmi_fill_residual.argtypes = [vp, vp, vp, ct.POINTER(d), vp]
     L.tmi_apply_jacobian.argtypes = [vp, vp, vp, vp, vp]
+    L.tmi_fill_residual_timed.argtypes = [vp, vp, vp, i, vp, vp]
     L.tmi_apply_jacobian_fp32.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_apply_jacobian_transpose.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_init_fields.argtypes = [vp, vp, vp]

@@ -560,6 +560,29 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
     });
 }
 
+int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int reps, double *stage_ms, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !F_d || !stage_ms || reps < 1) throw std::invalid_argument("bad argument");
+        hipStream_t s = (hipStream_t)stream;
+        hipEvent_t ev[6];
+        for (auto &e : ev) TMI_HIP_CHECK(hipEventCreate(&e));
+        for (int q = 0; q < 5; ++q) stage_ms[q] = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            launch_residual(*P, x_d, F_d, s, ev);
+            TMI_HIP_CHECK(hipEventSynchronize(ev[5]));
+            for (int q = 0; q < 5; ++q) {
+                float ms = 0.f;
+                TMI_HIP_CHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+                stage_ms[q] += ms / reps;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        return 0;
+    });
+}
+
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream)
 {
     return guarded([&] {

@@ -100,7 +100,7 @@ struct Problem {
 enum Mode { kResidual = 0, kLinear = 1 };
 
 // F = residual (all rows); sumsq optional (host)
-void launch_residual(Problem &P, const double *x, double *F, hipStream_t s);
+void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev = nullptr);
 // Jv for all rows, precision T for the stencil sums
 template <typename T>
 void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s);

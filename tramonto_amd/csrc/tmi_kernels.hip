@@ -693,15 +693,23 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
 }  // namespace
 
 // ---------------------------------------------------------------- public launchers
-void launch_residual(Problem &P, const double *x, double *F, hipStream_t s)
+// ev (optional): 6 events recorded between the stages pad_rho | rb_rows |
+// mf_rows | fmt_point+pad_adj | el_rows
+void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev)
 {
     const size_t N = P.N;
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
     pad_rho<double>(P, x + P.iu_rho * N, false, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[1], s));
     rb_rows<double>(P, x + P.iu_rb * N, nullptr, F + P.iu_rb * N, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[2], s));
     mf_rows<double>(P, P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
     el_rows<double>(P, false, x + P.iu_rho * N, x + P.iu_rho * N, P.nmf ? x + P.iu_mf * N : nullptr, nullptr,
                     F + P.iu_rho * N, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[5], s));
     TMI_HIP_CHECK(hipGetLastError());
 }
 

@@ -57,7 +57,25 @@ void legendre_roots(int n, double *xs, double *ws)
     }
 }
 
-GaussRule gauss_rule(int n)
+GaussRule make_gauss_rule(int n);
+
+// rules are pure functions of n: build each once
+const GaussRule &gauss_rule(int n)
+{
+    static const GaussRule r1 = make_gauss_rule(1), r3 = make_gauss_rule(3), r6 = make_gauss_rule(6),
+                           r12 = make_gauss_rule(12), r20 = make_gauss_rule(20), r40 = make_gauss_rule(40);
+    switch (n) {
+    case 1: return r1;
+    case 3: return r3;
+    case 6: return r6;
+    case 12: return r12;
+    case 20: return r20;
+    case 40: return r40;
+    default: throw std::runtime_error("unsupported Gauss rule");
+    }
+}
+
+GaussRule make_gauss_rule(int n)
 {
     GaussRule r;
     r.n = n;
@@ -320,10 +338,9 @@ class StencilBuilder {
     void interior_element(const double *xl, double rc2, double *nodal) const
     {
         const int ng = (kind_ == kUatt) ? 1 : (rc2 <= 4.000000001 ? 6 : 3);
-        const GaussRule g = gauss_rule(ng);
+        const GaussRule &g = gauss_rule(ng);
         const int nu = integrand_points(rc2);
-        GaussRule gu;
-        if (nu > 0) gu = gauss_rule(nu);
+        const GaussRule &gu = gauss_rule(nu > 0 ? nu : 1);
         const double inv_r2 = 1.0 / (radius_ * radius_);
         double pt[3];
         for (int i = 0; i < g.n; ++i) {
@@ -365,8 +382,7 @@ class StencilBuilder {
         else
             npt = (nd_ == 3) ? 1 : 20;
         const int nu = integrand_points(rc2);
-        GaussRule gu;
-        if (nu > 0) gu = gauss_rule(nu);
+        const GaussRule &gu = gauss_rule(nu > 0 ? nu : 1);
         const double inv_npt = 1.0 / (double)npt;
         const double inv_r2 = 1.0 / (radius_ * radius_);
         const bool delta = (kind_ == kDelta);

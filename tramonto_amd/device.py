@@ -92,6 +92,15 @@ class DeviceProblem:
                    "tmi_fill_residual")
         return F, s.value
 
+    def fill_residual_timed(self, x: torch.Tensor, reps: int = 10) -> np.ndarray:
+        """Mean per-stage milliseconds of `reps` fills (HIP events on the launch stream)."""
+        F = self.zeros()
+        ms = np.zeros(5)
+        _lib.check(_lib.lib().tmi_fill_residual_timed(self.handle, _ptr(x), _ptr(F), reps,
+                                                      ms.ctypes.data_as(ct.c_void_p), _stream()),
+                   "tmi_fill_residual_timed")
+        return ms
+
     def apply_jacobian(self, x: torch.Tensor, v: torch.Tensor, out: Optional[torch.Tensor] = None,
                        fp32: bool = False) -> torch.Tensor:
         out = self.zeros() if out is None else out
