@@ -55,6 +55,7 @@ def lib() -> ct.CDLL:
         L.tmo_create.restype = vp
         L.tmo_create.argtypes = [ct.POINTER(TmoParams), vp, vp]
         L.tmo_destroy.argtypes = [vp]
+        L.tmo_last_error.restype = ct.c_char_p
         L.tmo_nunk.argtypes = [vp]
         L.tmo_nnodes.argtypes = [vp]
         L.tmo_unk_layout.argtypes = [vp] + [ct.POINTER(i)] * 5
@@ -106,6 +107,8 @@ class Oracle:
         self.zero = None if zero_tf is None else np.ascontiguousarray(zero_tf, dtype=np.uint8)
         self.h = L.tmo_create(ct.byref(self.params), None if self.vext is None else _p(self.vext),
                               None if self.zero is None else _p(self.zero))
+        if not self.h:
+            raise ValueError("oracle: " + L.tmo_last_error().decode())
         self.nunk = L.tmo_nunk(self.h)
         self.N = L.tmo_nnodes(self.h)
         a = [ct.c_int() for _ in range(5)]

@@ -15,6 +15,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <setjmp.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -38,11 +39,22 @@ struct tmo_ctx {
     sten_t sten[3][NC * NC];
 };
 
+/* errors inside tmo_create unwind to it (it returns NULL); anywhere else they
+ * are fatal, since the oracle has no partial results to return */
+static jmp_buf g_create_env;
+static int g_in_create = 0;
+static char g_errmsg[256];
+static tmo_ctx *volatile g_pending = NULL;
+
 static void die(const char *msg)
 {
+    snprintf(g_errmsg, sizeof g_errmsg, "%s", msg);
+    if (g_in_create) longjmp(g_create_env, 1);
     fprintf(stderr, "tmo oracle: %s\n", msg);
     exit(2);
 }
+
+const char *tmo_last_error(void) { return g_errmsg; }
 
 int tmo_num_threads(void)
 {
@@ -830,12 +842,30 @@ static void bulk_constants(tmo_ctx *c)
 }
 
 /* ------------------------------------------------------------------------- */
+static tmo_ctx *create_impl(const tmo_params *p, const double *vext, const unsigned char *zero_tf);
+
 tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char *zero_tf)
+{
+    g_pending = NULL;
+    g_errmsg[0] = 0;
+    g_in_create = 1;
+    if (setjmp(g_create_env)) {
+        g_in_create = 0;
+        if (g_pending) tmo_destroy(g_pending);
+        return NULL;
+    }
+    tmo_ctx *c = create_impl(p, vext, zero_tf);
+    g_in_create = 0;
+    return c;
+}
+
+static tmo_ctx *create_impl(const tmo_params *p, const double *vext, const unsigned char *zero_tf)
 {
     if (p->lhard_surf) die("lhard_surf not supported");
     if (p->type_func != TMO_FMT1 && p->type_func != TMO_FMT3) die("type_func");
     if (p->ncomp < 1 || p->ncomp > NC || p->ndim < 1 || p->ndim > 3) die("dims");
     tmo_ctx *c = (tmo_ctx *)calloc(1, sizeof(tmo_ctx));
+    g_pending = c;
     c->p = *p;
     c->N = 1;
     for (int d = 0; d < p->ndim; d++) c->N *= p->nx[d];

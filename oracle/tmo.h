@@ -72,6 +72,8 @@ typedef struct {
  * Zero_density_TF[inode_box][icomp]).  Both may be NULL (no walls). */
 tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char *zero_tf);
 void tmo_destroy(tmo_ctx *c);
+/* message of the last failed tmo_create (NULL return) */
+const char *tmo_last_error(void);
 
 int tmo_nunk(const tmo_ctx *c);
 int tmo_nnodes(const tmo_ctx *c);

@@ -1,0 +1,53 @@
+"""Golden vectors from the reference's own test suite.
+
+Each Tramonto Example checks README "Key Output Parameters" with an absolute
+tolerance of 1e-3 and niter within +-4 (Examples/CMakeLists.txt:14-17,
+Examples/check_results/main.cpp:120-139).  The fixture
+tests/golden/examples_1d.json (written by make_fixtures.py) records the
+parameters and key outputs of the 1D soft-wall Rosenfeld + WCA examples; the
+oracle solves each to convergence with the reference's Newton rules and must
+reproduce the adsorption and the iteration count.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tramonto_amd.examples import adsorption, spec_from_tramonto
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "examples_1d.json")) as f:
+    EXAMPLES = json.load(f)
+
+
+def solve_oracle(g, name):
+    spec = spec_from_tramonto(g, name)
+    o = Oracle.from_spec(spec)
+    x0 = o.init_fields(spec.initial_x())
+    # resid_tol = 0 selects the reference's convergence test: weighted update norm <= 1
+    x, st = o.newton(x0, max_iter=g["max_nl_iter"], nl_rel_tol=g["nl_rel_tol"], nl_abs_tol=g["nl_abs_tol"],
+                     update_scaling=g["nl_update_scaling"], resid_tol=0.0, gmres_tol=1e-10)
+    return spec, o, x, st
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_example_key_outputs(name):
+    g = EXAMPLES[name]
+    spec, o, x, st = solve_oracle(g, name)
+    assert st["converged"], st
+    ads = adsorption(spec, g, x[: spec.ncomp])
+    ref = g["key_outputs"]
+    assert len(ref["adsorption"]) == spec.ncomp
+    for c in range(spec.ncomp):
+        assert abs(ads[c] - ref["adsorption"][c]) < 1e-3, (c, ads[c], ref["adsorption"][c])
+    assert abs(st["iters"] - ref["niter"]) <= 4, (st["iters"], ref["niter"])
+
+
+def test_example_converged_residual_is_zero():
+    g = EXAMPLES["LJ1_FMT1_WCA_1D"]
+    spec, o, x, st = solve_oracle(g, "LJ1")
+    x2, st2 = o.newton(x, resid_tol=1e-11, gmres_tol=1e-12)
+    assert st2["converged"]
+    assert np.linalg.norm(o.residual(x2)) < 1e-11
