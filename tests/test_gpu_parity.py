@@ -107,6 +107,38 @@ def test_jvp_matches_assembled_jacobian(case, device_problem):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", [c for c in CASES if c not in EXACT_ONLY])
+def test_vjp_matches_assembled_transpose(case, device_problem):
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 16)
+    r, c, v = o.jacobian_coo(x)
+    n = spec.nunk * spec.nnodes
+    JT = sps.coo_matrix((v, (c, r)), shape=(n, n)).tocsr()
+    rng = np.random.default_rng(8)
+    for trial in range(2):
+        w = rng.uniform(-1.0, 1.0, n)
+        ref = (JT @ w).reshape(spec.nunk, spec.nnodes)
+        got = dp.apply_jacobian_transpose(dev(x), dev(w.reshape(spec.nunk, spec.nnodes))).cpu().numpy()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < RTOL, (trial, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES)
+def test_vjp_is_adjoint_of_jvp(case, device_problem):
+    """<w, J v> = <J^T w, v> for the device products (all boundary types)."""
+    spec, o, dp = device_problem(case)
+    x = generic_state(spec, o, 17)
+    rng = np.random.default_rng(9)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    w = rng.uniform(-1.0, 1.0, x.shape)
+    Jv = dp.apply_jacobian(dev(x), dev(v)).cpu().numpy()
+    JTw = dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy()
+    a, b = float((w * Jv).sum()), float((JTw * v).sum())
+    assert abs(a - b) < 1e-12 * max(np.abs(w * Jv).sum(), 1.0), (a, b)
+
+
+@pytest.mark.gpu
 def test_jacobian_entries_1d(device_problem):
     """Entry-level parity: every column of J from unit-vector products."""
     spec, o, dp = device_problem("1d_reflect_wca")

@@ -79,13 +79,14 @@ void validate(const tmi_params &p)
     }
 }
 
-void set_geometry(Problem &P, PadGeom &pg, const int *halo)
+void set_geometry(Problem &P, PadGeom &pg, const int *halo, int scale = 1)
 {
     pg.n = 1;
     for (int d = 0; d < 3; ++d) {
         int n = (d < P.p.ndim) ? P.p.nx[d] : 1;
-        pg.h[d] = (d < P.p.ndim) ? halo[d] : 0;
-        if (d < P.p.ndim && P.p.bc[d][0] == TMI_PERIODIC && pg.h[d] > n)
+        pg.h[d] = (d < P.p.ndim) ? scale * halo[d] : 0;
+        // periodic images are read from one neighbouring period only
+        if (scale == 1 && d < P.p.ndim && P.p.bc[d][0] == TMI_PERIODIC && pg.h[d] > n)
             throw std::domain_error("stencil reach exceeds the periodic domain");
         pg.P[d] = n + 2 * pg.h[d];
         pg.n *= pg.P[d];
@@ -98,12 +99,17 @@ void upload_stencil(Problem &P, int type, int idx, int comp, DevStencilT<T> &ds)
     const HostStencil &hs = P.hs[type][idx];
     const int L = hs.L, nd = P.p.ndim;
     ds.L = L;
-    std::vector<int> lo(2 * (size_t)L);
+    std::vector<int> lo(4 * (size_t)L);
     std::vector<T> w(L), wv((size_t)L * nd);
+    const PadGeom *geoms[4] = {&P.pad_u, &P.pad_f, &P.pad_u2, &P.pad_f2};
     for (int k = 0; k < L; ++k) {
         const int *o = &hs.off[3 * (size_t)k];
-        lo[k] = o[0] + o[1] * P.pad_u.P[0] + o[2] * P.pad_u.P[0] * P.pad_u.P[1];
-        lo[L + k] = o[0] + o[1] * P.pad_f.P[0] + o[2] * P.pad_f.P[0] * P.pad_f.P[1];
+        for (int g = 0; g < 4; ++g) {
+            const PadGeom &pg = *geoms[g];
+            long long v = o[0] + (long long)o[1] * pg.P[0] + (long long)o[2] * pg.P[0] * pg.P[1];
+            if (v > INT32_MAX || v < INT32_MIN) throw std::domain_error("grid too large for 32-bit stencil offsets");
+            lo[(size_t)g * L + k] = (int)v;
+        }
         w[k] = (T)hs.w[k];
         for (int d = 0; d < nd; ++d) wv[(size_t)k * nd + d] = (T)(hs.w[k] * ((double)o[d] * P.p.dx[d] * P.bulk.inv_rad[comp]));
     }
@@ -160,6 +166,8 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
         }
     set_geometry(*P, P->pad_u, hu);
     set_geometry(*P, P->pad_f, hf);
+    set_geometry(*P, P->pad_u2, hu, 2);
+    set_geometry(*P, P->pad_f2, hf, 2);
 
     // node-major inputs -> component-major device rows
     const size_t N = P->N;
@@ -219,6 +227,10 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->wsd.adj_pad);
     (void)hipFree(P->wsf.rho_pad);
     (void)hipFree(P->wsf.adj_pad);
+    (void)hipFree(P->wsd.corr_u);
+    (void)hipFree(P->wsd.corr_f);
+    (void)hipFree(P->wsf.corr_u);
+    (void)hipFree(P->wsf.corr_f);
     (void)hipFree(P->d_red);
     (void)hipFree(P->d_scratch);
     if (P->h_red) (void)hipHostFree(P->h_red);
@@ -606,12 +618,9 @@ int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, 
 int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *w_d, double *JTw_d, void *stream)
 {
     return guarded([&] {
-        (void)handle;
-        (void)x_d;
-        (void)w_d;
-        (void)JTw_d;
-        (void)stream;
-        throw std::domain_error("transpose product not built yet");
+        Problem *P = as_problem(handle);
+        if (!x_d || !w_d || !JTw_d) throw std::invalid_argument("null device pointer");
+        launch_vjp<double>(*P, x_d, w_d, JTw_d, (hipStream_t)stream);
         return 0;
     });
 }

@@ -50,7 +50,7 @@ struct PadGeom {
 template <typename T>
 struct DevStencilT {
     int L = 0;
-    int *lo = nullptr;   // padded linear offsets
+    int *lo = nullptr;   // padded linear offsets, [4][L]: geometries pad_u, pad_f, pad_u2, pad_f2
     T *w = nullptr;      // weights
     T *wv = nullptr;     // [L][ndim] w * offset_d*dx_d/R (delta stencils only)
 };
@@ -69,6 +69,9 @@ template <typename T>
 struct Workspace {
     T *rho_pad = nullptr;    // [C][pad_u.n]   density (or direction) with halo
     T *adj_pad = nullptr;    // [nadj][pad_f.n] adjoint fields A_c, B_c,d, S3 with halo
+    // transpose products (allocated on first use): zero-padded row weights
+    T *corr_u = nullptr;     // [nrb + nmf][pad_u2.n]
+    T *corr_f = nullptr;     // [C][pad_f2.n]
 };
 
 struct Problem {
@@ -79,6 +82,7 @@ struct Problem {
     BulkConstants bulk;
     HostStencil hs[3][kNC * kNC];
     PadGeom pad_u, pad_f;   // density halo (max reach of all stencils), FMT halo
+    PadGeom pad_u2, pad_f2; // twice the halo: inputs of the transpose correlations
     // device tables
     DevStencilT<double> dsd[3][kNC * kNC];
     DevStencilT<float> dsf[3][kNC * kNC];

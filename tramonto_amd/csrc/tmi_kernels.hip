@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "tmi_device.hpp"
 #include "tmi_fmt.hpp"
 #include "tmi_internal.hpp"
 
@@ -31,97 +32,7 @@ namespace tmi {
 
 namespace {
 
-constexpr int kBlock = 256;
-
-struct Geo {
-    int nd, C, N;
-    int nx[3];
-    int bc[3][2];
-    int h[3], P[3];
-    long long np;
-};
-
-Geo make_geo(const Problem &P, const PadGeom &pg)
-{
-    Geo g;
-    g.nd = P.p.ndim;
-    g.C = P.p.ncomp;
-    g.N = P.N;
-    for (int d = 0; d < 3; ++d) {
-        g.nx[d] = (d < g.nd) ? P.p.nx[d] : 1;
-        g.bc[d][0] = P.p.bc[d][0];
-        g.bc[d][1] = P.p.bc[d][1];
-        g.h[d] = pg.h[d];
-        g.P[d] = pg.P[d];
-    }
-    g.np = pg.n;
-    return g;
-}
-
-struct CompConsts {
-    double i4pir[kNC], i4pirsq[kNC], rho_b[kNC], mu[kNC];
-    double Ab[kNC], S3b;    // bulk adjoint-field values
-};
-
-__device__ inline long long pad_pos(const Geo &g, int i)
-{
-    int x = i % g.nx[0];
-    int r = i / g.nx[0];
-    int y = r % g.nx[1];
-    int z = r / g.nx[1];
-    return (long long)(x + g.h[0]) + (long long)(y + g.h[1]) * g.P[0] + (long long)(z + g.h[2]) * g.P[0] * g.P[1];
-}
-
-// map a padded position to (code, source node, reflect mask); code 0: node,
-// -1: bulk, -2: wall.  Dimension order and codes as offset_to_node_box.
-__device__ inline int map_pad(const Geo &g, long long pos, int &src, int &refl)
-{
-    int c[3];
-    c[0] = (int)(pos % g.P[0]);
-    long long r = pos / g.P[0];
-    c[1] = (int)(r % g.P[1]);
-    c[2] = (int)(r / g.P[1]);
-    refl = 0;
-    int s[3];
-    for (int d = 0; d < 3; ++d) {
-        int v = c[d] - g.h[d];
-        const int n = g.nx[d];
-        if (d < g.nd) {
-            if (v < 0) {
-                const int b = g.bc[d][0];
-                if (b == TMI_IN_BULK) return -1;
-                if (b == TMI_IN_WALL) return -2;
-                if (b == TMI_PERIODIC) {
-                    v += n;
-                } else {
-                    refl |= 1 << d;
-                    v = -v;
-                    if (v >= n) {
-                        if (g.bc[d][1] == TMI_IN_BULK) return -1;
-                        return -2;
-                    }
-                }
-            } else if (v >= n) {
-                const int b = g.bc[d][1];
-                if (b == TMI_IN_BULK) return -1;
-                if (b == TMI_IN_WALL) return -2;
-                if (b == TMI_PERIODIC) {
-                    v -= n;
-                } else {
-                    refl |= 1 << d;
-                    v = 2 * (n - 1) - v;
-                    if (v < 0) {
-                        if (g.bc[d][0] == TMI_IN_BULK) return -1;
-                        return -2;
-                    }
-                }
-            }
-        }
-        s[d] = v;
-    }
-    src = s[0] + s[1] * g.nx[0] + s[2] * g.nx[0] * g.nx[1];
-    return 0;
-}
+using namespace dev;
 
 // ---------------------------------------------------------------- padding
 // rho_pad[c][pos] from in[c*N + node]; bulk value rho_b (residual) or 0 (linear)
@@ -229,13 +140,6 @@ __global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj)
 }
 
 // ---------------------------------------------------------------- stencil rows
-struct StenPtrs {
-    const int *lo;
-    const void *w;
-    const void *wv;
-    int L;
-};
-
 struct RbArgs {
     StenPtrs th[kNC], de[kNC];
 };
@@ -538,67 +442,6 @@ __global__ void k_apply_update(double *__restrict__ x, const double *__restrict_
         x[i] = xv + frac * d;
 }
 
-inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
-CompConsts make_consts(const Problem &P)
-{
-    CompConsts cc{};
-    const auto &b = P.bulk;
-    for (int c = 0; c < P.p.ncomp; ++c) {
-        cc.i4pir[c] = b.inv_4pir[c];
-        cc.i4pirsq[c] = b.inv_4pirsq[c];
-        cc.rho_b[c] = P.p.rho_b[c];
-        double mu = 0.0;
-        mu -= std::log(P.p.rho_b[c]);
-        mu -= b.betamu_hs_ex[c];
-        if (P.p.type_attr >= 0) mu -= b.betamu_att[c];
-        cc.mu[c] = mu;
-        cc.Ab[c] = b.dphi_b[0] * b.inv_4pirsq[c] + b.dphi_b[1] * b.inv_4pir[c] + b.dphi_b[2];
-    }
-    cc.S3b = b.dphi_b[3];
-    return cc;
-}
-
-template <typename T>
-const DevStencilT<T> &dev_sten(const Problem &P, int type, int idx);
-template <>
-const DevStencilT<double> &dev_sten<double>(const Problem &P, int type, int idx)
-{
-    return P.dsd[type][idx];
-}
-template <>
-const DevStencilT<float> &dev_sten<float>(const Problem &P, int type, int idx)
-{
-    return P.dsf[type][idx];
-}
-
-template <typename T>
-Workspace<T> &workspace(Problem &P);
-template <>
-Workspace<double> &workspace<double>(Problem &P)
-{
-    return P.wsd;
-}
-template <>
-Workspace<float> &workspace<float>(Problem &P)
-{
-    return P.wsf;
-}
-
-// lo tables: dev stencil arrays hold lo for the density geometry at lo[0..L)
-// and for the FMT geometry at lo[L..2L)
-template <typename T>
-StenPtrs sten_ptrs(const Problem &P, int type, int idx, bool fmt_geom)
-{
-    const DevStencilT<T> &s = dev_sten<T>(P, type, idx);
-    StenPtrs sp;
-    sp.L = s.L;
-    sp.lo = s.lo + (fmt_geom ? s.L : 0);
-    sp.w = s.w;
-    sp.wv = s.wv;
-    return sp;
-}
-
 template <typename T>
 void pad_rho(Problem &P, const double *in, bool linear, hipStream_t s)
 {
@@ -613,8 +456,8 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
     Geo gu = make_geo(P, P.pad_u);
     RbArgs ra{};
     for (int c = 0; c < P.p.ncomp; ++c) {
-        ra.th[c] = sten_ptrs<T>(P, TMI_THETA, c, false);
-        ra.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, false);
+        ra.th[c] = sten_ptrs<T>(P, TMI_THETA, c, kGeomU);
+        ra.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, kGeomU);
     }
     const CompConsts cc = make_consts(P);
     const T *rp = workspace<T>(P).rho_pad;
@@ -631,7 +474,7 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
     if (P.nmf == 0) return;
     Geo gu = make_geo(P, P.pad_u);
     UArgs ua{};
-    for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ua.u[q] = sten_ptrs<T>(P, TMI_UATT, q, false);
+    for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ua.u[q] = sten_ptrs<T>(P, TMI_UATT, q, kGeomU);
     hipLaunchKernelGGL(k_mf_rows<T>, dim3(grid_for(P.N), P.p.ncomp), dim3(kBlock), 0, s, gu, ua,
                        workspace<T>(P).rho_pad, in_mf, sub, out_mf);
 }
@@ -667,8 +510,8 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
     Geo gf = make_geo(P, P.pad_f);
     ElArgs ea{};
     for (int c = 0; c < P.p.ncomp; ++c) {
-        ea.th[c] = sten_ptrs<T>(P, TMI_THETA, c, true);
-        ea.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, true);
+        ea.th[c] = sten_ptrs<T>(P, TMI_THETA, c, kGeomF);
+        ea.de[c] = sten_ptrs<T>(P, TMI_DELTA, c, kGeomF);
     }
     const CompConsts cc = make_consts(P);
     const T *adj = workspace<T>(P).adj_pad;
