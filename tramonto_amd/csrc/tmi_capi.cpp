@@ -9,6 +9,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <cstdlib>
+#include <map>
 #include <vector>
 
 #include "tmi_internal.hpp"
@@ -121,6 +123,56 @@ void upload_stencil(Problem &P, int type, int idx, int comp, DevStencilT<T> &ds)
     upload(ds.wv, wv.data(), wv.size());
 }
 
+// z-column form of a 3D stencil (see DevColumnsT)
+template <typename T>
+void upload_columns(Problem &P, int type, int idx, DevColumnsT<T> &dc)
+{
+    const HostStencil &hs = P.hs[type][idx];
+    std::map<std::pair<int, int>, std::vector<std::pair<int, double>>> cols;
+    for (int k = 0; k < hs.L; ++k) {
+        const int *o = &hs.off[3 * (size_t)k];
+        cols[{o[1], o[0]}].push_back({o[2], hs.w[k]});
+    }
+    const int B = kColB;
+    std::vector<int> lo, woff, nchunk;
+    std::vector<T> wpad;
+    for (auto &kv : cols) {
+        int z0 = 1 << 30, z1 = -(1 << 30);
+        for (auto &e : kv.second) {
+            z0 = std::min(z0, e.first);
+            z1 = std::max(z1, e.first);
+        }
+        const int len = z1 - z0 + 1;
+        const int nk = (len + B - 1 + B - 1) / B;
+        const int dy = kv.first.first, dx = kv.first.second;
+        lo.push_back(dx + dy * P.pad_u.P[0] + z0 * P.pad_u.P[0] * P.pad_u.P[1]);
+        woff.push_back((int)wpad.size());
+        nchunk.push_back(nk);
+        std::vector<T> w((size_t)nk * B + B - 1, (T)0);
+        for (auto &e : kv.second) w[(size_t)(B - 1) + (e.first - z0)] = (T)e.second;
+        wpad.insert(wpad.end(), w.begin(), w.end());
+    }
+    dc.ncol = (int)lo.size();
+    dc.lo = dev_alloc<int>(lo.size());
+    dc.woff = dev_alloc<int>(woff.size());
+    dc.nchunk = dev_alloc<int>(nchunk.size());
+    dc.wpad = dev_alloc<T>(wpad.size());
+    upload(dc.lo, lo.data(), lo.size());
+    upload(dc.woff, woff.data(), woff.size());
+    upload(dc.nchunk, nchunk.data(), nchunk.size());
+    upload(dc.wpad, wpad.data(), wpad.size());
+}
+
+template <typename T>
+void free_columns(DevColumnsT<T> &dc)
+{
+    (void)hipFree(dc.lo);
+    (void)hipFree(dc.woff);
+    (void)hipFree(dc.nchunk);
+    (void)hipFree(dc.wpad);
+    dc = DevColumnsT<T>();
+}
+
 template <typename T>
 void free_stencil(DevStencilT<T> &ds)
 {
@@ -200,9 +252,23 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
                 upload_stencil(*P, TMI_UATT, q, q % C, P->dsd[TMI_UATT][q]);
                 upload_stencil(*P, TMI_UATT, q, q % C, P->dsf[TMI_UATT][q]);
             }
-        P->wsd.rho_pad = dev_alloc<double>((size_t)C * P->pad_u.n);
+        // column kernels for the attraction stencil on 3D grids (TMI_STENCIL_KERNEL=gather keeps the
+        // per-point gather kernel for comparison)
+        const char *kern = std::getenv("TMI_STENCIL_KERNEL");
+        if (nd == 3 && p.type_attr >= 0 && !(kern && std::string(kern) == "gather")) {
+            for (int q = 0; q < C * C; ++q) {
+                upload_columns(*P, TMI_UATT, q, P->cold[q]);
+                upload_columns(*P, TMI_UATT, q, P->colf[q]);
+            }
+            P->use_columns = true;
+        }
+        // column reads run up to 2*kColB planes past the last component's halo (times zero weights)
+        const size_t slack = (size_t)2 * kColB * P->pad_u.P[0] * P->pad_u.P[1];
+        P->wsd.rho_pad = dev_alloc<double>((size_t)C * P->pad_u.n + slack);
+        TMI_HIP_CHECK(hipMemset(P->wsd.rho_pad, 0, ((size_t)C * P->pad_u.n + slack) * sizeof(double)));
         P->wsd.adj_pad = dev_alloc<double>((size_t)P->nadj * P->pad_f.n);
-        P->wsf.rho_pad = dev_alloc<float>((size_t)C * P->pad_u.n);
+        P->wsf.rho_pad = dev_alloc<float>((size_t)C * P->pad_u.n + slack);
+        TMI_HIP_CHECK(hipMemset(P->wsf.rho_pad, 0, ((size_t)C * P->pad_u.n + slack) * sizeof(float)));
         P->wsf.adj_pad = dev_alloc<float>((size_t)P->nadj * P->pad_f.n);
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
@@ -219,6 +285,10 @@ void destroy_problem(Problem *P)
         for (int q = 0; q < kNC * kNC; ++q) {
             free_stencil(P->dsd[t][q]);
             free_stencil(P->dsf[t][q]);
+            if (t == 0) {
+                free_columns(P->cold[q]);
+                free_columns(P->colf[q]);
+            }
         }
     (void)hipFree(P->d_zero);
     (void)hipFree(P->d_elzero);

@@ -147,6 +147,19 @@ inline const DevStencilT<float> &dev_sten<float>(const Problem &P, int type, int
 }
 
 template <typename T>
+inline const DevColumnsT<T> &dev_cols(const Problem &P, int idx);
+template <>
+inline const DevColumnsT<double> &dev_cols<double>(const Problem &P, int idx)
+{
+    return P.cold[idx];
+}
+template <>
+inline const DevColumnsT<float> &dev_cols<float>(const Problem &P, int idx)
+{
+    return P.colf[idx];
+}
+
+template <typename T>
 inline Workspace<T> &workspace(Problem &P);
 template <>
 inline Workspace<double> &workspace<double>(Problem &P)

@@ -55,6 +55,20 @@ struct DevStencilT {
     T *wv = nullptr;     // [L][ndim] w * offset_d*dx_d/R (delta stencils only)
 };
 
+// A 3D stencil regrouped into z-columns for register-blocked sums: for each
+// distinct (dx, dy) the z-range [z0, z1] with dense weights, stored with
+// kColB-1 leading zeros and zero-padded to a whole number of kColB chunks
+// plus kColB-1 (so every chunk reads a full window of 2*kColB-1 weights).
+constexpr int kColB = 8;   // outputs per thread along z
+template <typename T>
+struct DevColumnsT {
+    int ncol = 0;
+    int *lo = nullptr;       // [ncol] linear offset of (dx, dy, z0) in the density geometry
+    int *woff = nullptr;     // [ncol] start of the column's padded weights
+    int *nchunk = nullptr;   // [ncol] number of kColB-chunks
+    T *wpad = nullptr;
+};
+
 struct Grid {
     int nd;
     int nx[3];
@@ -86,6 +100,10 @@ struct Problem {
     // device tables
     DevStencilT<double> dsd[3][kNC * kNC];
     DevStencilT<float> dsf[3][kNC * kNC];
+    // column form of the attraction stencils (3D only)
+    DevColumnsT<double> cold[kNC * kNC];
+    DevColumnsT<float> colf[kNC * kNC];
+    bool use_columns = false;
     unsigned char *d_zero = nullptr;     // [C][N] Zero_density_TF
     unsigned char *d_elzero = nullptr;   // [C][N] EL row replaced by x (zero density or V_ext = VEXT_MAX)
     double *d_vext = nullptr;            // [C][N]

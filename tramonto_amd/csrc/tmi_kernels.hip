@@ -234,6 +234,88 @@ __global__ void k_mf_rows(Geo g, UArgs ua, const T *__restrict__ rho_pad, const 
     out_mf[(size_t)c * g.N + i] = r;
 }
 
+// ---------------------------------------------------------------- 3D column kernels
+template <typename T>
+struct ColPtrs {
+    const int *lo, *woff, *nchunk;
+    const T *wpad;
+    int ncol;
+};
+
+template <typename T>
+struct ColArgs {
+    ColPtrs<T> u[kNC * kNC];
+};
+
+// Mean-field rows on a 3D grid with the attraction stencil in column form.
+// Workgroup: 64 x-lanes x S waves for one (y, z-block).  Each lane owns kColB
+// consecutive z outputs; wave s takes stencil columns s, s+S, ...; per column
+// the density is streamed along z once, so every loaded value feeds kColB
+// fused multiply-adds against wave-uniform (scalar-register) weights.  The S
+// partial sums are reduced through LDS.
+constexpr int kColS = 8;
+template <typename T, int B, int S>
+__global__ __launch_bounds__(64 * S) void k_mf_rows_cols(Geo g, ColArgs<T> ca, const T *__restrict__ rho_pad,
+                                                         const double *__restrict__ in_mf, const double *__restrict__ sub,
+                                                         double *__restrict__ out_mf)
+{
+    __shared__ T red[S - 1][B][64];
+    const int nx = g.nx[0], ny = g.nx[1], nz = g.nx[2];
+    const int lane = threadIdx.x, grp = threadIdx.y;
+    const int x = blockIdx.x * 64 + lane;
+    const int y = blockIdx.y;
+    const int zblocks = (nz + B - 1) / B;
+    const int c = blockIdx.z / zblocks;
+    const int zb = (blockIdx.z % zblocks) * B;
+    const bool active = (x < nx) && (y < ny);
+    const int xs = active ? x : 0, ys = active ? y : 0;
+    const long long PXY = (long long)g.P[0] * g.P[1];
+    const long long base = (long long)(xs + g.h[0]) + (long long)(ys + g.h[1]) * g.P[0] + (long long)(zb + g.h[2]) * PXY;
+    T acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    for (int cj = 0; cj < g.C; ++cj) {
+        const ColPtrs<T> cp = ca.u[c + g.C * cj];
+        const T *src0 = rho_pad + (size_t)cj * g.np + base;
+        for (int col = grp; col < cp.ncol; col += S) {
+            const T *s = src0 + cp.lo[col];
+            const T *w = cp.wpad + cp.woff[col];
+            const int nk = cp.nchunk[col];
+            for (int k = 0; k < nk; ++k) {
+                T v[B], wk[2 * B - 1];
+#pragma unroll
+                for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+#pragma unroll
+                for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
+#pragma unroll
+                for (int i = 0; i < B; ++i)
+#pragma unroll
+                    for (int b = 0; b < B; ++b) acc[b] += wk[i - b + B - 1] * v[i];
+            }
+        }
+    }
+    if (grp > 0) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) red[grp - 1][b][lane] = acc[b];
+    }
+    __syncthreads();
+    if (grp > 0 || !active) return;
+#pragma unroll
+    for (int s = 0; s < S - 1; ++s)
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[b] += red[s][b][lane];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int z = zb + b;
+        if (z >= nz) break;
+        const size_t row = (size_t)c * g.N + x + (size_t)nx * (y + (size_t)ny * z);
+        double r = acc[b];
+        if (in_mf) r -= in_mf[row];
+        if (sub) r -= sub[row];
+        out_mf[row] = r;
+    }
+}
+
 struct ElArgs {
     StenPtrs th[kNC], de[kNC];
 };
@@ -473,6 +555,18 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
 {
     if (P.nmf == 0) return;
     Geo gu = make_geo(P, P.pad_u);
+    if (P.use_columns) {
+        ColArgs<T> ca{};
+        for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) {
+            const DevColumnsT<T> &dc = dev_cols<T>(P, q);
+            ca.u[q] = ColPtrs<T>{dc.lo, dc.woff, dc.nchunk, dc.wpad, dc.ncol};
+        }
+        const int nz = P.p.nx[2];
+        dim3 grid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((nz + kColB - 1) / kColB) * P.p.ncomp);
+        hipLaunchKernelGGL((k_mf_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, ca,
+                           workspace<T>(P).rho_pad, in_mf, sub, out_mf);
+        return;
+    }
     UArgs ua{};
     for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ua.u[q] = sten_ptrs<T>(P, TMI_UATT, q, kGeomU);
     hipLaunchKernelGGL(k_mf_rows<T>, dim3(grid_for(P.N), P.p.ncomp), dim3(kBlock), 0, s, gu, ua,
