@@ -125,7 +125,7 @@ void upload_stencil(Problem &P, int type, int idx, int comp, DevStencilT<T> &ds)
 
 // z-column form of a 3D stencil (see DevColumnsT)
 template <typename T>
-void upload_columns(Problem &P, int type, int idx, DevColumnsT<T> &dc)
+void upload_columns(Problem &P, int type, int idx, int comp, DevColumnsT<T> &dc)
 {
     const HostStencil &hs = P.hs[type][idx];
     std::map<std::pair<int, int>, std::vector<std::pair<int, double>>> cols;
@@ -134,8 +134,11 @@ void upload_columns(Problem &P, int type, int idx, DevColumnsT<T> &dc)
         cols[{o[1], o[0]}].push_back({o[2], hs.w[k]});
     }
     const int B = kColB;
-    std::vector<int> lo, woff, nchunk;
-    std::vector<T> wpad;
+    const bool delta = (type == TMI_DELTA);
+    const double inv_r = P.bulk.inv_rad[comp];
+    const PadGeom *geoms[2] = {&P.pad_u, &P.pad_f};
+    std::vector<int> lo[2], woff, nchunk;
+    std::vector<T> wpad, wzpad, fxy;
     for (auto &kv : cols) {
         int z0 = 1 << 30, z1 = -(1 << 30);
         for (auto &e : kv.second) {
@@ -145,22 +148,42 @@ void upload_columns(Problem &P, int type, int idx, DevColumnsT<T> &dc)
         const int len = z1 - z0 + 1;
         const int nk = (len + B - 1 + B - 1) / B;
         const int dy = kv.first.first, dx = kv.first.second;
-        lo.push_back(dx + dy * P.pad_u.P[0] + z0 * P.pad_u.P[0] * P.pad_u.P[1]);
+        for (int g = 0; g < 2; ++g) {
+            const PadGeom &pg = *geoms[g];
+            lo[g].push_back(dx + dy * pg.P[0] + z0 * pg.P[0] * pg.P[1]);
+        }
+        if (wpad.size() > INT32_MAX) throw std::domain_error("stencil too large for 32-bit column offsets");
         woff.push_back((int)wpad.size());
         nchunk.push_back(nk);
-        std::vector<T> w((size_t)nk * B + B - 1, (T)0);
-        for (auto &e : kv.second) w[(size_t)(B - 1) + (e.first - z0)] = (T)e.second;
+        std::vector<T> w((size_t)nk * B + B - 1, (T)0), wz(w.size(), (T)0);
+        for (auto &e : kv.second) {
+            w[(size_t)(B - 1) + (e.first - z0)] = (T)e.second;
+            wz[(size_t)(B - 1) + (e.first - z0)] = (T)(e.second * ((double)e.first * P.p.dx[2] * inv_r));
+        }
         wpad.insert(wpad.end(), w.begin(), w.end());
+        if (delta) {
+            wzpad.insert(wzpad.end(), wz.begin(), wz.end());
+            fxy.push_back((T)((double)dx * P.p.dx[0] * inv_r));
+            fxy.push_back((T)((double)dy * P.p.dx[1] * inv_r));
+        }
     }
-    dc.ncol = (int)lo.size();
-    dc.lo = dev_alloc<int>(lo.size());
+    dc.ncol = (int)woff.size();
+    dc.nw = (long long)wpad.size();
+    if (delta) wpad.insert(wpad.end(), wzpad.begin(), wzpad.end());
+    std::vector<int> lo2(lo[0]);
+    lo2.insert(lo2.end(), lo[1].begin(), lo[1].end());
+    dc.lo = dev_alloc<int>(lo2.size());
     dc.woff = dev_alloc<int>(woff.size());
     dc.nchunk = dev_alloc<int>(nchunk.size());
     dc.wpad = dev_alloc<T>(wpad.size());
-    upload(dc.lo, lo.data(), lo.size());
+    upload(dc.lo, lo2.data(), lo2.size());
     upload(dc.woff, woff.data(), woff.size());
     upload(dc.nchunk, nchunk.data(), nchunk.size());
     upload(dc.wpad, wpad.data(), wpad.size());
+    if (delta) {
+        dc.fxy = dev_alloc<T>(fxy.size());
+        upload(dc.fxy, fxy.data(), fxy.size());
+    }
 }
 
 template <typename T>
@@ -170,6 +193,7 @@ void free_columns(DevColumnsT<T> &dc)
     (void)hipFree(dc.woff);
     (void)hipFree(dc.nchunk);
     (void)hipFree(dc.wpad);
+    (void)hipFree(dc.fxy);
     dc = DevColumnsT<T>();
 }
 
@@ -252,24 +276,38 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
                 upload_stencil(*P, TMI_UATT, q, q % C, P->dsd[TMI_UATT][q]);
                 upload_stencil(*P, TMI_UATT, q, q % C, P->dsf[TMI_UATT][q]);
             }
-        // column kernels for the attraction stencil on 3D grids (TMI_STENCIL_KERNEL=gather keeps the
-        // per-point gather kernel for comparison)
+        // column kernels on 3D grids (TMI_STENCIL_KERNEL=gather keeps the per-point gather kernels
+        // for comparison)
         const char *kern = std::getenv("TMI_STENCIL_KERNEL");
-        if (nd == 3 && p.type_attr >= 0 && !(kern && std::string(kern) == "gather")) {
-            for (int q = 0; q < C * C; ++q) {
-                upload_columns(*P, TMI_UATT, q, P->cold[q]);
-                upload_columns(*P, TMI_UATT, q, P->colf[q]);
-            }
+        if (nd == 3 && !(kern && std::string(kern) == "gather")) {
+            for (int c = 0; c < C; ++c)
+                for (int t : {TMI_DELTA, TMI_THETA}) {
+                    upload_columns(*P, t, c, c, P->cold[t][c]);
+                    upload_columns(*P, t, c, c, P->colf[t][c]);
+                }
+            if (p.type_attr >= 0)
+                for (int q = 0; q < C * C; ++q) {
+                    upload_columns(*P, TMI_UATT, q, q % C, P->cold[TMI_UATT][q]);
+                    upload_columns(*P, TMI_UATT, q, q % C, P->colf[TMI_UATT][q]);
+                }
             P->use_columns = true;
         }
-        // column reads run up to 2*kColB planes past the last component's halo (times zero weights)
-        const size_t slack = (size_t)2 * kColB * P->pad_u.P[0] * P->pad_u.P[1];
-        P->wsd.rho_pad = dev_alloc<double>((size_t)C * P->pad_u.n + slack);
-        TMI_HIP_CHECK(hipMemset(P->wsd.rho_pad, 0, ((size_t)C * P->pad_u.n + slack) * sizeof(double)));
-        P->wsd.adj_pad = dev_alloc<double>((size_t)P->nadj * P->pad_f.n);
-        P->wsf.rho_pad = dev_alloc<float>((size_t)C * P->pad_u.n + slack);
-        TMI_HIP_CHECK(hipMemset(P->wsf.rho_pad, 0, ((size_t)C * P->pad_u.n + slack) * sizeof(float)));
-        P->wsf.adj_pad = dev_alloc<float>((size_t)P->nadj * P->pad_f.n);
+        // column reads run up to 2*kColB planes past the last field's halo (times zero weights)
+        const size_t slack_u = (size_t)2 * kColB * P->pad_u.P[0] * P->pad_u.P[1];
+        const size_t slack_f = (size_t)2 * kColB * P->pad_f.P[0] * P->pad_f.P[1];
+        const size_t nu = (size_t)C * P->pad_u.n + slack_u, nf = (size_t)P->nadj * P->pad_f.n + slack_f;
+        P->wsd.rho_pad = dev_alloc<double>(nu);
+        P->wsd.adj_pad = dev_alloc<double>(nf);
+        P->wsf.rho_pad = dev_alloc<float>(nu);
+        P->wsf.adj_pad = dev_alloc<float>(nf);
+        TMI_HIP_CHECK(hipMemset(P->wsd.rho_pad, 0, nu * sizeof(double)));
+        TMI_HIP_CHECK(hipMemset(P->wsd.adj_pad, 0, nf * sizeof(double)));
+        TMI_HIP_CHECK(hipMemset(P->wsf.rho_pad, 0, nu * sizeof(float)));
+        TMI_HIP_CHECK(hipMemset(P->wsf.adj_pad, 0, nf * sizeof(float)));
+        if (P->use_columns && C > 1) {
+            P->wsd.rb_part = dev_alloc<double>((size_t)5 * C * N);
+            P->wsf.rb_part = dev_alloc<float>((size_t)5 * C * N);
+        }
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
     } catch (...) {
@@ -285,10 +323,8 @@ void destroy_problem(Problem *P)
         for (int q = 0; q < kNC * kNC; ++q) {
             free_stencil(P->dsd[t][q]);
             free_stencil(P->dsf[t][q]);
-            if (t == 0) {
-                free_columns(P->cold[q]);
-                free_columns(P->colf[q]);
-            }
+            free_columns(P->cold[t][q]);
+            free_columns(P->colf[t][q]);
         }
     (void)hipFree(P->d_zero);
     (void)hipFree(P->d_elzero);
@@ -297,6 +333,8 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->wsd.adj_pad);
     (void)hipFree(P->wsf.rho_pad);
     (void)hipFree(P->wsf.adj_pad);
+    (void)hipFree(P->wsd.rb_part);
+    (void)hipFree(P->wsf.rb_part);
     (void)hipFree(P->wsd.corr_u);
     (void)hipFree(P->wsd.corr_f);
     (void)hipFree(P->wsf.corr_u);

@@ -147,16 +147,16 @@ inline const DevStencilT<float> &dev_sten<float>(const Problem &P, int type, int
 }
 
 template <typename T>
-inline const DevColumnsT<T> &dev_cols(const Problem &P, int idx);
+inline const DevColumnsT<T> &dev_cols(const Problem &P, int type, int idx);
 template <>
-inline const DevColumnsT<double> &dev_cols<double>(const Problem &P, int idx)
+inline const DevColumnsT<double> &dev_cols<double>(const Problem &P, int type, int idx)
 {
-    return P.cold[idx];
+    return P.cold[type][idx];
 }
 template <>
-inline const DevColumnsT<float> &dev_cols<float>(const Problem &P, int idx)
+inline const DevColumnsT<float> &dev_cols<float>(const Problem &P, int type, int idx)
 {
-    return P.colf[idx];
+    return P.colf[type][idx];
 }
 
 template <typename T>

@@ -59,14 +59,18 @@ struct DevStencilT {
 // distinct (dx, dy) the z-range [z0, z1] with dense weights, stored with
 // kColB-1 leading zeros and zero-padded to a whole number of kColB chunks
 // plus kColB-1 (so every chunk reads a full window of 2*kColB-1 weights).
+// Delta stencils carry a second weight plane, w * dz*kz/R, and the column
+// constants dx*kx/R, dy*ky/R of the x and y vector weights.
 constexpr int kColB = 8;   // outputs per thread along z
 template <typename T>
 struct DevColumnsT {
     int ncol = 0;
-    int *lo = nullptr;       // [ncol] linear offset of (dx, dy, z0) in the density geometry
+    long long nw = 0;        // length of one weight plane
+    int *lo = nullptr;       // [2][ncol] linear offset of (dx, dy, z0): density geometry, FMT geometry
     int *woff = nullptr;     // [ncol] start of the column's padded weights
     int *nchunk = nullptr;   // [ncol] number of kColB-chunks
-    T *wpad = nullptr;
+    T *wpad = nullptr;       // [1 or 2][nw]
+    T *fxy = nullptr;        // [ncol][2] (delta stencils)
 };
 
 struct Grid {
@@ -83,6 +87,7 @@ template <typename T>
 struct Workspace {
     T *rho_pad = nullptr;    // [C][pad_u.n]   density (or direction) with halo
     T *adj_pad = nullptr;    // [nadj][pad_f.n] adjoint fields A_c, B_c,d, S3 with halo
+    T *rb_part = nullptr;    // [C][5][N] per-component rho-bar sums (3D column kernels, C > 1)
     // transpose products (allocated on first use): zero-padded row weights
     T *corr_u = nullptr;     // [nrb + nmf][pad_u2.n]
     T *corr_f = nullptr;     // [C][pad_f2.n]
@@ -100,9 +105,9 @@ struct Problem {
     // device tables
     DevStencilT<double> dsd[3][kNC * kNC];
     DevStencilT<float> dsf[3][kNC * kNC];
-    // column form of the attraction stencils (3D only)
-    DevColumnsT<double> cold[kNC * kNC];
-    DevColumnsT<float> colf[kNC * kNC];
+    // column form of the stencils (3D only), [type][idx] as the point tables
+    DevColumnsT<double> cold[3][kNC * kNC];
+    DevColumnsT<float> colf[3][kNC * kNC];
     bool use_columns = false;
     unsigned char *d_zero = nullptr;     // [C][N] Zero_density_TF
     unsigned char *d_elzero = nullptr;   // [C][N] EL row replaced by x (zero density or V_ext = VEXT_MAX)

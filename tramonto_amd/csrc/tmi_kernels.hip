@@ -235,84 +235,358 @@ __global__ void k_mf_rows(Geo g, UArgs ua, const T *__restrict__ rho_pad, const 
 }
 
 // ---------------------------------------------------------------- 3D column kernels
+// A column pass streams one input field along z for one stencil column and
+// adds, for each of the lane's B consecutive z outputs, the column's weights
+// times the loaded values (see DevColumnsT for the padded weight layout).
 template <typename T>
 struct ColPtrs {
     const int *lo, *woff, *nchunk;
-    const T *wpad;
+    const T *w, *wz, *fxy;
     int ncol;
 };
+
+template <typename T>
+inline ColPtrs<T> col_ptrs(const DevColumnsT<T> &dc, int geom)
+{
+    ColPtrs<T> cp;
+    cp.lo = dc.lo + (size_t)geom * dc.ncol;
+    cp.woff = dc.woff;
+    cp.nchunk = dc.nchunk;
+    cp.w = dc.wpad;
+    cp.wz = dc.fxy ? dc.wpad + dc.nw : nullptr;
+    cp.fxy = dc.fxy;
+    cp.ncol = dc.ncol;
+    return cp;
+}
 
 template <typename T>
 struct ColArgs {
     ColPtrs<T> u[kNC * kNC];
 };
 
-// Mean-field rows on a 3D grid with the attraction stencil in column form.
-// Workgroup: 64 x-lanes x S waves for one (y, z-block).  Each lane owns kColB
-// consecutive z outputs; wave s takes stencil columns s, s+S, ...; per column
-// the density is streamed along z once, so every loaded value feeds kColB
-// fused multiply-adds against wave-uniform (scalar-register) weights.  The S
-// partial sums are reduced through LDS.
+template <typename T>
+struct FmtColArgs {
+    ColPtrs<T> th[kNC], de[kNC];
+};
+
+// acc[b] += sum_z' w(z' - z_b) s(z')
+template <typename T, int B>
+__device__ __forceinline__ void col_sum(const T *__restrict__ s, const T *__restrict__ w, int nk, long long PXY,
+                                        T (&acc)[B])
+{
+    for (int k = 0; k < nk; ++k) {
+        T v[B], wk[2 * B - 1];
+#pragma unroll
+        for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] += wk[i - b + B - 1] * v[i];
+    }
+}
+
+// the same loads against two weight planes: a += w s, az += wz s
+template <typename T, int B>
+__device__ __forceinline__ void col_sum2(const T *__restrict__ s, const T *__restrict__ w, const T *__restrict__ wz,
+                                         int nk, long long PXY, T (&a)[B], T (&az)[B])
+{
+    for (int k = 0; k < nk; ++k) {
+        T v[B], wk[2 * B - 1];
+#pragma unroll
+        for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) a[b] += wk[i - b + B - 1] * v[i];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = wz[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) az[b] += wk[i - b + B - 1] * v[i];
+    }
+}
+
+// Sum the per-wave partial results of a workgroup into wave 0 through LDS,
+// one wave at a time (buf holds B*64 values).
+template <typename T, int B, int S>
+__device__ __forceinline__ void wave_reduce(T (&acc)[B], T *buf, int grp, int lane)
+{
+    for (int s = 1; s < S; ++s) {
+        if (grp == s) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) buf[b * 64 + lane] = acc[b];
+        }
+        __syncthreads();
+        if (grp == 0) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) acc[b] += buf[b * 64 + lane];
+        }
+        __syncthreads();
+    }
+}
+
+// Column-kernel launch shape: workgroup = 64 x-lanes x S waves for one
+// (y, z-block, component); each lane owns kColB consecutive z outputs and
+// wave s takes the stencil columns s, s+S, ...; the S partial sums meet in LDS.
 constexpr int kColS = 8;
+
+struct ColTile {
+    int lane, grp, x, y, c, zb;
+    bool active;
+    long long PXY, base;
+};
+
+template <int B>
+__device__ __forceinline__ ColTile col_tile(const Geo &g)
+{
+    ColTile t;
+    t.lane = threadIdx.x;
+    // the wave index is uniform across the wave: readfirstlane lets the
+    // compiler keep the column loops and their weights in scalar registers
+    t.grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    t.x = blockIdx.x * 64 + t.lane;
+    t.y = blockIdx.y;
+    const int zblocks = (g.nx[2] + B - 1) / B;
+    t.c = blockIdx.z / zblocks;
+    t.zb = (blockIdx.z % zblocks) * B;
+    t.active = (t.x < g.nx[0]) && (t.y < g.nx[1]);
+    const int xs = t.active ? t.x : 0, ys = t.active ? t.y : 0;
+    t.PXY = (long long)g.P[0] * g.P[1];
+    t.base = (long long)(xs + g.h[0]) + (long long)(ys + g.h[1]) * g.P[0] + (long long)(t.zb + g.h[2]) * t.PXY;
+    return t;
+}
+
+// Mean-field rows on a 3D grid with the attraction stencil in column form.
 template <typename T, int B, int S>
 __global__ __launch_bounds__(64 * S) void k_mf_rows_cols(Geo g, ColArgs<T> ca, const T *__restrict__ rho_pad,
                                                          const double *__restrict__ in_mf, const double *__restrict__ sub,
                                                          double *__restrict__ out_mf)
 {
-    __shared__ T red[S - 1][B][64];
-    const int nx = g.nx[0], ny = g.nx[1], nz = g.nx[2];
-    const int lane = threadIdx.x, grp = threadIdx.y;
-    const int x = blockIdx.x * 64 + lane;
-    const int y = blockIdx.y;
-    const int zblocks = (nz + B - 1) / B;
-    const int c = blockIdx.z / zblocks;
-    const int zb = (blockIdx.z % zblocks) * B;
-    const bool active = (x < nx) && (y < ny);
-    const int xs = active ? x : 0, ys = active ? y : 0;
-    const long long PXY = (long long)g.P[0] * g.P[1];
-    const long long base = (long long)(xs + g.h[0]) + (long long)(ys + g.h[1]) * g.P[0] + (long long)(zb + g.h[2]) * PXY;
+    __shared__ T red[B * 64];
+    const ColTile t = col_tile<B>(g);
     T acc[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
     for (int cj = 0; cj < g.C; ++cj) {
-        const ColPtrs<T> cp = ca.u[c + g.C * cj];
-        const T *src0 = rho_pad + (size_t)cj * g.np + base;
-        for (int col = grp; col < cp.ncol; col += S) {
-            const T *s = src0 + cp.lo[col];
-            const T *w = cp.wpad + cp.woff[col];
+        const ColPtrs<T> cp = ca.u[t.c + g.C * cj];
+        const T *src0 = rho_pad + (size_t)cj * g.np + t.base;
+        for (int col = t.grp; col < cp.ncol; col += S)
+            col_sum<T, B>(src0 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
+    }
+    wave_reduce<T, B, S>(acc, red, t.grp, t.lane);
+    if (t.grp > 0 || !t.active) return;
+    const int nx = g.nx[0], ny = g.nx[1], nz = g.nx[2];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int z = t.zb + b;
+        if (z >= nz) break;
+        const size_t row = (size_t)t.c * g.N + t.x + (size_t)nx * (t.y + (size_t)ny * z);
+        double r = acc[b];
+        if (in_mf) r -= in_mf[row];
+        if (sub) r -= sub[row];
+        out_mf[row] = r;
+    }
+}
+
+// rho-bar rows on a 3D grid, theta and delta stencils in column form.  One
+// workgroup handles one component; with one component the rows are written
+// directly (out_rb = sums - in_rb - sub), otherwise the five raw sums
+// (theta, delta, delta*k_x, delta*k_y, delta*k_z) go to part[c][5][N] and
+// k_rb_combine forms the rows.
+template <typename T, int B, int S>
+__global__ __launch_bounds__(64 * S) void k_rb_rows_cols(Geo g, FmtColArgs<T> fa, const T *__restrict__ rho_pad,
+                                                         CompConsts cc, const double *__restrict__ in_rb,
+                                                         const double *__restrict__ sub, double *__restrict__ out_rb,
+                                                         T *__restrict__ part)
+{
+    __shared__ T red[B * 64];
+    const ColTile t = col_tile<B>(g);
+    T n3[B], s2[B], vx[B], vy[B], vz[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) n3[b] = s2[b] = vx[b] = vy[b] = vz[b] = 0;
+    const T *src0 = rho_pad + (size_t)t.c * g.np + t.base;
+    {
+        const ColPtrs<T> cp = fa.th[t.c];
+        for (int col = t.grp; col < cp.ncol; col += S)
+            col_sum<T, B>(src0 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, n3);
+    }
+    {
+        const ColPtrs<T> cp = fa.de[t.c];
+        for (int col = t.grp; col < cp.ncol; col += S) {
+            T cs[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) cs[b] = 0;
+            const int wo = cp.woff[col];
+            col_sum2<T, B>(src0 + cp.lo[col], cp.w + wo, cp.wz + wo, cp.nchunk[col], t.PXY, cs, vz);
+            const T fx = cp.fxy[2 * col], fy = cp.fxy[2 * col + 1];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                s2[b] += cs[b];
+                vx[b] += fx * cs[b];
+                vy[b] += fy * cs[b];
+            }
+        }
+    }
+    wave_reduce<T, B, S>(n3, red, t.grp, t.lane);
+    wave_reduce<T, B, S>(s2, red, t.grp, t.lane);
+    wave_reduce<T, B, S>(vx, red, t.grp, t.lane);
+    wave_reduce<T, B, S>(vy, red, t.grp, t.lane);
+    wave_reduce<T, B, S>(vz, red, t.grp, t.lane);
+    if (t.grp > 0 || !t.active) return;
+    const int nx = g.nx[0], ny = g.nx[1], nz = g.nx[2];
+    const int c = t.c;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int z = t.zb + b;
+        if (z >= nz) break;
+        const size_t i = t.x + (size_t)nx * (t.y + (size_t)ny * z);
+        if (part) {
+            T *pp = part + (size_t)c * 5 * g.N + i;
+            pp[0] = n3[b];
+            pp[(size_t)g.N] = s2[b];
+            pp[(size_t)2 * g.N] = vx[b];
+            pp[(size_t)3 * g.N] = vy[b];
+            pp[(size_t)4 * g.N] = vz[b];
+            continue;
+        }
+        const T i4r = (T)cc.i4pir[c], i4r2 = (T)cc.i4pirsq[c];
+        const T v[3] = {vx[b], vy[b], vz[b]};
+        double vals[10];
+        vals[0] = n3[b];
+        vals[1] = s2[b];
+        vals[2] = i4r * s2[b];
+        vals[3] = i4r2 * s2[b];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            vals[4 + d] = v[d];
+            vals[7 + d] = i4r * v[d];
+        }
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+            double r = vals[q];
+            if (in_rb) r -= in_rb[(size_t)q * g.N + i];
+            if (sub) r -= sub[(size_t)q * g.N + i];
+            out_rb[(size_t)q * g.N + i] = r;
+        }
+    }
+}
+
+// rho-bar rows from the per-component sums of k_rb_rows_cols (3D)
+template <typename T>
+__global__ void k_rb_combine(Geo g, const T *__restrict__ part, CompConsts cc, const double *__restrict__ in_rb,
+                             const double *__restrict__ sub, double *__restrict__ out_rb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.N) return;
+    T n3 = 0, n2 = 0, n1 = 0, n0 = 0, v2[3] = {0, 0, 0}, v1[3] = {0, 0, 0};
+    for (int c = 0; c < g.C; ++c) {
+        const T *pp = part + (size_t)c * 5 * g.N + i;
+        const T st = pp[0], sd = pp[(size_t)g.N];
+        n3 += st;
+        n2 += sd;
+        n1 += (T)cc.i4pir[c] * sd;
+        n0 += (T)cc.i4pirsq[c] * sd;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T vd = pp[(size_t)(2 + d) * g.N];
+            v2[d] += vd;
+            v1[d] += (T)cc.i4pir[c] * vd;
+        }
+    }
+    double vals[10] = {n3, n2, n1, n0, v2[0], v2[1], v2[2], v1[0], v1[1], v1[2]};
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        double r = vals[q];
+        if (in_rb) r -= in_rb[(size_t)q * g.N + i];
+        if (sub) r -= sub[(size_t)q * g.N + i];
+        out_rb[(size_t)q * g.N + i] = r;
+    }
+}
+
+// Euler-Lagrange rows on a 3D grid, adjoint sums in column form (FMT-halo
+// geometry); the pointwise terms are those of k_el_rows.
+template <typename T, int B, int S>
+__global__ __launch_bounds__(64 * S) void k_el_rows_cols(Geo g, FmtColArgs<T> fa, const T *__restrict__ adj,
+                                                         CompConsts cc, int linear, int nwall,
+                                                         const double *__restrict__ x_rho, const double *__restrict__ in_rho,
+                                                         const double *__restrict__ in_mf, const double *__restrict__ vext,
+                                                         const unsigned char *__restrict__ elzero,
+                                                         const double *__restrict__ add, double *__restrict__ out_el)
+{
+    __shared__ T red[B * 64];
+    const ColTile t = col_tile<B>(g);
+    T acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    {
+        const T *S3 = adj + (size_t)(g.C * 4) * g.np + t.base;
+        const ColPtrs<T> cp = fa.th[t.c];
+        for (int col = t.grp; col < cp.ncol; col += S)
+            col_sum<T, B>(S3 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
+    }
+    {
+        const T *A0 = adj + (size_t)(t.c * 4) * g.np + t.base;
+        const size_t np = g.np;
+        const ColPtrs<T> cp = fa.de[t.c];
+        for (int col = t.grp; col < cp.ncol; col += S) {
+            const T *s = A0 + cp.lo[col];
+            const int wo = cp.woff[col];
+            const T *w = cp.w + wo, *wz = cp.wz + wo;
+            const T fx = cp.fxy[2 * col], fy = cp.fxy[2 * col + 1];
             const int nk = cp.nchunk[col];
             for (int k = 0; k < nk; ++k) {
-                T v[B], wk[2 * B - 1];
+                T a[B], bz[B], wk[2 * B - 1];
 #pragma unroll
-                for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+                for (int i = 0; i < B; ++i) {
+                    const long long o = (long long)(k * B + i) * t.PXY;
+                    a[i] = s[o] - fx * s[np + o] - fy * s[2 * np + o];
+                    bz[i] = s[3 * np + o];
+                }
 #pragma unroll
                 for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
 #pragma unroll
                 for (int i = 0; i < B; ++i)
 #pragma unroll
-                    for (int b = 0; b < B; ++b) acc[b] += wk[i - b + B - 1] * v[i];
+                    for (int b = 0; b < B; ++b) acc[b] += wk[i - b + B - 1] * a[i];
+#pragma unroll
+                for (int j = 0; j < 2 * B - 1; ++j) wk[j] = wz[k * B + j];
+#pragma unroll
+                for (int i = 0; i < B; ++i)
+#pragma unroll
+                    for (int b = 0; b < B; ++b) acc[b] -= wk[i - b + B - 1] * bz[i];
             }
         }
     }
-    if (grp > 0) {
-#pragma unroll
-        for (int b = 0; b < B; ++b) red[grp - 1][b][lane] = acc[b];
-    }
-    __syncthreads();
-    if (grp > 0 || !active) return;
-#pragma unroll
-    for (int s = 0; s < S - 1; ++s)
-#pragma unroll
-        for (int b = 0; b < B; ++b) acc[b] += red[s][b][lane];
+    wave_reduce<T, B, S>(acc, red, t.grp, t.lane);
+    if (t.grp > 0 || !t.active) return;
+    const int nx = g.nx[0], ny = g.nx[1], nz = g.nx[2];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int z = zb + b;
+        const int z = t.zb + b;
         if (z >= nz) break;
-        const size_t row = (size_t)c * g.N + x + (size_t)nx * (y + (size_t)ny * z);
-        double r = acc[b];
-        if (in_mf) r -= in_mf[row];
-        if (sub) r -= sub[row];
-        out_mf[row] = r;
+        const size_t row = (size_t)t.c * g.N + t.x + (size_t)nx * (t.y + (size_t)ny * z);
+        double r;
+        if (elzero[row]) {
+            r = in_rho ? in_rho[row] : 0.0;
+        } else {
+            r = (double)acc[b];
+            const double xv = x_rho[row];
+            if (!linear) {
+                r += log(xv) + cc.mu[t.c];
+                if (nwall > 0) r += vext[row];
+                if (in_mf) r += in_mf[row];
+            } else {
+                if (in_rho) r += in_rho[row] / xv;
+                if (in_mf) r += in_mf[row];
+            }
+        }
+        if (add) r += add[row];
+        out_el[row] = r;
     }
 }
 
@@ -543,6 +817,20 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
     }
     const CompConsts cc = make_consts(P);
     const T *rp = workspace<T>(P).rho_pad;
+    if (P.use_columns) {
+        FmtColArgs<T> fa{};
+        for (int c = 0; c < P.p.ncomp; ++c) {
+            fa.th[c] = col_ptrs(dev_cols<T>(P, TMI_THETA, c), kGeomU);
+            fa.de[c] = col_ptrs(dev_cols<T>(P, TMI_DELTA, c), kGeomU);
+        }
+        T *part = (P.p.ncomp > 1) ? workspace<T>(P).rb_part : nullptr;
+        dim3 grid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((P.p.nx[2] + kColB - 1) / kColB) * P.p.ncomp);
+        hipLaunchKernelGGL((k_rb_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, fa, rp, cc, in_rb, sub,
+                           out_rb, part);
+        if (part)
+            hipLaunchKernelGGL(k_rb_combine<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, part, cc, in_rb, sub, out_rb);
+        return;
+    }
     switch (P.p.ndim) {
     case 1: hipLaunchKernelGGL((k_rb_rows<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
     case 2: hipLaunchKernelGGL((k_rb_rows<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
@@ -557,10 +845,7 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
     Geo gu = make_geo(P, P.pad_u);
     if (P.use_columns) {
         ColArgs<T> ca{};
-        for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) {
-            const DevColumnsT<T> &dc = dev_cols<T>(P, q);
-            ca.u[q] = ColPtrs<T>{dc.lo, dc.woff, dc.nchunk, dc.wpad, dc.ncol};
-        }
+        for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ca.u[q] = col_ptrs(dev_cols<T>(P, TMI_UATT, q), kGeomU);
         const int nz = P.p.nx[2];
         dim3 grid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((nz + kColB - 1) / kColB) * P.p.ncomp);
         hipLaunchKernelGGL((k_mf_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, ca,
@@ -609,8 +894,19 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
     }
     const CompConsts cc = make_consts(P);
     const T *adj = workspace<T>(P).adj_pad;
-    dim3 grid(grid_for(P.N), P.p.ncomp);
     const int lin = linear ? 1 : 0;
+    if (P.use_columns) {
+        FmtColArgs<T> fa{};
+        for (int c = 0; c < P.p.ncomp; ++c) {
+            fa.th[c] = col_ptrs(dev_cols<T>(P, TMI_THETA, c), kGeomF);
+            fa.de[c] = col_ptrs(dev_cols<T>(P, TMI_DELTA, c), kGeomF);
+        }
+        dim3 cgrid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((P.p.nx[2] + kColB - 1) / kColB) * P.p.ncomp);
+        hipLaunchKernelGGL((k_el_rows_cols<T, kColB, kColS>), cgrid, dim3(64, kColS), 0, s, gf, fa, adj, cc, lin,
+                           P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
+        return;
+    }
+    dim3 grid(grid_for(P.N), P.p.ncomp);
     switch (P.p.ndim) {
     case 1:
         hipLaunchKernelGGL((k_el_rows<T, 1>), grid, dim3(kBlock), 0, s, gf, ea, adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf,
