@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+COPY_PROBE_BYTES = 512 << 20
+# HBM bytes per launch of the attraction-stencil kernel (k_mf_rows_cols<double>) on the 64^3
+# workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script with --copy-probe,
+# scaled by the counters' readings on the 512 MiB copy (x2.0 and x1.0): 54.67 + 2.00 MiB.
+MF_TRAFFIC_BYTES_64 = (54.67 + 2.00) * (1 << 20)
 
 
 def stencil_flops_per_node(dp, spec):
@@ -62,6 +67,8 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 512 MiB device copy three times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -73,6 +80,14 @@ def main():
 
     from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
     from tramonto_amd.problem import periodic_3d
+
+    if args.copy_probe:
+        a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     spec = periodic_3d(seed=2, n=args.n)
     dp = DeviceProblem(spec)
@@ -172,7 +187,10 @@ def main():
         "flops_per_node_direct": flops_node,
         "bytes_per_node_residual": bytes_node,
         "roofline": {"bound": "mfma", "kernel": stage_names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": MF_TRAFFIC_BYTES_64 if (args.n == 64 and stage_names[dom] == "mf_rows") else None,
+                     "traffic_note": "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
+                                     "512 MiB copy; algorithmic bytes per launch are 6 MiB",
                      "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
                              "peak = 78.6 TFLOP/s)"},
         "fill_hbm_frac": (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9),

@@ -331,15 +331,27 @@ __device__ __forceinline__ void wave_reduce(T (&acc)[B], T *buf, int grp, int la
 }
 
 // Column-kernel launch shape: workgroup = 64 x-lanes x S waves for one
-// (y, z-block, component); each lane owns kColB consecutive z outputs and
-// wave s takes the stencil columns s, s+S, ...; the S partial sums meet in LDS.
+// (x-block, y, z-block, component) tile; each lane owns kColB consecutive z
+// outputs and wave s takes the stencil columns s, s+S, ...; the S partial
+// sums meet in LDS.  The grid is 1D.  Workgroups are dealt round-robin to the
+// 8 XCDs, each with its own L2, so the block id is remapped to give every XCD
+// a contiguous range of tiles, and tiles are ordered x-block fastest, then
+// groups of kColGZ z-blocks by all y, so that a range is a compact (y, z)
+// region of the halo-padded input.
 constexpr int kColS = 8;
+constexpr int kXcd = 8;
+constexpr int kColGZ = 4;
 
 struct ColTile {
     int lane, grp, x, y, c, zb;
     bool active;
     long long PXY, base;
 };
+
+inline int col_tiles(const Problem &P)
+{
+    return ((P.p.nx[0] + 63) / 64) * P.p.nx[1] * ((P.p.nx[2] + kColB - 1) / kColB) * P.p.ncomp;
+}
 
 template <int B>
 __device__ __forceinline__ ColTile col_tile(const Geo &g)
@@ -349,15 +361,25 @@ __device__ __forceinline__ ColTile col_tile(const Geo &g)
     // the wave index is uniform across the wave: readfirstlane lets the
     // compiler keep the column loops and their weights in scalar registers
     t.grp = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    t.x = blockIdx.x * 64 + t.lane;
-    t.y = blockIdx.y;
+    const int nxb = (g.nx[0] + 63) / 64, ny = g.nx[1];
     const int zblocks = (g.nx[2] + B - 1) / B;
-    t.c = blockIdx.z / zblocks;
-    t.zb = (blockIdx.z % zblocks) * B;
-    t.active = (t.x < g.nx[0]) && (t.y < g.nx[1]);
-    const int xs = t.active ? t.x : 0, ys = t.active ? t.y : 0;
+    const int nb = nxb * ny * zblocks * g.C;
+    int L = blockIdx.x;
+    if (nb % kXcd == 0) L = (L % kXcd) * (nb / kXcd) + L / kXcd;
+    const int xb = L % nxb;
+    const int r = L / nxb;
+    t.c = r / (ny * zblocks);
+    const int q = r % (ny * zblocks);
+    const int zg = q / (ny * kColGZ);
+    const int gz = min(kColGZ, zblocks - zg * kColGZ);
+    const int q2 = q - zg * ny * kColGZ;
+    t.y = q2 / gz;
+    t.zb = (zg * kColGZ + q2 % gz) * B;
+    t.x = xb * 64 + t.lane;
+    t.active = (t.x < g.nx[0]);
+    const int xs = t.active ? t.x : 0;
     t.PXY = (long long)g.P[0] * g.P[1];
-    t.base = (long long)(xs + g.h[0]) + (long long)(ys + g.h[1]) * g.P[0] + (long long)(t.zb + g.h[2]) * t.PXY;
+    t.base = (long long)(xs + g.h[0]) + (long long)(t.y + g.h[1]) * g.P[0] + (long long)(t.zb + g.h[2]) * t.PXY;
     return t;
 }
 
@@ -824,7 +846,7 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
             fa.de[c] = col_ptrs(dev_cols<T>(P, TMI_DELTA, c), kGeomU);
         }
         T *part = (P.p.ncomp > 1) ? workspace<T>(P).rb_part : nullptr;
-        dim3 grid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((P.p.nx[2] + kColB - 1) / kColB) * P.p.ncomp);
+        dim3 grid(col_tiles(P));
         hipLaunchKernelGGL((k_rb_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, fa, rp, cc, in_rb, sub,
                            out_rb, part);
         if (part)
@@ -846,8 +868,7 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
     if (P.use_columns) {
         ColArgs<T> ca{};
         for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ca.u[q] = col_ptrs(dev_cols<T>(P, TMI_UATT, q), kGeomU);
-        const int nz = P.p.nx[2];
-        dim3 grid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((nz + kColB - 1) / kColB) * P.p.ncomp);
+        dim3 grid(col_tiles(P));
         hipLaunchKernelGGL((k_mf_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, ca,
                            workspace<T>(P).rho_pad, in_mf, sub, out_mf);
         return;
@@ -901,7 +922,7 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
             fa.th[c] = col_ptrs(dev_cols<T>(P, TMI_THETA, c), kGeomF);
             fa.de[c] = col_ptrs(dev_cols<T>(P, TMI_DELTA, c), kGeomF);
         }
-        dim3 cgrid((P.p.nx[0] + 63) / 64, P.p.nx[1], ((P.p.nx[2] + kColB - 1) / kColB) * P.p.ncomp);
+        dim3 cgrid(col_tiles(P));
         hipLaunchKernelGGL((k_el_rows_cols<T, kColB, kColS>), cgrid, dim3(64, kColS), 0, s, gf, fa, adj, cc, lin,
                            P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
         return;
