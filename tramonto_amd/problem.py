@@ -229,6 +229,94 @@ def slit_pore_1d(H: float, dx: float, sig=(1.0, 0.8), rho_b=(0.2, 0.2), eps_w: f
                        sigma_ff=S, eps_ff=E, cut_ff=R, rho_b=list(rho_b), vext=vext, zero_tf=zero, rho0=rho0)
 
 
+def gaussian_rough_profile(n: int, dx: float, amplitude: float, corr_len: float, seed: int) -> np.ndarray:
+    """Periodic random height profile h(y) on n nodes: SplitMix64 white noise
+    filtered by a Gaussian of correlation length corr_len (FFT, periodic), zero
+    mean, scaled so that max |h| = amplitude."""
+    rng = SplitMix64(seed)
+    white = rng.uniform(n, -1.0, 1.0)
+    k = 2 * math.pi * np.fft.fftfreq(n, d=dx)
+    filt = np.exp(-0.25 * (k * corr_len) ** 2)
+    h = np.real(np.fft.ifft(np.fft.fft(white) * filt))
+    h -= h.mean()
+    return amplitude * h / np.abs(h).max()
+
+
+def rough_wall_2d(seed: int = 4, nx: int = 256, ny: int = 256, dx: float = 0.05, eps_w: float = 1.5,
+                  amplitude: float = 0.5, corr_len: float = 2.0, rho_b: float = 0.5, cut: float = 2.5,
+                  wall_pos: float = 1.0, wall_cut: float = 3.0, z_excl: float = 0.62,
+                  vext_max: float = 10.0) -> ProblemSpec:
+    """2D Rosenfeld + WCA fluid (sigma = 1, eps = 1, cut-off `cut`, kT = 1) at a
+    rough LJ 9-3 wall on the x = 0 face.  The wall surface sits at
+    x_s(y) = wall_pos + h(y), h a Gaussian-correlated periodic profile
+    (gaussian_rough_profile); V_ext(x, y) = lj93_wall(x - x_s(y); eps_w,
+    sigma_w = 1, cut wall_cut).  Nodes with x - x_s(y) < z_excl are excluded
+    (Zero_density_TF, V_ext = VEXT_MAX); elsewhere V_ext < VEXT_MAX, so the
+    excluded set does not depend on eps_w.  Boundaries: IN_WALL at x = 0,
+    IN_BULK at x = L_x, periodic in y."""
+    yv = np.arange(ny) * dx
+    xv = np.arange(nx) * dx
+    h = gaussian_rough_profile(ny, dx, amplitude, corr_len, seed)
+    zs = xv[None, :] - (wall_pos + h[:, None])          # [ny, nx]
+    excl = zs < z_excl
+    v = lj93_wall(np.where(excl, 1.0, zs), eps_w, 1.0, wall_cut)
+    if np.any(v[~excl] >= vext_max):
+        raise ValueError("V_ext reaches VEXT_MAX outside the excluded region; raise z_excl")
+    v = np.where(excl, vext_max, v)
+    vext = v.reshape(-1)[:, None]                        # node = i + j*nx: x fastest
+    zero = excl.reshape(-1).astype(np.uint8)[:, None]
+    rng = SplitMix64(seed + 1)
+    rho = rho_b * np.exp(-np.minimum(vext[:, 0], 30.0)) * (1.0 + 0.02 * rng.uniform(nx * ny, -1.0, 1.0))
+    rho[zero[:, 0] == 1] = 0.0
+    return ProblemSpec(name="rough_wall_2d", ndim=2, ncomp=1, nx=[nx, ny], dx=[dx, dx],
+                       bc=[[IN_WALL, IN_BULK], [PERIODIC, PERIODIC]], type_func=FMT1, type_attr=0, nwall=1,
+                       vext_max=vext_max, hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.ones((1, 1)),
+                       cut_ff=np.full((1, 1), cut), rho_b=[rho_b], vext=vext, zero_tf=zero, rho0=rho[None, :])
+
+
+def dirichlet_split(total: float, k: int, seed: int) -> np.ndarray:
+    """total * (w_1..w_k), w ~ Dirichlet(1,...,1) from SplitMix64 (normalised exponentials)."""
+    e = -np.log(SplitMix64(seed).uniform(k, 1e-300, 1.0))
+    return total * e / e.sum()
+
+
+def solute_3d(seed: int = 5, n: int = 128, dx: float = 0.1, sig=(1.0, 0.9, 0.7), eta_b: float = 0.35,
+              radius: float = 4.0, eps_w: float = 1.0, cut_fac: float = 3.0, wall_cut: float = 3.0,
+              z_excl: float = 0.62, vext_max: float = 10.0) -> ProblemSpec:
+    """3D White Bear + WCA mixture (sigma_i = sig, eps = 1, cut-off 3 sigma_i,
+    Lorentz-Berthelot cross terms) around a spherical solute of `radius` at the
+    box centre, periodic box of n^3 nodes.  Solute-fluid potential: LJ 9-3 in
+    the surface distance s = r - radius (eps_w, sigma_w = sigma_i, cut wall_cut
+    sigma_i); nodes with s < z_excl sigma_i are excluded for that species.
+    Bulk packing fraction eta_b split over the species by seeded Dirichlet
+    weights, rho_i = eta_i / (pi sigma_i^3 / 6)."""
+    C = len(sig)
+    eta = dirichlet_split(eta_b, C, seed)
+    rho_b = [float(eta[c] / (math.pi * sig[c] ** 3 / 6.0)) for c in range(C)]
+    g = (np.arange(n) - 0.5 * (n - 1)) * dx
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")        # node = i + j n + k n^2: x fastest
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2).reshape(-1)
+    vext = np.zeros((r.size, C))
+    zero = np.zeros((r.size, C), dtype=np.uint8)
+    for c in range(C):
+        s = r - radius
+        excl = s < z_excl * sig[c]
+        v = lj93_wall(np.where(excl, sig[c], s), eps_w, sig[c], wall_cut * sig[c])
+        if np.any(v[~excl] >= vext_max):
+            raise ValueError("V_ext reaches VEXT_MAX outside the excluded region; raise z_excl")
+        vext[:, c] = np.where(excl, vext_max, v)
+        zero[:, c] = excl
+    S, E, R = lorentz_berthelot(list(sig), [1.0] * C, [cut_fac * s for s in sig])
+    rho0 = np.zeros((C, r.size))
+    for c in range(C):
+        rho0[c] = rho_b[c] * np.exp(-np.minimum(vext[:, c], 30.0))
+        rho0[c][zero[:, c] == 1] = 0.0
+    return ProblemSpec(name="solute_3d", ndim=3, ncomp=C, nx=[n, n, n], dx=[dx] * 3,
+                       bc=[[PERIODIC, PERIODIC]] * 3, type_func=FMT3, type_attr=0, nwall=1,
+                       vext_max=vext_max, hs_diam=list(sig), sigma_ff=S, eps_ff=E, cut_ff=R, rho_b=rho_b,
+                       vext=vext, zero_tf=zero, rho0=rho0)
+
+
 def small_case(kind: str, seed: int = 7) -> ProblemSpec:
     """Small state points the oracle evaluates in seconds (parity tests)."""
     if kind == "1d_hw":
@@ -287,4 +375,8 @@ def small_case(kind: str, seed: int = 7) -> ProblemSpec:
                            bc=[[IN_BULK, IN_BULK], [PERIODIC, PERIODIC], [REFLECT, IN_WALL]],
                            type_func=FMT3, type_attr=0, nwall=0, hs_diam=sig, sigma_ff=S, eps_ff=E,
                            cut_ff=R, rho_b=[0.3, 0.25], rho0=rho0)
+    if kind == "2d_rough_wall":
+        return rough_wall_2d(seed=seed, nx=40, ny=32, dx=0.1, cut=1.5, wall_cut=2.0)
+    if kind == "3d_solute_small":
+        return solute_3d(seed=seed, n=20, dx=0.25, sig=(1.0, 0.8), radius=1.2, cut_fac=1.2, wall_cut=2.0)
     raise KeyError(kind)
