@@ -143,6 +143,34 @@ int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *
                     int maxit, int precision, int *iters, void *stream);
 
 /* --------------------------------------------------------------------------
+ * Batched 1D problems.  No counterpart in the reference, which solves one
+ * problem per run (src/dft_main.c): npores independent 1D problems ("pores",
+ * IN_WALL at both ends of each) that share the fluid of *fluid (ndim = 1;
+ * fluid->nx[0], fluid->bc and fluid->rho_b are ignored) and differ in node
+ * count nodes[p], bulk densities rho_b[p][ncomp], V_ext and Zero_density_TF
+ * (vext, zero_tf: [N][ncomp] node-major over the packed nodes, pore 0 first,
+ * N = sum of nodes).  Each pore is the problem the reference would set up
+ * with those inputs (one fill_resid_and_matrix_control system per pore).
+ * The handle works with tmi_problem_info (nnodes = N), tmi_fill_residual,
+ * tmi_apply_jacobian[_fp32], tmi_init_fields, tmi_schur_solve and
+ * tmi_newton_solve; rows of x hold the packed nodes.  Transpose products
+ * return TMI_EUNSUP.
+ * ------------------------------------------------------------------------ */
+int tmi_batch_create(const tmi_params *fluid, int npores, const int *nodes, const double *rho_b, const double *vext,
+                     const unsigned char *zero_tf, void **handle);
+/* number of pores and first node of each pore (pore_off[npores+1], may be NULL) */
+int tmi_batch_info(void *handle, int *npores, int *pore_off);
+/* Newton-Krylov per pore in lockstep (newton_solver, src/dft_newton.c:131-196,
+ * applied to every pore): per-pore GMRES, damping and convergence; one
+ * Newton step runs as replayed HIP graphs (TMI_BATCH_GRAPH=0 launches the
+ * same kernels directly).  stats: iters = Newton steps taken, converged =
+ * all pores converged, gmres_iters_total = sum over pores.  Per-pore outputs
+ * ([npores], host, each may be NULL): Newton iterations, converged flag,
+ * GMRES iterations, final ||F_p||_2. */
+int tmi_batch_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi_newton_stats *stats,
+                           int *pore_iters, int *pore_converged, int *pore_gmres, double *pore_resid, void *stream);
+
+/* --------------------------------------------------------------------------
  * Reference ABI.  The handle is the LinProbMgr; the fill entry point uses
  * the problem registered with tmi_set_active_problem (the reference reads
  * its problem from globals).  Arrays are host double** in box layout,

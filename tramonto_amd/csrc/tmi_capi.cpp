@@ -206,14 +206,77 @@ void free_stencil(DevStencilT<T> &ds)
     ds = DevStencilT<T>();
 }
 
-Problem *create_problem(const tmi_params &p, const double *vext, const unsigned char *zero_tf)
+// Pores of a batched 1D problem: node counts and bulk densities [npores][C].
+struct BatchLayout {
+    int npores = 0;
+    const int *nodes = nullptr;
+    const double *rho_b = nullptr;
+};
+
+// The padded copy of a batched problem: pores back to back, each preceded
+// (and the last also followed) by a wall gap of h zero positions.
+void batch_geometry(Problem &P, PadGeom &pg, int h)
 {
+    const long long n = (long long)P.N + (long long)(P.npores + 1) * h;
+    if (n > INT32_MAX) throw std::domain_error("batch too large for 32-bit padded positions");
+    pg.h[0] = h;
+    pg.h[1] = pg.h[2] = 0;
+    pg.P[0] = (int)n;
+    pg.P[1] = pg.P[2] = 1;
+    pg.n = n;
+    std::vector<int> node_pad(P.N), pad_node((size_t)n, -1);
+    for (int q = 0; q < P.npores; ++q)
+        for (int i = P.pore_off[q]; i < P.pore_off[q + 1]; ++i) {
+            node_pad[i] = i + (q + 1) * h;
+            pad_node[(size_t)i + (size_t)(q + 1) * h] = i;
+        }
+    pg.d_node_pad = dev_alloc<int>(node_pad.size());
+    pg.d_pad_node = dev_alloc<int>(pad_node.size());
+    upload(pg.d_node_pad, node_pad.data(), node_pad.size());
+    upload(pg.d_pad_node, pad_node.data(), pad_node.size());
+}
+
+void free_geometry_maps(PadGeom &pg)
+{
+    (void)hipFree(pg.d_node_pad);
+    (void)hipFree(pg.d_pad_node);
+    pg.d_node_pad = pg.d_pad_node = nullptr;
+}
+
+void destroy_problem(Problem *P);
+
+Problem *create_problem(const tmi_params &p_in, const double *vext, const unsigned char *zero_tf,
+                        const BatchLayout *batch = nullptr)
+{
+    tmi_params p = p_in;
+    std::vector<int> pore_off;
+    if (batch) {
+        if (p.ndim != 1) throw std::invalid_argument("batched problems are 1D");
+        if (batch->npores < 1 || !batch->nodes || !batch->rho_b) throw std::invalid_argument("batch layout");
+        pore_off.assign(1, 0);
+        for (int q = 0; q < batch->npores; ++q) {
+            if (batch->nodes[q] < 1) throw std::invalid_argument("pore node count must be positive");
+            if ((long long)pore_off.back() + batch->nodes[q] > INT32_MAX) throw std::domain_error("batch too large");
+            pore_off.push_back(pore_off.back() + batch->nodes[q]);
+        }
+        p.nx[0] = pore_off.back();
+        p.bc[0][0] = p.bc[0][1] = TMI_IN_WALL;
+        for (int c = 0; c < p.ncomp; ++c) p.rho_b[c] = batch->rho_b[c];
+        for (int q = 0; q < batch->npores; ++q)
+            for (int c = 0; c < p.ncomp; ++c)
+                if (!(batch->rho_b[(size_t)q * p.ncomp + c] > 0.0)) throw std::invalid_argument("rho_b must be positive");
+    }
     validate(p);
     auto P = new Problem();
     P->p = p;
     const int C = p.ncomp, nd = p.ndim;
     P->N = 1;
     for (int d = 0; d < nd; ++d) P->N *= p.nx[d];
+    if (batch) {
+        P->npores = batch->npores;
+        P->pore_off = pore_off;
+        P->pore_rho_b.assign(batch->rho_b, batch->rho_b + (size_t)batch->npores * C);
+    }
     P->nrb = 4 + 2 * nd;
     P->nmf = (p.type_attr >= 0) ? C : 0;
     P->iu_rho = 0;
@@ -240,8 +303,10 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
                     if (t != TMI_UATT) hf[d] = std::max(hf[d], a);
                 }
         }
-    set_geometry(*P, P->pad_u, hu);
-    set_geometry(*P, P->pad_f, hf);
+    if (!batch) {
+        set_geometry(*P, P->pad_u, hu);
+        set_geometry(*P, P->pad_f, hf);
+    }
     set_geometry(*P, P->pad_u2, hu, 2);
     set_geometry(*P, P->pad_f2, hf, 2);
 
@@ -259,6 +324,33 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
             P->h_elzero[c * N + i] = (z || (p.nwall > 0 && v == p.vext_max)) ? 1 : 0;
         }
     try {
+        if (batch) {
+            batch_geometry(*P, P->pad_u, hu[0]);
+            batch_geometry(*P, P->pad_f, hf[0]);
+            std::vector<int> pore_of(N);
+            for (int q = 0; q < P->npores; ++q)
+                for (int i = P->pore_off[q]; i < P->pore_off[q + 1]; ++i) pore_of[i] = q;
+            // per-pore chemical-potential term, as make_consts forms it for one problem
+            std::vector<double> mu((size_t)C * N);
+            for (int q = 0; q < P->npores; ++q) {
+                tmi_params pq = p;
+                for (int c = 0; c < C; ++c) pq.rho_b[c] = P->pore_rho_b[(size_t)q * C + c];
+                BulkConstants bq;
+                compute_bulk_constants(pq, bq);
+                if (p.type_attr >= 0) compute_attraction_constants(pq, bq, P->hs);
+                for (int c = 0; c < C; ++c) {
+                    double m = -std::log(pq.rho_b[c]) - bq.betamu_hs_ex[c];
+                    if (p.type_attr >= 0) m -= bq.betamu_att[c];
+                    for (int i = P->pore_off[q]; i < P->pore_off[q + 1]; ++i) mu[(size_t)c * N + i] = m;
+                }
+            }
+            P->d_pore_off = dev_alloc<int>(P->pore_off.size());
+            P->d_pore_of = dev_alloc<int>(N);
+            P->d_mu = dev_alloc<double>(mu.size());
+            upload(P->d_pore_off, P->pore_off.data(), P->pore_off.size());
+            upload(P->d_pore_of, pore_of.data(), pore_of.size());
+            upload(P->d_mu, mu.data(), mu.size());
+        }
         P->d_zero = dev_alloc<unsigned char>(P->h_zero.size());
         P->d_elzero = dev_alloc<unsigned char>(P->h_elzero.size());
         P->d_vext = dev_alloc<double>(P->h_vext.size());
@@ -311,7 +403,7 @@ Problem *create_problem(const tmi_params &p, const double *vext, const unsigned 
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
     } catch (...) {
-        delete P;
+        destroy_problem(P);
         throw;
     }
     return P;
@@ -326,6 +418,12 @@ void destroy_problem(Problem *P)
             free_columns(P->cold[t][q]);
             free_columns(P->colf[t][q]);
         }
+    free_batch_work(*P);
+    free_geometry_maps(P->pad_u);
+    free_geometry_maps(P->pad_f);
+    (void)hipFree(P->d_pore_off);
+    (void)hipFree(P->d_pore_of);
+    (void)hipFree(P->d_mu);
     (void)hipFree(P->d_zero);
     (void)hipFree(P->d_elzero);
     (void)hipFree(P->d_vext);
@@ -728,6 +826,7 @@ int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *
     return guarded([&] {
         Problem *P = as_problem(handle);
         if (!x_d || !w_d || !JTw_d) throw std::invalid_argument("null device pointer");
+        if (P->npores > 0) throw std::domain_error("transpose products of batched problems are not implemented");
         launch_vjp<double>(*P, x_d, w_d, JTw_d, (hipStream_t)stream);
         return 0;
     });
@@ -756,7 +855,45 @@ int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi
     return guarded([&] {
         Problem *P = as_problem(handle);
         if (!opts || !stats) throw std::invalid_argument("null argument");
+        if (P->npores > 0)
+            return batch_newton_solve(*P, x_d, *opts, *stats, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
         return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
+    });
+}
+
+int tmi_batch_create(const tmi_params *fluid, int npores, const int *nodes, const double *rho_b, const double *vext,
+                     const unsigned char *zero_tf, void **handle)
+{
+    return guarded([&] {
+        if (!fluid || !handle || !nodes || !rho_b) throw std::invalid_argument("null argument");
+        BatchLayout bl;
+        bl.npores = npores;
+        bl.nodes = nodes;
+        bl.rho_b = rho_b;
+        *handle = create_problem(*fluid, vext, zero_tf, &bl);
+        return 0;
+    });
+}
+
+int tmi_batch_info(void *handle, int *npores, int *pore_off)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (npores) *npores = P->npores;
+        if (pore_off && P->npores > 0) std::memcpy(pore_off, P->pore_off.data(), sizeof(int) * P->pore_off.size());
+        return 0;
+    });
+}
+
+int tmi_batch_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi_newton_stats *stats,
+                           int *pore_iters, int *pore_converged, int *pore_gmres, double *pore_resid, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!opts || !stats || !x_d) throw std::invalid_argument("null argument");
+        if (P->npores < 1) throw std::invalid_argument("not a batched problem");
+        return batch_newton_solve(*P, x_d, *opts, *stats, pore_iters, pore_converged, pore_gmres, pore_resid,
+                                  (hipStream_t)stream);
     });
 }
 

@@ -19,6 +19,9 @@ struct Geo {
     int bc[3][2];
     int h[3], P[3];
     long long np;
+    const int *node_pad;   // batched problems: node -> padded position (else null)
+    const int *pad_node;   // batched problems: padded position -> node or -1 (else null)
+    const double *mu;      // batched problems: [C][N] chemical-potential term (else null)
 };
 
 inline Geo make_geo(const Problem &P, const PadGeom &pg)
@@ -35,6 +38,9 @@ inline Geo make_geo(const Problem &P, const PadGeom &pg)
         g.P[d] = pg.P[d];
     }
     g.np = pg.n;
+    g.node_pad = pg.d_node_pad;
+    g.pad_node = pg.d_pad_node;
+    g.mu = P.d_mu;
     return g;
 }
 
@@ -67,6 +73,7 @@ inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBl
 // padded position of node i
 __device__ inline long long pad_pos(const Geo &g, int i)
 {
+    if (g.node_pad) return g.node_pad[i];
     int x = i % g.nx[0];
     int r = i / g.nx[0];
     int y = r % g.nx[1];
@@ -79,6 +86,11 @@ __device__ inline long long pad_pos(const Geo &g, int i)
 // (src/dft_mesh_lib.c:203-297).
 __device__ inline int map_pad(const Geo &g, long long pos, int &src, int &refl)
 {
+    refl = 0;
+    if (g.pad_node) {   // batched: the gaps between pores are walls
+        src = g.pad_node[pos];
+        return src >= 0 ? 0 : -2;
+    }
     int c[3];
     c[0] = (int)(pos % g.P[0]);
     long long r = pos / g.P[0];

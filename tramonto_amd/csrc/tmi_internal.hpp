@@ -45,6 +45,9 @@ struct PadGeom {
     int h[3] = {0, 0, 0};
     int P[3] = {1, 1, 1};
     long long n = 0;   // P0*P1*P2
+    // batched 1D problems only: explicit maps between nodes and padded positions
+    int *d_node_pad = nullptr;   // [N] padded position of each node
+    int *d_pad_node = nullptr;   // [n] node at each padded position, -1 in the wall gaps
 };
 
 template <typename T>
@@ -81,6 +84,7 @@ struct Grid {
 };
 
 struct Problem;
+struct BatchWork;
 
 // kernel-side launch helpers (tmi_kernels.hip)
 template <typename T>
@@ -116,6 +120,17 @@ struct Problem {
     std::vector<double> h_vext;
     Workspace<double> wsd;
     Workspace<float> wsf;
+    // Batched 1D problems: npores independent pores (IN_WALL at both ends) that
+    // share the fluid, packed node-contiguously, pore p owning nodes
+    // [pore_off[p], pore_off[p+1]); in the padded copies consecutive pores are
+    // separated by a zero gap of the halo width.  npores == 0: one problem.
+    int npores = 0;
+    std::vector<int> pore_off;
+    int *d_pore_off = nullptr;     // [npores+1]
+    int *d_pore_of = nullptr;      // [N] pore of each node
+    double *d_mu = nullptr;        // [C][N] -ln rho_b - beta mu_hs,ex - beta mu_att of the node's pore
+    std::vector<double> pore_rho_b;   // [npores][C]
+    BatchWork *batch_work = nullptr;  // batched Newton buffers and graphs (tmi_batch.hip)
     // solver scratch
     double *d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -142,6 +157,11 @@ void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t
 void launch_schur_rhs(Problem &P, const double *x, const double *b, double *bS, hipStream_t s);
 // x1 = A12 x2 - b1 written into dx (full length), x2 already in dx[0:C*N]
 void launch_schur_backsub(Problem &P, const double *x, const double *b, double *dx, hipStream_t s);
+
+// batched problems (tmi_batch.hip): per-pore Newton-Krylov; per-pore outputs may be NULL
+int batch_newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_stats &st, int *pore_iters,
+                       int *pore_conv, int *pore_gmres, double *pore_resid, hipStream_t s);
+void free_batch_work(Problem &P);
 
 // vector helpers
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s);

@@ -661,7 +661,7 @@ __global__ void k_el_rows(Geo g, ElArgs ea, const T *__restrict__ adj, CompConst
         r = (double)acc;
         const double xv = x_rho[row];
         if (!linear) {
-            r += log(xv) + cc.mu[c];
+            r += log(xv) + (g.mu ? g.mu[row] : cc.mu[c]);
             if (nwall > 0) r += vext[row];
             if (in_mf) r += in_mf[row];
         } else {

@@ -111,7 +111,10 @@ extern "C" {
 int tmi_set_active_problem(void *problem_handle, void *linprobmgr_handle)
 {
     return guarded([&] {
-        g_active_problem = problem_handle ? problem_from_handle(problem_handle) : nullptr;
+        Problem *P = problem_handle ? problem_from_handle(problem_handle) : nullptr;
+        if (P && P->npores > 0)
+            throw std::invalid_argument("a batched problem has no single-problem LinProbMgr counterpart");
+        g_active_problem = P;
         g_active = (LinProbMgr *)linprobmgr_handle;
         if (g_active) g_active->prob = g_active_problem;
         return 0;

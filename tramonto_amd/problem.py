@@ -204,10 +204,14 @@ def periodic_3d(seed: int = 2, n: int = 64, dx: float = 0.1, rho_b: float = 0.6,
 
 
 def slit_pore_1d(H: float, dx: float, sig=(1.0, 0.8), rho_b=(0.2, 0.2), eps_w: float = 1.0,
-                 cut_fac: float = 2.5, seed: int = 3, vext_max: float = 10.0) -> ProblemSpec:
-    """Binary Rosenfeld + WCA fluid in a slit of width H between two LJ 9-3
-    walls (eps_w, sigma_wf = sigma_i, cut-off 2.5 sigma_i), IN_WALL outside.
-    Nodes with V_ext >= VEXT_MAX are zero-density nodes."""
+                 cut_fac: float = 2.5, seed: int = 3, vext_max: float = 10.0, eps_ff: float = 1.0) -> ProblemSpec:
+    """Binary Rosenfeld + WCA fluid (fluid-fluid eps_ff/kT, cut-off cut_fac
+    sigma_i) in a slit of width H between two LJ 9-3 walls (eps_w,
+    sigma_wf = sigma_i), IN_WALL outside.  Nodes with V_ext >= VEXT_MAX are
+    zero-density nodes.  Seeded initial densities rho_b exp(-max(V_ext, 0))
+    (1 + 0.05 U(-1, 1)): the Boltzmann factor only where the wall repels, since
+    amplifying dense states in the attractive well can start Newton past
+    packing (n3 > 1)."""
     C = len(sig)
     nx = nodes_along(H, dx, False)
     z = np.arange(nx) * dx
@@ -218,11 +222,11 @@ def slit_pore_1d(H: float, dx: float, sig=(1.0, 0.8), rho_b=(0.2, 0.2), eps_w: f
         v = np.where(v >= vext_max, vext_max, v)
         vext[:, c] = v
     zero = (vext >= vext_max).astype(np.uint8)
-    S, E, R = lorentz_berthelot(list(sig), [1.0] * C, [cut_fac * s for s in sig])
+    S, E, R = lorentz_berthelot(list(sig), [eps_ff] * C, [cut_fac * s for s in sig])
     rng = SplitMix64(seed)
     rho0 = np.zeros((C, nx))
     for c in range(C):
-        rho0[c] = rho_b[c] * np.exp(-np.minimum(vext[:, c], 50.0)) * (1.0 + 0.05 * rng.uniform(nx, -1, 1))
+        rho0[c] = rho_b[c] * np.exp(-np.clip(vext[:, c], 0.0, 50.0)) * (1.0 + 0.05 * rng.uniform(nx, -1, 1))
         rho0[c][zero[:, c] == 1] = 0.0
     return ProblemSpec(name="slit_pore_1d", ndim=1, ncomp=C, nx=[nx], dx=[dx], bc=[[IN_WALL, IN_WALL]],
                        type_func=FMT1, type_attr=0, nwall=2, vext_max=vext_max, hs_diam=list(sig),
