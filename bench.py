@@ -59,6 +59,159 @@ def cpu_baseline(spec, x_np):
                        f"({t * 1e3:.1f} ms each)")
 
 
+def bench_batched_pores(npores=256):
+    """The uneven slit-pore workload: every pore solved to ||F_p|| < 1e-8, (a) all
+    pores at once by the batched per-pore Newton-Krylov (graph replay and direct
+    launches, FP64 and mixed), (b) pore by pore with the plain single-problem
+    FP64 path.  Problem set-up is outside the timed regions."""
+    import os as _os
+
+    import numpy as np
+    import torch
+
+    from tramonto_amd.batch import DeviceBatch, slit_pore_batch
+    from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
+
+    b = slit_pore_batch(npores=npores)
+    db = DeviceBatch(b)
+    x0 = db.initial_state()
+    F = db.zeros()
+    for _ in range(3):
+        db.fill_residual(x0, F)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        db.fill_residual(x0, F)
+    e1.record()
+    torch.cuda.synchronize()
+    fill_ms = e0.elapsed_time(e1) / 20
+    out = {"workload": f"{npores} uneven 1D slit pores, H ~ U(4,30) sigma at dx = 0.02 sigma, binary "
+                       "(sigma 1.0/0.8) Rosenfeld FMT + WCA (cut-off 2.5 sigma_i, eps_ff/kT = 0.7), LJ 9-3 walls "
+                       "eps_w ~ U(0.5,2), rho_b,i ~ U(0.05,0.35)",
+           "pores": npores, "nodes": int(b.nnodes), "pore_nodes_min_max": [int(b.nodes.min()), int(b.nodes.max())],
+           "fill_ms": fill_ms, "fill_gnode_s": b.nnodes / (fill_ms * 1e-3) / 1e9,
+           "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
+                                     [float(v) for v in db.fill_residual_timed(x0, 10)]))}
+    sols = {}
+    for name, prec, graph in (("batched_fp64_graph", PREC_FP64, "1"), ("batched_fp64_direct", PREC_FP64, "0"),
+                              ("batched_mixed_graph", PREC_MIXED, "1"), ("batched_mixed_direct", PREC_MIXED, "0")):
+        _os.environ["TMI_BATCH_GRAPH"] = graph
+        xw = x0.clone()
+        db.newton(xw, max_iter=2, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)   # buffers and graphs
+        x = x0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = db.newton(x, max_iter=60, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[name] = {"seconds": wall, "newton_iters": st["iters"], "pores_converged": int(st["pore_converged"].sum()),
+                     "gmres_iters_total": st["gmres_iters"], "max_pore_resid": float(st["pore_resid"].max())}
+        sols[name] = x
+    _os.environ.pop("TMI_BATCH_GRAPH", None)
+    # plain path: one single-problem device Newton per pore
+    dps = [DeviceProblem(p) for p in b.pores]
+    xs = [dp.initial_state() for dp in dps]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nconv, iters = 0, 0
+    for dp, x in zip(dps, xs):
+        st = dp.newton(x, max_iter=60, resid_tol=1e-8, gmres_tol=1e-6)
+        nconv += int(st["converged"])
+        iters += st["iters"]
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    out["sequential_fp64"] = {"seconds": seq, "pores_converged": nconv, "newton_iters_total": iters}
+    xb = sols["batched_fp64_graph"].cpu().numpy()
+    xm = sols["batched_mixed_graph"].cpu().numpy()
+    C = b.ncomp
+    out["max_profile_diff_batched_vs_sequential"] = max(
+        float(np.abs(xp[:C] - x.cpu().numpy()[:C]).max()) for xp, x in zip(b.split(xb), xs))
+    out["max_profile_diff_mixed_vs_fp64"] = float(np.abs(xb[:C] - xm[:C]).max())
+    out["speedup_batched_fp64_vs_sequential"] = seq / out["batched_fp64_graph"]["seconds"]
+    out["speedup_batched_mixed_vs_sequential"] = seq / out["batched_mixed_graph"]["seconds"]
+    return out
+
+
+def bench_adjoint_2d():
+    """The rough-wall workload: 256 x 256 at dx = 0.05 sigma; Newton to
+    ||F|| < 1e-8, then dGamma/d(eps_w) and dGamma/d(rho_b) from one backward
+    pass of the implicit-function adjoint (transpose Schur GMRES with the
+    device transpose products); dGamma/d(beta mu) by the bulk equation of state."""
+    import numpy as np
+    import torch
+
+    from tramonto_amd.adjoint import (ParametricModel, dbetamu_drho_b, equilibrium_profile,
+                                      excess_adsorption)
+    from tramonto_amd.problem import rough_wall_2d
+
+    model = ParametricModel(lambda th: rough_wall_2d(eps_w=float(th[0]), rho_b=float(th[1])),
+                            newton_kwargs=dict(resid_tol=1e-8, gmres_tol=1e-7))
+    theta0 = np.array([1.5, 0.5])
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = equilibrium_profile(theta, model)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    spec = model.make_spec(theta0)
+    area = spec.nx[1] * spec.dx[1]
+    gam = excess_adsorption(x, spec, theta[1:2], area)
+    gam.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    dmu = dbetamu_drho_b(spec)[0, 0]
+    return {"workload": "2D 256x256 at dx = 0.05 sigma, Rosenfeld FMT + WCA (eps = 1, cut-off 2.5 sigma, kT = 1), "
+                        "rough LJ 9-3 wall eps_w = 1.5 (amplitude 0.5 sigma, correlation length 2 sigma), "
+                        "rho_b = 0.5",
+            "nodes": spec.nnodes, "newton_seconds": t1 - t0, "newton": model.last_newton,
+            "adjoint_seconds": t2 - t1, "adjoint": model.last_adjoint,
+            "gamma": float(gam.detach()), "dgamma_deps_w": float(theta.grad[0]),
+            "dgamma_drho_b": float(theta.grad[1]), "dgamma_dbetamu": float(theta.grad[1]) / dmu}
+
+
+def bench_solute_3d(n=128):
+    """The 3-component White Bear solute workload: mixed-precision and FP64
+    Newton-Krylov to ||F|| < 1e-8."""
+    import torch
+
+    from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
+    from tramonto_amd.problem import solute_3d
+
+    spec = solute_3d(n=n)
+    dp = DeviceProblem(spec)
+    x0 = dp.initial_state()
+    F = dp.zeros()
+    dp.fill_residual(x0, F)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        dp.fill_residual(x0, F)
+    e1.record()
+    torch.cuda.synchronize()
+    fill_ms = e0.elapsed_time(e1) / 3
+    out = {"workload": f"3D {n}^3 at dx = 0.1 sigma, 3-component White Bear FMT + WCA (sigma 1.0/0.9/0.7, "
+                       "cut-off 3 sigma_i), spherical solute radius 4 sigma with LJ 9-3 surface, eta_b = 0.35 "
+                       "split by seeded Dirichlet weights, periodic box",
+           "nodes": spec.nnodes, "unknowns_per_node": spec.nunk, "fill_ms": fill_ms,
+           "fill_gnode_s": spec.nnodes / (fill_ms * 1e-3) / 1e9,
+           "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
+                                     [float(v) for v in dp.fill_residual_timed(x0, 2)]))}
+    sols = {}
+    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+        x = x0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = dp.newton(x, max_iter=50, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)
+        torch.cuda.synchronize()
+        st["wall_s"] = time.perf_counter() - t0
+        out[name] = st
+        sols[name] = x
+    out["max_profile_diff_mixed_vs_fp64"] = float((sols["mixed"] - sols["fp64"])[: spec.ncomp].abs().max())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +220,9 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched slit-pore workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D rough-wall adjoint workload")
+    ap.add_argument("--solute", action="store_true", help="also run the 128^3 three-component solute workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 512 MiB device copy three times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -162,6 +318,14 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(spec, x0.cpu().numpy())
 
+    extra = {}
+    if not args.no_batch:
+        extra["batched_pores"] = bench_batched_pores()
+    if not args.no_adjoint:
+        extra["adjoint_2d"] = bench_adjoint_2d()
+    if args.solute:
+        extra["solute_3d"] = bench_solute_3d()
+
     out = {
         "metric": "Newton time-to-solve & fill Gnodes/s, mixed/batched vs plain FP64, res<1e-8",
         "value": fill_gnodes,
@@ -196,7 +360,8 @@ def main():
         "fill_hbm_frac": (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    out.update(extra)
+    print(json.dumps(out, default=float))
 
 
 if __name__ == "__main__":
