@@ -142,6 +142,13 @@ int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi
 int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *dx_d, double rtol, int restart,
                     int maxit, int precision, int *iters, void *stream);
 
+/* Launch shapes the autotuner chose for the 1D row kernels (1D problems and
+ * batches; no counterpart in the reference): shape[prec][kind][2] = outputs
+ * per lane and lanes per workgroup, ms[prec][kind] = measured time, prec 0
+ * FP64 / 1 FP32, kind 0 rho-bar rows / 1 mean-field rows / 2 Euler-Lagrange
+ * rows.  Zeros when the problem does not use the row kernels. */
+int tmi_row_tuning(void *handle, int *shape, double *ms);
+
 /* --------------------------------------------------------------------------
  * Batched 1D problems.  No counterpart in the reference, which solves one
  * problem per run (src/dft_main.c): npores independent 1D problems ("pores",

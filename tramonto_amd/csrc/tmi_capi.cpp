@@ -186,6 +186,42 @@ void upload_columns(Problem &P, int type, int idx, int comp, DevColumnsT<T> &dc)
     }
 }
 
+// dense row form of a 1D stencil (see DevRow1DT)
+template <typename T>
+void upload_row1d(Problem &P, int type, int idx, int comp, DevRow1DT<T> &dr)
+{
+    const HostStencil &hs = P.hs[type][idx];
+    if (hs.L == 0) return;
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int k = 0; k < hs.L; ++k) {
+        lo = std::min(lo, hs.off[3 * (size_t)k]);
+        hi = std::max(hi, hs.off[3 * (size_t)k]);
+    }
+    const int L = hi - lo + 1;
+    std::vector<T> w((size_t)L + 5 * kRowPad, (T)0), wv(w.size(), (T)0);
+    for (int k = 0; k < hs.L; ++k) {
+        const int o = hs.off[3 * (size_t)k];
+        w[(size_t)kRowPad + (o - lo)] = (T)hs.w[k];
+        wv[(size_t)kRowPad + (o - lo)] = (T)(hs.w[k] * ((double)o * P.p.dx[0] * P.bulk.inv_rad[comp]));
+    }
+    dr.lo0 = lo;
+    dr.L = L;
+    dr.w = dev_alloc<T>(w.size());
+    upload(dr.w, w.data(), w.size());
+    if (type == TMI_DELTA) {
+        dr.wv = dev_alloc<T>(wv.size());
+        upload(dr.wv, wv.data(), wv.size());
+    }
+}
+
+template <typename T>
+void free_row1d(DevRow1DT<T> &dr)
+{
+    (void)hipFree(dr.w);
+    (void)hipFree(dr.wv);
+    dr = DevRow1DT<T>();
+}
+
 template <typename T>
 void free_columns(DevColumnsT<T> &dc)
 {
@@ -384,9 +420,28 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                 }
             P->use_columns = true;
         }
-        // column reads run up to 2*kColB planes past the last field's halo (times zero weights)
-        const size_t slack_u = (size_t)2 * kColB * P->pad_u.P[0] * P->pad_u.P[1];
-        const size_t slack_f = (size_t)2 * kColB * P->pad_f.P[0] * P->pad_f.P[1];
+        // dense row kernels on 1D grids (TMI_STENCIL_KERNEL=gather keeps the gather kernels)
+        if (nd == 1 && !(kern && std::string(kern) == "gather")) {
+            for (int c = 0; c < C; ++c)
+                for (int t : {TMI_DELTA, TMI_THETA}) {
+                    upload_row1d(*P, t, c, c, P->rowd[t][c]);
+                    upload_row1d(*P, t, c, c, P->rowf[t][c]);
+                }
+            if (p.type_attr >= 0)
+                for (int q = 0; q < C * C; ++q) {
+                    upload_row1d(*P, TMI_UATT, q, q % C, P->rowd[TMI_UATT][q]);
+                    upload_row1d(*P, TMI_UATT, q, q % C, P->rowf[TMI_UATT][q]);
+                }
+            P->use_rows1d = true;
+        }
+        // blocked reads run past the last field's halo (times zero weights): up to 2 kColB
+        // planes (3D columns) or rows (2D), or 4 kRowPad positions (1D rows)
+        auto slack = [&](const PadGeom &pg) -> size_t {
+            if (nd == 3) return (size_t)2 * kColB * pg.P[0] * pg.P[1];
+            if (nd == 2) return (size_t)2 * kColB * pg.P[0];
+            return (size_t)4 * kRowPad;
+        };
+        const size_t slack_u = slack(P->pad_u), slack_f = slack(P->pad_f);
         const size_t nu = (size_t)C * P->pad_u.n + slack_u, nf = (size_t)P->nadj * P->pad_f.n + slack_f;
         P->wsd.rho_pad = dev_alloc<double>(nu);
         P->wsd.adj_pad = dev_alloc<double>(nf);
@@ -401,6 +456,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
             P->wsf.rb_part = dev_alloc<float>((size_t)5 * C * N);
         }
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
+        autotune_rows1d(*P);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
     } catch (...) {
         destroy_problem(P);
@@ -417,6 +473,8 @@ void destroy_problem(Problem *P)
             free_stencil(P->dsf[t][q]);
             free_columns(P->cold[t][q]);
             free_columns(P->colf[t][q]);
+            free_row1d(P->rowd[t][q]);
+            free_row1d(P->rowf[t][q]);
         }
     free_batch_work(*P);
     free_geometry_maps(P->pad_u);
@@ -858,6 +916,23 @@ int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi
         if (P->npores > 0)
             return batch_newton_solve(*P, x_d, *opts, *stats, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
         return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
+    });
+}
+
+int tmi_row_tuning(void *handle, int *shape, double *ms)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        for (int prec = 0; prec < 2; ++prec)
+            for (int k = 0; k < 3; ++k) {
+                const RowTune &t = P->tune[prec][k];
+                if (shape) {
+                    shape[(prec * 3 + k) * 2] = P->use_rows1d ? t.B : 0;
+                    shape[(prec * 3 + k) * 2 + 1] = P->use_rows1d ? t.wg : 0;
+                }
+                if (ms) ms[prec * 3 + k] = P->use_rows1d ? t.ms : 0.0;
+            }
+        return 0;
     });
 }
 

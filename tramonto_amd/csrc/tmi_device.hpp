@@ -172,6 +172,19 @@ inline const DevColumnsT<float> &dev_cols<float>(const Problem &P, int type, int
 }
 
 template <typename T>
+inline const DevRow1DT<T> &dev_row(const Problem &P, int type, int idx);
+template <>
+inline const DevRow1DT<double> &dev_row<double>(const Problem &P, int type, int idx)
+{
+    return P.rowd[type][idx];
+}
+template <>
+inline const DevRow1DT<float> &dev_row<float>(const Problem &P, int type, int idx)
+{
+    return P.rowf[type][idx];
+}
+
+template <typename T>
 inline Workspace<T> &workspace(Problem &P);
 template <>
 inline Workspace<double> &workspace<double>(Problem &P)

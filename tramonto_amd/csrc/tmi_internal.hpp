@@ -76,6 +76,25 @@ struct DevColumnsT {
     T *fxy = nullptr;        // [ncol][2] (delta stencils)
 };
 
+// A 1D stencil as dense weights over its offset range [lo0, lo0 + L): kRowPad
+// zeros, the L weights, then 4 kRowPad zeros, so that a kernel blocking B <=
+// kRowPad outputs per lane reads whole windows of 2B-1 weights without
+// bounds checks.  Delta stencils also carry the vector weights w k dx/R.
+constexpr int kRowPad = 16;
+template <typename T>
+struct DevRow1DT {
+    int lo0 = 0, L = 0;
+    T *w = nullptr;    // [kRowPad + L + 4 kRowPad]
+    T *wv = nullptr;   // same layout (delta stencils)
+};
+
+// Launch shape of a 1D row kernel chosen by the autotuner: outputs per lane
+// and lanes per workgroup.
+struct RowTune {
+    int B = 8, wg = 256;
+    double ms = 0.0;   // best measured time
+};
+
 struct Grid {
     int nd;
     int nx[3];
@@ -113,6 +132,11 @@ struct Problem {
     DevColumnsT<double> cold[3][kNC * kNC];
     DevColumnsT<float> colf[3][kNC * kNC];
     bool use_columns = false;
+    // dense row form of the stencils (1D only) and the tuned launch shapes
+    DevRow1DT<double> rowd[3][kNC * kNC];
+    DevRow1DT<float> rowf[3][kNC * kNC];
+    bool use_rows1d = false;
+    RowTune tune[2][3];   // [FP64, FP32][rb, mf, el]
     unsigned char *d_zero = nullptr;     // [C][N] Zero_density_TF
     unsigned char *d_elzero = nullptr;   // [C][N] EL row replaced by x (zero density or V_ext = VEXT_MAX)
     double *d_vext = nullptr;            // [C][N]
@@ -162,6 +186,10 @@ void launch_schur_backsub(Problem &P, const double *x, const double *b, double *
 int batch_newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_stats &st, int *pore_iters,
                        int *pore_conv, int *pore_gmres, double *pore_resid, hipStream_t s);
 void free_batch_work(Problem &P);
+
+// 1D row-kernel autotuner (tmi_kernels.hip): times every launch shape on the
+// problem's own buffers and stores the fastest in P.tune
+void autotune_rows1d(Problem &P);
 
 // vector helpers
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s);

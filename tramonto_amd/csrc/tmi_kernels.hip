@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 #include "tmi_device.hpp"
 #include "tmi_fmt.hpp"
@@ -612,6 +614,281 @@ __global__ __launch_bounds__(64 * S) void k_el_rows_cols(Geo g, FmtColArgs<T> fa
     }
 }
 
+// ---------------------------------------------------------------- 1D row kernels
+// Each lane owns B consecutive padded positions q0 .. q0+B-1 of the span
+// [qlo, qhi) and streams its inputs once through the dense weights of every
+// stencil (DevRow1DT): every loaded value feeds B fused multiply-adds against
+// weights in scalar registers.  Outputs go to node q - h (one problem) or to
+// pad_node[q] (a batch; gap positions are skipped).
+template <typename T>
+struct RowPtrs {
+    const T *w, *wv;
+    int lo0, L;
+};
+
+template <typename T>
+struct RowArgs {
+    RowPtrs<T> th[kNC], de[kNC], u[kNC * kNC];
+};
+
+template <typename T>
+inline RowPtrs<T> row_ptrs(const DevRow1DT<T> &d)
+{
+    return RowPtrs<T>{d.w, d.wv, d.lo0, d.L};
+}
+
+__device__ inline int row_node(const Geo &g, int q) { return g.pad_node ? g.pad_node[q] : q - g.h[0]; }
+
+// acc[b] +=/-= sum_j w_j s[b + lo0 + j] over the stencil's dense range
+template <typename T, int B, bool NEG = false>
+__device__ __forceinline__ void row_sum(const T *__restrict__ s, const T *__restrict__ wbase, int L, T (&acc)[B])
+{
+    const T *w = wbase + (kRowPad - (B - 1));
+    const int nk = (L + 2 * B - 2) / B;
+    for (int k = 0; k < nk; ++k) {
+        T v[B], wk[2 * B - 1];
+#pragma unroll
+        for (int i = 0; i < B; ++i) v[i] = s[k * B + i];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (NEG)
+                    acc[b] -= wk[i - b + B - 1] * v[i];
+                else
+                    acc[b] += wk[i - b + B - 1] * v[i];
+            }
+    }
+}
+
+// the same loads against the scalar and vector weights of a delta stencil
+template <typename T, int B>
+__device__ __forceinline__ void row_sum2(const T *__restrict__ s, const T *__restrict__ wbase,
+                                         const T *__restrict__ wvbase, int L, T (&a)[B], T (&av)[B])
+{
+    const T *w = wbase + (kRowPad - (B - 1)), *wv = wvbase + (kRowPad - (B - 1));
+    const int nk = (L + 2 * B - 2) / B;
+    for (int k = 0; k < nk; ++k) {
+        T v[B], wk[2 * B - 1];
+#pragma unroll
+        for (int i = 0; i < B; ++i) v[i] = s[k * B + i];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) a[b] += wk[i - b + B - 1] * v[i];
+#pragma unroll
+        for (int j = 0; j < 2 * B - 1; ++j) wk[j] = wv[k * B + j];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+#pragma unroll
+            for (int b = 0; b < B; ++b) av[b] += wk[i - b + B - 1] * v[i];
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_rb_rows_1d(Geo g, RowArgs<T> ra, const T *__restrict__ rho_pad, CompConsts cc,
+                             const double *__restrict__ in_rb, const double *__restrict__ sub,
+                             double *__restrict__ out_rb, int qlo, int qhi)
+{
+    const int q0 = qlo + (int)(blockIdx.x * blockDim.x + threadIdx.x) * B;
+    if (q0 >= qhi) return;
+    T n3[B], n2[B], n1[B], n0[B], v2[B], v1[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) n3[b] = n2[b] = n1[b] = n0[b] = v2[b] = v1[b] = 0;
+    for (int c = 0; c < g.C; ++c) {
+        const T *src = rho_pad + (size_t)c * g.np + q0;
+        T st[B], sd[B], sv[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) st[b] = sd[b] = sv[b] = 0;
+        const RowPtrs<T> th = ra.th[c], de = ra.de[c];
+        row_sum<T, B>(src + th.lo0, th.w, th.L, st);
+        row_sum2<T, B>(src + de.lo0, de.w, de.wv, de.L, sd, sv);
+        const T i4r = (T)cc.i4pir[c], i4r2 = (T)cc.i4pirsq[c];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            n3[b] += st[b];
+            n2[b] += sd[b];
+            n1[b] += i4r * sd[b];
+            n0[b] += i4r2 * sd[b];
+            v2[b] += sv[b];
+            v1[b] += i4r * sv[b];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int q = q0 + b;
+        if (q >= qhi) break;
+        const int i = row_node(g, q);
+        if (i < 0) continue;
+        const double vals[6] = {n3[b], n2[b], n1[b], n0[b], v2[b], v1[b]};
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double o = vals[r];
+            if (in_rb) o -= in_rb[(size_t)r * g.N + i];
+            if (sub) o -= sub[(size_t)r * g.N + i];
+            out_rb[(size_t)r * g.N + i] = o;
+        }
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_mf_rows_1d(Geo g, RowArgs<T> ra, const T *__restrict__ rho_pad, const double *__restrict__ in_mf,
+                             const double *__restrict__ sub, double *__restrict__ out_mf, int qlo, int qhi)
+{
+    const int q0 = qlo + (int)(blockIdx.x * blockDim.x + threadIdx.x) * B;
+    const int c = blockIdx.y;
+    if (q0 >= qhi) return;
+    T acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    for (int cj = 0; cj < g.C; ++cj) {
+        const RowPtrs<T> u = ra.u[c + g.C * cj];
+        row_sum<T, B>(rho_pad + (size_t)cj * g.np + q0 + u.lo0, u.w, u.L, acc);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int q = q0 + b;
+        if (q >= qhi) break;
+        const int i = row_node(g, q);
+        if (i < 0) continue;
+        const size_t row = (size_t)c * g.N + i;
+        double o = acc[b];
+        if (in_mf) o -= in_mf[row];
+        if (sub) o -= sub[row];
+        out_mf[row] = o;
+    }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(256) void k_el_rows_1d(Geo g, RowArgs<T> ra, const T *__restrict__ adj, CompConsts cc, int linear, int nwall,
+                             const double *__restrict__ x_rho, const double *__restrict__ in_rho,
+                             const double *__restrict__ in_mf, const double *__restrict__ vext,
+                             const unsigned char *__restrict__ elzero, const double *__restrict__ add,
+                             double *__restrict__ out_el, int qlo, int qhi)
+{
+    const int q0 = qlo + (int)(blockIdx.x * blockDim.x + threadIdx.x) * B;
+    const int c = blockIdx.y;
+    if (q0 >= qhi) return;
+    T acc[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) acc[b] = 0;
+    const RowPtrs<T> th = ra.th[c], de = ra.de[c];
+    const T *S3 = adj + (size_t)(g.C * 2) * g.np + q0;
+    const T *A = adj + (size_t)(c * 2) * g.np + q0;
+    row_sum<T, B>(S3 + th.lo0, th.w, th.L, acc);
+    row_sum<T, B>(A + de.lo0, de.w, de.L, acc);
+    row_sum<T, B, true>(A + g.np + de.lo0, de.wv, de.L, acc);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int q = q0 + b;
+        if (q >= qhi) break;
+        const int i = row_node(g, q);
+        if (i < 0) continue;
+        const size_t row = (size_t)c * g.N + i;
+        double r;
+        if (elzero[row]) {
+            r = in_rho ? in_rho[row] : 0.0;
+        } else {
+            r = (double)acc[b];
+            const double xv = x_rho[row];
+            if (!linear) {
+                r += log(xv) + (g.mu ? g.mu[row] : cc.mu[c]);
+                if (nwall > 0) r += vext[row];
+                if (in_mf) r += in_mf[row];
+            } else {
+                if (in_rho) r += in_rho[row] / xv;
+                if (in_mf) r += in_mf[row];
+            }
+        }
+        if (add) r += add[row];
+        out_el[row] = r;
+    }
+}
+
+template <typename T>
+RowArgs<T> row_args(const Problem &P)
+{
+    RowArgs<T> ra{};
+    for (int c = 0; c < P.p.ncomp; ++c) {
+        ra.th[c] = row_ptrs(dev_row<T>(P, TMI_THETA, c));
+        ra.de[c] = row_ptrs(dev_row<T>(P, TMI_DELTA, c));
+    }
+    if (P.nmf)
+        for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ra.u[q] = row_ptrs(dev_row<T>(P, TMI_UATT, q));
+    return ra;
+}
+
+// kernel kinds of the 1D row autotuner
+enum RowKind { kRowRb = 0, kRowMf = 1, kRowEl = 2 };
+
+template <typename T>
+inline const RowTune &row_tune(const Problem &P, int kind)
+{
+    return P.tune[sizeof(T) == 4 ? 1 : 0][kind];
+}
+
+inline dim3 row_grid(const PadGeom &pg, int B, int wg, int ny)
+{
+    const long long span = (long long)pg.P[0] - 2LL * pg.h[0];
+    const long long lanes = (span + B - 1) / B;
+    return dim3((unsigned)((lanes + wg - 1) / wg), (unsigned)ny);
+}
+
+#define TMI_ROW_DISPATCH(B_, LAUNCH) \
+    switch (B_) {                    \
+    case 4: LAUNCH(4); break;        \
+    case 16: LAUNCH(16); break;      \
+    default: LAUNCH(8); break;       \
+    }
+
+template <typename T>
+void rb_rows_1d(Problem &P, const RowTune &t, const double *in_rb, const double *sub, double *out_rb, hipStream_t s)
+{
+    Geo gu = make_geo(P, P.pad_u);
+    const RowArgs<T> ra = row_args<T>(P);
+    const CompConsts cc = make_consts(P);
+    const T *rp = workspace<T>(P).rho_pad;
+    const int qlo = P.pad_u.h[0], qhi = P.pad_u.P[0] - P.pad_u.h[0];
+#define L_(B) hipLaunchKernelGGL((k_rb_rows_1d<T, B>), row_grid(P.pad_u, B, t.wg, 1), dim3(t.wg), 0, s, gu, ra, rp, cc, \
+                                 in_rb, sub, out_rb, qlo, qhi)
+    TMI_ROW_DISPATCH(t.B, L_)
+#undef L_
+}
+
+template <typename T>
+void mf_rows_1d(Problem &P, const RowTune &t, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
+{
+    Geo gu = make_geo(P, P.pad_u);
+    const RowArgs<T> ra = row_args<T>(P);
+    const T *rp = workspace<T>(P).rho_pad;
+    const int qlo = P.pad_u.h[0], qhi = P.pad_u.P[0] - P.pad_u.h[0];
+#define L_(B) hipLaunchKernelGGL((k_mf_rows_1d<T, B>), row_grid(P.pad_u, B, t.wg, P.p.ncomp), dim3(t.wg), 0, s, gu, \
+                                 ra, rp, in_mf, sub, out_mf, qlo, qhi)
+    TMI_ROW_DISPATCH(t.B, L_)
+#undef L_
+}
+
+template <typename T>
+void el_rows_1d(Problem &P, const RowTune &t, bool linear, const double *x_rho, const double *in_rho,
+                const double *in_mf, const double *add, double *out_el, hipStream_t s)
+{
+    Geo gf = make_geo(P, P.pad_f);
+    const RowArgs<T> ra = row_args<T>(P);
+    const CompConsts cc = make_consts(P);
+    const T *adj = workspace<T>(P).adj_pad;
+    const int qlo = P.pad_f.h[0], qhi = P.pad_f.P[0] - P.pad_f.h[0];
+    const int lin = linear ? 1 : 0;
+#define L_(B) hipLaunchKernelGGL((k_el_rows_1d<T, B>), row_grid(P.pad_f, B, t.wg, P.p.ncomp), dim3(t.wg), 0, s, gf, \
+                                 ra, adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el, \
+                                 qlo, qhi)
+    TMI_ROW_DISPATCH(t.B, L_)
+#undef L_
+}
+
 struct ElArgs {
     StenPtrs th[kNC], de[kNC];
 };
@@ -839,6 +1116,10 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
     }
     const CompConsts cc = make_consts(P);
     const T *rp = workspace<T>(P).rho_pad;
+    if (P.use_rows1d) {
+        rb_rows_1d<T>(P, row_tune<T>(P, kRowRb), in_rb, sub, out_rb, s);
+        return;
+    }
     if (P.use_columns) {
         FmtColArgs<T> fa{};
         for (int c = 0; c < P.p.ncomp; ++c) {
@@ -864,6 +1145,10 @@ template <typename T>
 void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
 {
     if (P.nmf == 0) return;
+    if (P.use_rows1d) {
+        mf_rows_1d<T>(P, row_tune<T>(P, kRowMf), in_mf, sub, out_mf, s);
+        return;
+    }
     Geo gu = make_geo(P, P.pad_u);
     if (P.use_columns) {
         ColArgs<T> ca{};
@@ -916,6 +1201,10 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
     const CompConsts cc = make_consts(P);
     const T *adj = workspace<T>(P).adj_pad;
     const int lin = linear ? 1 : 0;
+    if (P.use_rows1d) {
+        el_rows_1d<T>(P, row_tune<T>(P, kRowEl), linear, x_rho, in_rho, in_mf, add, out_el, s);
+        return;
+    }
     if (P.use_columns) {
         FmtColArgs<T> fa{};
         for (int c = 0; c < P.p.ncomp; ++c) {
@@ -945,6 +1234,75 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- 1D row autotuner
+// Times every (outputs per lane, lanes per workgroup) shape of the three 1D
+// row kernels, in both precisions, on the problem's own padded buffers and
+// keeps the fastest (TMI_AUTOTUNE=0 keeps the defaults: 8 per lane, 256 lanes).
+void autotune_rows1d(Problem &P)
+{
+    if (!P.use_rows1d) return;
+    const char *env = std::getenv("TMI_AUTOTUNE");
+    if (env && std::string(env) == "0") return;
+    const size_t n = (size_t)P.nunk * P.N;
+    double *out = nullptr, *xz = nullptr;
+    TMI_HIP_CHECK(hipMalloc((void **)&out, n * sizeof(double)));
+    hipError_t ea = hipMalloc((void **)&xz, (size_t)P.p.ncomp * P.N * sizeof(double));
+    if (ea != hipSuccess) {
+        (void)hipFree(out);
+        TMI_HIP_CHECK(ea);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        TMI_HIP_CHECK(hipMemset(xz, 0, (size_t)P.p.ncomp * P.N * sizeof(double)));
+        TMI_HIP_CHECK(hipEventCreate(&e0));
+        TMI_HIP_CHECK(hipEventCreate(&e1));
+        const int Bs[3] = {4, 8, 16}, wgs[3] = {64, 128, 256};
+        for (int prec = 0; prec < 2; ++prec)
+            for (int kind = 0; kind < 3; ++kind) {
+                if (kind == kRowMf && P.nmf == 0) continue;
+                RowTune best;
+                best.ms = 1e30;
+                for (int B : Bs)
+                    for (int wg : wgs) {
+                        RowTune t;
+                        t.B = B;
+                        t.wg = wg;
+                        auto launch = [&] {
+                            if (prec == 0) {
+                                if (kind == kRowRb) rb_rows_1d<double>(P, t, nullptr, nullptr, out + (size_t)P.iu_rb * P.N, 0);
+                                if (kind == kRowMf) mf_rows_1d<double>(P, t, nullptr, nullptr, out + (size_t)P.iu_mf * P.N, 0);
+                                if (kind == kRowEl) el_rows_1d<double>(P, t, true, xz, nullptr, nullptr, nullptr, out, 0);
+                            } else {
+                                if (kind == kRowRb) rb_rows_1d<float>(P, t, nullptr, nullptr, out + (size_t)P.iu_rb * P.N, 0);
+                                if (kind == kRowMf) mf_rows_1d<float>(P, t, nullptr, nullptr, out + (size_t)P.iu_mf * P.N, 0);
+                                if (kind == kRowEl) el_rows_1d<float>(P, t, true, xz, nullptr, nullptr, nullptr, out, 0);
+                            }
+                        };
+                        launch();
+                        TMI_HIP_CHECK(hipEventRecord(e0, 0));
+                        for (int r = 0; r < 5; ++r) launch();
+                        TMI_HIP_CHECK(hipEventRecord(e1, 0));
+                        TMI_HIP_CHECK(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        TMI_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                        t.ms = ms / 5.0;
+                        if (t.ms < best.ms) best = t;
+                    }
+                P.tune[prec][kind] = best;
+            }
+    } catch (...) {
+        (void)hipFree(out);
+        (void)hipFree(xz);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipFree(out);
+    (void)hipFree(xz);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
 
 // ---------------------------------------------------------------- public launchers
 // ev (optional): 6 events recorded between the stages pad_rho | rb_rows |

@@ -81,6 +81,18 @@ class DeviceProblem:
         _lib.check(_lib.lib().tmi_bulk_get(self.handle, out.ctypes.data_as(ct.c_void_p)), "tmi_bulk_get")
         return out
 
+    def row_tuning(self) -> Dict:
+        """Autotuned launch shapes of the 1D row kernels: {precision: {kind: (per_lane, lanes, ms)}}."""
+        shape = np.zeros(12, dtype=np.int32)
+        ms = np.zeros(6)
+        _lib.check(_lib.lib().tmi_row_tuning(self.handle, shape.ctypes.data_as(ct.c_void_p),
+                                             ms.ctypes.data_as(ct.c_void_p)), "tmi_row_tuning")
+        out = {}
+        for p, pname in enumerate(("fp64", "fp32")):
+            out[pname] = {k: (int(shape[(p * 3 + i) * 2]), int(shape[(p * 3 + i) * 2 + 1]), float(ms[p * 3 + i]))
+                          for i, k in enumerate(("rb_rows", "mf_rows", "el_rows"))}
+        return out
+
     # ------------------------------------------------------------ device ops
     def zeros(self) -> torch.Tensor:
         return torch.zeros((self.nunk, self.nnodes), dtype=torch.float64, device="cuda")
