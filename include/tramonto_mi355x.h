@@ -76,6 +76,7 @@ typedef struct tmi_info {
     int nunk, nnodes;
     int iu_rho, iu_rb, n_rb, iu_mf, n_mf;
     int sten_len[3][TMI_NCOMP_MAX * TMI_NCOMP_MAX];
+    int mf_fft;   /* 1: the mean-field sums run as FFT convolutions (wide 2D/3D attraction stencils) */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */

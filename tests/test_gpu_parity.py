@@ -219,3 +219,59 @@ def test_newton_mixed_precision_converges(precision, device_problem):
     assert st["converged"], st
     xo, _ = o.newton(x0, resid_tol=1e-10)
     assert np.abs(xg.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+
+
+FFT_CASES = ["2d_periodic", "2d_wall_reflect", "2d_rough_wall", "3d_periodic_small", "3d_periodic_wb", "3d_mix_bulk",
+             "3d_solute_small"]
+
+
+@pytest.fixture(scope="module")
+def fft_pair():
+    """The same problem with the mean-field sums as FFT convolutions and as direct stencil sums."""
+    import os
+    from tramonto_amd.device import DeviceProblem
+    cache = {}
+
+    def get(case):
+        if case not in cache:
+            spec = small_case(case)
+            old = os.environ.get("TMI_MF_FFT")
+            try:
+                os.environ["TMI_MF_FFT"] = "1"
+                dfft = DeviceProblem(spec)
+                os.environ["TMI_MF_FFT"] = "0"
+                ddir = DeviceProblem(spec)
+            finally:
+                if old is None:
+                    os.environ.pop("TMI_MF_FFT", None)
+                else:
+                    os.environ["TMI_MF_FFT"] = old
+            assert dfft.info.mf_fft == 1 and ddir.info.mf_fft == 0
+            cache[case] = (spec, Oracle.from_spec(spec), dfft, ddir)
+        return cache[case]
+    return get
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", FFT_CASES)
+def test_fft_mean_field_residual_matches_oracle(case, fft_pair):
+    spec, o, dfft, ddir = fft_pair(case)
+    x = generic_state(spec, o, 31)
+    Fo = o.residual(x)
+    Fg = dfft.fill_residual(dev(x))[0].cpu().numpy()
+    assert np.abs(Fg - Fo).max() < RTOL * np.abs(Fo).max()
+    a, b = spec.ncomp + spec.nrb, spec.nunk   # mean-field rows against their own scale
+    assert np.abs(Fg[a:b] - Fo[a:b]).max() < RTOL * max(np.abs(Fo[a:b]).max(), 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", FFT_CASES)
+def test_fft_mean_field_jvp_matches_direct(case, fft_pair):
+    spec, o, dfft, ddir = fft_pair(case)
+    x = dev(generic_state(spec, o, 32))
+    rng = np.random.default_rng(33)
+    v = dev(rng.uniform(-1.0, 1.0, tuple(x.shape)))
+    for fp32, tol in ((False, RTOL), (True, 1e-5)):
+        a = dfft.apply_jacobian(x, v, fp32=fp32)
+        b = ddir.apply_jacobian(x, v, fp32=fp32)
+        assert float((a - b).abs().max() / b.abs().max()) < tol, (fp32,)

@@ -242,6 +242,10 @@ void free_stencil(DevStencilT<T> &ds)
     ds = DevStencilT<T>();
 }
 
+// attraction stencils with at least this many points in all (summed over the
+// component pairs) use the FFT form of the mean-field sums
+constexpr long long kFftMinStencil = 2000;
+
 // Pores of a batched 1D problem: node counts and bulk densities [npores][C].
 struct BatchLayout {
     int npores = 0;
@@ -457,6 +461,15 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         }
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         autotune_rows1d(*P);
+        // FFT form of the mean-field sums on 2D/3D grids when the attraction stencils are wide
+        // (TMI_MF_FFT=1 forces it, TMI_MF_FFT=0 keeps the direct sums)
+        if (nd >= 2 && p.type_attr >= 0) {
+            const char *fe = std::getenv("TMI_MF_FFT");
+            long long lsum = 0;
+            for (int q = 0; q < C * C; ++q) lsum += P->hs[TMI_UATT][q].L;
+            const bool want = fe ? std::string(fe) == "1" : lsum >= kFftMinStencil;
+            if (want) fft_setup(*P);
+        }
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
     } catch (...) {
         destroy_problem(P);
@@ -477,6 +490,7 @@ void destroy_problem(Problem *P)
             free_row1d(P->rowf[t][q]);
         }
     free_batch_work(*P);
+    fft_free(*P);
     free_geometry_maps(P->pad_u);
     free_geometry_maps(P->pad_f);
     (void)hipFree(P->d_pore_off);
@@ -736,6 +750,7 @@ int tmi_problem_info(void *handle, tmi_info *info)
         info->n_mf = P->nmf;
         for (int t = 0; t < 3; ++t)
             for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
+        info->mf_fft = P->fft ? 1 : 0;
         return 0;
     });
 }

@@ -104,6 +104,7 @@ struct Grid {
 
 struct Problem;
 struct BatchWork;
+struct FftPlan;
 
 // kernel-side launch helpers (tmi_kernels.hip)
 template <typename T>
@@ -155,6 +156,7 @@ struct Problem {
     double *d_mu = nullptr;        // [C][N] -ln rho_b - beta mu_hs,ex - beta mu_att of the node's pore
     std::vector<double> pore_rho_b;   // [npores][C]
     BatchWork *batch_work = nullptr;  // batched Newton buffers and graphs (tmi_batch.hip)
+    FftPlan *fft = nullptr;           // FFT form of the mean-field sums (tmi_fft.hip), 2D/3D
     // solver scratch
     double *d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -186,6 +188,12 @@ void launch_schur_backsub(Problem &P, const double *x, const double *b, double *
 int batch_newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_stats &st, int *pore_iters,
                        int *pore_conv, int *pore_gmres, double *pore_resid, hipStream_t s);
 void free_batch_work(Problem &P);
+
+// FFT form of the mean-field sums (tmi_fft.hip)
+void fft_setup(Problem &P);
+void fft_free(Problem &P);
+template <typename T>
+void mf_rows_fft(Problem &P, const T *rho_pad, const double *in_mf, const double *sub, double *out_mf, hipStream_t s);
 
 // 1D row-kernel autotuner (tmi_kernels.hip): times every launch shape on the
 // problem's own buffers and stores the fastest in P.tune

@@ -1145,6 +1145,10 @@ template <typename T>
 void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
 {
     if (P.nmf == 0) return;
+    if (P.fft) {
+        mf_rows_fft<T>(P, workspace<T>(P).rho_pad, in_mf, sub, out_mf, s);
+        return;
+    }
     if (P.use_rows1d) {
         mf_rows_1d<T>(P, row_tune<T>(P, kRowMf), in_mf, sub, out_mf, s);
         return;
