@@ -411,7 +411,12 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         // column kernels on 3D grids (TMI_STENCIL_KERNEL=gather keeps the per-point gather kernels
         // for comparison)
         const char *kern = std::getenv("TMI_STENCIL_KERNEL");
-        if (nd == 3 && !(kern && std::string(kern) == "gather")) {
+        // the column kernels address a column with 32-bit byte offsets: (2h + 1 + 2 kColB) planes
+        // of the padded grid must span less than 4 GiB
+        auto columns_fit = [&](const PadGeom &pg) {
+            return (double)(2 * pg.h[2] + 1 + 2 * kColB) * pg.P[0] * pg.P[1] * sizeof(double) < 4294967296.0;
+        };
+        if (nd == 3 && !(kern && std::string(kern) == "gather") && columns_fit(P->pad_u) && columns_fit(P->pad_f)) {
             for (int c = 0; c < C; ++c)
                 for (int t : {TMI_DELTA, TMI_THETA}) {
                     upload_columns(*P, t, c, c, P->cold[t][c]);

@@ -271,15 +271,45 @@ struct FmtColArgs {
     ColPtrs<T> th[kNC], de[kNC];
 };
 
-// acc[b] += sum_z' w(z' - z_b) s(z')
-template <typename T, int B>
-__device__ __forceinline__ void col_sum(const T *__restrict__ s, const T *__restrict__ w, int nk, long long PXY,
-                                        T (&acc)[B])
+// Raw buffer loads for the column kernels: the descriptor holds the
+// wave-uniform column start, the per-lane byte offset stays fixed in a vector
+// register and the z step goes into the scalar offset, so a load costs no
+// vector address arithmetic.  Descriptor word 3 is the gfx9-family raw
+// 32-bit format (0x00020000); the range is the full 32-bit offset space.
+typedef int tmi_v2i __attribute__((ext_vector_type(2)));
+
+__device__ inline __amdgpu_buffer_rsrc_t col_rsrc(const void *p)
 {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)0xffffffff, 0x00020000);
+}
+
+template <typename T>
+__device__ inline T rsrc_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ inline double rsrc_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    const tmi_v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(double, v);
+}
+template <>
+__device__ inline float rsrc_load<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    const int v = __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(float, v);
+}
+
+// acc[b] += sum_z' w(z' - z_b) s(z').  s is wave-uniform and lane the lane's
+// offset from it (see col_rsrc).
+template <typename T, int B>
+__device__ __forceinline__ void col_sum(const T *__restrict__ s, unsigned lane, const T *__restrict__ w, int nk,
+                                        long long PXY, T (&acc)[B])
+{
+    const __amdgpu_buffer_rsrc_t r = col_rsrc(s);
+    const unsigned voff = lane * (unsigned)sizeof(T), pstride = (unsigned)(PXY * (long long)sizeof(T));
     for (int k = 0; k < nk; ++k) {
         T v[B], wk[2 * B - 1];
 #pragma unroll
-        for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+        for (int i = 0; i < B; ++i) v[i] = rsrc_load<T>(r, voff, (unsigned)(k * B + i) * pstride);
 #pragma unroll
         for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
 #pragma unroll
@@ -291,13 +321,15 @@ __device__ __forceinline__ void col_sum(const T *__restrict__ s, const T *__rest
 
 // the same loads against two weight planes: a += w s, az += wz s
 template <typename T, int B>
-__device__ __forceinline__ void col_sum2(const T *__restrict__ s, const T *__restrict__ w, const T *__restrict__ wz,
-                                         int nk, long long PXY, T (&a)[B], T (&az)[B])
+__device__ __forceinline__ void col_sum2(const T *__restrict__ s, unsigned lane, const T *__restrict__ w,
+                                         const T *__restrict__ wz, int nk, long long PXY, T (&a)[B], T (&az)[B])
 {
+    const __amdgpu_buffer_rsrc_t r = col_rsrc(s);
+    const unsigned voff = lane * (unsigned)sizeof(T), pstride = (unsigned)(PXY * (long long)sizeof(T));
     for (int k = 0; k < nk; ++k) {
         T v[B], wk[2 * B - 1];
 #pragma unroll
-        for (int i = 0; i < B; ++i) v[i] = s[(long long)(k * B + i) * PXY];
+        for (int i = 0; i < B; ++i) v[i] = rsrc_load<T>(r, voff, (unsigned)(k * B + i) * pstride);
 #pragma unroll
         for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
 #pragma unroll
@@ -347,7 +379,7 @@ constexpr int kColGZ = 4;
 struct ColTile {
     int lane, grp, x, y, c, zb;
     bool active;
-    long long PXY, base;
+    long long PXY, ubase;   // ubase: padded position of the tile's lane 0 (wave-uniform)
 };
 
 inline int col_tiles(const Problem &P)
@@ -378,10 +410,11 @@ __device__ __forceinline__ ColTile col_tile(const Geo &g)
     t.y = q2 / gz;
     t.zb = (zg * kColGZ + q2 % gz) * B;
     t.x = xb * 64 + t.lane;
+    // lanes past the last x read inside the padded array (a row is followed by
+    // others or by the slack) and write nothing
     t.active = (t.x < g.nx[0]);
-    const int xs = t.active ? t.x : 0;
     t.PXY = (long long)g.P[0] * g.P[1];
-    t.base = (long long)(xs + g.h[0]) + (long long)(t.y + g.h[1]) * g.P[0] + (long long)(t.zb + g.h[2]) * t.PXY;
+    t.ubase = (long long)(xb * 64 + g.h[0]) + (long long)(t.y + g.h[1]) * g.P[0] + (long long)(t.zb + g.h[2]) * t.PXY;
     return t;
 }
 
@@ -398,9 +431,9 @@ __global__ __launch_bounds__(64 * S) void k_mf_rows_cols(Geo g, ColArgs<T> ca, c
     for (int b = 0; b < B; ++b) acc[b] = 0;
     for (int cj = 0; cj < g.C; ++cj) {
         const ColPtrs<T> cp = ca.u[t.c + g.C * cj];
-        const T *src0 = rho_pad + (size_t)cj * g.np + t.base;
+        const T *src0 = rho_pad + (size_t)cj * g.np + t.ubase;
         for (int col = t.grp; col < cp.ncol; col += S)
-            col_sum<T, B>(src0 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
+            col_sum<T, B>(src0 + cp.lo[col], t.lane, cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
     }
     wave_reduce<T, B, S>(acc, red, t.grp, t.lane);
     if (t.grp > 0 || !t.active) return;
@@ -433,11 +466,11 @@ __global__ __launch_bounds__(64 * S) void k_rb_rows_cols(Geo g, FmtColArgs<T> fa
     T n3[B], s2[B], vx[B], vy[B], vz[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) n3[b] = s2[b] = vx[b] = vy[b] = vz[b] = 0;
-    const T *src0 = rho_pad + (size_t)t.c * g.np + t.base;
+    const T *src0 = rho_pad + (size_t)t.c * g.np + t.ubase;
     {
         const ColPtrs<T> cp = fa.th[t.c];
         for (int col = t.grp; col < cp.ncol; col += S)
-            col_sum<T, B>(src0 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, n3);
+            col_sum<T, B>(src0 + cp.lo[col], t.lane, cp.w + cp.woff[col], cp.nchunk[col], t.PXY, n3);
     }
     {
         const ColPtrs<T> cp = fa.de[t.c];
@@ -446,7 +479,7 @@ __global__ __launch_bounds__(64 * S) void k_rb_rows_cols(Geo g, FmtColArgs<T> fa
 #pragma unroll
             for (int b = 0; b < B; ++b) cs[b] = 0;
             const int wo = cp.woff[col];
-            col_sum2<T, B>(src0 + cp.lo[col], cp.w + wo, cp.wz + wo, cp.nchunk[col], t.PXY, cs, vz);
+            col_sum2<T, B>(src0 + cp.lo[col], t.lane, cp.w + wo, cp.wz + wo, cp.nchunk[col], t.PXY, cs, vz);
             const T fx = cp.fxy[2 * col], fy = cp.fxy[2 * col + 1];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -548,13 +581,13 @@ __global__ __launch_bounds__(64 * S) void k_el_rows_cols(Geo g, FmtColArgs<T> fa
 #pragma unroll
     for (int b = 0; b < B; ++b) acc[b] = 0;
     {
-        const T *S3 = adj + (size_t)(g.C * 4) * g.np + t.base;
+        const T *S3 = adj + (size_t)(g.C * 4) * g.np + t.ubase;
         const ColPtrs<T> cp = fa.th[t.c];
         for (int col = t.grp; col < cp.ncol; col += S)
-            col_sum<T, B>(S3 + cp.lo[col], cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
+            col_sum<T, B>(S3 + cp.lo[col], t.lane, cp.w + cp.woff[col], cp.nchunk[col], t.PXY, acc);
     }
     {
-        const T *A0 = adj + (size_t)(t.c * 4) * g.np + t.base;
+        const T *A0 = adj + (size_t)(t.c * 4) * g.np + t.ubase;
         const size_t np = g.np;
         const ColPtrs<T> cp = fa.de[t.c];
         for (int col = t.grp; col < cp.ncol; col += S) {
@@ -563,13 +596,18 @@ __global__ __launch_bounds__(64 * S) void k_el_rows_cols(Geo g, FmtColArgs<T> fa
             const T *w = cp.w + wo, *wz = cp.wz + wo;
             const T fx = cp.fxy[2 * col], fy = cp.fxy[2 * col + 1];
             const int nk = cp.nchunk[col];
+            const __amdgpu_buffer_rsrc_t rA = col_rsrc(s), rX = col_rsrc(s + np), rY = col_rsrc(s + 2 * np),
+                                         rZ = col_rsrc(s + 3 * np);
+            const unsigned voff = (unsigned)t.lane * (unsigned)sizeof(T);
+            const unsigned pstride = (unsigned)(t.PXY * (long long)sizeof(T));
             for (int k = 0; k < nk; ++k) {
                 T a[B], bz[B], wk[2 * B - 1];
 #pragma unroll
                 for (int i = 0; i < B; ++i) {
-                    const long long o = (long long)(k * B + i) * t.PXY;
-                    a[i] = s[o] - fx * s[np + o] - fy * s[2 * np + o];
-                    bz[i] = s[3 * np + o];
+                    const unsigned so = (unsigned)(k * B + i) * pstride;
+                    a[i] = rsrc_load<T>(rA, voff, so) - fx * rsrc_load<T>(rX, voff, so) -
+                           fy * rsrc_load<T>(rY, voff, so);
+                    bz[i] = rsrc_load<T>(rZ, voff, so);
                 }
 #pragma unroll
                 for (int j = 0; j < 2 * B - 1; ++j) wk[j] = w[k * B + j];
