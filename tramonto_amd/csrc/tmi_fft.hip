@@ -42,9 +42,14 @@ struct FftPlan {
     int lim[3] = {1, 1, 1};   // FFT-grid positions below lim map onto the density copy
     int nst[3] = {0, 0, 0};
     int rad[3][24] = {};
-    long long n = 1;
+    long long n = 1;            // grid points M0 M1 M2
+    long long S1 = 1, S2 = 1;   // storage pitches of y rows and z planes (one element of padding each,
+                                // so that no stride of the strided passes is a power of two)
+    long long ns = 1;           // storage elements per field
     Cx<double> *kd = nullptr;   // [C*C][n] spectra, index icomp + C*jcomp as the stencils
     Cx<float> *kf = nullptr;
+    Cx<double> *twd[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / M_d), j < M_d
+    Cx<float> *twf[3] = {nullptr, nullptr, nullptr};
     void *buf = nullptr;        // 2 C n complex doubles: transforms of the inputs, then of the outputs
 };
 
@@ -54,7 +59,7 @@ using namespace dev;
 
 constexpr int kFftBlock = 256;
 constexpr size_t kFftLds = 48 * 1024;
-constexpr int kFftMaxLen = 1024;   // longest FFT axis (2 lines of complex doubles in kFftLds)
+constexpr int kFftMaxLen = 768;   // longest FFT axis: its twiddles and 2 lines of complex doubles fit kFftLds
 
 struct Radices {
     int r[24];
@@ -85,19 +90,6 @@ template <typename T>
 __device__ inline Cx<T> cis(Cx<T> a, int sign)
 {
     return sign > 0 ? Cx<T>{-a.y, a.x} : Cx<T>{a.y, -a.x};
-}
-
-template <typename T>
-__device__ inline void twiddle(T frac, T &c, T &s);
-template <>
-__device__ inline void twiddle<double>(double frac, double &c, double &s)
-{
-    sincospi(frac, &s, &c);
-}
-template <>
-__device__ inline void twiddle<float>(float frac, float &c, float &s)
-{
-    sincospif(frac, &s, &c);
 }
 
 // X_p = sum_q a_q exp(sign 2 pi i p q / R), in place
@@ -142,49 +134,63 @@ __device__ inline void dft_small(Cx<T> *a, int R, int sign)
 // One axis of a 3D transform, in place.  A workgroup takes lpb lines of the
 // axis (consecutive lines along x for the strided axes, so that the loads
 // are coalesced), runs every Stockham stage in LDS and writes back.
+// q = x / d for 0 <= x < 2^22 through a float reciprocal, corrected to exact
+__device__ inline int small_div(int x, int d, float inv)
+{
+    int q = (int)((float)x * inv);
+    if ((q + 1) * d <= x) ++q;
+    if (q * d > x) --q;
+    return q;
+}
+
 template <typename T>
-__global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, int M0, int M1, int M2, int axis,
-                                                        int nst, Radices rad, int sign, int lpb)
+__global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, int M0, int M1, int M2, long long S1,
+                                                        long long S2, int axis, int nst, Radices rad, int sign, int lpb,
+                                                        const Cx<T> *__restrict__ twtab)
 {
     extern __shared__ char smem_raw[];
-    Cx<T> *bufA = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = axis == 0 ? M0 : (axis == 1 ? M1 : M2);
     const int ld = M + 1;
+    Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);   // [M] twiddles exp(-2 pi i j / M)
+    Cx<T> *bufA = tw + M;
     Cx<T> *bufB = bufA + (size_t)lpb * ld;
+    for (int j = threadIdx.x; j < M; j += kFftBlock) tw[j] = twtab[j];
     // line geometry
-    long long base0;
-    long long step;   // between elements of a line
-    long long lstep;  // between consecutive lines of the group
-    int nl;           // lines in this group
+    long long base0 = 0;
+    long long step = 1;   // between elements of a line (strided axes)
+    int nl;               // lines in this group
+    int l0 = 0;           // first line of the group (axis 0: line = y + M1 z)
     if (axis == 0) {
-        const long long nlines = (long long)M1 * M2;
-        const long long l0 = (long long)blockIdx.x * lpb;
-        nl = (int)((nlines - l0) < lpb ? (nlines - l0) : lpb);
-        base0 = l0 * M0;
-        step = 1;
-        lstep = M0;
+        const int nlines = M1 * M2;
+        l0 = blockIdx.x * lpb;
+        nl = (nlines - l0) < lpb ? (nlines - l0) : lpb;
     } else {
         const int xg = (M0 + lpb - 1) / lpb;
         const int other = blockIdx.x / xg, x0 = (blockIdx.x % xg) * lpb;
         nl = (M0 - x0) < lpb ? (M0 - x0) : lpb;
-        lstep = 1;
         if (axis == 1) {
-            base0 = x0 + (long long)M0 * M1 * other;
-            step = M0;
+            base0 = x0 + S2 * other;
+            step = S1;
         } else {
-            base0 = x0 + (long long)M0 * other;
-            step = (long long)M0 * M1;
+            base0 = x0 + S1 * other;
+            step = S2;
         }
     }
+    const float inv_m = 1.0f / (float)M, inv_nl = 1.0f / (float)nl, inv_m1 = 1.0f / (float)M1;
+    // storage index of element j of line i of an axis-0 group
+    auto row0 = [&](int i) {
+        const int l = l0 + i, z = small_div(l, M1, inv_m1);
+        return (long long)(l - z * M1) * S1 + (long long)z * S2;
+    };
     // load
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
-            const int i = e / M, j = e - i * M;
-            bufA[i * ld + j] = data[base0 + (long long)i * lstep + j];
+            const int i = small_div(e, M, inv_m), j = e - i * M;
+            bufA[i * ld + j] = data[row0(i) + j];
         }
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
-            const int i = e % nl, j = e / nl;
+            const int j = small_div(e, nl, inv_nl), i = e - j * nl;
             bufA[i * ld + j] = data[base0 + i + (long long)j * step];
         }
     }
@@ -193,21 +199,23 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     for (int s = 0; s < nst; ++s) {
         const int R = rad.r[s];
         const int nb = M / R;
+        const int tstep = M / (Ns * R);   // twiddle exp(sign 2 pi i r k / (Ns R)) = tw[r k tstep]
+        const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
         for (int t = threadIdx.x; t < nl * nb; t += kFftBlock) {
-            const int i = t / nb, jj = t - i * nb;
-            const int k = jj % Ns;
+            const int i = small_div(t, nb, inv_nb), jj = t - i * nb;
+            const int q = small_div(jj, Ns, inv_ns), k = jj - q * Ns;
             Cx<T> v[5];
             const Cx<T> *src = bufA + (size_t)i * ld;
             for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
             if (Ns > 1) {
                 for (int r = 1; r < R; ++r) {
-                    T c, sn;
-                    twiddle<T>((T)(sign * 2.0 * (double)(r * k) / (double)(Ns * R)), c, sn);
-                    v[r] = cmul(v[r], Cx<T>{c, sn});
+                    Cx<T> w = tw[r * k * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    v[r] = cmul(v[r], w);
                 }
             }
             dft_small(v, R, sign);
-            const int idxD = (jj / Ns) * Ns * R + k;
+            const int idxD = q * Ns * R + k;
             Cx<T> *dst = bufB + (size_t)i * ld;
             for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
         }
@@ -220,12 +228,12 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     // store
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
-            const int i = e / M, j = e - i * M;
-            data[base0 + (long long)i * lstep + j] = bufA[i * ld + j];
+            const int i = small_div(e, M, inv_m), j = e - i * M;
+            data[row0(i) + j] = bufA[i * ld + j];
         }
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
-            const int i = e % nl, j = e / nl;
+            const int j = small_div(e, nl, inv_nl), i = e - j * nl;
             data[base0 + i + (long long)j * step] = bufA[i * ld + j];
         }
     }
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
 
 struct FftGeo {
     int M[3], src[3], lim[3], off[3];
-    long long n;
+    long long n, S1, S2, ns;
 };
 
 // inputs: R_c'(m) = density copy at the FFT-grid position (0 beyond the copy)
@@ -249,7 +257,8 @@ __global__ void k_fft_gather(FftGeo f, Geo g, const T *__restrict__ rho_pad, Cx<
     const long long p = in ? (long long)(m0 + f.src[0]) + (long long)(m1 + f.src[1]) * g.P[0] +
                                  (long long)(m2 + f.src[2]) * g.P[0] * g.P[1]
                            : 0;
-    for (int c = 0; c < g.C; ++c) R[(size_t)c * f.n + m] = Cx<T>{in ? rho_pad[(size_t)c * g.np + p] : (T)0, (T)0};
+    const long long q = m0 + m1 * f.S1 + m2 * f.S2;
+    for (int c = 0; c < g.C; ++c) R[(size_t)c * f.ns + q] = Cx<T>{in ? rho_pad[(size_t)c * g.np + p] : (T)0, (T)0};
 }
 
 // O_c = sum_c' K[c + C c'] R_c'
@@ -277,10 +286,9 @@ __global__ void k_fft_extract(FftGeo f, Geo g, const Cx<T> *__restrict__ O, cons
     const int x = i % g.nx[0];
     const int r = i / g.nx[0];
     const int y = r % g.nx[1], z = r / g.nx[1];
-    const long long m = (long long)(x + f.off[0]) + (long long)(y + f.off[1]) * f.M[0] +
-                        (long long)(z + f.off[2]) * f.M[0] * f.M[1];
+    const long long m = (long long)(x + f.off[0]) + (long long)(y + f.off[1]) * f.S1 + (long long)(z + f.off[2]) * f.S2;
     const size_t row = (size_t)c * g.N + i;
-    double v = (double)O[(size_t)c * f.n + m].x;
+    double v = (double)O[(size_t)c * f.ns + m].x;
     if (in_mf) v -= in_mf[row];
     if (sub) v -= sub[row];
     out_mf[row] = v;
@@ -333,22 +341,46 @@ void factor_radices(int M, int *rad, int &nst)
 }
 
 template <typename T>
+inline const Cx<T> *twiddles(const FftPlan &F, int axis);
+template <>
+inline const Cx<double> *twiddles<double>(const FftPlan &F, int axis)
+{
+    return F.twd[axis];
+}
+template <>
+inline const Cx<float> *twiddles<float>(const FftPlan &F, int axis)
+{
+    return F.twf[axis];
+}
+
+// Lines per workgroup: as many as the LDS budget allows, but few enough that
+// the pass has about kFftGroups workgroups (the whole chip busy).
+constexpr long long kFftGroups = 1024;
+
+template <typename T>
 void fft3d(const FftPlan &F, Cx<T> *data, int sign, hipStream_t s)
 {
     for (int axis = 0; axis < 3; ++axis) {
         const int M = F.M[axis];
         if (M == 1) continue;
-        const int lpb = (int)std::max<size_t>(1, std::min<size_t>(64, kFftLds / (2 * sizeof(Cx<T>) * (M + 1))));
+        const long long lines = F.n / M;
+        const long long lds_max = (long long)((kFftLds / sizeof(Cx<T>) - M) / (2 * (M + 1)));
+        long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines / kFftGroups));
+        lpb = std::min<long long>(lpb, 64);
+        if (axis > 0) {   // strided axes: whole 128-byte lines of consecutive x when the budget allows
+            const long long line_elems = 128 / (long long)sizeof(Cx<T>);
+            if (lds_max >= line_elems) lpb = std::max(line_elems, (lpb / line_elems) * line_elems);
+        }
         long long groups;
         if (axis == 0)
-            groups = ((long long)F.M[1] * F.M[2] + lpb - 1) / lpb;
+            groups = (lines + lpb - 1) / lpb;
         else
             groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
         Radices rad;
         for (int i = 0; i < 24; ++i) rad.r[i] = F.rad[axis][i];
-        const size_t lds = 2 * sizeof(Cx<T>) * (size_t)lpb * (M + 1);
+        const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
         hipLaunchKernelGGL(k_fft_axis<T>, dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
-                           F.M[2], axis, F.nst[axis], rad, sign, lpb);
+                           F.M[2], F.S1, F.S2, axis, F.nst[axis], rad, sign, (int)lpb, twiddles<T>(F, axis));
     }
 }
 
@@ -362,6 +394,9 @@ FftGeo fft_geo(const FftPlan &F)
         f.off[d] = F.off[d];
     }
     f.n = F.n;
+    f.S1 = F.S1;
+    f.S2 = F.S2;
+    f.ns = F.ns;
     return f;
 }
 
@@ -395,7 +430,24 @@ void fft_setup(Problem &P)
             }
         }
         F->n = (long long)F->M[0] * F->M[1] * F->M[2];
-        const size_t n = (size_t)F->n;
+        F->S1 = F->M[0] + (F->M[1] > 1 ? 1 : 0);
+        F->S2 = F->S1 * (F->M[1] + (F->M[2] > 1 ? 1 : 0));
+        F->ns = F->S2 * F->M[2];
+        const size_t n = (size_t)F->ns;
+        for (int d = 0; d < 3; ++d) {
+            const int M = F->M[d];
+            std::vector<Cx<double>> td(M);
+            std::vector<Cx<float>> tf(M);
+            for (int j = 0; j < M; ++j) {
+                const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)M;
+                td[j] = Cx<double>{(double)cosl(a), (double)sinl(a)};
+                tf[j] = Cx<float>{(float)td[j].x, (float)td[j].y};
+            }
+            TMI_HIP_CHECK(hipMalloc((void **)&F->twd[d], M * sizeof(Cx<double>)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->twf[d], M * sizeof(Cx<float>)));
+            TMI_HIP_CHECK(hipMemcpy(F->twd[d], td.data(), M * sizeof(Cx<double>), hipMemcpyHostToDevice));
+            TMI_HIP_CHECK(hipMemcpy(F->twf[d], tf.data(), M * sizeof(Cx<float>), hipMemcpyHostToDevice));
+        }
         TMI_HIP_CHECK(hipMalloc(&F->buf, 2 * (size_t)C * n * sizeof(Cx<double>)));
         TMI_HIP_CHECK(hipMalloc((void **)&F->kd, (size_t)C * C * n * sizeof(Cx<double>)));
         TMI_HIP_CHECK(hipMalloc((void **)&F->kf, (size_t)C * C * n * sizeof(Cx<float>)));
@@ -404,21 +456,21 @@ void fft_setup(Problem &P)
         for (int q = 0; q < C * C; ++q) {
             std::fill(host.begin(), host.end(), Cx<double>{0.0, 0.0});
             const HostStencil &hs = P.hs[TMI_UATT][q];
+            const long long pitch[3] = {1, F->S1, F->S2};
             for (int k = 0; k < hs.L; ++k) {
-                long long idx = 0, mul = 1;
+                long long idx = 0;
                 for (int d = 0; d < 3; ++d) {
                     const int o = (d < nd) ? hs.off[3 * (size_t)k + d] : 0;
                     int j = (-o) % F->M[d];
                     if (j < 0) j += F->M[d];
-                    idx += (long long)j * mul;
-                    mul *= F->M[d];
+                    idx += (long long)j * pitch[d];
                 }
                 host[(size_t)idx].x += hs.w[k];
             }
             Cx<double> *kq = F->kd + (size_t)q * n;
             TMI_HIP_CHECK(hipMemcpy(kq, host.data(), n * sizeof(Cx<double>), hipMemcpyHostToDevice));
             fft3d<double>(*F, kq, -1, 0);
-            hipLaunchKernelGGL(k_scale_cx, dim3(grid_for((long long)n)), dim3(kBlock), 0, 0, kq, 1.0 / (double)n,
+            hipLaunchKernelGGL(k_scale_cx, dim3(grid_for((long long)n)), dim3(kBlock), 0, 0, kq, 1.0 / (double)F->n,
                                (long long)n);
             hipLaunchKernelGGL(k_cast_spectrum<float>, dim3(grid_for((long long)n)), dim3(kBlock), 0, 0, kq,
                                F->kf + (size_t)q * n, (long long)n);
@@ -429,6 +481,10 @@ void fft_setup(Problem &P)
         (void)hipFree(F->buf);
         (void)hipFree(F->kd);
         (void)hipFree(F->kf);
+        for (int d = 0; d < 3; ++d) {
+            (void)hipFree(F->twd[d]);
+            (void)hipFree(F->twf[d]);
+        }
         delete F;
         throw;
     }
@@ -441,6 +497,10 @@ void fft_free(Problem &P)
     (void)hipFree(P.fft->buf);
     (void)hipFree(P.fft->kd);
     (void)hipFree(P.fft->kf);
+    for (int d = 0; d < 3; ++d) {
+        (void)hipFree(P.fft->twd[d]);
+        (void)hipFree(P.fft->twf[d]);
+    }
     delete P.fft;
     P.fft = nullptr;
 }
@@ -451,7 +511,7 @@ void mf_rows_fft(Problem &P, const T *rho_pad, const double *in_mf, const double
 {
     const FftPlan &F = *P.fft;
     const int C = P.p.ncomp;
-    const size_t n = (size_t)F.n;
+    const size_t n = (size_t)F.ns;
     Cx<T> *R = reinterpret_cast<Cx<T> *>(F.buf);
     Cx<T> *O = R + (size_t)C * n;
     const Cx<T> *K = sizeof(T) == 8 ? reinterpret_cast<const Cx<T> *>(F.kd) : reinterpret_cast<const Cx<T> *>(F.kf);
@@ -459,7 +519,7 @@ void mf_rows_fft(Problem &P, const T *rho_pad, const double *in_mf, const double
     const FftGeo fg = fft_geo(F);
     hipLaunchKernelGGL(k_fft_gather<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, rho_pad, R);
     for (int c = 0; c < C; ++c) fft3d<T>(F, R + (size_t)c * n, -1, s);
-    hipLaunchKernelGGL(k_fft_mul<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, C, F.n, K, R, O);
+    hipLaunchKernelGGL(k_fft_mul<T>, dim3(grid_for(F.ns)), dim3(kBlock), 0, s, C, F.ns, K, R, O);
     for (int c = 0; c < C; ++c) fft3d<T>(F, O + (size_t)c * n, +1, s);
     hipLaunchKernelGGL(k_fft_extract<T>, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, gu, O, in_mf, sub, out_mf);
     TMI_HIP_CHECK(hipGetLastError());
