@@ -27,10 +27,14 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 COPY_PROBE_BYTES = 512 << 20
-# HBM bytes per launch of the attraction-stencil kernel (k_mf_rows_cols<double>) on the 64^3
-# workload: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script with --copy-probe,
-# scaled by the counters' readings on the 512 MiB copy (x2.0 and x1.0): 54.67 + 2.00 MiB.
-MF_TRAFFIC_BYTES_64 = (54.67 + 2.00) * (1 << 20)
+# HBM bytes per launch of the stencil kernels on the 64^3 workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script with --copy-probe --no-batch --no-adjoint, scaled by the counters'
+# readings on the 512 MiB copy (x2.0 and x1.0), in MiB (fetch + write).  Algorithmic bytes per launch:
+# el_rows reads the five adjoint fields with their FMT halo (5 x 74^3 x 8 B = 15.5 MiB) and x, writes
+# one row (2 MiB each); rb_rows reads the density copy (11.3 MiB with the attraction halo, of which
+# the FMT reach touches 3.1 MiB) and writes ten rows (20 MiB).
+STAGE_TRAFFIC_MIB_64 = {"el_rows": 33.98 + 2.00, "rb_rows": 26.52 + 20.00}
+STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
 
 
 def stencil_flops_per_node(dp, spec):
@@ -289,8 +293,12 @@ def main():
 
     # ---------------------------------------------------------------- roofline of the dominant kernel
     flops_node, f_fwd, f_adj, f_mf = stencil_flops_per_node(dp, spec)
-    per_stage_flops = [0.0, f_fwd, f_mf, 0.0, f_adj]
-    dom = int(np.argmax(stages))
+    # the FFT form of the mean-field sums is HBM-bound passes, not direct-form flops: it is not
+    # priced against the FP64 roof (its time is still in fill_stage_ms)
+    mf_fft = bool(dp.info.mf_fft)
+    per_stage_flops = [0.0, f_fwd, 0.0 if mf_fft else f_mf, 0.0, f_adj]
+    priced = [i for i in range(5) if per_stage_flops[i] > 0]
+    dom = max(priced, key=lambda i: stages[i])
     dom_flops = per_stage_flops[dom] * N
     achieved = dom_flops / (stages[dom] * 1e-3) / 1e12 if dom_flops > 0 else 0.0
     C = spec.ncomp
@@ -348,13 +356,16 @@ def main():
         "fill_stage_ms": dict(zip(stage_names, [float(s) for s in stages])),
         "jvp": jvp,
         "newton": newton,
+        "mean_field_form": "fft" if mf_fft else "direct",
         "flops_per_node_direct": flops_node,
         "bytes_per_node_residual": bytes_node,
         "roofline": {"bound": "mfma", "kernel": stage_names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": MF_TRAFFIC_BYTES_64 if (args.n == 64 and stage_names[dom] == "mf_rows") else None,
-                     "traffic_note": "HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
-                                     "512 MiB copy; algorithmic bytes per launch are 6 MiB",
+                     "traffic": (STAGE_TRAFFIC_MIB_64[stage_names[dom]] * (1 << 20)
+                                 if (args.n == 64 and stage_names[dom] in STAGE_TRAFFIC_MIB_64) else None),
+                     "traffic_note": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
+                                      f"512 MiB copy; algorithmic bytes per launch "
+                                      f"{STAGE_ALGO_MIB_64.get(stage_names[dom], float('nan')):.1f} MiB"),
                      "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
                              "peak = 78.6 TFLOP/s)"},
         "fill_hbm_frac": (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9),

@@ -43,8 +43,8 @@ struct FftPlan {
     int nst[3] = {0, 0, 0};
     int rad[3][24] = {};
     long long n = 1;            // grid points M0 M1 M2
-    long long S1 = 1, S2 = 1;   // storage pitches of y rows and z planes (one element of padding each,
-                                // so that no stride of the strided passes is a power of two)
+    long long S1 = 1, S2 = 1;   // storage pitches of y rows and z planes (row_pitch; one row of padding
+                                // per plane), so that no stride of the strided passes is a power of two
     long long ns = 1;           // storage elements per field
     Cx<double> *kd = nullptr;   // [C*C][n] spectra, index icomp + C*jcomp as the stencils
     Cx<float> *kf = nullptr;
@@ -307,6 +307,15 @@ __global__ void k_scale_cx(Cx<double> *__restrict__ a, double s, long long n)
     if (m < n) a[m] = Cx<double>{a[m].x * s, a[m].y * s};
 }
 
+// Row pitch of the FFT storage: a multiple of 16 elements (rows start on 128-byte lines in complex
+// float and complex double), at least one 16-element pad past the row, and not a power of two.
+long long row_pitch(int M0)
+{
+    long long s = ((M0 + 15) / 16) * 16 + 16;
+    if ((s & (s - 1)) == 0) s += 16;
+    return s;
+}
+
 int nice_size(int n)
 {
     for (int m = std::max(n, 1);; ++m) {
@@ -430,7 +439,7 @@ void fft_setup(Problem &P)
             }
         }
         F->n = (long long)F->M[0] * F->M[1] * F->M[2];
-        F->S1 = F->M[0] + (F->M[1] > 1 ? 1 : 0);
+        F->S1 = F->M[1] > 1 ? row_pitch(F->M[0]) : F->M[0];
         F->S2 = F->S1 * (F->M[1] + (F->M[2] > 1 ? 1 : 0));
         F->ns = F->S2 * F->M[2];
         const size_t n = (size_t)F->ns;

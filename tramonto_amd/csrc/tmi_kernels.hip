@@ -1281,9 +1281,12 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
 // Times every (outputs per lane, lanes per workgroup) shape of the three 1D
 // row kernels, in both precisions, on the problem's own padded buffers and
 // keeps the fastest (TMI_AUTOTUNE=0 keeps the defaults: 8 per lane, 256 lanes).
+// Problems under kAutotuneMinNodes keep the defaults: their fills are bound by
+// launch latency, and timing 54 shapes would cost more than it could save.
+constexpr int kAutotuneMinNodes = 16384;
 void autotune_rows1d(Problem &P)
 {
-    if (!P.use_rows1d) return;
+    if (!P.use_rows1d || P.N < kAutotuneMinNodes) return;
     const char *env = std::getenv("TMI_AUTOTUNE");
     if (env && std::string(env) == "0") return;
     const size_t n = (size_t)P.nunk * P.N;
