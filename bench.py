@@ -150,7 +150,7 @@ def bench_adjoint_2d():
     from tramonto_amd.problem import rough_wall_2d
 
     model = ParametricModel(lambda th: rough_wall_2d(eps_w=float(th[0]), rho_b=float(th[1])),
-                            newton_kwargs=dict(resid_tol=1e-8, gmres_tol=1e-7))
+                            newton_kwargs=dict(resid_tol=1e-8, gmres_tol=1e-7), in_place=True)
     theta0 = np.array([1.5, 0.5])
     theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
     torch.cuda.synchronize()

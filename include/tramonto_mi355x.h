@@ -120,6 +120,17 @@ int tmi_bulk_get(void *handle, double *out60);
 int tmi_host_stencil(const tmi_params *p, int isten, int icomp, int jcomp, int *len, int *off, double *w);
 int tmi_host_bulk(const tmi_params *p, double *out60);
 
+/* Parameter updates in place (no counterpart in the reference, which rebuilds
+ * its globals between continuation steps, src/dft_continuation.c; these serve
+ * continuation in a bulk or wall parameter and the adjoint harness).
+ * set_rho_b: new bulk densities rho_b[ncomp]; the chemical potentials and the
+ * bulk boundary values follow, stencils and tables are kept.  set_vext: new
+ * V_ext [nnodes][ncomp] (host, node-major as at creation); Zero_density_TF is
+ * kept and rows with V_ext = VEXT_MAX become fixed rows as at creation.  Both
+ * wait for the device to be idle first. */
+int tmi_problem_set_rho_b(void *handle, const double *rho_b);
+int tmi_problem_set_vext(void *handle, const double *vext);
+
 /* F(x): the residual of every equation (the reference inserts -F into the
  * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
  * fill_resid_and_matrix_control returns. */

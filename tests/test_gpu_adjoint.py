@@ -16,13 +16,13 @@ torch = pytest.importorskip("torch")
 from tramonto_amd.problem import rough_wall_2d, small_case  # noqa: E402
 
 
-def rough_model(nx=40, ny=32, dx=0.1):
+def rough_model(nx=40, ny=32, dx=0.1, in_place=False):
     from tramonto_amd.adjoint import ParametricModel
 
     def make(theta):
         return rough_wall_2d(seed=7, nx=nx, ny=ny, dx=dx, cut=1.5, wall_cut=2.0, eps_w=float(theta[0]),
                              rho_b=float(theta[1]))
-    return ParametricModel(make)
+    return ParametricModel(make, in_place=in_place)
 
 
 @pytest.mark.gpu
@@ -57,9 +57,24 @@ def test_transpose_solve_residual():
 
 
 @pytest.mark.gpu
-def test_adsorption_sensitivities_match_finite_differences():
-    from tramonto_amd.adjoint import dbetamu_drho_b, equilibrium_profile, excess_adsorption
+def test_in_place_parameter_updates_match_new_problems():
+    """set_rho_b / set_vext on a problem give the residual of a problem built with the new values."""
     model = rough_model()
+    dp = model.problem(np.array([1.5, 0.5]))
+    x = dp.initial_state()
+    new = model.make_spec(np.array([1.2, 0.45]))
+    dp.set_rho_b(new.rho_b)
+    dp.set_vext(new.vext)
+    Fa, _ = dp.fill_residual(x)
+    Fb, _ = model.problem(np.array([1.2, 0.45])).fill_residual(x)
+    assert float((Fa - Fb).abs().max()) == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("in_place", [False, True])
+def test_adsorption_sensitivities_match_finite_differences(in_place):
+    from tramonto_amd.adjoint import dbetamu_drho_b, equilibrium_profile, excess_adsorption
+    model = rough_model(in_place=in_place)
     theta0 = np.array([1.5, 0.5])
     theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
     x = equilibrium_profile(theta, model)

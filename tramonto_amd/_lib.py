@@ -71,6 +71,7 @@ DECLARED_SYMBOLS = [
     "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning",
+    "tmi_problem_set_rho_b", "tmi_problem_set_vext",
     "fill_resid_and_matrix_control",
     "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
     "dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids", "dft_hardsphere_lin_prob_mgr_setdensityequationids",
@@ -112,6 +113,8 @@ def lib() -> ct.CDLL:
     L.tmi_schur_solve.argtypes = [vp, vp, vp, vp, d, i, i, i, ct.POINTER(i), vp]
     L.tmi_set_active_problem.argtypes = [vp, vp]
     L.tmi_row_tuning.argtypes = [vp, vp, vp]
+    L.tmi_problem_set_rho_b.argtypes = [vp, vp]
+    L.tmi_problem_set_vext.argtypes = [vp, vp]
     L.tmi_batch_create.argtypes = [ct.POINTER(TmiParams), i, vp, vp, vp, vp, ct.POINTER(vp)]
     L.tmi_batch_info.argtypes = [vp, ct.POINTER(i), vp]
     L.tmi_batch_newton_solve.argtypes = [vp, vp, ct.POINTER(TmiNewtonOpts), ct.POINTER(TmiNewtonStats), vp, vp, vp,

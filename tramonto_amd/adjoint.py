@@ -151,9 +151,13 @@ class ParametricModel:
     """
 
     def __init__(self, make_spec: Callable[[np.ndarray], ProblemSpec], rel_step: float = 1e-6,
-                 newton_kwargs: Optional[dict] = None):
+                 newton_kwargs: Optional[dict] = None, in_place: bool = False):
+        """in_place: theta changes only the bulk densities and V_ext (not the zero-density flags),
+        so dF/dtheta can update the solved problem in place (tmi_problem_set_rho_b / _set_vext)
+        instead of building the problems at theta +- h."""
         self.make_spec = make_spec
         self.rel_step = rel_step
+        self.in_place = in_place
         self.newton_kwargs = dict(max_iter=60, resid_tol=1e-11, gmres_tol=1e-9, precision=PREC_FP64)
         if newton_kwargs:
             self.newton_kwargs.update(newton_kwargs)
@@ -161,16 +165,34 @@ class ParametricModel:
     def problem(self, theta: np.ndarray) -> DeviceProblem:
         return DeviceProblem(self.make_spec(np.asarray(theta, dtype=np.float64)))
 
-    def dF_dtheta(self, theta: np.ndarray, x: torch.Tensor) -> List[torch.Tensor]:
+    def _apply(self, dp: DeviceProblem, theta: np.ndarray) -> None:
+        spec = self.make_spec(np.asarray(theta, dtype=np.float64))
+        if (spec.zero_tf is None) != (dp.spec.zero_tf is None) or (
+                spec.zero_tf is not None and not np.array_equal(spec.zero_tf, dp.spec.zero_tf)):
+            raise ValueError("theta changes the zero-density flags: not an in-place parameter")
+        dp.set_rho_b(spec.rho_b)
+        if spec.nwall > 0:
+            dp.set_vext(spec.vext)
+
+    def dF_dtheta(self, theta: np.ndarray, x: torch.Tensor, dp: Optional[DeviceProblem] = None) -> List[torch.Tensor]:
         out = []
+        inplace = self.in_place and dp is not None
         for k in range(theta.size):
             h = self.rel_step * max(abs(theta[k]), 1.0)
             tp, tm = theta.copy(), theta.copy()
             tp[k] += h
             tm[k] -= h
-            Fp, _ = self.problem(tp).fill_residual(x)
-            Fm, _ = self.problem(tm).fill_residual(x)
+            if inplace:
+                self._apply(dp, tp)
+                Fp, _ = dp.fill_residual(x)
+                self._apply(dp, tm)
+                Fm, _ = dp.fill_residual(x)
+            else:
+                Fp, _ = self.problem(tp).fill_residual(x)
+                Fm, _ = self.problem(tm).fill_residual(x)
             out.append((Fp - Fm) / (2 * h))
+        if inplace:
+            self._apply(dp, theta)
         return out
 
 
@@ -195,7 +217,7 @@ class EquilibriumProfile(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         lam, info = transpose_solve(ctx.dp, x, gx.contiguous())
         ctx.model.last_adjoint = info
-        dF = ctx.model.dF_dtheta(ctx.theta, x)
+        dF = ctx.model.dF_dtheta(ctx.theta, x, ctx.dp)
         grad = torch.tensor([-float((lam * d).sum()) for d in dF], dtype=torch.float64)
         return grad, None, None
 

@@ -910,6 +910,43 @@ int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *
     });
 }
 
+int tmi_problem_set_rho_b(void *handle, const double *rho_b)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!rho_b) throw std::invalid_argument("null rho_b");
+        if (P->npores > 0) throw std::domain_error("a batch holds one set of bulk densities per pore");
+        for (int c = 0; c < P->p.ncomp; ++c)
+            if (!(rho_b[c] > 0.0)) throw std::invalid_argument("rho_b must be positive");
+        TMI_HIP_CHECK(hipDeviceSynchronize());   // launches in flight read the old constants by value only
+        for (int c = 0; c < P->p.ncomp; ++c) P->p.rho_b[c] = rho_b[c];
+        compute_bulk_constants(P->p, P->bulk);
+        if (P->p.type_attr >= 0) compute_attraction_constants(P->p, P->bulk, P->hs);
+        return 0;
+    });
+}
+
+int tmi_problem_set_vext(void *handle, const double *vext)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!vext) throw std::invalid_argument("null vext");
+        if (P->p.nwall <= 0) throw std::invalid_argument("the problem has no external field (nwall = 0)");
+        const size_t N = P->N;
+        const int C = P->p.ncomp;
+        for (size_t i = 0; i < N; ++i)
+            for (int c = 0; c < C; ++c) {
+                const double v = vext[i * C + c];
+                P->h_vext[c * N + i] = v;
+                P->h_elzero[c * N + i] = (P->h_zero[c * N + i] || v == P->p.vext_max) ? 1 : 0;
+            }
+        TMI_HIP_CHECK(hipDeviceSynchronize());
+        upload(P->d_vext, P->h_vext.data(), P->h_vext.size());
+        upload(P->d_elzero, P->h_elzero.data(), P->h_elzero.size());
+        return 0;
+    });
+}
+
 int tmi_init_fields(void *handle, double *x_d, void *stream)
 {
     return guarded([&] {

@@ -12,6 +12,7 @@ library's.  Tensors are float64, shape [nunk, nnodes], on cuda:0.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as ct
 from typing import Dict, Optional, Tuple
 
@@ -37,7 +38,7 @@ def _ptr(t: torch.Tensor) -> ct.c_void_p:
 class DeviceProblem:
     def __init__(self, spec: ProblemSpec):
         L = _lib.lib()
-        self.spec = spec
+        self.spec = copy.copy(spec)   # parameter updates (set_rho_b, set_vext) change this copy only
         self.params = _lib.fill_params(_lib.TmiParams(), spec.as_params())
         C, N = spec.ncomp, spec.nnodes
         vext = None if spec.vext is None else np.ascontiguousarray(spec.vext, dtype=np.float64).reshape(N, C)
@@ -92,6 +93,18 @@ class DeviceProblem:
             out[pname] = {k: (int(shape[(p * 3 + i) * 2]), int(shape[(p * 3 + i) * 2 + 1]), float(ms[p * 3 + i]))
                           for i, k in enumerate(("rb_rows", "mf_rows", "el_rows"))}
         return out
+
+    def set_rho_b(self, rho_b) -> None:
+        """New bulk densities, in place (chemical potentials and bulk boundary values follow)."""
+        a = np.ascontiguousarray(np.asarray(rho_b, dtype=np.float64)[: self.spec.ncomp])
+        _lib.check(_lib.lib().tmi_problem_set_rho_b(self.handle, a.ctypes.data_as(ct.c_void_p)), "tmi_problem_set_rho_b")
+        self.spec.rho_b = [float(v) for v in a]
+
+    def set_vext(self, vext: np.ndarray) -> None:
+        """New external field [nnodes, ncomp], in place (zero-density flags unchanged)."""
+        a = np.ascontiguousarray(np.asarray(vext, dtype=np.float64).reshape(self.nnodes, self.spec.ncomp))
+        _lib.check(_lib.lib().tmi_problem_set_vext(self.handle, a.ctypes.data_as(ct.c_void_p)), "tmi_problem_set_vext")
+        self.spec.vext = a
 
     # ------------------------------------------------------------ device ops
     def zeros(self) -> torch.Tensor:
