@@ -174,6 +174,8 @@ def periodic_3d(seed: int = 2, n: int = 64, dx: float = 0.1, rho_b: float = 0.6,
     f = (1/modes) sum_m cos(k_m . r + phi_m); each k_m = 2 pi n_m / L is a lattice
     vector of the periodic box with wavelength in [2, 6] sigma."""
     L = n * dx
+    if L < 2.0:
+        raise ValueError("periodic_3d: the box (n dx) must be at least 2 sigma for modes of wavelength 2-6 sigma")
     rng = SplitMix64(seed)
     ks = []
     while len(ks) < modes:
