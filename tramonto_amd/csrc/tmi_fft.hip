@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "tmi_device.hpp"
@@ -40,8 +41,7 @@ struct FftPlan {
     int off[3] = {0, 0, 0};   // FFT-grid coordinate of node 0
     int src[3] = {0, 0, 0};   // density-copy coordinate of FFT-grid position 0
     int lim[3] = {1, 1, 1};   // FFT-grid positions below lim map onto the density copy
-    int nst[3] = {0, 0, 0};
-    int rad[3][24] = {};
+    int rc[3][4] = {};          // Stockham stages per axis: counts of radix 4, 2, 3 and 5, run in that order
     long long n = 1;            // grid points M0 M1 M2
     long long S1 = 1, S2 = 1;   // storage pitches of y rows and z planes (row_pitch; one row of padding
                                 // per plane), so that no stride of the strided passes is a power of two
@@ -61,8 +61,9 @@ constexpr int kFftBlock = 256;
 constexpr size_t kFftLds = 48 * 1024;
 constexpr int kFftMaxLen = 768;   // longest FFT axis: its twiddles and 2 lines of complex doubles fit kFftLds
 
-struct Radices {
-    int r[24];
+// Stockham stages per axis in the order they run: radix 4, then 2, then 3, then 5
+struct RadixCounts {
+    int n4, n2, n3, n5;
 };
 
 template <typename T>
@@ -85,44 +86,48 @@ __device__ inline Cx<T> cscale(Cx<T> a, T s)
 {
     return Cx<T>{a.x * s, a.y * s};
 }
-// i * a * sign
-template <typename T>
-__device__ inline Cx<T> cis(Cx<T> a, int sign)
+// i * a * SIGN
+template <int SIGN, typename T>
+__device__ inline Cx<T> cis(Cx<T> a)
 {
-    return sign > 0 ? Cx<T>{-a.y, a.x} : Cx<T>{a.y, -a.x};
+    if constexpr (SIGN > 0)
+        return Cx<T>{-a.y, a.x};
+    else
+        return Cx<T>{a.y, -a.x};
 }
 
-// X_p = sum_q a_q exp(sign 2 pi i p q / R), in place
-template <typename T>
-__device__ inline void dft_small(Cx<T> *a, int R, int sign)
+// X_p = sum_q a_q exp(SIGN 2 pi i p q / R), in place, in registers
+template <int R, int SIGN, typename T>
+__device__ inline void dft_small(Cx<T> *a)
 {
-    if (R == 2) {
+    if constexpr (R == 2) {
         const Cx<T> t = a[0];
         a[0] = cadd(t, a[1]);
         a[1] = csub(t, a[1]);
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
         const Cx<T> t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
-        const Cx<T> t2 = cadd(a[1], a[3]), t3 = cis(csub(a[1], a[3]), sign);
+        const Cx<T> t2 = cadd(a[1], a[3]), t3 = cis<SIGN>(csub(a[1], a[3]));
         a[0] = cadd(t0, t2);
         a[2] = csub(t0, t2);
         a[1] = cadd(t1, t3);
         a[3] = csub(t1, t3);
-    } else if (R == 3) {
+    } else if constexpr (R == 3) {
         const T h = (T)0.86602540378443864676;   // sqrt(3)/2
-        const Cx<T> s = cadd(a[1], a[2]), d = cis(cscale(csub(a[1], a[2]), h), sign);
+        const Cx<T> s = cadd(a[1], a[2]), d = cis<SIGN>(cscale(csub(a[1], a[2]), h));
         const Cx<T> m = csub(a[0], cscale(s, (T)0.5));
         a[0] = cadd(a[0], s);
         a[1] = cadd(m, d);
         a[2] = csub(m, d);
-    } else {   // R == 5
+    } else {
+        static_assert(R == 5, "radix 2, 3, 4 or 5");
         const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
         const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
         const Cx<T> p1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
         const Cx<T> p2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
         const Cx<T> b1 = cadd(a[0], cadd(cscale(p1, c1), cscale(p2, c2)));
         const Cx<T> b2 = cadd(a[0], cadd(cscale(p1, c2), cscale(p2, c1)));
-        const Cx<T> e1 = cis(cadd(cscale(d1, s1), cscale(d2, s2)), sign);
-        const Cx<T> e2 = cis(csub(cscale(d1, s2), cscale(d2, s1)), sign);
+        const Cx<T> e1 = cis<SIGN>(cadd(cscale(d1, s1), cscale(d2, s2)));
+        const Cx<T> e2 = cis<SIGN>(csub(cscale(d1, s2), cscale(d2, s1)));
         a[0] = cadd(a[0], cadd(p1, p2));
         a[1] = cadd(b1, e1);
         a[4] = csub(b1, e1);
@@ -131,9 +136,6 @@ __device__ inline void dft_small(Cx<T> *a, int R, int sign)
     }
 }
 
-// One axis of a 3D transform, in place.  A workgroup takes lpb lines of the
-// axis (consecutive lines along x for the strided axes, so that the loads
-// are coalesced), runs every Stockham stage in LDS and writes back.
 // q = x / d for 0 <= x < 2^22 through a float reciprocal, corrected to exact
 __device__ inline int small_div(int x, int d, float inv)
 {
@@ -143,9 +145,44 @@ __device__ inline int small_div(int x, int d, float inv)
     return q;
 }
 
-template <typename T>
+// One radix-R Stockham stage over nl lines of length M held in LDS (row pitch
+// ld): src -> dst, with Ns the length of the sub-transforms already done.
+// The butterfly is a compile-time radix, so its operands stay in registers.
+template <int R, int SIGN, typename T>
+__device__ inline void stockham_stage(const Cx<T> *__restrict__ src, Cx<T> *__restrict__ dst, int ld, int nl, int M,
+                                      int Ns, const Cx<T> *__restrict__ tw)
+{
+    const int nb = M / R;
+    const int tstep = M / (Ns * R);   // twiddle exp(SIGN 2 pi i r k / (Ns R)) = tw[r k tstep]
+    const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
+    for (int t = threadIdx.x; t < nl * nb; t += kFftBlock) {
+        const int i = small_div(t, nb, inv_nb), jj = t - i * nb;
+        const int q = small_div(jj, Ns, inv_ns), k = jj - q * Ns;
+        const Cx<T> *a = src + (size_t)i * ld + jj;
+        Cx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = a[r * nb];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                Cx<T> w = tw[r * k * tstep];
+                if constexpr (SIGN > 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_small<R, SIGN>(v);
+        Cx<T> *b = dst + (size_t)i * ld + q * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[r * Ns] = v[r];
+    }
+}
+
+// One axis of a 3D transform, in place.  A workgroup takes lpb lines of the
+// axis (consecutive lines along x for the strided axes, so that the loads
+// are coalesced), runs every Stockham stage in LDS and writes back.
+template <typename T, int SIGN>
 __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, int M0, int M1, int M2, long long S1,
-                                                        long long S2, int axis, int nst, Radices rad, int sign, int lpb,
+                                                        long long S2, int axis, RadixCounts rc, int lpb,
                                                         const Cx<T> *__restrict__ twtab)
 {
     extern __shared__ char smem_raw[];
@@ -196,35 +233,21 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     }
     __syncthreads();
     int Ns = 1;
-    for (int s = 0; s < nst; ++s) {
-        const int R = rad.r[s];
-        const int nb = M / R;
-        const int tstep = M / (Ns * R);   // twiddle exp(sign 2 pi i r k / (Ns R)) = tw[r k tstep]
-        const float inv_nb = 1.0f / (float)nb, inv_ns = 1.0f / (float)Ns;
-        for (int t = threadIdx.x; t < nl * nb; t += kFftBlock) {
-            const int i = small_div(t, nb, inv_nb), jj = t - i * nb;
-            const int q = small_div(jj, Ns, inv_ns), k = jj - q * Ns;
-            Cx<T> v[5];
-            const Cx<T> *src = bufA + (size_t)i * ld;
-            for (int r = 0; r < R; ++r) v[r] = src[jj + r * nb];
-            if (Ns > 1) {
-                for (int r = 1; r < R; ++r) {
-                    Cx<T> w = tw[r * k * tstep];
-                    if (sign > 0) w.y = -w.y;
-                    v[r] = cmul(v[r], w);
-                }
-            }
-            dft_small(v, R, sign);
-            const int idxD = q * Ns * R + k;
-            Cx<T> *dst = bufB + (size_t)i * ld;
-            for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
+    auto run = [&](auto radix, int count) {
+        constexpr int R = decltype(radix)::value;
+        for (int s = 0; s < count; ++s) {
+            stockham_stage<R, SIGN>(bufA, bufB, ld, nl, M, Ns, tw);
+            __syncthreads();
+            Cx<T> *tmp = bufA;
+            bufA = bufB;
+            bufB = tmp;
+            Ns *= R;
         }
-        __syncthreads();
-        Cx<T> *tmp = bufA;
-        bufA = bufB;
-        bufB = tmp;
-        Ns *= R;
-    }
+    };
+    run(std::integral_constant<int, 4>{}, rc.n4);
+    run(std::integral_constant<int, 2>{}, rc.n2);
+    run(std::integral_constant<int, 3>{}, rc.n3);
+    run(std::integral_constant<int, 5>{}, rc.n5);
     // store
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
@@ -326,27 +349,19 @@ int nice_size(int n)
     }
 }
 
-void factor_radices(int M, int *rad, int &nst)
+// M = 4^c[0] 2^c[1] 3^c[2] 5^c[3] with c[1] <= 1
+void factor_radices(int M, int *c)
 {
-    nst = 0;
     int r = M;
-    while (r % 4 == 0) {
-        rad[nst++] = 4;
-        r /= 4;
+    const int radix[4] = {4, 2, 3, 5};
+    for (int i = 0; i < 4; ++i) {
+        c[i] = 0;
+        while (r % radix[i] == 0) {
+            ++c[i];
+            r /= radix[i];
+        }
     }
-    while (r % 2 == 0) {
-        rad[nst++] = 2;
-        r /= 2;
-    }
-    while (r % 3 == 0) {
-        rad[nst++] = 3;
-        r /= 3;
-    }
-    while (r % 5 == 0) {
-        rad[nst++] = 5;
-        r /= 5;
-    }
-    if (r != 1 || nst > 24) throw std::logic_error("FFT length must factor into 2, 3 and 5");
+    if (r != 1) throw std::logic_error("FFT length must factor into 2, 3 and 5");
 }
 
 template <typename T>
@@ -385,11 +400,14 @@ void fft3d(const FftPlan &F, Cx<T> *data, int sign, hipStream_t s)
             groups = (lines + lpb - 1) / lpb;
         else
             groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
-        Radices rad;
-        for (int i = 0; i < 24; ++i) rad.r[i] = F.rad[axis][i];
+        const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
         const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
-        hipLaunchKernelGGL(k_fft_axis<T>, dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
-                           F.M[2], F.S1, F.S2, axis, F.nst[axis], rad, sign, (int)lpb, twiddles<T>(F, axis));
+        if (sign < 0)
+            hipLaunchKernelGGL((k_fft_axis<T, -1>), dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
+                               F.M[2], F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+        else
+            hipLaunchKernelGGL((k_fft_axis<T, 1>), dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
+                               F.M[2], F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
     }
 }
 
@@ -432,7 +450,7 @@ void fft_setup(Problem &P)
                 F->src[d] = 0;
                 F->lim[d] = n + 2 * h;
             }
-            factor_radices(F->M[d], F->rad[d], F->nst[d]);
+            factor_radices(F->M[d], F->rc[d]);
             if (F->M[d] > kFftMaxLen) {   // one line must fit the LDS budget: keep the direct sums
                 delete F;
                 return;
