@@ -77,6 +77,7 @@ typedef struct tmi_info {
     int iu_rho, iu_rb, n_rb, iu_mf, n_mf;
     int sten_len[3][TMI_NCOMP_MAX * TMI_NCOMP_MAX];
     int mf_fft;   /* 1: the mean-field sums run as FFT convolutions (wide 2D/3D attraction stencils) */
+    int fmt_fft;  /* 1: the FMT sums (rho-bar and Euler-Lagrange rows) run as FFT convolutions (2D/3D, when faster) */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */

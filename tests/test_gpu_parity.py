@@ -227,51 +227,88 @@ FFT_CASES = ["2d_periodic", "2d_wall_reflect", "2d_rough_wall", "3d_periodic_sma
 
 @pytest.fixture(scope="module")
 def fft_pair():
-    """The same problem with the mean-field sums as FFT convolutions and as direct stencil sums."""
+    """The same problem with every stencil sum as an FFT convolution ("fft"),
+    with only the mean-field sums in FFT form ("mf") and with direct stencil
+    sums only ("direct")."""
     import os
     from tramonto_amd.device import DeviceProblem
     cache = {}
+    forms = {"fft": ("1", "1"), "mf": ("1", "0"), "direct": ("0", "0")}
 
     def get(case):
         if case not in cache:
             spec = small_case(case)
-            old = os.environ.get("TMI_MF_FFT")
+            old = {k: os.environ.get(k) for k in ("TMI_MF_FFT", "TMI_FMT_FFT")}
+            probs = {}
             try:
-                os.environ["TMI_MF_FFT"] = "1"
-                dfft = DeviceProblem(spec)
-                os.environ["TMI_MF_FFT"] = "0"
-                ddir = DeviceProblem(spec)
+                for name, (mf, fmt) in forms.items():
+                    os.environ["TMI_MF_FFT"], os.environ["TMI_FMT_FFT"] = mf, fmt
+                    probs[name] = DeviceProblem(spec)
             finally:
-                if old is None:
-                    os.environ.pop("TMI_MF_FFT", None)
-                else:
-                    os.environ["TMI_MF_FFT"] = old
-            assert dfft.info.mf_fft == 1 and ddir.info.mf_fft == 0
-            cache[case] = (spec, Oracle.from_spec(spec), dfft, ddir)
+                for k, v in old.items():
+                    if v is None:
+                        os.environ.pop(k, None)
+                    else:
+                        os.environ[k] = v
+            for name, (mf, fmt) in forms.items():
+                assert (probs[name].info.mf_fft, probs[name].info.fmt_fft) == (int(mf), int(fmt)), name
+            cache[case] = (spec, Oracle.from_spec(spec), probs)
         return cache[case]
     return get
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fft", "mf"])
 @pytest.mark.parametrize("case", FFT_CASES)
-def test_fft_mean_field_residual_matches_oracle(case, fft_pair):
-    spec, o, dfft, ddir = fft_pair(case)
+def test_fft_residual_matches_oracle(case, form, fft_pair):
+    spec, o, probs = fft_pair(case)
     x = generic_state(spec, o, 31)
     Fo = o.residual(x)
-    Fg = dfft.fill_residual(dev(x))[0].cpu().numpy()
+    Fg = probs[form].fill_residual(dev(x))[0].cpu().numpy()
     assert np.abs(Fg - Fo).max() < RTOL * np.abs(Fo).max()
-    a, b = spec.ncomp + spec.nrb, spec.nunk   # mean-field rows against their own scale
-    assert np.abs(Fg[a:b] - Fo[a:b]).max() < RTOL * max(np.abs(Fo[a:b]).max(), 1.0)
+    # rho-bar and mean-field rows against their own scale
+    for a, b in ((spec.ncomp, spec.ncomp + spec.nrb), (spec.ncomp + spec.nrb, spec.nunk)):
+        if b > a:
+            assert np.abs(Fg[a:b] - Fo[a:b]).max() < RTOL * max(np.abs(Fo[a:b]).max(), 1.0), (a, b)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fft", "mf"])
 @pytest.mark.parametrize("case", FFT_CASES)
-def test_fft_mean_field_jvp_matches_direct(case, fft_pair):
-    spec, o, dfft, ddir = fft_pair(case)
+def test_fft_jvp_matches_direct(case, form, fft_pair):
+    spec, o, probs = fft_pair(case)
     x = dev(generic_state(spec, o, 32))
     rng = np.random.default_rng(33)
     v = dev(rng.uniform(-1.0, 1.0, tuple(x.shape)))
     for fp32, tol in ((False, RTOL), (True, 1e-5)):
-        a = dfft.apply_jacobian(x, v, fp32=fp32)
-        b = ddir.apply_jacobian(x, v, fp32=fp32)
+        a = probs[form].apply_jacobian(x, v, fp32=fp32)
+        b = probs["direct"].apply_jacobian(x, v, fp32=fp32)
         assert float((a - b).abs().max() / b.abs().max()) < tol, (fp32,)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2d_rough_wall", "3d_periodic_wb", "3d_solute_small"])
+def test_fft_schur_solve_matches_direct(case, fft_pair):
+    """One Newton step's Schur-complement solve (rhs, GMRES on the Schur
+    operator, back substitution) with every sum in FFT form."""
+    spec, o, probs = fft_pair(case)
+    x = dev(generic_state(spec, o, 34))
+    dxs = []
+    for form in ("fft", "direct"):
+        F, _ = probs[form].fill_residual(x)
+        dx, its = probs[form].schur_solve(x, F, rtol=1e-12, restart=200, maxit=2000)
+        dxs.append(dx)
+    assert float((dxs[0] - dxs[1]).abs().max() / dxs[1].abs().max()) < 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["3d_periodic_small", "2d_rough_wall"])
+def test_fft_newton_matches_oracle(case, fft_pair):
+    spec, o, probs = fft_pair(case)
+    x0 = o.init_fields(spec.initial_x())
+    xg = probs["fft"].init_fields(dev(spec.initial_x()))
+    assert np.abs(xg.cpu().numpy() - x0).max() < RTOL * np.abs(x0).max()
+    st = probs["fft"].newton(xg, resid_tol=1e-10, gmres_tol=1e-8, max_iter=60)
+    assert st["converged"], st
+    xo, _ = o.newton(x0, resid_tol=1e-10)
+    assert np.abs(xg.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8

@@ -466,16 +466,21 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         }
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         autotune_rows1d(*P);
-        // FFT form of the mean-field sums on 2D/3D grids when the attraction stencils are wide
-        // (TMI_MF_FFT=1 forces it, TMI_MF_FFT=0 keeps the direct sums)
-        if (nd >= 2 && p.type_attr >= 0) {
-            const char *fe = std::getenv("TMI_MF_FFT");
-            long long lsum = 0;
-            for (int q = 0; q < C * C; ++q) lsum += P->hs[TMI_UATT][q].L;
-            const bool want = fe ? std::string(fe) == "1" : lsum >= kFftMinStencil;
-            if (want) fft_setup(*P);
-        }
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
+        // FFT forms of the stencil sums on 2D/3D grids: the mean-field sums when the attraction
+        // stencils are wide (TMI_MF_FFT=1 forces it, 0 keeps the direct sums), the FMT sums when
+        // that form fills faster (TMI_FMT_FFT=1 / 0 forces the FFT / direct form)
+        if (nd >= 2) {
+            const char *fe = std::getenv("TMI_MF_FFT"), *fm = std::getenv("TMI_FMT_FFT");
+            long long lsum = 0;
+            if (p.type_attr >= 0)
+                for (int q = 0; q < C * C; ++q) lsum += P->hs[TMI_UATT][q].L;
+            const bool mf = p.type_attr >= 0 && (fe ? std::string(fe) == "1" : lsum >= kFftMinStencil);
+            const bool fmt = !(fm && std::string(fm) == "0");
+            fft_setup(*P, mf, fmt);
+            choose_fmt_form(*P);
+            if (P->fft && !fft_mf(*P) && !fft_fmt(*P)) fft_free(*P);
+        }
     } catch (...) {
         destroy_problem(P);
         throw;
@@ -755,7 +760,8 @@ int tmi_problem_info(void *handle, tmi_info *info)
         info->n_mf = P->nmf;
         for (int t = 0; t < 3; ++t)
             for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
-        info->mf_fft = P->fft ? 1 : 0;
+        info->mf_fft = fft_mf(*P) ? 1 : 0;
+        info->fmt_fft = fft_fmt(*P) ? 1 : 0;
         return 0;
     });
 }

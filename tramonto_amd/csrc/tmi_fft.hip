@@ -1,21 +1,33 @@
-// The mean-field (attraction) sums of 2D/3D problems as FFT convolutions.
+// The stencil sums of 2D/3D problems as FFT convolutions.
 //
-// The mean-field rows are a correlation of the densities with the attraction
-// stencils, F_MF,c(i) = sum_c' sum_k u_cc'(k) rho_c'(i + k) - x_MF,c(i)
-// (load_mean_field, src/dft_fill_mf.c:38-63, through
-// resid_and_Jac_sten_fill_sum_Ncomp, src/dft_fill_CLSmf.c:38-171).  With tens
-// of thousands of stencil points per node the direct form is bound by FP64
-// FMAs; as a circular convolution on an FFT grid it costs a few HBM-bound
-// passes instead.
+// Every stencil sum of the fill is a correlation of a halo-padded field with a
+// stencil, out(i) = sum_k w(k) in(i + o_k):
+//   rho-bar rows         n_q(i)  = sum_c a_qc sum_k w_c(k) rho_c(i + o_k) over
+//                        the theta, delta and vector-delta stencils
+//                        (load_rho_bar_s/_v, src/dft_fill_FMT.c:195-347);
+//   mean-field rows      MF_c(i) = sum_c' sum_k u_cc'(k) rho_c'(i + o_k)
+//                        (load_mean_field, src/dft_fill_mf.c:38-63, through
+//                        resid_and_Jac_sten_fill_sum_Ncomp, src/dft_fill_CLSmf.c:38-171);
+//   Euler-Lagrange sums  E_c(i)  = sum_k [w_d(k) A_c - wv(k).B_c + w_t(k) S3](i + o_k)
+//                        (load_nonlocal_hs_rosen_rb, src/dft_fill_FMT.c:41-189).
+// With hundreds (FMT) to tens of thousands (attraction) of stencil points per
+// node the direct sums are bound by FP64 FMAs; on an FFT grid each sum is a
+// product of spectra and the fill costs a few HBM-bound passes instead.
 //
 // FFT grid, per dimension: a periodic axis whose node count factors into 2,
 // 3 and 5 is used as is (a circular convolution is the periodic image sum);
-// any other axis uses the halo-padded copy of the density (the halo holds the
+// any other axis uses the halo-padded copy of the field (the halo holds the
 // boundary values offset_to_node_box gives) extended to a 2^a 3^b 5^c length.
 // Interior outputs of a padded axis never wrap, because the halo is the
 // stencil reach.  The transforms are mixed-radix Stockham passes in LDS, one
-// launch per axis; the kernel spectra (flipped stencils, scaled by 1/M) are
-// computed once at set-up.
+// launch per axis for a batch of fields.
+//
+// The flipped stencils are even (theta, delta, attraction) or odd (vector
+// delta), so their spectra are real or imaginary: set-up keeps one real table
+// per stencil, scaled by 1/M, after checking that the other part vanishes.
+// Real fields are transformed two at a time, one as the real and one as the
+// imaginary part (split again in the product kernels from Z(k) and Z(-k)),
+// and pairs of real outputs share one inverse transform the same way.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,21 +48,51 @@ struct alignas(2 * sizeof(T)) Cx {
     T x, y;
 };
 
+constexpr int kMaxOut = 10 + kNC;   // outputs of the density product: 4 + 2*3 rho-bar rows and kNC mean-field rows
+constexpr int kKindMf = 8;          // output kinds: 0 theta, 1 delta, 2 + d vector delta d, kKindMf + c mean-field row c
+
+// Outputs of the product of the density spectra: Y_o = sum_c coef[o][c] K_kind[o],c rho_c
+// (FMT kinds) or Y_o = sum_c' U_c,c' rho_c' (mean-field row c = kind - kKindMf)
+struct FwdDesc {
+    int C = 0, nout = 0;
+    int kind[kMaxOut] = {};
+    double coef[kMaxOut][kNC] = {};
+};
+
+// Rows written from the outputs: rho-bar row q (q < nrb) = scale[q] Y_src[q],
+// then nmf mean-field rows
+struct RowMap {
+    int nrb = 0, nmf = 0;
+    int src[kMaxOut] = {};
+    double scale[kMaxOut] = {};
+};
+
 struct FftPlan {
-    int M[3] = {1, 1, 1};     // FFT grid
-    int off[3] = {0, 0, 0};   // FFT-grid coordinate of node 0
-    int src[3] = {0, 0, 0};   // density-copy coordinate of FFT-grid position 0
-    int lim[3] = {1, 1, 1};   // FFT-grid positions below lim map onto the density copy
+    int nd = 0, C = 0;
+    int M[3] = {1, 1, 1};       // FFT grid
+    int off[3] = {0, 0, 0};     // FFT-grid coordinate of node 0
+    // per geometry (0: density halo pad_u, 1: FMT halo pad_f): FFT-grid positions
+    // lo <= m < lim map onto the padded copy at m + src; the rest are zero
+    int lo[2][3] = {}, lim[2][3] = {{1, 1, 1}, {1, 1, 1}}, src[2][3] = {};
+    bool wraps = true;          // every axis periodic with M = n: halos are not read
     int rc[3][4] = {};          // Stockham stages per axis: counts of radix 4, 2, 3 and 5, run in that order
     long long n = 1;            // grid points M0 M1 M2
     long long S1 = 1, S2 = 1;   // storage pitches of y rows and z planes (row_pitch; one row of padding
                                 // per plane), so that no stride of the strided passes is a power of two
     long long ns = 1;           // storage elements per field
-    Cx<double> *kd = nullptr;   // [C*C][n] spectra, index icomp + C*jcomp as the stencils
-    Cx<float> *kf = nullptr;
+    bool mf = false;            // mean-field sums in FFT form
+    bool fmt_ok = false;        // FMT spectra built
+    bool fmt = false;           // FMT sums (rho-bar, Euler-Lagrange) in FFT form
+    double *ud = nullptr;       // [C*C][ns] attraction spectra (real), index icomp + C*jcomp as the stencils
+    float *uf = nullptr;
+    double *td = nullptr;       // [C][2+nd][ns] theta, delta (real) and vector-delta d (imaginary part) spectra
+    float *tf = nullptr;
     Cx<double> *twd[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / M_d), j < M_d
     Cx<float> *twf[3] = {nullptr, nullptr, nullptr};
-    void *buf = nullptr;        // 2 C n complex doubles: transforms of the inputs, then of the outputs
+    void *zbuf = nullptr;       // transforms of the input pairs
+    void *obuf = nullptr;       // products, then the output pairs
+    FwdDesc fwd[2];             // [0] mean-field rows only, [1] rho-bar rows then mean-field rows
+    RowMap rows[2];
 };
 
 namespace {
@@ -177,15 +219,17 @@ __device__ inline void stockham_stage(const Cx<T> *__restrict__ src, Cx<T> *__re
     }
 }
 
-// One axis of a 3D transform, in place.  A workgroup takes lpb lines of the
-// axis (consecutive lines along x for the strided axes, so that the loads
-// are coalesced), runs every Stockham stage in LDS and writes back.
+// One axis of a 3D transform of a batch of fields, in place.  A workgroup
+// takes lpb lines of the axis (consecutive lines along x for the strided
+// axes, so that the loads are coalesced), runs every Stockham stage in LDS and
+// writes back.
 template <typename T, int SIGN>
-__global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, int M0, int M1, int M2, long long S1,
-                                                        long long S2, int axis, RadixCounts rc, int lpb,
-                                                        const Cx<T> *__restrict__ twtab)
+__global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, long long fstride, int M0, int M1,
+                                                        int M2, long long S1, long long S2, int axis, RadixCounts rc,
+                                                        int lpb, const Cx<T> *__restrict__ twtab)
 {
     extern __shared__ char smem_raw[];
+    data += (long long)blockIdx.y * fstride;   // one field of the batch per grid row
     const int M = axis == 0 ? M0 : (axis == 1 ? M1 : M2);
     const int ld = M + 1;
     Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);   // [M] twiddles exp(-2 pi i j / M)
@@ -263,71 +307,213 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
 }
 
 struct FftGeo {
-    int M[3], src[3], lim[3], off[3];
+    int M[3], off[3];
+    int lo[3], lim[3], src[3];
     long long n, S1, S2, ns;
 };
 
-// inputs: R_c'(m) = density copy at the FFT-grid position (0 beyond the copy)
+// storage index of FFT-grid point m and of its negative -m
+struct KIdx {
+    long long q, qn;
+};
+__device__ inline KIdx k_index(const FftGeo &f, long long m)
+{
+    const int m0 = (int)(m % f.M[0]);
+    const long long r = m / f.M[0];
+    const int m1 = (int)(r % f.M[1]), m2 = (int)(r / f.M[1]);
+    const int n0 = m0 ? f.M[0] - m0 : 0, n1 = m1 ? f.M[1] - m1 : 0, n2 = m2 ? f.M[2] - m2 : 0;
+    return KIdx{m0 + m1 * f.S1 + m2 * f.S2, n0 + n1 * f.S1 + n2 * f.S2};
+}
+
+// spectrum of the real field carried as the real (odd = 0) or imaginary
+// (odd = 1) part of a pair transform, from Z(k) = a and Z(-k) = b
 template <typename T>
-__global__ void k_fft_gather(FftGeo f, Geo g, const T *__restrict__ rho_pad, Cx<T> *__restrict__ R)
+__device__ inline Cx<T> unpack(Cx<T> a, Cx<T> b, int odd)
+{
+    return odd ? Cx<T>{(T)0.5 * (a.y + b.y), (T)0.5 * (b.x - a.x)} : Cx<T>{(T)0.5 * (a.x + b.x), (T)0.5 * (a.y - b.y)};
+}
+
+// Z_j = f_2j + i f_2j+1 at every FFT-grid position (0 off the padded copy)
+template <typename T>
+__global__ void k_gather_pairs(FftGeo f, Geo g, const T *__restrict__ pad, int nfield, Cx<T> *__restrict__ Z)
 {
     const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (m >= f.n) return;
     const int m0 = (int)(m % f.M[0]);
     const long long r = m / f.M[0];
     const int m1 = (int)(r % f.M[1]), m2 = (int)(r / f.M[1]);
-    const bool in = m0 < f.lim[0] && m1 < f.lim[1] && m2 < f.lim[2];
+    const bool in = m0 >= f.lo[0] && m0 < f.lim[0] && m1 >= f.lo[1] && m1 < f.lim[1] && m2 >= f.lo[2] && m2 < f.lim[2];
     const long long p = in ? (long long)(m0 + f.src[0]) + (long long)(m1 + f.src[1]) * g.P[0] +
                                  (long long)(m2 + f.src[2]) * g.P[0] * g.P[1]
                            : 0;
     const long long q = m0 + m1 * f.S1 + m2 * f.S2;
-    for (int c = 0; c < g.C; ++c) R[(size_t)c * f.ns + q] = Cx<T>{in ? rho_pad[(size_t)c * g.np + p] : (T)0, (T)0};
-}
-
-// O_c = sum_c' K[c + C c'] R_c'
-template <typename T>
-__global__ void k_fft_mul(int C, long long n, const Cx<T> *__restrict__ K, const Cx<T> *__restrict__ R,
-                          Cx<T> *__restrict__ O)
-{
-    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (m >= n) return;
-    for (int c = 0; c < C; ++c) {
-        Cx<T> acc{(T)0, (T)0};
-        for (int cj = 0; cj < C; ++cj) acc = cadd(acc, cmul(K[(size_t)(c + C * cj) * n + m], R[(size_t)cj * n + m]));
-        O[(size_t)c * n + m] = acc;
+    for (int j = 0; 2 * j < nfield; ++j) {
+        const T re = in ? pad[(size_t)(2 * j) * g.np + p] : (T)0;
+        const T im = (in && 2 * j + 1 < nfield) ? pad[(size_t)(2 * j + 1) * g.np + p] : (T)0;
+        Z[(size_t)j * f.ns + q] = Cx<T>{re, im};
     }
 }
 
-// out_mf[c][i] = Re O_c(node i) - in_mf - sub
+// Products of the density spectra (pairs Z) with the stencil spectra, the
+// outputs packed in pairs O_j = Y_2j + i Y_2j+1
+template <typename T, int ND>
+__global__ void k_fwd_mul(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z, const T *__restrict__ tk,
+                          const T *__restrict__ uk, Cx<T> *__restrict__ O)
+{
+    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (m >= f.n) return;
+    const KIdx ix = k_index(f, m);
+    Cx<T> X[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        X[c] = Cx<T>{(T)0, (T)0};
+        if (c < d.C) {
+            const Cx<T> *z = Z + (size_t)(c >> 1) * f.ns;
+            X[c] = unpack(z[ix.q], z[ix.qn], c & 1);
+        }
+    }
+    Cx<T> acc{(T)0, (T)0};
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o) {
+        if (o < d.nout) {
+            const int kind = d.kind[o];
+            Cx<T> y{(T)0, (T)0};
+            if (kind < kKindMf) {
+#pragma unroll
+                for (int c = 0; c < kNC; ++c) {
+                    if (c < d.C) {
+                        const T w = (T)d.coef[o][c] * tk[((size_t)c * (2 + ND) + kind) * f.ns + ix.q];
+                        if (kind < 2)
+                            y = cadd(y, cscale(X[c], w));
+                        else   // imaginary spectrum: (i w) X
+                            y = cadd(y, Cx<T>{-w * X[c].y, w * X[c].x});
+                    }
+                }
+            } else {
+                const int row = kind - kKindMf;
+#pragma unroll
+                for (int c = 0; c < kNC; ++c)
+                    if (c < d.C) y = cadd(y, cscale(X[c], uk[(size_t)(row + d.C * c) * f.ns + ix.q]));
+            }
+            if ((o & 1) == 0) {
+                acc = y;
+                if (o + 1 == d.nout) O[(size_t)(o >> 1) * f.ns + ix.q] = acc;
+            } else {
+                O[(size_t)(o >> 1) * f.ns + ix.q] = Cx<T>{acc.x - y.y, acc.y + y.x};
+            }
+        }
+    }
+}
+
+// rho-bar and mean-field rows at the nodes: row = Y - in - sub
 template <typename T>
-__global__ void k_fft_extract(FftGeo f, Geo g, const Cx<T> *__restrict__ O, const double *__restrict__ in_mf,
-                              const double *__restrict__ sub, double *__restrict__ out_mf)
+__global__ void k_fwd_extract(FftGeo f, Geo g, RowMap rm, const Cx<T> *__restrict__ O, const double *__restrict__ in_rb,
+                              const double *__restrict__ sub_rb, double *__restrict__ out_rb,
+                              const double *__restrict__ in_mf, const double *__restrict__ sub_mf,
+                              double *__restrict__ out_mf)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int c = blockIdx.y;
+    const int row = blockIdx.y;
     if (i >= g.N) return;
     const int x = i % g.nx[0];
     const int r = i / g.nx[0];
     const int y = r % g.nx[1], z = r / g.nx[1];
     const long long m = (long long)(x + f.off[0]) + (long long)(y + f.off[1]) * f.S1 + (long long)(z + f.off[2]) * f.S2;
-    const size_t row = (size_t)c * g.N + i;
-    double v = (double)O[(size_t)c * f.ns + m].x;
-    if (in_mf) v -= in_mf[row];
-    if (sub) v -= sub[row];
-    out_mf[row] = v;
+    const int o = rm.src[row];
+    const Cx<T> v = O[(size_t)(o >> 1) * f.ns + m];
+    double val = (double)((o & 1) ? v.y : v.x) * rm.scale[row];
+    if (row < rm.nrb) {
+        const size_t at = (size_t)row * g.N + i;
+        if (in_rb) val -= in_rb[at];
+        if (sub_rb) val -= sub_rb[at];
+        out_rb[at] = val;
+    } else {
+        const size_t at = (size_t)(row - rm.nrb) * g.N + i;
+        if (in_mf) val -= in_mf[at];
+        if (sub_mf) val -= sub_mf[at];
+        out_mf[at] = val;
+    }
 }
 
+// Euler-Lagrange sums of the adjoint fields (pairs Z, fields A_c, B_c,d in
+// the order c (1 + ND) + {0, 1 + d}, then S3):
+// E_c = K_delta,c A_c - sum_d (i K'_c,d) B_c,d + K_theta,c S3, packed in pairs
+template <typename T, int ND>
+__global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *__restrict__ tk, Cx<T> *__restrict__ O)
+{
+    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (m >= f.n) return;
+    const KIdx ix = k_index(f, m);
+    auto field = [&](int j) {
+        const Cx<T> *z = Z + (size_t)(j >> 1) * f.ns;
+        return unpack(z[ix.q], z[ix.qn], j & 1);
+    };
+    const Cx<T> S3 = field(C * (1 + ND));
+    Cx<T> acc{(T)0, (T)0};
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        if (c < C) {
+            const T *t = tk + (size_t)c * (2 + ND) * f.ns + ix.q;
+            Cx<T> e = cadd(cscale(S3, t[0]), cscale(field(c * (1 + ND)), t[f.ns]));
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) {
+                const T kv = t[(size_t)(2 + dd) * f.ns];
+                const Cx<T> B = field(c * (1 + ND) + 1 + dd);
+                e = cadd(e, Cx<T>{kv * B.y, -kv * B.x});   // -(i kv) B
+            }
+            if ((c & 1) == 0) {
+                acc = e;
+                if (c + 1 == C) O[(size_t)(c >> 1) * f.ns + ix.q] = acc;
+            } else {
+                O[(size_t)(c >> 1) * f.ns + ix.q] = Cx<T>{acc.x - e.y, acc.y + e.x};
+            }
+        }
+    }
+}
+
+// Euler-Lagrange rows from the sums E_c, as k_el_rows:
+//  residual: E + ln x + mu [+ V_ext] [+ x_MF]      ; zero rows: x   (in_rho)
+//  linear  : E + v/x + v_MF                        ; zero rows: v
 template <typename T>
-__global__ void k_cast_spectrum(const Cx<double> *__restrict__ a, Cx<T> *__restrict__ b, long long n)
+__global__ void k_el_extract(FftGeo f, Geo g, CompConsts cc, int linear, int nwall, const Cx<T> *__restrict__ O,
+                             const double *__restrict__ x_rho, const double *__restrict__ in_rho,
+                             const double *__restrict__ in_mf, const double *__restrict__ vext,
+                             const unsigned char *__restrict__ elzero, const double *__restrict__ add,
+                             double *__restrict__ out_el)
 {
-    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (m < n) b[m] = Cx<T>{(T)a[m].x, (T)a[m].y};
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= g.N) return;
+    const size_t row = (size_t)c * g.N + i;
+    double r;
+    if (elzero[row]) {
+        r = in_rho ? in_rho[row] : 0.0;
+    } else {
+        const int x = i % g.nx[0];
+        const int rr = i / g.nx[0];
+        const int y = rr % g.nx[1], z = rr / g.nx[1];
+        const long long m =
+            (long long)(x + f.off[0]) + (long long)(y + f.off[1]) * f.S1 + (long long)(z + f.off[2]) * f.S2;
+        const Cx<T> v = O[(size_t)(c >> 1) * f.ns + m];
+        r = (double)((c & 1) ? v.y : v.x);
+        const double xv = x_rho[row];
+        if (!linear) {
+            r += log(xv) + (g.mu ? g.mu[row] : cc.mu[c]);
+            if (nwall > 0) r += vext[row];
+            if (in_mf) r += in_mf[row];
+        } else {
+            if (in_rho) r += in_rho[row] / xv;
+            if (in_mf) r += in_mf[row];
+        }
+    }
+    if (add) r += add[row];
+    out_el[row] = r;
 }
 
-__global__ void k_scale_cx(Cx<double> *__restrict__ a, double s, long long n)
+__global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long long n)
 {
     const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (m < n) a[m] = Cx<double>{a[m].x * s, a[m].y * s};
+    if (m < n) b[m] = (float)a[m];
 }
 
 // Row pitch of the FFT storage: a multiple of 16 elements (rows start on 128-byte lines in complex
@@ -381,15 +567,17 @@ inline const Cx<float> *twiddles<float>(const FftPlan &F, int axis)
 // the pass has about kFftGroups workgroups (the whole chip busy).
 constexpr long long kFftGroups = 1024;
 
+// in-place transform of nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled)
 template <typename T>
-void fft3d(const FftPlan &F, Cx<T> *data, int sign, hipStream_t s)
+void fft3d(const FftPlan &F, Cx<T> *data, int nfield, int sign, hipStream_t s)
 {
+    if (nfield <= 0) return;
     for (int axis = 0; axis < 3; ++axis) {
         const int M = F.M[axis];
         if (M == 1) continue;
         const long long lines = F.n / M;
         const long long lds_max = (long long)((kFftLds / sizeof(Cx<T>) - M) / (2 * (M + 1)));
-        long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines / kFftGroups));
+        long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines * nfield / kFftGroups));
         lpb = std::min<long long>(lpb, 64);
         if (axis > 0) {   // strided axes: whole 128-byte lines of consecutive x when the budget allows
             const long long line_elems = 128 / (long long)sizeof(Cx<T>);
@@ -402,23 +590,25 @@ void fft3d(const FftPlan &F, Cx<T> *data, int sign, hipStream_t s)
             groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
         const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
         const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
+        const dim3 grid((unsigned)groups, (unsigned)nfield);
         if (sign < 0)
-            hipLaunchKernelGGL((k_fft_axis<T, -1>), dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
-                               F.M[2], F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+            hipLaunchKernelGGL((k_fft_axis<T, -1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2],
+                               F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
         else
-            hipLaunchKernelGGL((k_fft_axis<T, 1>), dim3((unsigned)groups), dim3(kFftBlock), lds, s, data, F.M[0], F.M[1],
-                               F.M[2], F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+            hipLaunchKernelGGL((k_fft_axis<T, 1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2],
+                               F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
     }
 }
 
-FftGeo fft_geo(const FftPlan &F)
+FftGeo fft_geo(const FftPlan &F, int which)
 {
     FftGeo f;
     for (int d = 0; d < 3; ++d) {
         f.M[d] = F.M[d];
-        f.src[d] = F.src[d];
-        f.lim[d] = F.lim[d];
         f.off[d] = F.off[d];
+        f.lo[d] = F.lo[which][d];
+        f.lim[d] = F.lim[which][d];
+        f.src[d] = F.src[which][d];
     }
     f.n = F.n;
     f.S1 = F.S1;
@@ -427,32 +617,85 @@ FftGeo fft_geo(const FftPlan &F)
     return f;
 }
 
+template <typename T>
+inline const T *attr_spectra(const FftPlan &F)
+{
+    if constexpr (sizeof(T) == 8)
+        return reinterpret_cast<const T *>(F.ud);
+    else
+        return reinterpret_cast<const T *>(F.uf);
+}
+template <typename T>
+inline const T *fmt_spectra(const FftPlan &F)
+{
+    if constexpr (sizeof(T) == 8)
+        return reinterpret_cast<const T *>(F.td);
+    else
+        return reinterpret_cast<const T *>(F.tf);
+}
+
+void free_plan(FftPlan *F)
+{
+    if (!F) return;
+    (void)hipFree(F->zbuf);
+    (void)hipFree(F->obuf);
+    (void)hipFree(F->ud);
+    (void)hipFree(F->uf);
+    (void)hipFree(F->td);
+    (void)hipFree(F->tf);
+    for (int d = 0; d < 3; ++d) {
+        (void)hipFree(F->twd[d]);
+        (void)hipFree(F->twf[d]);
+    }
+    delete F;
+}
+
+// Relative size of the part of a flipped-stencil spectrum that must vanish
+// (imaginary for even stencils, real for odd ones) above which the FFT form
+// is refused.
+constexpr double kSpectrumParityTol = 1e-9;
+
 }  // namespace
 
-// Plan, spectra and buffers of the FFT mean-field path (P.fft stays null when an
-// axis would be longer than kFftMaxLen).
-void fft_setup(Problem &P)
+// Plan, spectra and buffers of the FFT forms.  mf: the mean-field sums;
+// fmt: build the FMT spectra (the FMT sums switch to the FFT form with
+// fft_set_fmt).  P.fft stays null when an axis would be longer than
+// kFftMaxLen or a stencil spectrum fails its parity check.
+void fft_setup(Problem &P, bool mf, bool fmt)
 {
     const int nd = P.p.ndim, C = P.p.ncomp;
+    mf = mf && P.nmf > 0;
+    if (!mf && !fmt) return;
     auto *F = new FftPlan();
     try {
+        F->nd = nd;
+        F->C = C;
         for (int d = 0; d < nd; ++d) {
-            const int n = P.p.nx[d], h = P.pad_u.h[d];
+            const int n = P.p.nx[d], h = P.pad_u.h[d], hf = P.pad_f.h[d];
             const bool per = P.p.bc[d][0] == TMI_PERIODIC;
             if (per && nice_size(n) == n) {
                 F->M[d] = n;
                 F->off[d] = 0;
-                F->src[d] = h;
-                F->lim[d] = n;
+                for (int w = 0; w < 2; ++w) {
+                    F->lo[w][d] = 0;
+                    F->lim[w][d] = n;
+                }
+                F->src[0][d] = h;
+                F->src[1][d] = hf;
             } else {
                 F->M[d] = nice_size(n + 2 * h);
                 F->off[d] = h;
-                F->src[d] = 0;
-                F->lim[d] = n + 2 * h;
+                F->lo[0][d] = 0;
+                F->lim[0][d] = n + 2 * h;
+                F->src[0][d] = 0;
+                F->lo[1][d] = h - hf;
+                F->lim[1][d] = h + n + hf;
+                F->src[1][d] = hf - h;
+                F->wraps = false;
             }
             factor_radices(F->M[d], F->rc[d]);
             if (F->M[d] > kFftMaxLen) {   // one line must fit the LDS budget: keep the direct sums
-                delete F;
+                free_plan(F);
                 return;
             }
         }
@@ -460,7 +703,7 @@ void fft_setup(Problem &P)
         F->S1 = F->M[1] > 1 ? row_pitch(F->M[0]) : F->M[0];
         F->S2 = F->S1 * (F->M[1] + (F->M[2] > 1 ? 1 : 0));
         F->ns = F->S2 * F->M[2];
-        const size_t n = (size_t)F->ns;
+        const size_t ns = (size_t)F->ns;
         for (int d = 0; d < 3; ++d) {
             const int M = F->M[d];
             std::vector<Cx<double>> td(M);
@@ -475,14 +718,63 @@ void fft_setup(Problem &P)
             TMI_HIP_CHECK(hipMemcpy(F->twd[d], td.data(), M * sizeof(Cx<double>), hipMemcpyHostToDevice));
             TMI_HIP_CHECK(hipMemcpy(F->twf[d], tf.data(), M * sizeof(Cx<float>), hipMemcpyHostToDevice));
         }
-        TMI_HIP_CHECK(hipMalloc(&F->buf, 2 * (size_t)C * n * sizeof(Cx<double>)));
-        TMI_HIP_CHECK(hipMalloc((void **)&F->kd, (size_t)C * C * n * sizeof(Cx<double>)));
-        TMI_HIP_CHECK(hipMalloc((void **)&F->kf, (size_t)C * C * n * sizeof(Cx<float>)));
-        // flipped stencil g(-k mod M) = u(k) on the FFT grid, transformed, scaled by 1/n
-        std::vector<Cx<double>> host(n);
-        for (int q = 0; q < C * C; ++q) {
+        // output descriptors
+        const int ND = nd, nadj = C * (1 + ND) + 1;
+        const auto &b = P.bulk;
+        FwdDesc &f0 = F->fwd[0], &f1 = F->fwd[1];
+        RowMap &r0 = F->rows[0], &r1 = F->rows[1];
+        f0.C = f1.C = C;
+        if (C == 1) {   // n1, n0 and v1 are multiples of n2 and v2
+            f1.kind[f1.nout] = 0, f1.coef[f1.nout++][0] = 1.0;
+            f1.kind[f1.nout] = 1, f1.coef[f1.nout++][0] = 1.0;
+            for (int d = 0; d < ND; ++d) f1.kind[f1.nout] = 2 + d, f1.coef[f1.nout++][0] = 1.0;
+            const int src[10] = {0, 1, 1, 1, 2, 3, 4, 2, 3, 4};
+            const double sc[4] = {1.0, 1.0, b.inv_4pir[0], b.inv_4pirsq[0]};
+            r1.nrb = 4 + 2 * ND;
+            for (int q = 0; q < 4; ++q) r1.src[q] = src[q], r1.scale[q] = sc[q];
+            for (int d = 0; d < ND; ++d) {
+                r1.src[4 + d] = 2 + d, r1.scale[4 + d] = 1.0;
+                r1.src[4 + ND + d] = 2 + d, r1.scale[4 + ND + d] = b.inv_4pir[0];
+            }
+        } else {
+            auto add = [&](int kind, const double *coef) {
+                f1.kind[f1.nout] = kind;
+                for (int c = 0; c < C; ++c) f1.coef[f1.nout][c] = coef ? coef[c] : 1.0;
+                r1.src[f1.nout] = f1.nout;
+                r1.scale[f1.nout] = 1.0;
+                ++f1.nout;
+            };
+            add(0, nullptr);
+            add(1, nullptr);
+            add(1, b.inv_4pir);
+            add(1, b.inv_4pirsq);
+            for (int d = 0; d < ND; ++d) add(2 + d, nullptr);
+            for (int d = 0; d < ND; ++d) add(2 + d, b.inv_4pir);
+            r1.nrb = f1.nout;
+        }
+        if (mf) {
+            for (int c = 0; c < C; ++c) {
+                f0.kind[f0.nout] = kKindMf + c;
+                r0.src[r0.nmf] = f0.nout++;
+                r0.scale[r0.nmf++] = 1.0;
+                f1.kind[f1.nout] = kKindMf + c;
+                r1.src[r1.nrb + r1.nmf] = f1.nout++;
+                r1.scale[r1.nrb + r1.nmf++] = 1.0;
+            }
+        }
+        const int zpairs = std::max((C + 1) / 2, fmt ? (nadj + 1) / 2 : 0);
+        const int opairs = std::max((f0.nout + 1) / 2, fmt ? std::max((f1.nout + 1) / 2, (C + 1) / 2) : 0);
+        TMI_HIP_CHECK(hipMalloc(&F->zbuf, (size_t)zpairs * ns * sizeof(Cx<double>)));
+        TMI_HIP_CHECK(hipMalloc(&F->obuf, (size_t)opairs * ns * sizeof(Cx<double>)));
+        // spectra: flipped stencil g(-o mod M) = w(o) on the FFT grid, transformed (in zbuf), the
+        // real (even) or imaginary (odd) part kept, scaled by 1/n
+        std::vector<Cx<double>> host(ns);
+        std::vector<double> part(ns);
+        Cx<double> *work = reinterpret_cast<Cx<double> *>(F->zbuf);
+        bool parity_ok = true;
+        // vd < 0: the weights w; vd = d: the vector weights w o_d dx_d / R_comp (odd)
+        auto spectrum = [&](const HostStencil &hs, int vd, int comp, double *dst_d, float *dst_f) {
             std::fill(host.begin(), host.end(), Cx<double>{0.0, 0.0});
-            const HostStencil &hs = P.hs[TMI_UATT][q];
             const long long pitch[3] = {1, F->S1, F->S2};
             for (int k = 0; k < hs.L; ++k) {
                 long long idx = 0;
@@ -492,66 +784,139 @@ void fft_setup(Problem &P)
                     if (j < 0) j += F->M[d];
                     idx += (long long)j * pitch[d];
                 }
-                host[(size_t)idx].x += hs.w[k];
+                double w = hs.w[k];
+                if (vd >= 0) w = hs.w[k] * ((double)hs.off[3 * (size_t)k + vd] * P.p.dx[vd] * P.bulk.inv_rad[comp]);
+                host[(size_t)idx].x += w;
             }
-            Cx<double> *kq = F->kd + (size_t)q * n;
-            TMI_HIP_CHECK(hipMemcpy(kq, host.data(), n * sizeof(Cx<double>), hipMemcpyHostToDevice));
-            fft3d<double>(*F, kq, -1, 0);
-            hipLaunchKernelGGL(k_scale_cx, dim3(grid_for((long long)n)), dim3(kBlock), 0, 0, kq, 1.0 / (double)F->n,
-                               (long long)n);
-            hipLaunchKernelGGL(k_cast_spectrum<float>, dim3(grid_for((long long)n)), dim3(kBlock), 0, 0, kq,
-                               F->kf + (size_t)q * n, (long long)n);
+            TMI_HIP_CHECK(hipMemcpy(work, host.data(), ns * sizeof(Cx<double>), hipMemcpyHostToDevice));
+            fft3d<double>(*F, work, 1, -1, 0);
+            TMI_HIP_CHECK(hipMemcpy(host.data(), work, ns * sizeof(Cx<double>), hipMemcpyDeviceToHost));
+            double keep = 0.0, drop = 0.0;
+            const double inv = 1.0 / (double)F->n;
+            for (size_t q = 0; q < ns; ++q) {
+                const double a = vd < 0 ? host[q].x : host[q].y, z = vd < 0 ? host[q].y : host[q].x;
+                keep = std::max(keep, std::fabs(a));
+                drop = std::max(drop, std::fabs(z));
+                part[q] = a * inv;
+            }
+            if (drop > kSpectrumParityTol * keep) parity_ok = false;
+            TMI_HIP_CHECK(hipMemcpy(dst_d, part.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_to_float, dim3(grid_for((long long)ns)), dim3(kBlock), 0, 0, dst_d, dst_f,
+                               (long long)ns);
+        };
+        if (mf) {
+            TMI_HIP_CHECK(hipMalloc((void **)&F->ud, (size_t)C * C * ns * sizeof(double)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->uf, (size_t)C * C * ns * sizeof(float)));
+            for (int q = 0; q < C * C; ++q)
+                spectrum(P.hs[TMI_UATT][q], -1, q % C, F->ud + (size_t)q * ns, F->uf + (size_t)q * ns);
+            if (!parity_ok) {   // keep the direct sums
+                free_plan(F);
+                return;
+            }
+            F->mf = true;
+        }
+        if (fmt) {
+            const size_t nk = (size_t)C * (2 + ND);
+            TMI_HIP_CHECK(hipMalloc((void **)&F->td, nk * ns * sizeof(double)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->tf, nk * ns * sizeof(float)));
+            for (int c = 0; c < C; ++c) {
+                const size_t base = (size_t)c * (2 + ND);
+                spectrum(P.hs[TMI_THETA][c], -1, c, F->td + base * ns, F->tf + base * ns);
+                spectrum(P.hs[TMI_DELTA][c], -1, c, F->td + (base + 1) * ns, F->tf + (base + 1) * ns);
+                for (int d = 0; d < ND; ++d)
+                    spectrum(P.hs[TMI_DELTA][c], d, c, F->td + (base + 2 + d) * ns, F->tf + (base + 2 + d) * ns);
+            }
+            F->fmt_ok = parity_ok;
         }
         TMI_HIP_CHECK(hipGetLastError());
         TMI_HIP_CHECK(hipDeviceSynchronize());
     } catch (...) {
-        (void)hipFree(F->buf);
-        (void)hipFree(F->kd);
-        (void)hipFree(F->kf);
-        for (int d = 0; d < 3; ++d) {
-            (void)hipFree(F->twd[d]);
-            (void)hipFree(F->twf[d]);
-        }
-        delete F;
+        free_plan(F);
         throw;
+    }
+    if (!F->mf && !F->fmt_ok) {
+        free_plan(F);
+        return;
     }
     P.fft = F;
 }
 
 void fft_free(Problem &P)
 {
-    if (!P.fft) return;
-    (void)hipFree(P.fft->buf);
-    (void)hipFree(P.fft->kd);
-    (void)hipFree(P.fft->kf);
-    for (int d = 0; d < 3; ++d) {
-        (void)hipFree(P.fft->twd[d]);
-        (void)hipFree(P.fft->twf[d]);
-    }
-    delete P.fft;
+    free_plan(P.fft);
     P.fft = nullptr;
 }
 
-// mean-field rows from the halo-padded densities in workspace<T>(P).rho_pad
+bool fft_mf(const Problem &P) { return P.fft && P.fft->mf; }
+bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
+bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
+bool fft_wraps(const Problem &P) { return P.fft && P.fft->wraps; }
+void fft_set_fmt(Problem &P, bool on)
+{
+    if (P.fft) P.fft->fmt = on && P.fft->fmt_ok;
+}
+
+// Density products from the halo-padded densities in workspace<T>(P).rho_pad:
+// with_rb: the rho-bar rows (out_rb = sums - in_rb - sub_rb), then the
+// mean-field rows when they are in FFT form; else the mean-field rows only.
 template <typename T>
-void mf_rows_fft(Problem &P, const T *rho_pad, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
+void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_rb, double *out_rb,
+               const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s)
 {
     const FftPlan &F = *P.fft;
+    const int w = with_rb ? 1 : 0;
+    const FwdDesc &d = F.fwd[w];
+    const RowMap &rm = F.rows[w];
+    if (d.nout == 0) return;
     const int C = P.p.ncomp;
-    const size_t n = (size_t)F.ns;
-    Cx<T> *R = reinterpret_cast<Cx<T> *>(F.buf);
-    Cx<T> *O = R + (size_t)C * n;
-    const Cx<T> *K = sizeof(T) == 8 ? reinterpret_cast<const Cx<T> *>(F.kd) : reinterpret_cast<const Cx<T> *>(F.kf);
+    Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
+    Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gu = make_geo(P, P.pad_u);
-    const FftGeo fg = fft_geo(F);
-    hipLaunchKernelGGL(k_fft_gather<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, rho_pad, R);
-    for (int c = 0; c < C; ++c) fft3d<T>(F, R + (size_t)c * n, -1, s);
-    hipLaunchKernelGGL(k_fft_mul<T>, dim3(grid_for(F.ns)), dim3(kBlock), 0, s, C, F.ns, K, R, O);
-    for (int c = 0; c < C; ++c) fft3d<T>(F, O + (size_t)c * n, +1, s);
-    hipLaunchKernelGGL(k_fft_extract<T>, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, gu, O, in_mf, sub, out_mf);
+    const FftGeo fg = fft_geo(F, 0);
+    hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, workspace<T>(P).rho_pad, C, Z);
+    fft3d<T>(F, Z, (C + 1) / 2, -1, s);
+    if (P.p.ndim == 2)
+        hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                           attr_spectra<T>(F), O);
+    else
+        hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                           attr_spectra<T>(F), O);
+    fft3d<T>(F, O, (d.nout + 1) / 2, +1, s);
+    hipLaunchKernelGGL(k_fwd_extract<T>, dim3(grid_for(P.N), rm.nrb + rm.nmf), dim3(kBlock), 0, s, fg, gu, rm, O, in_rb,
+                       sub_rb, out_rb, in_mf, sub_mf, out_mf);
     TMI_HIP_CHECK(hipGetLastError());
 }
-template void mf_rows_fft<double>(Problem &, const double *, const double *, const double *, double *, hipStream_t);
-template void mf_rows_fft<float>(Problem &, const float *, const double *, const double *, double *, hipStream_t);
+template void fft_rb_mf<double>(Problem &, bool, const double *, const double *, double *, const double *,
+                                const double *, double *, hipStream_t);
+template void fft_rb_mf<float>(Problem &, bool, const double *, const double *, double *, const double *,
+                               const double *, double *, hipStream_t);
+
+// Euler-Lagrange rows from the adjoint fields in workspace<T>(P).adj_pad (as el_rows)
+template <typename T>
+void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
+            double *out_el, hipStream_t s)
+{
+    const FftPlan &F = *P.fft;
+    const int C = P.p.ncomp, nd = P.p.ndim, nadj = C * (1 + nd) + 1;
+    Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
+    Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
+    const Geo gf = make_geo(P, P.pad_f);
+    const FftGeo fg = fft_geo(F, 1);
+    hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gf, workspace<T>(P).adj_pad, nadj,
+                       Z);
+    fft3d<T>(F, Z, (nadj + 1) / 2, -1, s);
+    if (nd == 2)
+        hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
+    else
+        hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
+    fft3d<T>(F, O, (C + 1) / 2, +1, s);
+    hipLaunchKernelGGL(k_el_extract<T>, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, gf, make_consts(P),
+                       linear ? 1 : 0, P.p.nwall, O, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+template void fft_el<double>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
+                             hipStream_t);
+template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
+                            hipStream_t);
 
 }  // namespace tmi

@@ -156,7 +156,7 @@ struct Problem {
     double *d_mu = nullptr;        // [C][N] -ln rho_b - beta mu_hs,ex - beta mu_att of the node's pore
     std::vector<double> pore_rho_b;   // [npores][C]
     BatchWork *batch_work = nullptr;  // batched Newton buffers and graphs (tmi_batch.hip)
-    FftPlan *fft = nullptr;           // FFT form of the mean-field sums (tmi_fft.hip), 2D/3D
+    FftPlan *fft = nullptr;           // FFT forms of the stencil sums (tmi_fft.hip), 2D/3D
     // solver scratch
     double *d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -189,11 +189,22 @@ int batch_newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newt
                        int *pore_conv, int *pore_gmres, double *pore_resid, hipStream_t s);
 void free_batch_work(Problem &P);
 
-// FFT form of the mean-field sums (tmi_fft.hip)
-void fft_setup(Problem &P);
+// FFT forms of the stencil sums (tmi_fft.hip)
+void fft_setup(Problem &P, bool mf, bool fmt);
 void fft_free(Problem &P);
+bool fft_mf(const Problem &P);              // mean-field sums in FFT form
+bool fft_fmt(const Problem &P);             // FMT sums (rho-bar, Euler-Lagrange) in FFT form
+bool fft_fmt_available(const Problem &P);   // FMT spectra built
+bool fft_wraps(const Problem &P);           // every axis periodic on the FFT grid (halos unused)
+void fft_set_fmt(Problem &P, bool on);
 template <typename T>
-void mf_rows_fft(Problem &P, const T *rho_pad, const double *in_mf, const double *sub, double *out_mf, hipStream_t s);
+void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_rb, double *out_rb,
+               const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s);
+template <typename T>
+void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
+            double *out_el, hipStream_t s);
+// times the fill with the FMT sums in both forms and keeps the faster (tmi_kernels.hip)
+void choose_fmt_form(Problem &P);
 
 // 1D row-kernel autotuner (tmi_kernels.hip): times every launch shape on the
 // problem's own buffers and stores the fastest in P.tune

@@ -1183,8 +1183,8 @@ template <typename T>
 void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf, hipStream_t s)
 {
     if (P.nmf == 0) return;
-    if (P.fft) {
-        mf_rows_fft<T>(P, workspace<T>(P).rho_pad, in_mf, sub, out_mf, s);
+    if (fft_mf(P)) {
+        fft_rb_mf<T>(P, false, nullptr, nullptr, nullptr, in_mf, sub, out_mf, s);
         return;
     }
     if (P.use_rows1d) {
@@ -1206,6 +1206,24 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
                        workspace<T>(P).rho_pad, in_mf, sub, out_mf);
 }
 
+// rho-bar rows and mean-field rows from the same padded densities: one
+// forward transform serves both when the FMT sums are in FFT form.  ev
+// (optional) is recorded between the two.
+template <typename T>
+void rb_mf_rows(Problem &P, const double *in_rb, const double *sub_rb, double *out_rb, const double *in_mf,
+                const double *sub_mf, double *out_mf, hipStream_t s, hipEvent_t ev = nullptr)
+{
+    if (fft_fmt(P)) {
+        fft_rb_mf<T>(P, true, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf, s);
+        if (ev) TMI_HIP_CHECK(hipEventRecord(ev, s));
+        if (!fft_mf(P)) mf_rows<T>(P, in_mf, sub_mf, out_mf, s);
+        return;
+    }
+    rb_rows<T>(P, in_rb, sub_rb, out_rb, s);
+    if (ev) TMI_HIP_CHECK(hipEventRecord(ev, s));
+    mf_rows<T>(P, in_mf, sub_mf, out_mf, s);
+}
+
 template <typename T>
 void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear, hipStream_t s)
 {
@@ -1214,6 +1232,13 @@ void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear,
     T *adj = workspace<T>(P).adj_pad;
     const int lin = linear ? 1 : 0;
     const int func = P.p.type_func;
+    if (fft_fmt(P) && fft_wraps(P)) {   // the FFT form reads the interior only
+        if (P.p.ndim == 2)
+            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        else
+            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        return;
+    }
     switch (P.p.ndim) {
     case 1:
         hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
@@ -1234,6 +1259,10 @@ template <typename T>
 void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
              double *out_el, hipStream_t s)
 {
+    if (fft_fmt(P)) {
+        fft_el<T>(P, linear, x_rho, in_rho, in_mf, add, out_el, s);
+        return;
+    }
     Geo gf = make_geo(P, P.pad_f);
     ElArgs ea{};
     for (int c = 0; c < P.p.ncomp; ++c) {
@@ -1349,6 +1378,51 @@ void autotune_rows1d(Problem &P)
     (void)hipEventDestroy(e1);
 }
 
+// ---------------------------------------------------------------- FMT form
+// Times residual fills with the FMT sums as direct stencil sums and as FFT
+// convolutions (state: every unknown 4.8e-4, a positive byte pattern) and
+// keeps the faster form.  TMI_FMT_FFT=1 / 0 forces the FFT / direct form.
+void choose_fmt_form(Problem &P)
+{
+    if (!fft_fmt_available(P)) return;
+    const char *env = std::getenv("TMI_FMT_FFT");
+    if (env) {
+        fft_set_fmt(P, std::string(env) == "1");
+        return;
+    }
+    const size_t n = (size_t)P.nunk * P.N;
+    double *x = nullptr, *F = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms[2] = {0.f, 0.f};
+    try {
+        TMI_HIP_CHECK(hipMalloc((void **)&x, n * sizeof(double)));
+        TMI_HIP_CHECK(hipMalloc((void **)&F, n * sizeof(double)));
+        TMI_HIP_CHECK(hipMemset(x, 0x3f, n * sizeof(double)));
+        TMI_HIP_CHECK(hipEventCreate(&e0));
+        TMI_HIP_CHECK(hipEventCreate(&e1));
+        for (int form = 0; form < 2; ++form) {
+            fft_set_fmt(P, form == 1);
+            launch_residual(P, x, F, 0);
+            TMI_HIP_CHECK(hipEventRecord(e0, 0));
+            for (int r = 0; r < 3; ++r) launch_residual(P, x, F, 0);
+            TMI_HIP_CHECK(hipEventRecord(e1, 0));
+            TMI_HIP_CHECK(hipEventSynchronize(e1));
+            TMI_HIP_CHECK(hipEventElapsedTime(&ms[form], e0, e1));
+        }
+    } catch (...) {
+        (void)hipFree(x);
+        (void)hipFree(F);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipFree(x);
+    (void)hipFree(F);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fft_set_fmt(P, ms[1] < ms[0]);
+}
+
 // ---------------------------------------------------------------- public launchers
 // ev (optional): 6 events recorded between the stages pad_rho | rb_rows |
 // mf_rows | fmt_point+pad_adj | el_rows
@@ -1358,9 +1432,8 @@ void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipE
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
     pad_rho<double>(P, x + P.iu_rho * N, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[1], s));
-    rb_rows<double>(P, x + P.iu_rb * N, nullptr, F + P.iu_rb * N, s);
-    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[2], s));
-    mf_rows<double>(P, P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s);
+    rb_mf_rows<double>(P, x + P.iu_rb * N, nullptr, F + P.iu_rb * N, P.nmf ? x + P.iu_mf * N : nullptr, nullptr,
+                       F + P.iu_mf * N, s, ev ? ev[2] : nullptr);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
@@ -1375,8 +1448,8 @@ void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStr
 {
     const size_t N = P.N;
     pad_rho<T>(P, v + P.iu_rho * N, true, s);
-    rb_rows<T>(P, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N, s);
-    mf_rows<T>(P, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_mf * N, s);
+    rb_mf_rows<T>(P, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr,
+                  Jv + P.iu_mf * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, v + P.iu_rb * N, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, v + P.iu_rho * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_rho * N,
                s);
@@ -1389,9 +1462,8 @@ void launch_init_fields(Problem &P, double *x, hipStream_t s)
 {
     const size_t N = P.N;
     pad_rho<double>(P, x + P.iu_rho * N, false, s);
-    rb_rows<double>(P, nullptr, nullptr, x + P.iu_rb * N, s);
+    rb_mf_rows<double>(P, nullptr, nullptr, x + P.iu_rb * N, nullptr, nullptr, x + P.iu_mf * N, s);
     hipLaunchKernelGGL(k_cap_n3, dim3(grid_for(P.N)), dim3(kBlock), 0, s, x + P.iu_rb * N, P.N);
-    mf_rows<double>(P, nullptr, nullptr, x + P.iu_mf * N, s);
     TMI_HIP_CHECK(hipGetLastError());
 }
 
@@ -1401,8 +1473,7 @@ void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t
 {
     const size_t N = P.N;
     pad_rho<T>(P, x2, true, s);
-    rb_rows<T>(P, nullptr, nullptr, t1, s);
-    mf_rows<T>(P, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
+    rb_mf_rows<T>(P, nullptr, nullptr, t1, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, t1, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, x2, P.nmf ? t1 + (size_t)P.nrb * N : nullptr, nullptr, y2, s);
     TMI_HIP_CHECK(hipGetLastError());
@@ -1425,8 +1496,8 @@ void launch_schur_backsub(Problem &P, const double *x, const double *b, double *
     (void)x;
     const size_t N = P.N;
     pad_rho<double>(P, dx + P.iu_rho * N, true, s);
-    rb_rows<double>(P, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, s);
-    if (P.nmf) mf_rows<double>(P, nullptr, b + P.iu_mf * N, dx + P.iu_mf * N, s);
+    rb_mf_rows<double>(P, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, nullptr, P.nmf ? b + P.iu_mf * N : nullptr,
+                       P.nmf ? dx + P.iu_mf * N : nullptr, s);
     TMI_HIP_CHECK(hipGetLastError());
 }
 
