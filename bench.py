@@ -35,6 +35,9 @@ COPY_PROBE_BYTES = 512 << 20
 # the FMT reach touches 3.1 MiB) and writes ten rows (20 MiB).
 STAGE_TRAFFIC_MIB_64 = {"el_rows": 33.98 + 2.00, "rb_rows": 26.52 + 20.00}
 STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
+# Mean HBM bytes per k_fft_axis launch of one 64^3 fill in FFT form (fetch + write, MiB), from the same
+# counter runs; None until measured.
+FFT_TRAFFIC_MIB_64 = None
 
 
 def stencil_flops_per_node(dp, spec):
@@ -273,7 +276,6 @@ def main():
 
     # per-stage kernel times (HIP events on the launch stream)
     stages = dp.fill_residual_timed(x0, reps=max(5, args.steps // 5))
-    stage_names = ["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"]
 
     # ---------------------------------------------------------------- JVP throughput
     v = torch.rand_like(x0)
@@ -293,16 +295,42 @@ def main():
 
     # ---------------------------------------------------------------- roofline of the dominant kernel
     flops_node, f_fwd, f_adj, f_mf = stencil_flops_per_node(dp, spec)
-    # the FFT form of the mean-field sums is HBM-bound passes, not direct-form flops: it is not
-    # priced against the FP64 roof (its time is still in fill_stage_ms)
-    mf_fft = bool(dp.info.mf_fft)
-    per_stage_flops = [0.0, f_fwd, 0.0 if mf_fft else f_mf, 0.0, f_adj]
-    priced = [i for i in range(5) if per_stage_flops[i] > 0]
-    dom = max(priced, key=lambda i: stages[i])
-    dom_flops = per_stage_flops[dom] * N
-    achieved = dom_flops / (stages[dom] * 1e-3) / 1e12 if dom_flops > 0 else 0.0
+    mf_fft, fmt_fft = bool(dp.info.mf_fft), bool(dp.info.fmt_fft)
     C = spec.ncomp
     bytes_node = 8.0 * (2 * spec.nunk + C) + C
+    if fmt_fft:
+        # every stencil sum is an FFT convolution: the axis passes (k_fft_axis) dominate; each pass
+        # reads and writes every grid point of every field once (32 B per complex-double point)
+        stage_names = ["pad_rho", "rb_mf_rows_fft", "mf_rows_direct", "fmt_point", "el_rows_fft"]
+        ft = dp.fft_timed(reps=max(5, args.steps // 5))
+        achieved = ft["bytes"] / (ft["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_fft_axis", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                    "traffic": (FFT_TRAFFIC_MIB_64 * (1 << 20) if (args.n == 64 and FFT_TRAFFIC_MIB_64) else None),
+                    "launch_ms": ft["ms"] / ft["launches"], "launches_per_fill": ft["launches"],
+                    "algorithmic_bytes_per_launch": ft["bytes"] / ft["launches"],
+                    "share_of_fill": ft["ms"] / ms_fill,
+                    "traffic_note": ("mean HBM bytes per k_fft_axis launch of a fill from rocprofv3 --pmc "
+                                     "FETCH_SIZE/WRITE_SIZE, scaled by a 512 MiB copy"),
+                    "note": "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"}
+    else:
+        # direct stencil sums; the FFT form of the mean-field sums is HBM-bound passes, not
+        # direct-form flops, so it is not priced against the FP64 roof (its time is in fill_stage_ms)
+        stage_names = ["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"]
+        per_stage_flops = [0.0, f_fwd, 0.0 if mf_fft else f_mf, 0.0, f_adj]
+        priced = [i for i in range(5) if per_stage_flops[i] > 0]
+        dom = max(priced, key=lambda i: stages[i])
+        dom_flops = per_stage_flops[dom] * N
+        achieved = dom_flops / (stages[dom] * 1e-3) / 1e12 if dom_flops > 0 else 0.0
+        roofline = {"bound": "mfma", "kernel": stage_names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                    "traffic": (STAGE_TRAFFIC_MIB_64[stage_names[dom]] * (1 << 20)
+                                if (args.n == 64 and stage_names[dom] in STAGE_TRAFFIC_MIB_64) else None),
+                    "traffic_note": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
+                                     f"512 MiB copy; algorithmic bytes per launch "
+                                     f"{STAGE_ALGO_MIB_64.get(stage_names[dom], float('nan')):.1f} MiB"),
+                    "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
+                            "peak = 78.6 TFLOP/s)"}
 
     # ---------------------------------------------------------------- Newton time-to-solve
     newton = {}
@@ -357,17 +385,10 @@ def main():
         "jvp": jvp,
         "newton": newton,
         "mean_field_form": "fft" if mf_fft else "direct",
+        "fmt_form": "fft" if fmt_fft else "direct",
         "flops_per_node_direct": flops_node,
         "bytes_per_node_residual": bytes_node,
-        "roofline": {"bound": "mfma", "kernel": stage_names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": (STAGE_TRAFFIC_MIB_64[stage_names[dom]] * (1 << 20)
-                                 if (args.n == 64 and stage_names[dom] in STAGE_TRAFFIC_MIB_64) else None),
-                     "traffic_note": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
-                                      f"512 MiB copy; algorithmic bytes per launch "
-                                      f"{STAGE_ALGO_MIB_64.get(stage_names[dom], float('nan')):.1f} MiB"),
-                     "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
-                             "peak = 78.6 TFLOP/s)"},
+        "roofline": roofline,
         "fill_hbm_frac": (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9),
         "cpu_baseline": cpu,
     }

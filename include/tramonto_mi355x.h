@@ -78,11 +78,16 @@ typedef struct tmi_info {
     int sten_len[3][TMI_NCOMP_MAX * TMI_NCOMP_MAX];
     int mf_fft;   /* 1: the mean-field sums run as FFT convolutions (wide 2D/3D attraction stencils) */
     int fmt_fft;  /* 1: the FMT sums (rho-bar and Euler-Lagrange rows) run as FFT convolutions (2D/3D, when faster) */
+    int precond;  /* preconditioner of the Schur GMRES: TMI_PRECOND_DIAG or TMI_PRECOND_BULK */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */
 #define TMI_PREC_FP64 0
 #define TMI_PREC_MIXED 1   /* FP32 stencil sums and JVPs, FP64 residual/update/norm */
+
+/* right preconditioners of the Schur-complement GMRES */
+#define TMI_PRECOND_DIAG 0   /* the A22 diagonal, as the reference's HardSphere LinProbMgr */
+#define TMI_PRECOND_BULK 1   /* A22 diagonal times the inverse bulk Schur symbol (periodic FFT grids) */
 
 typedef struct tmi_newton_opts {
     int max_iter;
@@ -131,6 +136,17 @@ int tmi_host_bulk(const tmi_params *p, double *out60);
  * wait for the device to be idle first. */
 int tmi_problem_set_rho_b(void *handle, const double *rho_b);
 int tmi_problem_set_vext(void *handle, const double *vext);
+/* Preconditioner of the Schur GMRES (no counterpart in the reference, whose
+ * HardSphere LinProbMgr preconditions with the A22 diagonal,
+ * src/dft_HardSphereLinProbMgr.cpp:345-428): TMI_PRECOND_DIAG keeps that;
+ * TMI_PRECOND_BULK, the default where available, multiplies the diagonal by
+ * the inverse symbol of the Schur operator at the uniform bulk state,
+ * M^-1 v = F^-1[S_b(k)^-1 diag(1/rho_b) F(diag(rho) v)], exact for a uniform
+ * fluid.  It exists when every axis is periodic with an FFT-friendly length
+ * (2^a 3^b 5^c), the FFT forms are in use and the bulk is stable (S_b(k)
+ * positive definite); TMI_PRECOND=diag in the environment at creation
+ * selects the diagonal.  Fails with TMI_PRECOND_BULK where unavailable. */
+int tmi_problem_set_preconditioner(void *handle, int kind);
 
 /* F(x): the residual of every equation (the reference inserts -F into the
  * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
@@ -140,6 +156,14 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
  * stage_ms[0..4] receive the mean over `reps` fills of: density halo,
  * rho-bar rows, mean-field rows, pointwise FMT + adjoint halo, EL rows. */
 int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int reps, double *stage_ms, void *stream);
+/* Measurement hook, no counterpart in the reference: when the FMT sums run in
+ * FFT form (tmi_info.fmt_fft), times the FFT axis passes of one residual fill
+ * on the problem's work buffers with HIP events on `stream`.  *ms receives
+ * their mean time per fill over `reps` fills, *bytes their algorithmic
+ * traffic per fill (each pass reads and writes every grid point of every
+ * field once, 16 B complex doubles) and *launches the passes per fill.
+ * Fails when the FMT sums run as direct stencil sums. */
+int tmi_fft_timed(void *handle, int reps, double *ms, double *bytes, int *launches, void *stream);
 /* Jv = J(x) v, the assembled-Jacobian product, src/dft_Schur_Epetra_Operator.cpp:149 (ApplyGlobal) */
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
 /* same product with FP32 stencil sums and FP32 accumulation (FP64 pointwise

@@ -1,0 +1,145 @@
+"""The bulk-circulant Schur preconditioner (run on an MI355X).
+
+GMRES is right-preconditioned, so the preconditioner changes the Krylov
+iterates, not the solution: every solve must reach the same Newton update and
+the same converged profile as the reference's A22-diagonal preconditioner
+(and the oracle) to the solver tolerance.  At the uniform bulk state the
+preconditioner is the exact inverse of the Schur operator, so GMRES must
+converge in one iteration there.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle.oracle import Oracle  # noqa: E402
+from tramonto_amd.problem import small_case  # noqa: E402
+
+PERIODIC = ["2d_periodic", "3d_periodic_small", "3d_periodic_wb", "3d_mix_periodic", "3d_solute_small"]
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+@pytest.fixture(scope="module")
+def problems():
+    from tramonto_amd.device import DeviceProblem
+    cache = {}
+
+    def get(case):
+        if case not in cache:
+            spec = small_case(case)
+            cache[case] = (spec, Oracle.from_spec(spec), DeviceProblem(spec))
+        return cache[case]
+    return get
+
+
+def generic_state(spec, o, seed):
+    x = o.init_fields(spec.initial_x())
+    rng = np.random.default_rng(seed)
+    x[spec.ncomp:] *= 1.0 + 0.01 * rng.uniform(-1.0, 1.0, x[spec.ncomp:].shape)
+    return x
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", PERIODIC)
+def test_bulk_preconditioner_is_default_on_periodic_grids(case, problems):
+    from tramonto_amd.device import PRECOND_BULK
+    spec, o, dp = problems(case)
+    assert dp.info.precond == PRECOND_BULK
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["3d_periodic_small", "3d_mix_periodic", "2d_periodic"])
+def test_bulk_preconditioner_exact_at_bulk(case, problems):
+    """At x = uniform bulk (no zero rows) the preconditioned Schur operator is the identity."""
+    from tramonto_amd.device import PRECOND_BULK, PRECOND_DIAG
+    spec, o, dp = problems(case)
+    x = np.zeros((spec.nunk, spec.nnodes))
+    for c in range(spec.ncomp):
+        x[c] = spec.rho_b[c]
+    xg = dp.init_fields(dev(x))
+    rng = np.random.default_rng(5)
+    F = dev(rng.uniform(-1.0, 1.0, (spec.nunk, spec.nnodes)))
+    try:
+        dp.set_preconditioner(PRECOND_DIAG)
+        _, its_diag = dp.schur_solve(xg, F, rtol=1e-10)
+        dp.set_preconditioner(PRECOND_BULK)
+        _, its_bulk = dp.schur_solve(xg, F, rtol=1e-10)
+    finally:
+        dp.set_preconditioner(PRECOND_BULK)
+    assert its_bulk <= 2 < its_diag, (its_bulk, its_diag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", PERIODIC)
+def test_bulk_preconditioner_same_update(case, problems):
+    from tramonto_amd.device import PRECOND_BULK, PRECOND_DIAG
+    spec, o, dp = problems(case)
+    x = dev(generic_state(spec, o, 41))
+    F, _ = dp.fill_residual(x)
+    out = {}
+    try:
+        for kind in (PRECOND_DIAG, PRECOND_BULK):
+            dp.set_preconditioner(kind)
+            out[kind] = dp.schur_solve(x, F, rtol=1e-12, restart=200, maxit=2000)
+    finally:
+        dp.set_preconditioner(PRECOND_BULK)
+    a, b = out[PRECOND_BULK][0], out[PRECOND_DIAG][0]
+    assert float((a - b).abs().max() / b.abs().max()) < 1e-7
+    assert out[PRECOND_BULK][1] < out[PRECOND_DIAG][1], (out[PRECOND_BULK][1], out[PRECOND_DIAG][1])
+
+
+@pytest.fixture(scope="module")
+def oracle_profiles(problems):
+    cache = {}
+
+    def get(case):
+        if case not in cache:
+            spec, o, _ = problems(case)
+            x0 = o.init_fields(spec.initial_x())
+            xo, st = o.newton(x0, resid_tol=1e-10)
+            assert st["converged"], st
+            cache[case] = (x0, xo)
+        return cache[case]
+    return get
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", ["3d_periodic_small", "3d_mix_periodic", "3d_solute_small"])
+def test_bulk_preconditioned_newton_matches_oracle(case, precision, problems, oracle_profiles):
+    spec, o, dp = problems(case)
+    x0, xo = oracle_profiles(case)
+    xg = dev(x0)
+    st = dp.newton(xg, resid_tol=1e-10, gmres_tol=1e-8, max_iter=60, precision=precision)
+    assert st["converged"], st
+    assert np.abs(xg.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_bulk_preconditioner_follows_rho_b(problems):
+    """After set_rho_b the symbol is rebuilt: still exact at the new bulk state."""
+    spec, o, dp = problems("3d_periodic_small")
+    old = list(spec.rho_b)
+    try:
+        dp.set_rho_b([0.45])
+        x = np.zeros((spec.nunk, spec.nnodes))
+        x[0] = 0.45
+        xg = dp.init_fields(dev(x))
+        F = dev(np.random.default_rng(6).uniform(-1.0, 1.0, (spec.nunk, spec.nnodes)))
+        _, its = dp.schur_solve(xg, F, rtol=1e-10)
+        assert its <= 2, its
+    finally:
+        dp.set_rho_b(old)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2d_wall_reflect", "2d_rough_wall", "3d_mix_bulk"])
+def test_bulk_preconditioner_unavailable_with_walls(case, problems):
+    from tramonto_amd.device import PRECOND_BULK, PRECOND_DIAG
+    spec, o, dp = problems(case)
+    assert dp.info.precond == PRECOND_DIAG
+    with pytest.raises(RuntimeError, match="periodic FFT grid"):
+        dp.set_preconditioner(PRECOND_BULK)

@@ -31,7 +31,7 @@ class TmiInfo(ct.Structure):
     _fields_ = [
         ("nunk", ct.c_int), ("nnodes", ct.c_int), ("iu_rho", ct.c_int), ("iu_rb", ct.c_int), ("n_rb", ct.c_int),
         ("iu_mf", ct.c_int), ("n_mf", ct.c_int), ("sten_len", (ct.c_int * (NC * NC)) * 3), ("mf_fft", ct.c_int),
-        ("fmt_fft", ct.c_int),
+        ("fmt_fft", ct.c_int), ("precond", ct.c_int),
     ]
 
 
@@ -69,10 +69,10 @@ _lib: Optional[ct.CDLL] = None
 # exported symbols declared in include/tramonto_mi355x.h
 DECLARED_SYMBOLS = [
     "tmi_last_error", "tmi_version", "tmi_problem_create", "tmi_problem_destroy", "tmi_problem_info",
-    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
+    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning",
-    "tmi_problem_set_rho_b", "tmi_problem_set_vext",
+    "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner",
     "fill_resid_and_matrix_control",
     "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
     "dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids", "dft_hardsphere_lin_prob_mgr_setdensityequationids",
@@ -107,6 +107,7 @@ def lib() -> ct.CDLL:
     L.tmi_fill_residual.argtypes = [vp, vp, vp, ct.POINTER(d), vp]
     L.tmi_apply_jacobian.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_fill_residual_timed.argtypes = [vp, vp, vp, i, vp, vp]
+    L.tmi_fft_timed.argtypes = [vp, i, ct.POINTER(d), ct.POINTER(d), ct.POINTER(i), vp]
     L.tmi_apply_jacobian_fp32.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_apply_jacobian_transpose.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_init_fields.argtypes = [vp, vp, vp]
@@ -116,6 +117,7 @@ def lib() -> ct.CDLL:
     L.tmi_row_tuning.argtypes = [vp, vp, vp]
     L.tmi_problem_set_rho_b.argtypes = [vp, vp]
     L.tmi_problem_set_vext.argtypes = [vp, vp]
+    L.tmi_problem_set_preconditioner.argtypes = [vp, i]
     L.tmi_batch_create.argtypes = [ct.POINTER(TmiParams), i, vp, vp, vp, vp, ct.POINTER(vp)]
     L.tmi_batch_info.argtypes = [vp, ct.POINTER(i), vp]
     L.tmi_batch_newton_solve.argtypes = [vp, vp, ct.POINTER(TmiNewtonOpts), ct.POINTER(TmiNewtonStats), vp, vp, vp,

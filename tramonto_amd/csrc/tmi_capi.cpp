@@ -479,7 +479,9 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
             const bool fmt = !(fm && std::string(fm) == "0");
             fft_setup(*P, mf, fmt);
             choose_fmt_form(*P);
-            if (P->fft && !fft_mf(*P) && !fft_fmt(*P)) fft_free(*P);
+            // an unused plan is kept on wrapping grids for the bulk preconditioner
+            if (P->fft && !fft_mf(*P) && !fft_fmt(*P) && !fft_wraps(*P)) fft_free(*P);
+            precond_setup(*P);
         }
     } catch (...) {
         destroy_problem(P);
@@ -557,7 +559,10 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     double *w = z + n;                                 // n
     double *dinv = w + n;                              // n
     double *t1 = dinv + n;                             // (nrb + nmf) N
-    dev_a22_inverse_diag(P, x, dinv, s);
+    // right preconditioner: the A22 diagonal (dinv), or its bulk-circulant correction on wrapping FFT grids
+    const bool bulk_pc = precond_active(P);
+    const double *x_rho = x + (size_t)P.iu_rho * P.N;
+    if (!bulk_pc) dev_a22_inverse_diag(P, x, dinv, s);
     TMI_HIP_CHECK(hipMemsetAsync(x2, 0, sizeof(double) * n, s));
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
     const double bn0 = std::sqrt(dev_dot(P, bS, bS, n, s));
@@ -585,7 +590,10 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
         int k = 0;
         bool done = false;
         for (; k < m && its < maxit; ++k) {
-            dev_mul(dinv, V + (size_t)k * n, z, n, s);
+            if (bulk_pc)
+                precond_apply(P, x_rho, V + (size_t)k * n, z, s);
+            else
+                dev_mul(dinv, V + (size_t)k * n, z, n, s);
             if (prec == TMI_PREC_MIXED)
                 launch_schur_apply<float>(P, x, z, t1, w, s);
             else
@@ -631,7 +639,10 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
         TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
         TMI_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * n, s));
         dev_gemv_add(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
-        dev_mul(dinv, z, w, n, s);
+        if (bulk_pc)
+            precond_apply(P, x_rho, z, w, s);
+        else
+            dev_mul(dinv, z, w, n, s);
         dev_axpy(1.0, w, x2, n, s);
         TMI_HIP_CHECK(hipStreamSynchronize(s));
         if (done) break;
@@ -762,6 +773,7 @@ int tmi_problem_info(void *handle, tmi_info *info)
             for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
         info->mf_fft = fft_mf(*P) ? 1 : 0;
         info->fmt_fft = fft_fmt(*P) ? 1 : 0;
+        info->precond = precond_active(*P) ? 1 : 0;
         return 0;
     });
 }
@@ -885,6 +897,16 @@ int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int re
     });
 }
 
+int tmi_fft_timed(void *handle, int reps, double *ms, double *bytes, int *launches, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!ms || !bytes || !launches || reps < 1) throw std::invalid_argument("bad argument");
+        fft_time_passes(*P, reps, ms, bytes, launches, (hipStream_t)stream);
+        return 0;
+    });
+}
+
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream)
 {
     return guarded([&] {
@@ -928,6 +950,7 @@ int tmi_problem_set_rho_b(void *handle, const double *rho_b)
         for (int c = 0; c < P->p.ncomp; ++c) P->p.rho_b[c] = rho_b[c];
         compute_bulk_constants(P->p, P->bulk);
         if (P->p.type_attr >= 0) compute_attraction_constants(P->p, P->bulk, P->hs);
+        precond_setup(*P);   // the bulk Schur symbol depends on rho_b
         return 0;
     });
 }
@@ -949,6 +972,18 @@ int tmi_problem_set_vext(void *handle, const double *vext)
         TMI_HIP_CHECK(hipDeviceSynchronize());
         upload(P->d_vext, P->h_vext.data(), P->h_vext.size());
         upload(P->d_elzero, P->h_elzero.data(), P->h_elzero.size());
+        return 0;
+    });
+}
+
+int tmi_problem_set_preconditioner(void *handle, int kind)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (kind != TMI_PRECOND_DIAG && kind != TMI_PRECOND_BULK) throw std::invalid_argument("preconditioner kind");
+        if (kind == TMI_PRECOND_BULK && !precond_available(*P))
+            throw std::domain_error("the bulk preconditioner needs a periodic FFT grid and a stable bulk");
+        precond_enable(*P, kind == TMI_PRECOND_BULK);
         return 0;
     });
 }

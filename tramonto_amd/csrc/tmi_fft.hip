@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -93,6 +94,11 @@ struct FftPlan {
     void *obuf = nullptr;       // products, then the output pairs
     FwdDesc fwd[2];             // [0] mean-field rows only, [1] rho-bar rows then mean-field rows
     RowMap rows[2];
+    // bulk-circulant Schur preconditioner (wrapping grids): Q(k) = S_b(k)^-1 diag(1/rho_b) / n
+    Cx<double> *pq = nullptr;   // [C*C][ns], entry (i, j) at i + C j
+    bool pc_ok = false;         // Q built
+    int pc_want = -1;           // wanted (TMI_PRECOND=diag: no; -1 until the first set-up reads it)
+    bool pc_on = false;         // GMRES uses it: built and wanted
 };
 
 namespace {
@@ -510,6 +516,78 @@ __global__ void k_el_extract(FftGeo f, Geo g, CompConsts cc, int linear, int nwa
     out_el[row] = r;
 }
 
+// Preconditioner input u_c = rho_c v_c on the free rows (0 on the zero rows),
+// as pairs on a wrapping FFT grid (grid point = node)
+__global__ void k_pc_gather(FftGeo f, Geo g, const double *__restrict__ x_rho, const double *__restrict__ v,
+                            const unsigned char *__restrict__ elzero, Cx<double> *__restrict__ Z)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.N) return;
+    const int x = i % g.nx[0];
+    const int r = i / g.nx[0];
+    const int y = r % g.nx[1], z = r / g.nx[1];
+    const long long q = x + y * f.S1 + z * f.S2;
+    for (int j = 0; 2 * j < g.C; ++j) {
+        const size_t a = (size_t)(2 * j) * g.N + i, b = a + g.N;
+        const double re = elzero[a] ? 0.0 : x_rho[a] * v[a];
+        const double im = (2 * j + 1 < g.C && !elzero[b]) ? x_rho[b] * v[b] : 0.0;
+        Z[(size_t)j * f.ns + q] = Cx<double>{re, im};
+    }
+}
+
+// Y = Q(k) U(k), outputs packed in pairs
+__global__ void k_pc_mul(FftGeo f, int C, const Cx<double> *__restrict__ Z, const Cx<double> *__restrict__ Q,
+                         Cx<double> *__restrict__ O)
+{
+    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (m >= f.n) return;
+    const KIdx ix = k_index(f, m);
+    Cx<double> X[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        X[c] = Cx<double>{0.0, 0.0};
+        if (c < C) {
+            const Cx<double> *z = Z + (size_t)(c >> 1) * f.ns;
+            X[c] = unpack(z[ix.q], z[ix.qn], c & 1);
+        }
+    }
+    Cx<double> acc{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < kNC; ++i) {
+        if (i < C) {
+            Cx<double> y{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < kNC; ++j)
+                if (j < C) y = cadd(y, cmul(Q[(size_t)(i + C * j) * f.ns + ix.q], X[j]));
+            if ((i & 1) == 0) {
+                acc = y;
+                if (i + 1 == C) O[(size_t)(i >> 1) * f.ns + ix.q] = acc;
+            } else {
+                O[(size_t)(i >> 1) * f.ns + ix.q] = Cx<double>{acc.x - y.y, acc.y + y.x};
+            }
+        }
+    }
+}
+
+// z_c = M^-1 v on the free rows, v on the zero rows
+__global__ void k_pc_extract(FftGeo f, Geo g, const Cx<double> *__restrict__ O, const double *__restrict__ v,
+                             const unsigned char *__restrict__ elzero, double *__restrict__ zout)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= g.N) return;
+    const size_t row = (size_t)c * g.N + i;
+    if (elzero[row]) {
+        zout[row] = v[row];
+        return;
+    }
+    const int x = i % g.nx[0];
+    const int r = i / g.nx[0];
+    const int y = r % g.nx[1], z = r / g.nx[1];
+    const Cx<double> o = O[(size_t)(c >> 1) * f.ns + x + y * f.S1 + z * f.S2];
+    zout[row] = (c & 1) ? o.y : o.x;
+}
+
 __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long long n)
 {
     const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -643,6 +721,7 @@ void free_plan(FftPlan *F)
     (void)hipFree(F->uf);
     (void)hipFree(F->td);
     (void)hipFree(F->tf);
+    (void)hipFree(F->pq);
     for (int d = 0; d < 3; ++d) {
         (void)hipFree(F->twd[d]);
         (void)hipFree(F->twf[d]);
@@ -918,5 +997,229 @@ template void fft_el<double>(Problem &, bool, const double *, const double *, co
                              hipStream_t);
 template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
                             hipStream_t);
+
+// Times the transforms of one residual fill in FFT form on the plan's work
+// buffers (the density pairs forward, the output pairs back, the adjoint
+// pairs forward, the Euler-Lagrange pairs back), with HIP events on s: *ms is
+// their mean time per fill over reps fills, *bytes their algorithmic traffic
+// (every axis pass reads and writes each grid point of each field once) and
+// *launches the axis-pass launches per fill.  The buffers' contents are
+// scratch between products.
+void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
+{
+    if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
+    const FftPlan &F = *P.fft;
+    const int C = P.p.ncomp, nadj = C * (1 + P.p.ndim) + 1;
+    Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
+    Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
+    struct Set {
+        Cx<double> *buf;
+        int nfield, sign;
+    };
+    const Set sets[4] = {{Z, (C + 1) / 2, -1}, {O, (F.fwd[1].nout + 1) / 2, +1}, {Z, (nadj + 1) / 2, -1}, {O, (C + 1) / 2, +1}};
+    int axes = 0;
+    for (int d = 0; d < 3; ++d) axes += F.M[d] > 1 ? 1 : 0;
+    auto fill = [&] {
+        for (const Set &t : sets) fft3d<double>(F, t.buf, t.nfield, t.sign, s);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float t = 0.f;
+    try {
+        TMI_HIP_CHECK(hipEventCreate(&e0));
+        TMI_HIP_CHECK(hipEventCreate(&e1));
+        fill();
+        TMI_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) fill();
+        TMI_HIP_CHECK(hipEventRecord(e1, s));
+        TMI_HIP_CHECK(hipEventSynchronize(e1));
+        TMI_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    double b = 0.0;
+    for (const Set &q : sets) b += 2.0 * sizeof(Cx<double>) * (double)F.n * q.nfield * axes;
+    *ms = (double)t / reps;
+    *bytes = b;
+    *launches = 4 * axes;
+}
+
+// ---------------------------------------------------------------- bulk-circulant Schur preconditioner
+// On a wrapping FFT grid the Schur operator S = A22 + A21 A12 at a uniform
+// bulk state is a block convolution: S_b v = sum_j r_j * v_j with r_j = S_b e_j
+// (unit impulse of component j at node 0), symbol S_b(k)_ij = FFT(r_j,i)(k).
+// With S ~ D^-1 + K and D = diag(rho) (the reference's A22 diagonal
+// preconditioner), S^-1 ~ S_b^-1 diag(1/rho_b) D, exact at the bulk state;
+// GMRES applies it on the right: M^-1 v = F^-1[S_b(k)^-1 diag(1/rho_b) F(D v)]
+// on the free rows, v on the zero rows (whose Schur rows are the identity).
+// Built when every axis wraps and the Hermitian part of S_b(k) is positive
+// definite at every k (a stable bulk); else GMRES keeps the diagonal.
+void precond_setup(Problem &P)
+{
+    if (!P.fft) return;
+    FftPlan &F = *P.fft;
+    if (F.pc_want < 0) {
+        const char *env = std::getenv("TMI_PRECOND");
+        F.pc_want = (env && std::string(env) == "diag") ? 0 : 1;
+    }
+    F.pc_ok = F.pc_on = false;
+    (void)hipFree(F.pq);
+    F.pq = nullptr;
+    if (!F.wraps || P.npores > 0) return;
+    const int C = P.p.ncomp;
+    const size_t N = P.N, ns = (size_t)F.ns, nq = (size_t)P.nrb + P.nmf;
+    double *xb = nullptr, *e = nullptr, *t1 = nullptr, *y = nullptr;
+    unsigned char *flags = nullptr;
+    unsigned char *zero_save = P.d_zero, *elzero_save = P.d_elzero;
+    std::vector<std::complex<double>> S((size_t)C * C * ns);
+    try {
+        TMI_HIP_CHECK(hipMalloc((void **)&xb, (size_t)P.nunk * N * sizeof(double)));
+        TMI_HIP_CHECK(hipMalloc((void **)&e, (size_t)C * N * sizeof(double)));
+        TMI_HIP_CHECK(hipMalloc((void **)&t1, std::max<size_t>(nq, 1) * N * sizeof(double)));
+        TMI_HIP_CHECK(hipMalloc((void **)&y, (size_t)C * N * sizeof(double)));
+        TMI_HIP_CHECK(hipMalloc((void **)&flags, (size_t)C * N));
+        TMI_HIP_CHECK(hipMemset(flags, 0, (size_t)C * N));
+        {
+            std::vector<double> h((size_t)P.nunk * N, 0.0);
+            for (int c = 0; c < C; ++c)
+                std::fill(h.begin() + (size_t)(P.iu_rho + c) * N, h.begin() + (size_t)(P.iu_rho + c + 1) * N,
+                          P.p.rho_b[c]);
+            TMI_HIP_CHECK(hipMemcpy(xb, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        P.d_zero = P.d_elzero = flags;   // the bulk operator has no zero rows
+        launch_init_fields(P, xb, 0);
+        std::vector<double> hy((size_t)C * N);
+        std::vector<Cx<double>> grid(ns);
+        Cx<double> *work = reinterpret_cast<Cx<double> *>(F.zbuf);
+        for (int j = 0; j < C; ++j) {
+            TMI_HIP_CHECK(hipMemset(e, 0, (size_t)C * N * sizeof(double)));
+            const double one = 1.0;
+            TMI_HIP_CHECK(hipMemcpy(e + (size_t)j * N, &one, sizeof(double), hipMemcpyHostToDevice));
+            launch_schur_apply<double>(P, xb, e, t1, y, 0);
+            TMI_HIP_CHECK(hipMemcpy(hy.data(), y, hy.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < C; ++i) {
+                std::fill(grid.begin(), grid.end(), Cx<double>{0.0, 0.0});
+                for (size_t node = 0; node < N; ++node) {
+                    const size_t x = node % P.p.nx[0], r = node / P.p.nx[0];
+                    const size_t yy = P.p.ndim > 1 ? r % P.p.nx[1] : 0, z = P.p.ndim > 1 ? r / P.p.nx[1] : 0;
+                    grid[x + yy * F.S1 + z * F.S2].x = hy[(size_t)i * N + node];
+                }
+                TMI_HIP_CHECK(hipMemcpy(work, grid.data(), ns * sizeof(Cx<double>), hipMemcpyHostToDevice));
+                fft3d<double>(F, work, 1, -1, 0);
+                TMI_HIP_CHECK(hipMemcpy(grid.data(), work, ns * sizeof(Cx<double>), hipMemcpyDeviceToHost));
+                for (size_t q = 0; q < ns; ++q) S[(size_t)(i + C * j) * ns + q] = {grid[q].x, grid[q].y};
+            }
+        }
+        P.d_zero = zero_save;
+        P.d_elzero = elzero_save;
+    } catch (...) {
+        P.d_zero = zero_save;
+        P.d_elzero = elzero_save;
+        (void)hipFree(xb);
+        (void)hipFree(e);
+        (void)hipFree(t1);
+        (void)hipFree(y);
+        (void)hipFree(flags);
+        throw;
+    }
+    (void)hipFree(xb);
+    (void)hipFree(e);
+    (void)hipFree(t1);
+    (void)hipFree(y);
+    (void)hipFree(flags);
+    // per grid point: Cholesky of the Hermitian part (positive definite), then Q = S^-1 diag(1/rho_b) / n
+    std::vector<Cx<double>> hq((size_t)C * C * ns, Cx<double>{0.0, 0.0});
+    const double inv_n = 1.0 / (double)F.n;
+    using cd = std::complex<double>;
+    for (int m2 = 0; m2 < F.M[2]; ++m2)
+        for (int m1 = 0; m1 < F.M[1]; ++m1)
+            for (int m0 = 0; m0 < F.M[0]; ++m0) {
+                const size_t q = m0 + m1 * F.S1 + m2 * F.S2;
+                cd A[kNC][kNC], Hm[kNC][kNC], Inv[kNC][kNC];
+                for (int i = 0; i < C; ++i)
+                    for (int j = 0; j < C; ++j) A[i][j] = S[(size_t)(i + C * j) * ns + q];
+                for (int i = 0; i < C; ++i)
+                    for (int j = 0; j < C; ++j) Hm[i][j] = 0.5 * (A[i][j] + std::conj(A[j][i]));
+                for (int k = 0; k < C; ++k) {   // in-place Cholesky, lower triangle
+                    double dkk = Hm[k][k].real();
+                    for (int l = 0; l < k; ++l) dkk -= std::norm(Hm[k][l]);
+                    if (!(dkk > 0.0)) return;   // not positive definite: keep the diagonal preconditioner
+                    const double lkk = std::sqrt(dkk);
+                    Hm[k][k] = lkk;
+                    for (int i = k + 1; i < C; ++i) {
+                        cd sum = Hm[i][k];
+                        for (int l = 0; l < k; ++l) sum -= Hm[i][l] * std::conj(Hm[k][l]);
+                        Hm[i][k] = sum / lkk;
+                    }
+                }
+                // Gauss-Jordan with partial pivoting: Inv = A^-1
+                for (int i = 0; i < C; ++i)
+                    for (int j = 0; j < C; ++j) Inv[i][j] = (i == j) ? 1.0 : 0.0;
+                for (int k = 0; k < C; ++k) {
+                    int piv = k;
+                    for (int i = k + 1; i < C; ++i)
+                        if (std::abs(A[i][k]) > std::abs(A[piv][k])) piv = i;
+                    if (std::abs(A[piv][k]) == 0.0) return;
+                    for (int j = 0; j < C; ++j) {
+                        std::swap(A[k][j], A[piv][j]);
+                        std::swap(Inv[k][j], Inv[piv][j]);
+                    }
+                    const cd d = 1.0 / A[k][k];
+                    for (int j = 0; j < C; ++j) {
+                        A[k][j] *= d;
+                        Inv[k][j] *= d;
+                    }
+                    for (int i = 0; i < C; ++i) {
+                        if (i == k) continue;
+                        const cd f = A[i][k];
+                        for (int j = 0; j < C; ++j) {
+                            A[i][j] -= f * A[k][j];
+                            Inv[i][j] -= f * Inv[k][j];
+                        }
+                    }
+                }
+                for (int i = 0; i < C; ++i)
+                    for (int j = 0; j < C; ++j) {
+                        const cd v = Inv[i][j] * (inv_n / P.p.rho_b[j]);
+                        hq[(size_t)(i + C * j) * ns + q] = Cx<double>{v.real(), v.imag()};
+                    }
+            }
+    TMI_HIP_CHECK(hipMalloc((void **)&F.pq, hq.size() * sizeof(Cx<double>)));
+    TMI_HIP_CHECK(hipMemcpy(F.pq, hq.data(), hq.size() * sizeof(Cx<double>), hipMemcpyHostToDevice));
+    F.pc_ok = true;
+    F.pc_on = F.pc_want != 0;
+}
+
+bool precond_available(const Problem &P) { return P.fft && P.fft->pc_ok; }
+bool precond_active(const Problem &P) { return P.fft && P.fft->pc_on; }
+void precond_enable(Problem &P, bool on)
+{
+    if (on && !precond_available(P))
+        throw std::domain_error("the bulk preconditioner needs a periodic FFT grid and a stable bulk");
+    if (P.fft) {
+        P.fft->pc_want = on ? 1 : 0;
+        P.fft->pc_on = on;
+    }
+}
+
+// z = M^-1 v (bulk-circulant preconditioner), x_rho the densities of the Newton iterate
+void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s)
+{
+    const FftPlan &F = *P.fft;
+    const int C = P.p.ncomp;
+    Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
+    Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
+    const Geo g = make_geo(P, P.pad_u);
+    const FftGeo fg = fft_geo(F, 0);
+    hipLaunchKernelGGL(k_pc_gather, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, g, x_rho, v, P.d_elzero, Z);
+    fft3d<double>(F, Z, (C + 1) / 2, -1, s);
+    hipLaunchKernelGGL(k_pc_mul, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, F.pq, O);
+    fft3d<double>(F, O, (C + 1) / 2, +1, s);
+    hipLaunchKernelGGL(k_pc_extract, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, g, O, v, P.d_elzero, z);
+    TMI_HIP_CHECK(hipGetLastError());
+}
 
 }  // namespace tmi

@@ -203,6 +203,13 @@ void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
             double *out_el, hipStream_t s);
+void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s);
+// bulk-circulant Schur preconditioner on wrapping FFT grids (tmi_fft.hip)
+void precond_setup(Problem &P);
+bool precond_available(const Problem &P);
+bool precond_active(const Problem &P);
+void precond_enable(Problem &P, bool on);
+void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s);
 // times the fill with the FMT sums in both forms and keeps the faster (tmi_kernels.hip)
 void choose_fmt_form(Problem &P);
 

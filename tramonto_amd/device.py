@@ -23,6 +23,7 @@ from . import _lib
 from .problem import ProblemSpec
 
 PREC_FP64, PREC_MIXED = 0, 1
+PRECOND_DIAG, PRECOND_BULK = 0, 1
 
 
 def _stream() -> ct.c_void_p:
@@ -106,6 +107,12 @@ class DeviceProblem:
         _lib.check(_lib.lib().tmi_problem_set_vext(self.handle, a.ctypes.data_as(ct.c_void_p)), "tmi_problem_set_vext")
         self.spec.vext = a
 
+    def set_preconditioner(self, kind: int) -> None:
+        """Schur GMRES preconditioner: PRECOND_DIAG (the A22 diagonal) or PRECOND_BULK
+        (diagonal times the inverse bulk Schur symbol, periodic FFT grids)."""
+        _lib.check(_lib.lib().tmi_problem_set_preconditioner(self.handle, int(kind)), "tmi_problem_set_preconditioner")
+        self.info.precond = int(kind)
+
     # ------------------------------------------------------------ device ops
     def zeros(self) -> torch.Tensor:
         return torch.zeros((self.nunk, self.nnodes), dtype=torch.float64, device="cuda")
@@ -125,6 +132,14 @@ class DeviceProblem:
                                                       ms.ctypes.data_as(ct.c_void_p), _stream()),
                    "tmi_fill_residual_timed")
         return ms
+
+    def fft_timed(self, reps: int = 10) -> Dict:
+        """FFT axis passes of one residual fill in FFT form (HIP events on the launch
+        stream): mean ms per fill, algorithmic bytes per fill, launches per fill."""
+        ms, nbytes, launches = ct.c_double(), ct.c_double(), ct.c_int()
+        _lib.check(_lib.lib().tmi_fft_timed(self.handle, reps, ct.byref(ms), ct.byref(nbytes), ct.byref(launches),
+                                            _stream()), "tmi_fft_timed")
+        return dict(ms=ms.value, bytes=nbytes.value, launches=launches.value)
 
     def apply_jacobian(self, x: torch.Tensor, v: torch.Tensor, out: Optional[torch.Tensor] = None,
                        fp32: bool = False) -> torch.Tensor:

@@ -381,6 +381,15 @@ def small_case(kind: str, seed: int = 7) -> ProblemSpec:
                            bc=[[IN_BULK, IN_BULK], [PERIODIC, PERIODIC], [REFLECT, IN_WALL]],
                            type_func=FMT3, type_attr=0, nwall=0, hs_diam=sig, sigma_ff=S, eps_ff=E,
                            cut_ff=R, rho_b=[0.3, 0.25], rho0=rho0)
+    if kind == "3d_mix_periodic":
+        n = 12
+        sp = periodic_3d(seed=seed, n=n, dx=0.25, cut=1.2, func=FMT3)
+        sig = [1.0, 0.8]
+        S, E, R = lorentz_berthelot(sig, [1.0, 1.0], [1.2, 1.0])
+        rho0 = np.stack([0.5 * sp.rho0[0], 0.4 * sp.rho0[0]])
+        return ProblemSpec(name="3d_mix_periodic", ndim=3, ncomp=2, nx=[n, n, n], dx=[0.25] * 3,
+                           bc=[[PERIODIC, PERIODIC]] * 3, type_func=FMT3, type_attr=0, nwall=0, hs_diam=sig,
+                           sigma_ff=S, eps_ff=E, cut_ff=R, rho_b=[0.3, 0.24], rho0=rho0)
     if kind == "2d_rough_wall":
         return rough_wall_2d(seed=seed, nx=40, ny=32, dx=0.1, cut=1.5, wall_cut=2.0)
     if kind == "3d_solute_small":
