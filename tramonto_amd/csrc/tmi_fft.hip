@@ -360,6 +360,26 @@ __global__ void k_gather_pairs(FftGeo f, Geo g, const T *__restrict__ pad, int n
     }
 }
 
+// Z_j = rho_2j + i rho_2j+1 from the node arrays (0 where Zero_density_TF) on a
+// wrapping FFT grid (grid point = node): the density halo is never read there
+template <typename T>
+__global__ void k_gather_nodes(FftGeo f, Geo g, const double *__restrict__ in, const unsigned char *__restrict__ zero,
+                               Cx<T> *__restrict__ Z)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.N) return;
+    const int x = i % g.nx[0];
+    const int r = i / g.nx[0];
+    const int y = r % g.nx[1], z = r / g.nx[1];
+    const long long q = x + y * f.S1 + z * f.S2;
+    for (int j = 0; 2 * j < g.C; ++j) {
+        const size_t a = (size_t)(2 * j) * g.N + i, b = a + g.N;
+        const T re = zero[a] ? (T)0 : (T)in[a];
+        const T im = (2 * j + 1 < g.C && !zero[b]) ? (T)in[b] : (T)0;
+        Z[(size_t)j * f.ns + q] = Cx<T>{re, im};
+    }
+}
+
 // Products of the density spectra (pairs Z) with the stencil spectra, the
 // outputs packed in pairs O_j = Y_2j + i Y_2j+1
 template <typename T, int ND>
@@ -926,6 +946,14 @@ void fft_free(Problem &P)
     P.fft = nullptr;
 }
 
+FftPairs fft_pairs(const Problem &P)
+{
+    return FftPairs{P.fft->zbuf, P.fft->ns, P.fft->S1, P.fft->S2};
+}
+bool fft_node_input(const Problem &P)
+{
+    return P.fft && P.fft->fmt && P.fft->wraps && (P.nmf == 0 || P.fft->mf);
+}
 bool fft_mf(const Problem &P) { return P.fft && P.fft->mf; }
 bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
@@ -939,8 +967,8 @@ void fft_set_fmt(Problem &P, bool on)
 // with_rb: the rho-bar rows (out_rb = sums - in_rb - sub_rb), then the
 // mean-field rows when they are in FFT form; else the mean-field rows only.
 template <typename T>
-void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_rb, double *out_rb,
-               const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s)
+void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *in_rb, const double *sub_rb,
+               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s)
 {
     const FftPlan &F = *P.fft;
     const int w = with_rb ? 1 : 0;
@@ -952,7 +980,11 @@ void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_
     Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gu = make_geo(P, P.pad_u);
     const FftGeo fg = fft_geo(F, 0);
-    hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, workspace<T>(P).rho_pad, C, Z);
+    if (rho_nodes)
+        hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
+    else
+        hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, workspace<T>(P).rho_pad,
+                           C, Z);
     fft3d<T>(F, Z, (C + 1) / 2, -1, s);
     if (P.p.ndim == 2)
         hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
@@ -965,12 +997,13 @@ void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_
                        sub_rb, out_rb, in_mf, sub_mf, out_mf);
     TMI_HIP_CHECK(hipGetLastError());
 }
-template void fft_rb_mf<double>(Problem &, bool, const double *, const double *, double *, const double *,
-                                const double *, double *, hipStream_t);
-template void fft_rb_mf<float>(Problem &, bool, const double *, const double *, double *, const double *,
-                               const double *, double *, hipStream_t);
+template void fft_rb_mf<double>(Problem &, const double *, bool, const double *, const double *, double *,
+                                const double *, const double *, double *, hipStream_t);
+template void fft_rb_mf<float>(Problem &, const double *, bool, const double *, const double *, double *,
+                               const double *, const double *, double *, hipStream_t);
 
-// Euler-Lagrange rows from the adjoint fields in workspace<T>(P).adj_pad (as el_rows)
+// Euler-Lagrange rows from the adjoint fields (as el_rows): in workspace<T>(P).adj_pad,
+// or, on wrapping grids, already paired in the transform buffer (fft_pairs)
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
             double *out_el, hipStream_t s)
@@ -981,8 +1014,9 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
     Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gf = make_geo(P, P.pad_f);
     const FftGeo fg = fft_geo(F, 1);
-    hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gf, workspace<T>(P).adj_pad, nadj,
-                       Z);
+    if (!F.wraps)   // on wrapping grids fmt_point wrote the pairs (fft_pairs)
+        hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gf, workspace<T>(P).adj_pad,
+                           nadj, Z);
     fft3d<T>(F, Z, (nadj + 1) / 2, -1, s);
     if (nd == 2)
         hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);

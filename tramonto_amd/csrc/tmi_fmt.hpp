@@ -27,7 +27,7 @@ struct FmtScalars {
     bool cut;    // White Bear below the n3 cut-off
 };
 
-__host__ __device__ inline FmtScalars fmt_scalars(int func, double n3)
+__host__ __device__ __attribute__((always_inline)) inline FmtScalars fmt_scalars(int func, double n3)
 {
     FmtScalars s;
     const double one_m = 1.0 - n3;
@@ -60,7 +60,7 @@ __host__ __device__ inline FmtScalars fmt_scalars(int func, double n3)
 
 // dPhi/dn in n order
 template <int ND>
-__host__ __device__ inline void fmt_dphi(int func, const double *n, double *out)
+__host__ __device__ __attribute__((always_inline)) inline void fmt_dphi(int func, const double *n, double *out)
 {
     constexpr double inv_pi = 0.318309886183790671537767526745028724;
     const FmtScalars s = fmt_scalars(func, n[3]);
@@ -71,15 +71,13 @@ __host__ __device__ inline void fmt_dphi(int func, const double *n, double *out)
         d22 += n[4 + ND + d] * n[4 + ND + d];
     }
     const double a = s.a, a2 = a * a;
+    // every element is written once, selects instead of branches: the array stays in registers
+    const double o2 = n[1] * a + (n[2] * n[2] - d22) * s.f0 * (inv_pi / 12.0);
+    const double o3 = n[0] * a + (n[1] * n[2] - d12) * a2 + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f1 * (inv_pi / 36.0);
     out[0] = s.lnA;
     out[1] = n[2] * a;
-    if (s.cut) {
-        out[2] = 0.0;
-        out[3] = 0.0;
-    } else {
-        out[2] = n[1] * a + (n[2] * n[2] - d22) * s.f0 * (inv_pi / 12.0);
-        out[3] = n[0] * a + (n[1] * n[2] - d12) * a2 + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f1 * (inv_pi / 36.0);
-    }
+    out[2] = s.cut ? 0.0 : o2;
+    out[3] = s.cut ? 0.0 : o3;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         out[4 + d] = -n[4 + ND + d] * a;
@@ -89,15 +87,11 @@ __host__ __device__ inline void fmt_dphi(int func, const double *n, double *out)
 
 // out = H(n) dn (exact Hessian of Phi; zero below the White Bear cut-off)
 template <int ND>
-__host__ __device__ inline void fmt_hvp(int func, const double *n, const double *dn, double *out)
+__host__ __device__ __attribute__((always_inline)) inline void fmt_hvp(int func, const double *n, const double *dn, double *out)
 {
     constexpr double inv_pi = 0.318309886183790671537767526745028724;
     const FmtScalars s = fmt_scalars(func, n[3]);
-    if (s.cut) {
-#pragma unroll
-        for (int q = 0; q < 4 + 2 * ND; ++q) out[q] = 0.0;
-        return;
-    }
+    const bool z = s.cut;   // every element is written once, through selects (no early return)
     double d12 = 0.0, d22 = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -119,13 +113,15 @@ __host__ __device__ inline void fmt_hvp(int func, const double *n, const double 
         const double dv1 = dn[4 + d], dv2 = dn[4 + ND + d];
         o2 += -v2 * c2v * dv2;
         o3 += -v2 * a2 * dv1 + (-v1 * a2 - n2 * v2 * c3v) * dv2;
-        out[4 + d] = -v2 * a2 * dn[3] - a * dv2;
-        out[4 + ND + d] = -v2 * c2v * dn[2] + (-v1 * a2 - n2 * v2 * c3v) * dn[3] - a * dv1 - n2 * c2v * dv2;
+        const double ov1 = -v2 * a2 * dn[3] - a * dv2;
+        const double ov2 = -v2 * c2v * dn[2] + (-v1 * a2 - n2 * v2 * c3v) * dn[3] - a * dv1 - n2 * c2v * dv2;
+        out[4 + d] = z ? 0.0 : ov1;
+        out[4 + ND + d] = z ? 0.0 : ov2;
     }
-    out[0] = a * dn[3];
-    out[1] = a * dn[2] + n2 * a2 * dn[3];
-    out[2] = o2;
-    out[3] = o3;
+    out[0] = z ? 0.0 : a * dn[3];
+    out[1] = z ? 0.0 : a * dn[2] + n2 * a2 * dn[3];
+    out[2] = z ? 0.0 : o2;
+    out[3] = z ? 0.0 : o3;
 }
 
 }  // namespace tmi

@@ -197,9 +197,17 @@ bool fft_fmt(const Problem &P);             // FMT sums (rho-bar, Euler-Lagrange
 bool fft_fmt_available(const Problem &P);   // FMT spectra built
 bool fft_wraps(const Problem &P);           // every axis periodic on the FFT grid (halos unused)
 void fft_set_fmt(Problem &P, bool on);
+// density products; rho_nodes (node arrays, fft_node_input) or, if null, workspace<T>(P).rho_pad
 template <typename T>
-void fft_rb_mf(Problem &P, bool with_rb, const double *in_rb, const double *sub_rb, double *out_rb,
-               const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s);
+void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *in_rb, const double *sub_rb,
+               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s);
+// pair buffer of a wrapping FFT grid: field f of node (x, y, z) at element 2 ((f / 2) ns + x + y S1 + z S2) + f % 2
+struct FftPairs {
+    void *buf;
+    long long ns, S1, S2;
+};
+FftPairs fft_pairs(const Problem &P);
+bool fft_node_input(const Problem &P);   // every density product reads the node arrays (no density halo)
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
             double *out_el, hipStream_t s);

@@ -59,10 +59,11 @@ __global__ void k_pad_rho(Geo g, const double *__restrict__ in, const unsigned c
 }
 
 // pointwise FMT: residual (dphi) or linear (H dn); writes the adjoint fields
-// at the interior positions of the FMT-halo geometry
+// at the interior positions of the FMT-halo geometry, or (pairs.buf set) as
+// pairs of the transform buffer of a wrapping FFT grid
 template <typename T, int ND>
 __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, const double *__restrict__ dnrb, int linear,
-                            CompConsts cc, T *__restrict__ adj)
+                            CompConsts cc, T *__restrict__ adj, FftPairs pairs)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.N) return;
@@ -97,15 +98,36 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
         }
         fmt_hvp<ND>(func, n, dn, out);
     }
-    const long long p = pad_pos(g, i);
     const int C = g.C;
-    for (int c = 0; c < C; ++c) {
-        adj[(size_t)(c * (1 + ND)) * g.np + p] = (T)(out[0] * cc.i4pirsq[c] + out[1] * cc.i4pir[c] + out[2]);
-#pragma unroll
-        for (int d = 0; d < ND; ++d)
-            adj[(size_t)(c * (1 + ND) + 1 + d) * g.np + p] = (T)(out[4 + d] * cc.i4pir[c] + out[4 + ND + d]);
+    long long p, fstride;
+    T *dst;
+    if (pairs.buf) {   // field f at 2 ((f / 2) ns + q) + f % 2
+        const int x = i % g.nx[0];
+        const int r = i / g.nx[0];
+        p = 2 * ((long long)x + (r % g.nx[1]) * pairs.S1 + (r / g.nx[1]) * pairs.S2);
+        fstride = pairs.ns;
+        dst = reinterpret_cast<T *>(pairs.buf);
+    } else {
+        p = pad_pos(g, i);
+        fstride = g.np;
+        dst = adj;
     }
-    adj[(size_t)(C * (1 + ND)) * g.np + p] = (T)out[3];
+    auto put = [&](int f, T v) {
+        if (pairs.buf)
+            dst[(size_t)(f >> 1) * 2 * fstride + p + (f & 1)] = v;
+        else
+            dst[(size_t)f * fstride + p] = v;
+    };
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {   // compile-time component index: the constants stay in registers
+        if (c < C) {
+            put(c * (1 + ND), (T)(out[0] * cc.i4pirsq[c] + out[1] * cc.i4pir[c] + out[2]));
+#pragma unroll
+            for (int d = 0; d < ND; ++d) put(c * (1 + ND) + 1 + d, (T)(out[4 + d] * cc.i4pir[c] + out[4 + ND + d]));
+        }
+    }
+    put(C * (1 + ND), (T)out[3]);
+    if (pairs.buf && ((C * (1 + ND) + 1) & 1)) put(C * (1 + ND) + 1, (T)0);   // odd field count: empty partner
 }
 
 template <typename T, int ND>
@@ -1184,7 +1206,7 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
 {
     if (P.nmf == 0) return;
     if (fft_mf(P)) {
-        fft_rb_mf<T>(P, false, nullptr, nullptr, nullptr, in_mf, sub, out_mf, s);
+        fft_rb_mf<T>(P, nullptr, false, nullptr, nullptr, nullptr, in_mf, sub, out_mf, s);
         return;
     }
     if (P.use_rows1d) {
@@ -1206,21 +1228,27 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
                        workspace<T>(P).rho_pad, in_mf, sub, out_mf);
 }
 
-// rho-bar rows and mean-field rows from the same padded densities: one
-// forward transform serves both when the FMT sums are in FFT form.  ev
-// (optional) is recorded between the two.
+// rho-bar rows and mean-field rows of the densities rho (node arrays [C][N];
+// residual: bulk halo, linear: zero halo): one forward transform serves both
+// when the FMT sums are in FFT form, and on wrapping grids it reads the node
+// arrays without a density halo.  ev_pad and ev_mid (optional) are recorded
+// after the density halo and between the two row blocks.
 template <typename T>
-void rb_mf_rows(Problem &P, const double *in_rb, const double *sub_rb, double *out_rb, const double *in_mf,
-                const double *sub_mf, double *out_mf, hipStream_t s, hipEvent_t ev = nullptr)
+void rb_mf_rows(Problem &P, const double *rho, bool linear, const double *in_rb, const double *sub_rb, double *out_rb,
+                const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s, hipEvent_t ev_pad = nullptr,
+                hipEvent_t ev_mid = nullptr)
 {
+    const bool nodes = fft_node_input(P);
+    if (!nodes) pad_rho<T>(P, rho, linear, s);
+    if (ev_pad) TMI_HIP_CHECK(hipEventRecord(ev_pad, s));
     if (fft_fmt(P)) {
-        fft_rb_mf<T>(P, true, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf, s);
-        if (ev) TMI_HIP_CHECK(hipEventRecord(ev, s));
+        fft_rb_mf<T>(P, nodes ? rho : nullptr, true, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf, s);
+        if (ev_mid) TMI_HIP_CHECK(hipEventRecord(ev_mid, s));
         if (!fft_mf(P)) mf_rows<T>(P, in_mf, sub_mf, out_mf, s);
         return;
     }
     rb_rows<T>(P, in_rb, sub_rb, out_rb, s);
-    if (ev) TMI_HIP_CHECK(hipEventRecord(ev, s));
+    if (ev_mid) TMI_HIP_CHECK(hipEventRecord(ev_mid, s));
     mf_rows<T>(P, in_mf, sub_mf, out_mf, s);
 }
 
@@ -1232,24 +1260,26 @@ void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear,
     T *adj = workspace<T>(P).adj_pad;
     const int lin = linear ? 1 : 0;
     const int func = P.p.type_func;
-    if (fft_fmt(P) && fft_wraps(P)) {   // the FFT form reads the interior only
+    if (fft_fmt(P) && fft_wraps(P)) {   // straight into the transform buffer as pairs: no halo
+        const FftPairs fp = fft_pairs(P);
         if (P.p.ndim == 2)
-            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp);
         else
-            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp);
         return;
     }
+    const FftPairs nop{nullptr, 0, 0, 0};
     switch (P.p.ndim) {
     case 1:
-        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
         hipLaunchKernelGGL((k_pad_adj<T, 1>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
         break;
     case 2:
-        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
         hipLaunchKernelGGL((k_pad_adj<T, 2>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
         break;
     default:
-        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
         hipLaunchKernelGGL((k_pad_adj<T, 3>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
         break;
     }
@@ -1430,10 +1460,9 @@ void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipE
 {
     const size_t N = P.N;
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
-    pad_rho<double>(P, x + P.iu_rho * N, false, s);
-    if (ev) TMI_HIP_CHECK(hipEventRecord(ev[1], s));
-    rb_mf_rows<double>(P, x + P.iu_rb * N, nullptr, F + P.iu_rb * N, P.nmf ? x + P.iu_mf * N : nullptr, nullptr,
-                       F + P.iu_mf * N, s, ev ? ev[2] : nullptr);
+    rb_mf_rows<double>(P, x + P.iu_rho * N, false, x + P.iu_rb * N, nullptr, F + P.iu_rb * N,
+                       P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s, ev ? ev[1] : nullptr,
+                       ev ? ev[2] : nullptr);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
@@ -1447,9 +1476,8 @@ template <typename T>
 void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s)
 {
     const size_t N = P.N;
-    pad_rho<T>(P, v + P.iu_rho * N, true, s);
-    rb_mf_rows<T>(P, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr,
-                  Jv + P.iu_mf * N, s);
+    rb_mf_rows<T>(P, v + P.iu_rho * N, true, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N,
+                  P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_mf * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, v + P.iu_rb * N, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, v + P.iu_rho * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_rho * N,
                s);
@@ -1461,8 +1489,8 @@ template void launch_jvp<float>(Problem &, const double *, const double *, doubl
 void launch_init_fields(Problem &P, double *x, hipStream_t s)
 {
     const size_t N = P.N;
-    pad_rho<double>(P, x + P.iu_rho * N, false, s);
-    rb_mf_rows<double>(P, nullptr, nullptr, x + P.iu_rb * N, nullptr, nullptr, x + P.iu_mf * N, s);
+    rb_mf_rows<double>(P, x + P.iu_rho * N, false, nullptr, nullptr, x + P.iu_rb * N, nullptr, nullptr,
+                       x + P.iu_mf * N, s);
     hipLaunchKernelGGL(k_cap_n3, dim3(grid_for(P.N)), dim3(kBlock), 0, s, x + P.iu_rb * N, P.N);
     TMI_HIP_CHECK(hipGetLastError());
 }
@@ -1472,8 +1500,7 @@ template <typename T>
 void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t1, double *y2, hipStream_t s)
 {
     const size_t N = P.N;
-    pad_rho<T>(P, x2, true, s);
-    rb_mf_rows<T>(P, nullptr, nullptr, t1, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
+    rb_mf_rows<T>(P, x2, true, nullptr, nullptr, t1, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, t1, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, x2, P.nmf ? t1 + (size_t)P.nrb * N : nullptr, nullptr, y2, s);
     TMI_HIP_CHECK(hipGetLastError());
@@ -1495,9 +1522,8 @@ void launch_schur_backsub(Problem &P, const double *x, const double *b, double *
 {
     (void)x;
     const size_t N = P.N;
-    pad_rho<double>(P, dx + P.iu_rho * N, true, s);
-    rb_mf_rows<double>(P, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, nullptr, P.nmf ? b + P.iu_mf * N : nullptr,
-                       P.nmf ? dx + P.iu_mf * N : nullptr, s);
+    rb_mf_rows<double>(P, dx + P.iu_rho * N, true, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, nullptr,
+                       P.nmf ? b + P.iu_mf * N : nullptr, P.nmf ? dx + P.iu_mf * N : nullptr, s);
     TMI_HIP_CHECK(hipGetLastError());
 }
 
