@@ -11,9 +11,13 @@ Jacobian-vector product throughput (FP64 and FP32), the Newton
 time-to-solve to ||F||_2 < 1e-8 for the plain FP64 path and the mixed path
 (FP32 inner Krylov products, FP64 residual/update), the per-stage kernel
 times of one fill measured with HIP events, the roofline of the dominant
-kernel, and the CPU baseline (the oracle's OpenMP fill on the host cores).
+kernel (the FFT axis passes when the stencil sums run in FFT form, else the
+direct stencil sum), and the CPU baseline (the oracle's OpenMP fill on the
+host cores), followed by the batched slit-pore, 2D adjoint and (--solute)
+128^3 solute workloads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-newton] [--no-cpu]
+                       [--no-batch] [--no-adjoint] [--solute] [--fill-only] [--copy-probe]
 """
 import argparse
 import json
@@ -35,9 +39,10 @@ COPY_PROBE_BYTES = 512 << 20
 # the FMT reach touches 3.1 MiB) and writes ten rows (20 MiB).
 STAGE_TRAFFIC_MIB_64 = {"el_rows": 33.98 + 2.00, "rb_rows": 26.52 + 20.00}
 STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
-# Mean HBM bytes per k_fft_axis launch of one 64^3 fill in FFT form (fetch + write, MiB), from the same
-# counter runs; None until measured.
-FFT_TRAFFIC_MIB_64 = None
+# Mean HBM bytes per k_fft_axis launch of one 64^3 fill in FFT form (fetch + write, MiB): counter runs of
+# this script with --fill-only --steps 50 --warmup 2 --copy-probe under TMI_FMT_FFT=1, mean over the last
+# 720 axis-pass dispatches (60 fills' worth), scaled as above.  Algorithmic: 16 MiB per launch.
+FFT_TRAFFIC_MIB_64 = 8.121 + 8.000
 
 
 def stencil_flops_per_node(dp, spec):
@@ -230,6 +235,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched slit-pore workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D rough-wall adjoint workload")
     ap.add_argument("--solute", action="store_true", help="also run the 128^3 three-component solute workload")
+    ap.add_argument("--fill-only", action="store_true",
+                    help="only the fill throughput, its stages and the roofline (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 512 MiB device copy three times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -281,7 +288,7 @@ def main():
     v = torch.rand_like(x0)
     Jv = dp.zeros()
     jvp = {}
-    for fp32 in (False, True):
+    for fp32 in (() if args.fill_only else (False, True)):
         for _ in range(args.warmup):
             dp.apply_jacobian(x0, v, Jv, fp32=fp32)
         torch.cuda.synchronize()
@@ -334,7 +341,7 @@ def main():
 
     # ---------------------------------------------------------------- Newton time-to-solve
     newton = {}
-    if not args.no_newton:
+    if not (args.no_newton or args.fill_only):
         for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
             xw = x0.clone()
             dp.newton(xw, max_iter=1, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)   # allocate scratch
@@ -351,15 +358,15 @@ def main():
         del newton["fp64_x"], newton["mixed_x"]
 
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.fill_only):
         cpu = cpu_baseline(spec, x0.cpu().numpy())
 
     extra = {}
-    if not args.no_batch:
+    if not (args.no_batch or args.fill_only):
         extra["batched_pores"] = bench_batched_pores()
-    if not args.no_adjoint:
+    if not (args.no_adjoint or args.fill_only):
         extra["adjoint_2d"] = bench_adjoint_2d()
-    if args.solute:
+    if args.solute and not args.fill_only:
         extra["solute_3d"] = bench_solute_3d()
 
     out = {
