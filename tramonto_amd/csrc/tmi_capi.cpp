@@ -564,7 +564,9 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     const double *x_rho = x + (size_t)P.iu_rho * P.N;
     if (!bulk_pc) dev_a22_inverse_diag(P, x, dinv, s);
     TMI_HIP_CHECK(hipMemsetAsync(x2, 0, sizeof(double) * n, s));
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1), h2(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
+    // device slots of the Arnoldi coefficients (past the partial sums of the dot products)
+    double *dh = P.d_red + 1024 * 1024 / 2, *dh2 = dh + 512, *dhn = dh + 1024;
     const double bn0 = std::sqrt(dev_dot(P, bS, bS, n, s));
     const double bn = bn0 > 0.0 ? bn0 : 1.0;
     int its = 0;
@@ -598,19 +600,22 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
                 launch_schur_apply<float>(P, x, z, t1, w, s);
             else
                 launch_schur_apply<double>(P, x, z, t1, w, s);
-            dev_multi_dot(P, V, n, k + 1, w, n, h.data(), s);
-            TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, h.data(), sizeof(double) * (k + 1),
-                                         hipMemcpyHostToDevice, s));
-            dev_gemv_sub(V, n, k + 1, P.d_red + 1024 * 1024 / 2, w, n, s);
-            dev_multi_dot(P, V, n, k + 1, w, n, h2.data(), s);
-            TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, h2.data(), sizeof(double) * (k + 1),
-                                         hipMemcpyHostToDevice, s));
-            dev_gemv_sub(V, n, k + 1, P.d_red + 1024 * 1024 / 2, w, n, s);
-            for (int j = 0; j <= k; ++j) h[j] += h2[j];
-            const double hn = std::sqrt(dev_dot(P, w, w, n, s));
+            // classical Gram-Schmidt and one reorthogonalisation pass with the coefficients kept
+            // on the device; one host synchronisation per step, for the Hessenberg column
+            dev_multi_dot_async(P, V, n, k + 1, w, n, dh, s);
+            dev_gemv_sub(V, n, k + 1, dh, w, n, s);
+            dev_multi_dot_async(P, V, n, k + 1, w, n, dh2, s);
+            dev_gemv_sub(V, n, k + 1, dh2, w, n, s);
+            dev_multi_dot_async(P, w, 0, 1, w, n, dhn, s);
+            dev_scale_by_norm(dhn, w, V + (size_t)(k + 1) * n, n, s);
+            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, dh, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red + 512, dh2, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red + 1023, dhn, sizeof(double), hipMemcpyDeviceToHost, s));
+            TMI_HIP_CHECK(hipStreamSynchronize(s));
+            for (int j = 0; j <= k; ++j) h[j] = P.h_red[j] + P.h_red[512 + j];
+            const double hn = std::sqrt(P.h_red[1023]);
             for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = h[j];
             H[(size_t)(k + 1) * m + k] = hn;
-            if (hn > 0.0) dev_scale_copy(1.0 / hn, w, V + (size_t)(k + 1) * n, n, s);
             for (int j = 0; j < k; ++j) {
                 double a = H[(size_t)j * m + k], b = H[(size_t)(j + 1) * m + k];
                 H[(size_t)j * m + k] = cs[j] * a + sn[j] * b;

@@ -227,6 +227,10 @@ void autotune_rows1d(Problem &P);
 
 // vector helpers
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s);
+// out_d[j] = V_j . w on the device (partial sums in P.d_red[0, k*1024)), no host synchronisation
+void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out_d,
+                         hipStream_t s);
+void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s);
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);

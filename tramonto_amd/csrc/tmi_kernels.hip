@@ -1078,6 +1078,16 @@ __global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b
 }
 
 // w -= V h (k columns)
+// y = x / sqrt(*nrm2) (0 when *nrm2 is 0): the next Arnoldi vector without a host round trip
+__global__ void k_scale_by_norm(const double *__restrict__ nrm2, const double *__restrict__ x, double *__restrict__ y,
+                                long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double q = *nrm2;
+    y[i] = q > 0.0 ? x[i] * (1.0 / sqrt(q)) : 0.0;
+}
+
 __global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, const double *__restrict__ h,
                            double *__restrict__ w, long long n)
 {
@@ -1541,6 +1551,19 @@ void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const doubl
     TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, res, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     TMI_HIP_CHECK(hipStreamSynchronize(s));
     for (int j = 0; j < k; ++j) out[j] = P.h_red[j];
+}
+
+void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out_d,
+                         hipStream_t s)
+{
+    unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d);
+}
+
+void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scale_by_norm, dim3(grid_for(n)), dim3(kBlock), 0, s, nrm2_d, x, y, n);
 }
 
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s)
