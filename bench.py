@@ -207,6 +207,8 @@ def bench_solute_3d(n=128):
                        "cut-off 3 sigma_i), spherical solute radius 4 sigma with LJ 9-3 surface, eta_b = 0.35 "
                        "split by seeded Dirichlet weights, periodic box",
            "nodes": spec.nnodes, "unknowns_per_node": spec.nunk, "fill_ms": fill_ms,
+           "fmt_form": ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft],
+           "precond": ["A22 diagonal", "bulk symbol"][dp.info.precond],
            "fill_gnode_s": spec.nnodes / (fill_ms * 1e-3) / 1e9,
            "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
                                      [float(v) for v in dp.fill_residual_timed(x0, 2)]))}
@@ -392,7 +394,8 @@ def main():
         "jvp": jvp,
         "newton": newton,
         "mean_field_form": "fft" if mf_fft else "direct",
-        "fmt_form": "fft" if fmt_fft else "direct",
+        "fmt_form": ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft],
+        "precond": ["A22 diagonal", "bulk symbol"][dp.info.precond],
         "flops_per_node_direct": flops_node,
         "bytes_per_node_residual": bytes_node,
         "roofline": roofline,

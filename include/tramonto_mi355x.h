@@ -77,7 +77,8 @@ typedef struct tmi_info {
     int iu_rho, iu_rb, n_rb, iu_mf, n_mf;
     int sten_len[3][TMI_NCOMP_MAX * TMI_NCOMP_MAX];
     int mf_fft;   /* 1: the mean-field sums run as FFT convolutions (wide 2D/3D attraction stencils) */
-    int fmt_fft;  /* 1: the FMT sums (rho-bar and Euler-Lagrange rows) run as FFT convolutions (2D/3D, when faster) */
+    int fmt_fft;  /* FMT sums (rho-bar and Euler-Lagrange rows): 0 direct, 1 FFT convolutions, 2 FFT with the x pass
+                     fused with the spectral products (2D/3D, whichever fills faster) */
     int precond;  /* preconditioner of the Schur GMRES: TMI_PRECOND_DIAG or TMI_PRECOND_BULK */
 } tmi_info;
 

@@ -233,16 +233,16 @@ def fft_pair():
     import os
     from tramonto_amd.device import DeviceProblem
     cache = {}
-    forms = {"fft": ("1", "1"), "mf": ("1", "0"), "direct": ("0", "0")}
+    forms = {"fft": ("1", "1", "1"), "fft_unfused": ("1", "1", "0"), "mf": ("1", "0", "1"), "direct": ("0", "0", "1")}
 
     def get(case):
         if case not in cache:
             spec = small_case(case)
-            old = {k: os.environ.get(k) for k in ("TMI_MF_FFT", "TMI_FMT_FFT")}
+            old = {k: os.environ.get(k) for k in ("TMI_MF_FFT", "TMI_FMT_FFT", "TMI_FFT_XFUSE")}
             probs = {}
             try:
-                for name, (mf, fmt) in forms.items():
-                    os.environ["TMI_MF_FFT"], os.environ["TMI_FMT_FFT"] = mf, fmt
+                for name, (mf, fmt, fuse) in forms.items():
+                    os.environ["TMI_MF_FFT"], os.environ["TMI_FMT_FFT"], os.environ["TMI_FFT_XFUSE"] = mf, fmt, fuse
                     probs[name] = DeviceProblem(spec)
             finally:
                 for k, v in old.items():
@@ -250,15 +250,16 @@ def fft_pair():
                         os.environ.pop(k, None)
                     else:
                         os.environ[k] = v
-            for name, (mf, fmt) in forms.items():
-                assert (probs[name].info.mf_fft, probs[name].info.fmt_fft) == (int(mf), int(fmt)), name
+            for name, (mf, fmt, fuse) in forms.items():
+                want_fmt = 0 if fmt == "0" else (2 if fuse == "1" else 1)
+                assert (probs[name].info.mf_fft, probs[name].info.fmt_fft) == (int(mf), want_fmt), name
             cache[case] = (spec, Oracle.from_spec(spec), probs)
         return cache[case]
     return get
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["fft", "mf"])
+@pytest.mark.parametrize("form", ["fft", "fft_unfused", "mf"])
 @pytest.mark.parametrize("case", FFT_CASES)
 def test_fft_residual_matches_oracle(case, form, fft_pair):
     spec, o, probs = fft_pair(case)
@@ -273,7 +274,7 @@ def test_fft_residual_matches_oracle(case, form, fft_pair):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["fft", "mf"])
+@pytest.mark.parametrize("form", ["fft", "fft_unfused", "mf"])
 @pytest.mark.parametrize("case", FFT_CASES)
 def test_fft_jvp_matches_direct(case, form, fft_pair):
     spec, o, probs = fft_pair(case)

@@ -777,7 +777,7 @@ int tmi_problem_info(void *handle, tmi_info *info)
         for (int t = 0; t < 3; ++t)
             for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
         info->mf_fft = fft_mf(*P) ? 1 : 0;
-        info->fmt_fft = fft_fmt(*P) ? 1 : 0;
+        info->fmt_fft = fft_fmt(*P) ? (fft_fused(*P) ? 2 : 1) : 0;
         info->precond = precond_active(*P) ? 1 : 0;
         return 0;
     });

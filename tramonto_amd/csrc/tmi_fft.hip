@@ -68,6 +68,21 @@ struct RowMap {
     double scale[kMaxOut] = {};
 };
 
+// Terms of a fused x-axis product: output o += coef (times i when imag) K_t(k) X_f(k), K_t a real
+// spectrum table (t >= 0: FMT table t; t < 0: attraction table -t-1), grouped by input field f,
+// terms first[f] .. first[f+1]-1
+constexpr int kMaxTerms = 64;
+constexpr int kMaxIn = kNC * 4 + 1;   // adjoint fields C(1+D)+1
+struct Term {
+    int o, t, imag;
+    double coef;
+};
+struct TermList {
+    int nin = 0, nout = 0;
+    int first[kMaxIn + 2] = {};   // one past the last field is the running end while a list is built
+    Term term[kMaxTerms] = {};
+};
+
 struct FftPlan {
     int nd = 0, C = 0;
     int M[3] = {1, 1, 1};       // FFT grid
@@ -94,6 +109,9 @@ struct FftPlan {
     void *obuf = nullptr;       // products, then the output pairs
     FwdDesc fwd[2];             // [0] mean-field rows only, [1] rho-bar rows then mean-field rows
     RowMap rows[2];
+    TermList tfwd[2], tel;      // the same products as terms of the fused x pass
+    int xfit[2] = {0, 0};       // lines per workgroup of the fused x pass that fit, density / adjoint product
+    int xlpb[2] = {0, 0};       // in use (0: separate x passes and product kernel)
     // bulk-circulant Schur preconditioner (wrapping grids): Q(k) = S_b(k)^-1 diag(1/rho_b) / n
     Cx<double> *pq = nullptr;   // [C*C][ns], entry (i, j) at i + C j
     bool pc_ok = false;         // Q built
@@ -225,6 +243,30 @@ __device__ inline void stockham_stage(const Cx<T> *__restrict__ src, Cx<T> *__re
     }
 }
 
+// Every Stockham stage of nl lines of length M held in LDS (row pitch ld),
+// ping-ponging between a and b; returns the buffer holding the transform.
+template <int SIGN, typename T>
+__device__ inline Cx<T> *fft_lines(Cx<T> *a, Cx<T> *b, int ld, int nl, int M, RadixCounts rc, const Cx<T> *tw)
+{
+    int Ns = 1;
+    auto run = [&](auto radix, int count) {
+        constexpr int R = decltype(radix)::value;
+        for (int s = 0; s < count; ++s) {
+            stockham_stage<R, SIGN>(a, b, ld, nl, M, Ns, tw);
+            __syncthreads();
+            Cx<T> *tmp = a;
+            a = b;
+            b = tmp;
+            Ns *= R;
+        }
+    };
+    run(std::integral_constant<int, 4>{}, rc.n4);
+    run(std::integral_constant<int, 2>{}, rc.n2);
+    run(std::integral_constant<int, 3>{}, rc.n3);
+    run(std::integral_constant<int, 5>{}, rc.n5);
+    return a;
+}
+
 // One axis of a 3D transform of a batch of fields, in place.  A workgroup
 // takes lpb lines of the axis (consecutive lines along x for the strided
 // axes, so that the loads are coalesced), runs every Stockham stage in LDS and
@@ -282,22 +324,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
         }
     }
     __syncthreads();
-    int Ns = 1;
-    auto run = [&](auto radix, int count) {
-        constexpr int R = decltype(radix)::value;
-        for (int s = 0; s < count; ++s) {
-            stockham_stage<R, SIGN>(bufA, bufB, ld, nl, M, Ns, tw);
-            __syncthreads();
-            Cx<T> *tmp = bufA;
-            bufA = bufB;
-            bufB = tmp;
-            Ns *= R;
-        }
-    };
-    run(std::integral_constant<int, 4>{}, rc.n4);
-    run(std::integral_constant<int, 2>{}, rc.n2);
-    run(std::integral_constant<int, 3>{}, rc.n3);
-    run(std::integral_constant<int, 5>{}, rc.n5);
+    bufA = fft_lines<SIGN>(bufA, bufB, ld, nl, M, rc, tw);
     // store
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
@@ -497,6 +524,86 @@ __global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *
     }
 }
 
+// Fused x pass of a product, after the y and z passes of the inputs and before those of the
+// outputs: a workgroup takes lpb x-lines (y, z) of every input pair and their mirror lines
+// (-y, -z), transforms them along x, splits each pair into its two real fields' spectra
+// (X(k) from Z(k) and Z(-k)), accumulates the terms into the output pairs in LDS, transforms
+// those back along x and stores them.
+template <typename T>
+__global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict__ Zin, Cx<T> *__restrict__ Oout,
+                                                         long long ns, int M0, int M1, int M2, long long S1,
+                                                         long long S2, RadixCounts rc, int lpb,
+                                                         const Cx<T> *__restrict__ twtab, const T *__restrict__ tk,
+                                                         const T *__restrict__ uk, TermList tl)
+{
+    extern __shared__ char smem_raw[];
+    const int M = M0, ld = M + 1;
+    Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);
+    // lines, then their mirror lines, transformed as one batch of 2 lpb lines; scratch of the same size
+    Cx<T> *bA = tw + M, *bB = bA + (size_t)lpb * ld, *bS = bB + (size_t)lpb * ld;
+    Cx<T> *out = bS + (size_t)2 * lpb * ld;   // [output pair][lpb][ld]
+    const int nop = (tl.nout + 1) / 2, nip = (tl.nin + 1) / 2;
+    for (int j = threadIdx.x; j < M; j += kFftBlock) tw[j] = twtab[j];
+    for (int e = threadIdx.x; e < nop * lpb * ld; e += kFftBlock) out[e] = Cx<T>{(T)0, (T)0};
+    const int l0 = blockIdx.x * lpb;
+    const int nl = min(lpb, M1 * M2 - l0);
+    const float inv_m = 1.0f / (float)M, inv_m1 = 1.0f / (float)M1;
+    auto yz = [&](int i, int &y, int &z) {
+        const int l = l0 + i;
+        z = small_div(l, M1, inv_m1);
+        y = l - z * M1;
+    };
+    for (int jp = 0; jp < nip; ++jp) {
+        const Cx<T> *src = Zin + (size_t)jp * ns;
+        __syncthreads();
+        for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
+            const int i = small_div(e, M, inv_m), j = e - i * M;
+            int y, z;
+            yz(i, y, z);
+            const int ym = y ? M1 - y : 0, zm = z ? M2 - z : 0;
+            bA[i * ld + j] = src[y * S1 + z * S2 + j];
+            bB[i * ld + j] = src[ym * S1 + zm * S2 + j];
+        }
+        __syncthreads();
+        // all 2 lpb lines (rows past nl hold stale values that are never read)
+        const Cx<T> *fa = fft_lines<-1>(bA, bS, ld, 2 * lpb, M, rc, tw);
+        const Cx<T> *fb = fa + (size_t)lpb * ld;
+        for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
+            const int i = small_div(e, M, inv_m), k = e - i * M;
+            int y, z;
+            yz(i, y, z);
+            const long long q = k + y * S1 + z * S2;
+            const Cx<T> a = fa[i * ld + k], b = fb[i * ld + (k ? M - k : 0)];
+            for (int h = 0; h < 2; ++h) {
+                const int f = 2 * jp + h;
+                if (f >= tl.nin) break;
+                const Cx<T> X = unpack(a, b, h);
+                for (int t = tl.first[f]; t < tl.first[f + 1]; ++t) {
+                    const Term tm = tl.term[t];
+                    const T kv = (T)tm.coef * (tm.t >= 0 ? tk[(size_t)tm.t * ns + q] : uk[(size_t)(-tm.t - 1) * ns + q]);
+                    const Cx<T> yv = tm.imag ? Cx<T>{-kv * X.y, kv * X.x} : Cx<T>{kv * X.x, kv * X.y};
+                    Cx<T> &acc = out[((size_t)(tm.o >> 1) * lpb + i) * ld + k];
+                    acc = (tm.o & 1) ? Cx<T>{acc.x - yv.y, acc.y + yv.x} : cadd(acc, yv);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // output pairs back along x, up to 4 pairs (4 lpb lines, the size of the input region) at a time
+    for (int p0 = 0; p0 < nop; p0 += 4) {
+        const int np = min(4, nop - p0);
+        const Cx<T> *r = fft_lines<1>(out + (size_t)p0 * lpb * ld, bA, ld, np * lpb, M, rc, tw);
+        for (int e = threadIdx.x; e < np * nl * M; e += kFftBlock) {
+            const int pl = small_div(e, nl * M, 1.0f / (float)(nl * M)), r2 = e - pl * nl * M;
+            const int i = small_div(r2, M, inv_m), j = r2 - i * M;
+            int y, z;
+            yz(i, y, z);
+            Oout[(size_t)(p0 + pl) * ns + y * S1 + z * S2 + j] = r[((size_t)pl * lpb + i) * ld + j];
+        }
+        __syncthreads();
+    }
+}
+
 // Euler-Lagrange rows from the sums E_c, as k_el_rows:
 //  residual: E + ln x + mu [+ V_ext] [+ x_MF]      ; zero rows: x   (in_rho)
 //  linear  : E + v/x + v_MF                        ; zero rows: v
@@ -665,37 +772,68 @@ inline const Cx<float> *twiddles<float>(const FftPlan &F, int axis)
 // the pass has about kFftGroups workgroups (the whole chip busy).
 constexpr long long kFftGroups = 1024;
 
-// in-place transform of nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled)
+// one axis pass over nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled)
+template <typename T>
+void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s)
+{
+    const int M = F.M[axis];
+    if (M == 1 || nfield <= 0) return;
+    const long long lines = F.n / M;
+    const long long lds_max = (long long)((kFftLds / sizeof(Cx<T>) - M) / (2 * (M + 1)));
+    long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines * nfield / kFftGroups));
+    lpb = std::min<long long>(lpb, 64);
+    if (axis > 0) {   // strided axes: whole 128-byte lines of consecutive x when the budget allows
+        const long long line_elems = 128 / (long long)sizeof(Cx<T>);
+        if (lds_max >= line_elems) lpb = std::max(line_elems, (lpb / line_elems) * line_elems);
+    }
+    long long groups;
+    if (axis == 0)
+        groups = (lines + lpb - 1) / lpb;
+    else
+        groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
+    const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
+    const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
+    const dim3 grid((unsigned)groups, (unsigned)nfield);
+    if (sign < 0)
+        hipLaunchKernelGGL((k_fft_axis<T, -1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1,
+                           F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+    else
+        hipLaunchKernelGGL((k_fft_axis<T, 1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1,
+                           F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+}
+
+// in-place 3D transform of nfield fields
 template <typename T>
 void fft3d(const FftPlan &F, Cx<T> *data, int nfield, int sign, hipStream_t s)
 {
-    if (nfield <= 0) return;
-    for (int axis = 0; axis < 3; ++axis) {
-        const int M = F.M[axis];
-        if (M == 1) continue;
-        const long long lines = F.n / M;
-        const long long lds_max = (long long)((kFftLds / sizeof(Cx<T>) - M) / (2 * (M + 1)));
-        long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines * nfield / kFftGroups));
-        lpb = std::min<long long>(lpb, 64);
-        if (axis > 0) {   // strided axes: whole 128-byte lines of consecutive x when the budget allows
-            const long long line_elems = 128 / (long long)sizeof(Cx<T>);
-            if (lds_max >= line_elems) lpb = std::max(line_elems, (lpb / line_elems) * line_elems);
-        }
-        long long groups;
-        if (axis == 0)
-            groups = (lines + lpb - 1) / lpb;
-        else
-            groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
-        const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
-        const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
-        const dim3 grid((unsigned)groups, (unsigned)nfield);
-        if (sign < 0)
-            hipLaunchKernelGGL((k_fft_axis<T, -1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2],
-                               F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
-        else
-            hipLaunchKernelGGL((k_fft_axis<T, 1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2],
-                               F.S1, F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
-    }
+    for (int axis = 0; axis < 3; ++axis) fft_pass<T>(F, data, nfield, sign, axis, s);
+}
+
+// LDS budget of the fused x pass
+constexpr size_t kXprodLds = 64 * 1024;
+
+// Lines per workgroup of the fused x pass for nop output pairs: the largest power of two up to 8
+// whose LDS fits kXprodLds and that leaves about kFftGroups workgroups; 0 when one line does not fit.
+int xprod_lpb(const FftPlan &F, int nop)
+{
+    const size_t M = (size_t)F.M[0], per_line = (4 + (size_t)nop) * (M + 1) * sizeof(Cx<double>);
+    const long long lines = F.n / F.M[0];
+    int lpb = 8;
+    while (lpb > 1 && (M * sizeof(Cx<double>) + lpb * per_line > kXprodLds || lines / lpb < kFftGroups)) lpb /= 2;
+    return (M * sizeof(Cx<double>) + lpb * per_line <= kXprodLds) ? lpb : 0;
+}
+
+// product of input pairs Z (y and z already transformed) into output pairs O (y and z still to invert)
+template <typename T>
+void fft_xprod(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, int lpb, const T *tk, const T *uk,
+               hipStream_t s)
+{
+    const int M = F.M[0], nop = (tl.nout + 1) / 2;
+    const long long lines = F.n / M;
+    const RadixCounts rc{F.rc[0][0], F.rc[0][1], F.rc[0][2], F.rc[0][3]};
+    const size_t lds = sizeof(Cx<T>) * ((size_t)M + (4 + (size_t)nop) * lpb * (M + 1));
+    hipLaunchKernelGGL(k_fft_xprod<T>, dim3((unsigned)((lines + lpb - 1) / lpb)), dim3(kFftBlock), lds, s, Z, O, F.ns,
+                       F.M[0], F.M[1], F.M[2], F.S1, F.S2, rc, lpb, twiddles<T>(F, 0), tk, uk, tl);
 }
 
 FftGeo fft_geo(const FftPlan &F, int which)
@@ -861,6 +999,49 @@ void fft_setup(Problem &P, bool mf, bool fmt)
                 r1.scale[r1.nrb + r1.nmf++] = 1.0;
             }
         }
+        // the same products as term lists of the fused x pass
+        // the current field nin owns terms first[nin] .. first[nin+1]-1; first[nin+1] is the running end
+        auto add_term = [&](TermList &tl, int o, int t, int imag, double coef) {
+            if (tl.first[tl.nin + 1] >= kMaxTerms) throw std::logic_error("too many product terms");
+            tl.term[tl.first[tl.nin + 1]++] = Term{o, t, imag, coef};
+        };
+        auto close_input = [&](TermList &tl) {   // the next input field starts
+            if (tl.nin + 2 > kMaxIn + 1) throw std::logic_error("too many product inputs");
+            ++tl.nin;
+            tl.first[tl.nin + 1] = tl.first[tl.nin];
+        };
+        for (int w = 0; w < 2; ++w) {
+            const FwdDesc &d = F->fwd[w];
+            TermList &tl = F->tfwd[w];
+            tl.nout = d.nout;
+            for (int c = 0; c < C; ++c) {
+                for (int o = 0; o < d.nout; ++o) {
+                    const int kind = d.kind[o];
+                    if (kind < kKindMf) {
+                        if (d.coef[o][c] != 0.0) add_term(tl, o, c * (2 + ND) + kind, kind >= 2 ? 1 : 0, d.coef[o][c]);
+                    } else {
+                        add_term(tl, o, -((kind - kKindMf) + C * c) - 1, 0, 1.0);
+                    }
+                }
+                close_input(tl);
+            }
+        }
+        {
+            TermList &tl = F->tel;
+            tl.nout = C;
+            for (int c = 0; c < C; ++c) {
+                add_term(tl, c, c * (2 + ND) + 1, 0, 1.0);   // A_c: delta
+                close_input(tl);
+                for (int d = 0; d < ND; ++d) {
+                    add_term(tl, c, c * (2 + ND) + 2 + d, 1, -1.0);   // B_c,d: -(i K'_d)
+                    close_input(tl);
+                }
+            }
+            for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND), 0, 1.0);   // S3: theta of every component
+            close_input(tl);
+        }
+        F->xfit[0] = F->xlpb[0] = xprod_lpb(*F, std::max((F->fwd[0].nout + 1) / 2, (F->fwd[1].nout + 1) / 2));
+        F->xfit[1] = F->xlpb[1] = xprod_lpb(*F, (C + 1) / 2);
         const int zpairs = std::max((C + 1) / 2, fmt ? (nadj + 1) / 2 : 0);
         const int opairs = std::max((f0.nout + 1) / 2, fmt ? std::max((f1.nout + 1) / 2, (C + 1) / 2) : 0);
         TMI_HIP_CHECK(hipMalloc(&F->zbuf, (size_t)zpairs * ns * sizeof(Cx<double>)));
@@ -956,8 +1137,15 @@ bool fft_node_input(const Problem &P)
 }
 bool fft_mf(const Problem &P) { return P.fft && P.fft->mf; }
 bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
+bool fft_fused(const Problem &P) { return P.fft && (P.fft->xlpb[0] || P.fft->xlpb[1]); }
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
 bool fft_wraps(const Problem &P) { return P.fft && P.fft->wraps; }
+bool fft_fused_available(const Problem &P) { return P.fft && (P.fft->xfit[0] || P.fft->xfit[1]); }
+void fft_set_fused(Problem &P, bool on)
+{
+    if (!P.fft) return;
+    for (int w = 0; w < 2; ++w) P.fft->xlpb[w] = on ? P.fft->xfit[w] : 0;
+}
 void fft_set_fmt(Problem &P, bool on)
 {
     if (P.fft) P.fft->fmt = on && P.fft->fmt_ok;
@@ -985,14 +1173,23 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
     else
         hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, workspace<T>(P).rho_pad,
                            C, Z);
-    fft3d<T>(F, Z, (C + 1) / 2, -1, s);
-    if (P.p.ndim == 2)
-        hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
-                           attr_spectra<T>(F), O);
-    else
-        hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
-                           attr_spectra<T>(F), O);
-    fft3d<T>(F, O, (d.nout + 1) / 2, +1, s);
+    const int nip = (C + 1) / 2, nop = (d.nout + 1) / 2;
+    if (F.xlpb[0]) {   // z, y passes; fused x pass with the product; y, z passes back
+        fft_pass<T>(F, Z, nip, -1, 2, s);
+        fft_pass<T>(F, Z, nip, -1, 1, s);
+        fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        fft_pass<T>(F, O, nop, +1, 1, s);
+        fft_pass<T>(F, O, nop, +1, 2, s);
+    } else {
+        fft3d<T>(F, Z, nip, -1, s);
+        if (P.p.ndim == 2)
+            hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                               attr_spectra<T>(F), O);
+        else
+            hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                               attr_spectra<T>(F), O);
+        fft3d<T>(F, O, nop, +1, s);
+    }
     hipLaunchKernelGGL(k_fwd_extract<T>, dim3(grid_for(P.N), rm.nrb + rm.nmf), dim3(kBlock), 0, s, fg, gu, rm, O, in_rb,
                        sub_rb, out_rb, in_mf, sub_mf, out_mf);
     TMI_HIP_CHECK(hipGetLastError());
@@ -1017,12 +1214,21 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
     if (!F.wraps)   // on wrapping grids fmt_point wrote the pairs (fft_pairs)
         hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gf, workspace<T>(P).adj_pad,
                            nadj, Z);
-    fft3d<T>(F, Z, (nadj + 1) / 2, -1, s);
-    if (nd == 2)
-        hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
-    else
-        hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
-    fft3d<T>(F, O, (C + 1) / 2, +1, s);
+    const int nip = (nadj + 1) / 2, nop = (C + 1) / 2;
+    if (F.xlpb[1]) {
+        fft_pass<T>(F, Z, nip, -1, 2, s);
+        fft_pass<T>(F, Z, nip, -1, 1, s);
+        fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        fft_pass<T>(F, O, nop, +1, 1, s);
+        fft_pass<T>(F, O, nop, +1, 2, s);
+    } else {
+        fft3d<T>(F, Z, nip, -1, s);
+        if (nd == 2)
+            hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
+        else
+            hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
+        fft3d<T>(F, O, nop, +1, s);
+    }
     hipLaunchKernelGGL(k_el_extract<T>, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, gf, make_consts(P),
                        linear ? 1 : 0, P.p.nwall, O, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
     TMI_HIP_CHECK(hipGetLastError());
@@ -1032,13 +1238,13 @@ template void fft_el<double>(Problem &, bool, const double *, const double *, co
 template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
                             hipStream_t);
 
-// Times the transforms of one residual fill in FFT form on the plan's work
+// Times the axis passes of one residual fill in FFT form on the plan's work
 // buffers (the density pairs forward, the output pairs back, the adjoint
-// pairs forward, the Euler-Lagrange pairs back), with HIP events on s: *ms is
-// their mean time per fill over reps fills, *bytes their algorithmic traffic
-// (every axis pass reads and writes each grid point of each field once) and
-// *launches the axis-pass launches per fill.  The buffers' contents are
-// scratch between products.
+// pairs forward, the Euler-Lagrange pairs back; with the fused x pass, the y
+// and z passes only), with HIP events on s: *ms is their mean time per fill
+// over reps fills, *bytes their algorithmic traffic (every axis pass reads and
+// writes each grid point of each field once) and *launches the axis-pass
+// launches per fill.  The buffers' contents are scratch between products.
 void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
 {
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
@@ -1049,12 +1255,26 @@ void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launc
     struct Set {
         Cx<double> *buf;
         int nfield, sign;
+        bool fused;
     };
-    const Set sets[4] = {{Z, (C + 1) / 2, -1}, {O, (F.fwd[1].nout + 1) / 2, +1}, {Z, (nadj + 1) / 2, -1}, {O, (C + 1) / 2, +1}};
-    int axes = 0;
-    for (int d = 0; d < 3; ++d) axes += F.M[d] > 1 ? 1 : 0;
+    const Set sets[4] = {{Z, (C + 1) / 2, -1, F.xlpb[0] > 0},
+                         {O, (F.fwd[1].nout + 1) / 2, +1, F.xlpb[0] > 0},
+                         {Z, (nadj + 1) / 2, -1, F.xlpb[1] > 0},
+                         {O, (C + 1) / 2, +1, F.xlpb[1] > 0}};
+    int count = 0;
+    double b = 0.0;
+    for (const Set &t : sets)
+        for (int axis = 0; axis < 3; ++axis)
+            if (F.M[axis] > 1 && !(t.fused && axis == 0)) {
+                ++count;
+                b += 2.0 * sizeof(Cx<double>) * (double)F.n * t.nfield;
+            }
     auto fill = [&] {
-        for (const Set &t : sets) fft3d<double>(F, t.buf, t.nfield, t.sign, s);
+        for (const Set &t : sets)
+            for (int a = 0; a < 3; ++a) {
+                const int axis = t.sign < 0 ? 2 - a : a;   // forward z, y, x; back x, y, z
+                if (!(t.fused && axis == 0)) fft_pass<double>(F, t.buf, t.nfield, t.sign, axis, s);
+            }
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float t = 0.f;
@@ -1074,11 +1294,9 @@ void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launc
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    double b = 0.0;
-    for (const Set &q : sets) b += 2.0 * sizeof(Cx<double>) * (double)F.n * q.nfield * axes;
     *ms = (double)t / reps;
     *bytes = b;
-    *launches = 4 * axes;
+    *launches = count;
 }
 
 // ---------------------------------------------------------------- bulk-circulant Schur preconditioner

@@ -197,6 +197,9 @@ bool fft_fmt(const Problem &P);             // FMT sums (rho-bar, Euler-Lagrange
 bool fft_fmt_available(const Problem &P);   // FMT spectra built
 bool fft_wraps(const Problem &P);           // every axis periodic on the FFT grid (halos unused)
 void fft_set_fmt(Problem &P, bool on);
+bool fft_fused_available(const Problem &P);   // the fused x pass fits the LDS budget
+bool fft_fused(const Problem &P);             // products through the fused x pass
+void fft_set_fused(Problem &P, bool on);      // products through the fused x pass
 // density products; rho_nodes (node arrays, fft_node_input) or, if null, workspace<T>(P).rho_pad
 template <typename T>
 void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *in_rb, const double *sub_rb,

@@ -1419,29 +1419,37 @@ void autotune_rows1d(Problem &P)
 }
 
 // ---------------------------------------------------------------- FMT form
-// Times residual fills with the FMT sums as direct stencil sums and as FFT
-// convolutions (state: every unknown 4.8e-4, a positive byte pattern) and
-// keeps the faster form.  TMI_FMT_FFT=1 / 0 forces the FFT / direct form.
+// Times residual fills with the FMT sums as direct stencil sums, as FFT
+// convolutions with separate x passes and product kernels, and with the fused
+// x pass (state: every unknown 4.8e-4, a positive byte pattern), and keeps
+// the fastest.  TMI_FMT_FFT=1 / 0 forces the FFT / direct form (the fused x
+// pass is then used where it fits, unless TMI_FFT_XFUSE=0).
 void choose_fmt_form(Problem &P)
 {
     if (!fft_fmt_available(P)) return;
-    const char *env = std::getenv("TMI_FMT_FFT");
-    if (env) {
+    const char *env = std::getenv("TMI_FMT_FFT"), *xf = std::getenv("TMI_FFT_XFUSE");
+    if (env) {   // TMI_FFT_XFUSE=0 keeps separate x passes and product kernels
         fft_set_fmt(P, std::string(env) == "1");
+        fft_set_fused(P, !(xf && std::string(xf) == "0"));
         return;
     }
     const size_t n = (size_t)P.nunk * P.N;
     double *x = nullptr, *F = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    float ms[2] = {0.f, 0.f};
+    float ms[3] = {0.f, 0.f, 0.f};
+    const int nform = fft_fused_available(P) ? 3 : 2;
+    auto set_form = [&](int form) {
+        fft_set_fmt(P, form > 0);
+        fft_set_fused(P, form == 2);
+    };
     try {
         TMI_HIP_CHECK(hipMalloc((void **)&x, n * sizeof(double)));
         TMI_HIP_CHECK(hipMalloc((void **)&F, n * sizeof(double)));
         TMI_HIP_CHECK(hipMemset(x, 0x3f, n * sizeof(double)));
         TMI_HIP_CHECK(hipEventCreate(&e0));
         TMI_HIP_CHECK(hipEventCreate(&e1));
-        for (int form = 0; form < 2; ++form) {
-            fft_set_fmt(P, form == 1);
+        for (int form = 0; form < nform; ++form) {
+            set_form(form);
             launch_residual(P, x, F, 0);
             TMI_HIP_CHECK(hipEventRecord(e0, 0));
             for (int r = 0; r < 3; ++r) launch_residual(P, x, F, 0);
@@ -1460,7 +1468,10 @@ void choose_fmt_form(Problem &P)
     (void)hipFree(F);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    fft_set_fmt(P, ms[1] < ms[0]);
+    int best = 0;
+    for (int form = 1; form < nform; ++form)
+        if (ms[form] < ms[best]) best = form;
+    set_form(best);
 }
 
 // ---------------------------------------------------------------- public launchers
