@@ -110,6 +110,9 @@ struct FftPlan {
     FwdDesc fwd[2];             // [0] mean-field rows only, [1] rho-bar rows then mean-field rows
     RowMap rows[2];
     TermList tfwd[2], tel;      // the same products as terms of the fused x pass
+    // rows fed by each output of the density product (fused last pass): [w][output][k]
+    int onrow[2][kMaxOut] = {}, orow[2][kMaxOut][3] = {};
+    double oscale[2][kMaxOut][3] = {};
     int xfit[2] = {0, 0};       // lines per workgroup of the fused x pass that fit, density / adjoint product
     int xlpb[2] = {0, 0};       // in use (0: separate x passes and product kernel)
     // bulk-circulant Schur preconditioner (wrapping grids): Q(k) = S_b(k)^-1 diag(1/rho_b) / n
@@ -267,17 +270,45 @@ __device__ inline Cx<T> *fft_lines(Cx<T> *a, Cx<T> *b, int ld, int nl, int M, Ra
     return a;
 }
 
+// Fused ends of a product on a wrapping grid (grid point = node), for the
+// strided passes: kPassGather reads the density pairs of the first forward
+// pass straight from the node arrays (0 where Zero_density_TF); kPassRows
+// writes the rows of the last inverse pass straight from the output pairs:
+// output o feeds rows row[o][0..nrow[o]) with factors scale[o][.], rho-bar
+// rows (< nrb) minus in_rb and sub_rb, mean-field rows minus in_mf and sub_mf.
+// kPassEl writes the Euler-Lagrange rows of components 2 field and 2 field + 1
+// with their local terms, as k_el_extract.
+constexpr int kPassPlain = 0, kPassGather = 1, kPassRows = 2, kPassEl = 3;
+struct PassIO {
+    const double *rho = nullptr;
+    const unsigned char *zero = nullptr;
+    int C = 0, nx0 = 1, nx1 = 1, N = 0;
+    int nout = 0, nrb = 0;
+    int nrow[kMaxOut] = {};
+    int row[kMaxOut][3] = {};
+    double scale[kMaxOut][3] = {};
+    const double *in_rb = nullptr, *sub_rb = nullptr, *in_mf = nullptr, *sub_mf = nullptr;
+    double *out_rb = nullptr, *out_mf = nullptr;
+    // kPassEl
+    const double *x_rho = nullptr, *el_in_rho = nullptr, *el_in_mf = nullptr, *vext = nullptr, *add = nullptr;
+    const unsigned char *elzero = nullptr;
+    double *out_el = nullptr;
+    double mu[kNC] = {};
+    int linear = 0, nwall = 0;
+};
+
 // One axis of a 3D transform of a batch of fields, in place.  A workgroup
 // takes lpb lines of the axis (consecutive lines along x for the strided
 // axes, so that the loads are coalesced), runs every Stockham stage in LDS and
 // writes back.
-template <typename T, int SIGN>
+template <typename T, int SIGN, int MODE>
 __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data, long long fstride, int M0, int M1,
                                                         int M2, long long S1, long long S2, int axis, RadixCounts rc,
-                                                        int lpb, const Cx<T> *__restrict__ twtab)
+                                                        int lpb, const Cx<T> *__restrict__ twtab, PassIO io)
 {
     extern __shared__ char smem_raw[];
-    data += (long long)blockIdx.y * fstride;   // one field of the batch per grid row
+    const int field = blockIdx.y;
+    data += (long long)field * fstride;   // one field of the batch per grid row
     const int M = axis == 0 ? M0 : (axis == 1 ? M1 : M2);
     const int ld = M + 1;
     Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);   // [M] twiddles exp(-2 pi i j / M)
@@ -289,13 +320,15 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     long long step = 1;   // between elements of a line (strided axes)
     int nl;               // lines in this group
     int l0 = 0;           // first line of the group (axis 0: line = y + M1 z)
+    int x0 = 0, other = 0;
     if (axis == 0) {
         const int nlines = M1 * M2;
         l0 = blockIdx.x * lpb;
         nl = (nlines - l0) < lpb ? (nlines - l0) : lpb;
     } else {
         const int xg = (M0 + lpb - 1) / lpb;
-        const int other = blockIdx.x / xg, x0 = (blockIdx.x % xg) * lpb;
+        other = blockIdx.x / xg;
+        x0 = (blockIdx.x % xg) * lpb;
         nl = (M0 - x0) < lpb ? (M0 - x0) : lpb;
         if (axis == 1) {
             base0 = x0 + S2 * other;
@@ -311,6 +344,11 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
         const int l = l0 + i, z = small_div(l, M1, inv_m1);
         return (long long)(l - z * M1) * S1 + (long long)z * S2;
     };
+    // node of element j of line i of a strided group (wrapping grids)
+    auto node = [&](int i, int j) {
+        const int x = x0 + i, y = axis == 1 ? j : other, z = axis == 1 ? other : j;
+        return x + io.nx0 * (y + io.nx1 * z);
+    };
     // load
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
@@ -320,7 +358,14 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
             const int j = small_div(e, nl, inv_nl), i = e - j * nl;
-            bufA[i * ld + j] = data[base0 + i + (long long)j * step];
+            if constexpr (MODE == kPassGather) {
+                const size_t a = (size_t)(2 * field) * io.N + node(i, j), b = a + io.N;
+                const T re = io.zero[a] ? (T)0 : (T)io.rho[a];
+                const T im = (2 * field + 1 < io.C && !io.zero[b]) ? (T)io.rho[b] : (T)0;
+                bufA[i * ld + j] = Cx<T>{re, im};
+            } else {
+                bufA[i * ld + j] = data[base0 + i + (long long)j * step];
+            }
         }
     }
     __syncthreads();
@@ -334,7 +379,57 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
             const int j = small_div(e, nl, inv_nl), i = e - j * nl;
-            data[base0 + i + (long long)j * step] = bufA[i * ld + j];
+            if constexpr (MODE == kPassEl) {
+                const Cx<T> v = bufA[i * ld + j];
+                const int nd = node(i, j);
+                for (int h = 0; h < 2; ++h) {
+                    const int c = 2 * field + h;
+                    if (c >= io.C) break;
+                    const size_t row = (size_t)c * io.N + nd;
+                    double r;
+                    if (io.elzero[row]) {
+                        r = io.el_in_rho ? io.el_in_rho[row] : 0.0;
+                    } else {
+                        r = (double)(h ? v.y : v.x);
+                        const double xv = io.x_rho[row];
+                        if (!io.linear) {
+                            r += log(xv) + io.mu[c];
+                            if (io.nwall > 0) r += io.vext[row];
+                            if (io.el_in_mf) r += io.el_in_mf[row];
+                        } else {
+                            if (io.el_in_rho) r += io.el_in_rho[row] / xv;
+                            if (io.el_in_mf) r += io.el_in_mf[row];
+                        }
+                    }
+                    if (io.add) r += io.add[row];
+                    io.out_el[row] = r;
+                }
+            } else if constexpr (MODE == kPassRows) {
+                const Cx<T> v = bufA[i * ld + j];
+                const int nd = node(i, j);
+                for (int h = 0; h < 2; ++h) {
+                    const int o = 2 * field + h;
+                    if (o >= io.nout) break;
+                    const double val = (double)(h ? v.y : v.x);
+                    for (int r = 0; r < io.nrow[o]; ++r) {
+                        const int row = io.row[o][r];
+                        double out = val * io.scale[o][r];
+                        if (row < io.nrb) {
+                            const size_t at = (size_t)row * io.N + nd;
+                            if (io.in_rb) out -= io.in_rb[at];
+                            if (io.sub_rb) out -= io.sub_rb[at];
+                            io.out_rb[at] = out;
+                        } else {
+                            const size_t at = (size_t)(row - io.nrb) * io.N + nd;
+                            if (io.in_mf) out -= io.in_mf[at];
+                            if (io.sub_mf) out -= io.sub_mf[at];
+                            io.out_mf[at] = out;
+                        }
+                    }
+                }
+            } else {
+                data[base0 + i + (long long)j * step] = bufA[i * ld + j];
+            }
         }
     }
 }
@@ -578,9 +673,15 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
                 const int f = 2 * jp + h;
                 if (f >= tl.nin) break;
                 const Cx<T> X = unpack(a, b, h);
+                int last = 0x7fffffff;   // terms are sorted by table: each table is read once per point
+                T kt = (T)0;
                 for (int t = tl.first[f]; t < tl.first[f + 1]; ++t) {
                     const Term tm = tl.term[t];
-                    const T kv = (T)tm.coef * (tm.t >= 0 ? tk[(size_t)tm.t * ns + q] : uk[(size_t)(-tm.t - 1) * ns + q]);
+                    if (tm.t != last) {
+                        kt = tm.t >= 0 ? tk[(size_t)tm.t * ns + q] : uk[(size_t)(-tm.t - 1) * ns + q];
+                        last = tm.t;
+                    }
+                    const T kv = (T)tm.coef * kt;
                     const Cx<T> yv = tm.imag ? Cx<T>{-kv * X.y, kv * X.x} : Cx<T>{kv * X.x, kv * X.y};
                     Cx<T> &acc = out[((size_t)(tm.o >> 1) * lpb + i) * ld + k];
                     acc = (tm.o & 1) ? Cx<T>{acc.x - yv.y, acc.y + yv.x} : cadd(acc, yv);
@@ -772,9 +873,11 @@ inline const Cx<float> *twiddles<float>(const FftPlan &F, int axis)
 // the pass has about kFftGroups workgroups (the whole chip busy).
 constexpr long long kFftGroups = 1024;
 
-// one axis pass over nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled)
+// one axis pass over nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled);
+// mode kPassGather / kPassRows (strided axes of a wrapping grid) with io
 template <typename T>
-void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s)
+void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s, int mode = kPassPlain,
+              const PassIO &io = PassIO())
 {
     const int M = F.M[axis];
     if (M == 1 || nfield <= 0) return;
@@ -794,12 +897,24 @@ void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hip
     const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
     const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
     const dim3 grid((unsigned)groups, (unsigned)nfield);
-    if (sign < 0)
-        hipLaunchKernelGGL((k_fft_axis<T, -1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1,
-                           F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
-    else
-        hipLaunchKernelGGL((k_fft_axis<T, 1>), grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1,
-                           F.S2, axis, rc, (int)lpb, twiddles<T>(F, axis));
+    if (mode != kPassPlain && (axis == 0 || !F.wraps)) throw std::logic_error("fused pass ends need a strided axis of a wrapping grid");
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1, F.S2, axis, rc,
+                           (int)lpb, twiddles<T>(F, axis), io);
+    };
+    if (sign < 0) {
+        if (mode == kPassGather)
+            go(k_fft_axis<T, -1, kPassGather>);
+        else
+            go(k_fft_axis<T, -1, kPassPlain>);
+    } else {
+        if (mode == kPassRows)
+            go(k_fft_axis<T, 1, kPassRows>);
+        else if (mode == kPassEl)
+            go(k_fft_axis<T, 1, kPassEl>);
+        else
+            go(k_fft_axis<T, 1, kPassPlain>);
+    }
 }
 
 // in-place 3D transform of nfield fields
@@ -1005,8 +1120,10 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             if (tl.first[tl.nin + 1] >= kMaxTerms) throw std::logic_error("too many product terms");
             tl.term[tl.first[tl.nin + 1]++] = Term{o, t, imag, coef};
         };
-        auto close_input = [&](TermList &tl) {   // the next input field starts
+        auto close_input = [&](TermList &tl) {   // sort the field's terms by table; the next field starts
             if (tl.nin + 2 > kMaxIn + 1) throw std::logic_error("too many product inputs");
+            std::stable_sort(tl.term + tl.first[tl.nin], tl.term + tl.first[tl.nin + 1],
+                             [](const Term &a, const Term &b) { return a.t < b.t; });
             ++tl.nin;
             tl.first[tl.nin + 1] = tl.first[tl.nin];
         };
@@ -1039,6 +1156,15 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             }
             for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND), 0, 1.0);   // S3: theta of every component
             close_input(tl);
+        }
+        for (int w = 0; w < 2; ++w) {
+            const RowMap &rm = F->rows[w];
+            for (int r = 0; r < rm.nrb + rm.nmf; ++r) {
+                const int o = rm.src[r];
+                if (F->onrow[w][o] >= 3) throw std::logic_error("more than 3 rows per product output");
+                F->orow[w][o][F->onrow[w][o]] = r;
+                F->oscale[w][o][F->onrow[w][o]++] = rm.scale[r];
+            }
         }
         F->xfit[0] = F->xlpb[0] = xprod_lpb(*F, std::max((F->fwd[0].nout + 1) / 2, (F->fwd[1].nout + 1) / 2));
         F->xfit[1] = F->xlpb[1] = xprod_lpb(*F, (C + 1) / 2);
@@ -1168,19 +1294,58 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
     Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gu = make_geo(P, P.pad_u);
     const FftGeo fg = fft_geo(F, 0);
-    if (rho_nodes)
-        hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
-    else
-        hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu, workspace<T>(P).rho_pad,
-                           C, Z);
     const int nip = (C + 1) / 2, nop = (d.nout + 1) / 2;
     if (F.xlpb[0]) {   // z, y passes; fused x pass with the product; y, z passes back
-        fft_pass<T>(F, Z, nip, -1, 2, s);
-        fft_pass<T>(F, Z, nip, -1, 1, s);
+        // on a wrapping grid the first pass reads the node arrays and the last one writes the rows
+        const bool ends = F.wraps && rho_nodes;
+        PassIO io;
+        io.C = C;
+        io.nx0 = P.p.nx[0];
+        io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
+        io.N = P.N;
+        if (ends) {
+            io.rho = rho_nodes;
+            io.zero = P.d_zero;
+            io.nout = d.nout;
+            io.nrb = rm.nrb;
+            for (int o = 0; o < d.nout; ++o) {
+                io.nrow[o] = F.onrow[w][o];
+                for (int k = 0; k < 3; ++k) {
+                    io.row[o][k] = F.orow[w][o][k];
+                    io.scale[o][k] = F.oscale[w][o][k];
+                }
+            }
+            io.in_rb = in_rb;
+            io.sub_rb = sub_rb;
+            io.out_rb = out_rb;
+            io.in_mf = in_mf;
+            io.sub_mf = sub_mf;
+            io.out_mf = out_mf;
+        } else if (rho_nodes) {
+            hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
+        } else {
+            hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu,
+                               workspace<T>(P).rho_pad, C, Z);
+        }
+        const int fwd_axes[2] = {2, 1}, inv_axes[2] = {1, 2};
+        int first = -1, last = -1;
+        for (int a : fwd_axes)
+            if (F.M[a] > 1 && first < 0) first = a;
+        for (int a : inv_axes)
+            if (F.M[a] > 1) last = a;
+        for (int a : fwd_axes) fft_pass<T>(F, Z, nip, -1, a, s, (ends && a == first) ? kPassGather : kPassPlain, io);
         fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
-        fft_pass<T>(F, O, nop, +1, 1, s);
-        fft_pass<T>(F, O, nop, +1, 2, s);
+        for (int a : inv_axes) fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassRows : kPassPlain, io);
+        if (ends) {
+            TMI_HIP_CHECK(hipGetLastError());
+            return;
+        }
     } else {
+        if (rho_nodes)
+            hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
+        else
+            hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu,
+                               workspace<T>(P).rho_pad, C, Z);
         fft3d<T>(F, Z, nip, -1, s);
         if (P.p.ndim == 2)
             hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
@@ -1216,11 +1381,37 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
                            nadj, Z);
     const int nip = (nadj + 1) / 2, nop = (C + 1) / 2;
     if (F.xlpb[1]) {
+        // on a wrapping grid the last pass writes the Euler-Lagrange rows
+        const bool ends = F.wraps;
+        PassIO io;
+        io.C = C;
+        io.nx0 = P.p.nx[0];
+        io.nx1 = nd > 1 ? P.p.nx[1] : 1;
+        io.N = P.N;
+        if (ends) {
+            const CompConsts cc = make_consts(P);
+            io.x_rho = x_rho;
+            io.el_in_rho = in_rho;
+            io.el_in_mf = in_mf;
+            io.vext = P.d_vext;
+            io.add = add;
+            io.elzero = P.d_elzero;
+            io.out_el = out_el;
+            for (int c = 0; c < C; ++c) io.mu[c] = cc.mu[c];
+            io.linear = linear ? 1 : 0;
+            io.nwall = P.p.nwall;
+        }
+        int last = -1;
+        for (int a : {1, 2})
+            if (F.M[a] > 1) last = a;
         fft_pass<T>(F, Z, nip, -1, 2, s);
         fft_pass<T>(F, Z, nip, -1, 1, s);
         fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
-        fft_pass<T>(F, O, nop, +1, 1, s);
-        fft_pass<T>(F, O, nop, +1, 2, s);
+        for (int a : {1, 2}) fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassEl : kPassPlain, io);
+        if (ends) {
+            TMI_HIP_CHECK(hipGetLastError());
+            return;
+        }
     } else {
         fft3d<T>(F, Z, nip, -1, s);
         if (nd == 2)
@@ -1238,13 +1429,12 @@ template void fft_el<double>(Problem &, bool, const double *, const double *, co
 template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
                             hipStream_t);
 
-// Times the axis passes of one residual fill in FFT form on the plan's work
-// buffers (the density pairs forward, the output pairs back, the adjoint
-// pairs forward, the Euler-Lagrange pairs back; with the fused x pass, the y
-// and z passes only), with HIP events on s: *ms is their mean time per fill
-// over reps fills, *bytes their algorithmic traffic (every axis pass reads and
-// writes each grid point of each field once) and *launches the axis-pass
-// launches per fill.  The buffers' contents are scratch between products.
+// Times the plain axis passes (k_fft_axis without fused ends) of one
+// residual fill in FFT form on the plan's work buffers, in the fill's order
+// and batch sizes, with HIP events on s: *ms is their mean time per fill over
+// reps fills, *bytes their algorithmic traffic (every axis pass reads and
+// writes each grid point of each field once) and *launches the passes per
+// fill.  The buffers' contents are scratch between products.
 void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
 {
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
@@ -1252,20 +1442,30 @@ void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launc
     const int C = P.p.ncomp, nadj = C * (1 + P.p.ndim) + 1;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
+    // which passes run with fused ends (those are other kernels): the density product's first and last
+    // strided passes when it reads the node arrays, the Euler-Lagrange product's last one on wrapping grids
+    const bool rb_ends = F.xlpb[0] && fft_node_input(P), el_ends = F.xlpb[1] && F.wraps;
+    int first_strided = -1, last_strided = -1;
+    for (int a : {2, 1})
+        if (F.M[a] > 1 && first_strided < 0) first_strided = a;
+    for (int a : {1, 2})
+        if (F.M[a] > 1) last_strided = a;
     struct Set {
         Cx<double> *buf;
         int nfield, sign;
         bool fused;
+        int skip;   // axis run as a fused-end pass, -1 if none
     };
-    const Set sets[4] = {{Z, (C + 1) / 2, -1, F.xlpb[0] > 0},
-                         {O, (F.fwd[1].nout + 1) / 2, +1, F.xlpb[0] > 0},
-                         {Z, (nadj + 1) / 2, -1, F.xlpb[1] > 0},
-                         {O, (C + 1) / 2, +1, F.xlpb[1] > 0}};
+    const Set sets[4] = {{Z, (C + 1) / 2, -1, F.xlpb[0] > 0, rb_ends ? first_strided : -1},
+                         {O, (F.fwd[1].nout + 1) / 2, +1, F.xlpb[0] > 0, rb_ends ? last_strided : -1},
+                         {Z, (nadj + 1) / 2, -1, F.xlpb[1] > 0, -1},
+                         {O, (C + 1) / 2, +1, F.xlpb[1] > 0, el_ends ? last_strided : -1}};
+    auto runs = [&](const Set &t, int axis) { return F.M[axis] > 1 && !(t.fused && axis == 0) && axis != t.skip; };
     int count = 0;
     double b = 0.0;
     for (const Set &t : sets)
         for (int axis = 0; axis < 3; ++axis)
-            if (F.M[axis] > 1 && !(t.fused && axis == 0)) {
+            if (runs(t, axis)) {
                 ++count;
                 b += 2.0 * sizeof(Cx<double>) * (double)F.n * t.nfield;
             }
@@ -1273,7 +1473,7 @@ void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launc
         for (const Set &t : sets)
             for (int a = 0; a < 3; ++a) {
                 const int axis = t.sign < 0 ? 2 - a : a;   // forward z, y, x; back x, y, z
-                if (!(t.fused && axis == 0)) fft_pass<double>(F, t.buf, t.nfield, t.sign, axis, s);
+                if (runs(t, axis)) fft_pass<double>(F, t.buf, t.nfield, t.sign, axis, s);
             }
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
