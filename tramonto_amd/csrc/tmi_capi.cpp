@@ -703,8 +703,7 @@ int newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_sta
     try {
         int iter = 0;
         while (true) {
-            launch_residual(P, x, F, s);
-            const double rn = std::sqrt(dev_dot(P, F, F, (long long)ntot, s));
+            const double rn = std::sqrt(residual_sumsq(P, x, F, s));
             st.resid_norm = rn;
             if (o.verbose) std::fprintf(stderr, "[tmi newton] iter %d ||F|| = %.6e\n", iter, rn);
             if (!(rn == rn)) throw std::runtime_error("residual is NaN");
@@ -873,8 +872,10 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
         Problem *P = as_problem(handle);
         if (!x_d || !F_d) throw std::invalid_argument("null device pointer");
         hipStream_t s = (hipStream_t)stream;
-        launch_residual(*P, x_d, F_d, s);
-        if (sumsq) *sumsq = dev_dot(*P, F_d, F_d, (long long)P->nunk * P->N, s);
+        if (sumsq)
+            *sumsq = residual_sumsq(*P, x_d, F_d, s);
+        else
+            launch_residual(*P, x_d, F_d, s);
         return 0;
     });
 }

@@ -113,6 +113,8 @@ struct FftPlan {
     // rows fed by each output of the density product (fused last pass): [w][output][k]
     int onrow[2][kMaxOut] = {}, orow[2][kMaxOut][3] = {};
     double oscale[2][kMaxOut][3] = {};
+    double *ss = nullptr;       // partial sums of F^2 from the fused last passes (density, then Euler-Lagrange rows)
+    long long ss_cap = 0;
     int xfit[2] = {0, 0};       // lines per workgroup of the fused x pass that fit, density / adjoint product
     int xlpb[2] = {0, 0};       // in use (0: separate x passes and product kernel)
     // bulk-circulant Schur preconditioner (wrapping grids): Q(k) = S_b(k)^-1 diag(1/rho_b) / n
@@ -295,7 +297,22 @@ struct PassIO {
     double *out_el = nullptr;
     double mu[kNC] = {};
     int linear = 0, nwall = 0;
+    // kPassRows, kPassEl: per-workgroup partial sums of the squares of the rows written
+    double *ss = nullptr;
 };
+
+// sum over the workgroup (kFftBlock threads), valid in thread 0
+__device__ inline double fft_block_sum(double v, double *sh)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kFftBlock / 64; ++w) t += sh[w];
+    return t;
+}
 
 // One axis of a 3D transform of a batch of fields, in place.  A workgroup
 // takes lpb lines of the axis (consecutive lines along x for the strided
@@ -371,6 +388,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     __syncthreads();
     bufA = fft_lines<SIGN>(bufA, bufB, ld, nl, M, rc, tw);
     // store
+    double ssq = 0.0;   // kPassRows, kPassEl: squares of the rows written
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
             const int i = small_div(e, M, inv_m), j = e - i * M;
@@ -403,6 +421,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
                     }
                     if (io.add) r += io.add[row];
                     io.out_el[row] = r;
+                    ssq += r * r;
                 }
             } else if constexpr (MODE == kPassRows) {
                 const Cx<T> v = bufA[i * ld + j];
@@ -425,11 +444,19 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
                             if (io.sub_mf) out -= io.sub_mf[at];
                             io.out_mf[at] = out;
                         }
+                        ssq += out * out;
                     }
                 }
             } else {
                 data[base0 + i + (long long)j * step] = bufA[i * ld + j];
             }
+        }
+    }
+    if constexpr (MODE == kPassRows || MODE == kPassEl) {
+        if (io.ss) {
+            __shared__ double sh[kFftBlock / 64];
+            const double t = fft_block_sum(ssq, sh);
+            if (threadIdx.x == 0) io.ss[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
         }
     }
 }
@@ -875,12 +902,14 @@ constexpr long long kFftGroups = 1024;
 
 // one axis pass over nfield fields ns elements apart; sign -1 forward, +1 inverse (unscaled);
 // mode kPassGather / kPassRows (strided axes of a wrapping grid) with io
+// lines per workgroup and workgroups per field of an axis pass
+struct PassShape {
+    long long lpb, groups;
+};
 template <typename T>
-void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s, int mode = kPassPlain,
-              const PassIO &io = PassIO())
+PassShape pass_shape(const FftPlan &F, int axis, int nfield)
 {
     const int M = F.M[axis];
-    if (M == 1 || nfield <= 0) return;
     const long long lines = F.n / M;
     const long long lds_max = (long long)((kFftLds / sizeof(Cx<T>) - M) / (2 * (M + 1)));
     long long lpb = std::max<long long>(1, std::min<long long>(lds_max, lines * nfield / kFftGroups));
@@ -894,6 +923,18 @@ void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hip
         groups = (lines + lpb - 1) / lpb;
     else
         groups = (long long)((F.M[0] + lpb - 1) / lpb) * (axis == 1 ? F.M[2] : F.M[1]);
+    return PassShape{lpb, groups};
+}
+
+// returns the number of workgroups launched
+template <typename T>
+long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s, int mode = kPassPlain,
+                   const PassIO &io = PassIO())
+{
+    const int M = F.M[axis];
+    if (M == 1 || nfield <= 0) return 0;
+    const PassShape ps = pass_shape<T>(F, axis, nfield);
+    const long long lpb = ps.lpb, groups = ps.groups;
     const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
     const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
     const dim3 grid((unsigned)groups, (unsigned)nfield);
@@ -915,6 +956,7 @@ void fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hip
         else
             go(k_fft_axis<T, 1, kPassPlain>);
     }
+    return groups * nfield;
 }
 
 // in-place 3D transform of nfield fields
@@ -995,6 +1037,7 @@ void free_plan(FftPlan *F)
     (void)hipFree(F->td);
     (void)hipFree(F->tf);
     (void)hipFree(F->pq);
+    (void)hipFree(F->ss);
     for (int d = 0; d < 3; ++d) {
         (void)hipFree(F->twd[d]);
         (void)hipFree(F->twf[d]);
@@ -1172,6 +1215,18 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         const int opairs = std::max((f0.nout + 1) / 2, fmt ? std::max((f1.nout + 1) / 2, (C + 1) / 2) : 0);
         TMI_HIP_CHECK(hipMalloc(&F->zbuf, (size_t)zpairs * ns * sizeof(Cx<double>)));
         TMI_HIP_CHECK(hipMalloc(&F->obuf, (size_t)opairs * ns * sizeof(Cx<double>)));
+        if (F->wraps) {   // partial sums of F^2 of the fused last passes (FP64 fills)
+            int last = -1;
+            for (int a : {1, 2})
+                if (F->M[a] > 1) last = a;
+            auto most = [&](int pairs) {   // workgroups of the pass over at most `pairs` fields
+                long long m = 0;
+                for (int nf = 1; nf <= pairs; ++nf) m = std::max(m, pass_shape<double>(*F, last, nf).groups * nf);
+                return m;
+            };
+            F->ss_cap = most((std::max(F->fwd[0].nout, F->fwd[1].nout) + 1) / 2) + most((C + 1) / 2);
+            TMI_HIP_CHECK(hipMalloc((void **)&F->ss, (size_t)F->ss_cap * sizeof(double)));
+        }
         // spectra: flipped stencil g(-o mod M) = w(o) on the FFT grid, transformed (in zbuf), the
         // real (even) or imaginary (odd) part kept, scaled by 1/n
         std::vector<Cx<double>> host(ns);
@@ -1262,6 +1317,11 @@ bool fft_node_input(const Problem &P)
     return P.fft && P.fft->fmt && P.fft->wraps && (P.nmf == 0 || P.fft->mf);
 }
 bool fft_mf(const Problem &P) { return P.fft && P.fft->mf; }
+double *fft_ss_buffer(const Problem &P, long long *cap)
+{
+    if (cap) *cap = P.fft ? P.fft->ss_cap : 0;
+    return P.fft ? P.fft->ss : nullptr;
+}
 bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
 bool fft_fused(const Problem &P) { return P.fft && (P.fft->xlpb[0] || P.fft->xlpb[1]); }
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
@@ -1282,8 +1342,10 @@ void fft_set_fmt(Problem &P, bool on)
 // mean-field rows when they are in FFT form; else the mean-field rows only.
 template <typename T>
 void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *in_rb, const double *sub_rb,
-               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s)
+               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s, double *ss,
+               long long *nss)
 {
+    if (nss) *nss = 0;
     const FftPlan &F = *P.fft;
     const int w = with_rb ? 1 : 0;
     const FwdDesc &d = F.fwd[w];
@@ -1335,8 +1397,12 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
             if (F.M[a] > 1) last = a;
         for (int a : fwd_axes) fft_pass<T>(F, Z, nip, -1, a, s, (ends && a == first) ? kPassGather : kPassPlain, io);
         fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
-        for (int a : inv_axes) fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassRows : kPassPlain, io);
+        if (ends) io.ss = ss;
+        long long blocks = 0;
+        for (int a : inv_axes)
+            blocks = fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassRows : kPassPlain, io);
         if (ends) {
+            if (ss && nss) *nss = blocks;
             TMI_HIP_CHECK(hipGetLastError());
             return;
         }
@@ -1360,16 +1426,17 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
     TMI_HIP_CHECK(hipGetLastError());
 }
 template void fft_rb_mf<double>(Problem &, const double *, bool, const double *, const double *, double *,
-                                const double *, const double *, double *, hipStream_t);
+                                const double *, const double *, double *, hipStream_t, double *, long long *);
 template void fft_rb_mf<float>(Problem &, const double *, bool, const double *, const double *, double *,
-                               const double *, const double *, double *, hipStream_t);
+                               const double *, const double *, double *, hipStream_t, double *, long long *);
 
 // Euler-Lagrange rows from the adjoint fields (as el_rows): in workspace<T>(P).adj_pad,
 // or, on wrapping grids, already paired in the transform buffer (fft_pairs)
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
-            double *out_el, hipStream_t s)
+            double *out_el, hipStream_t s, double *ss, long long *nss)
 {
+    if (nss) *nss = 0;
     const FftPlan &F = *P.fft;
     const int C = P.p.ncomp, nd = P.p.ndim, nadj = C * (1 + nd) + 1;
     Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
@@ -1407,8 +1474,11 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
         fft_pass<T>(F, Z, nip, -1, 2, s);
         fft_pass<T>(F, Z, nip, -1, 1, s);
         fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
-        for (int a : {1, 2}) fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassEl : kPassPlain, io);
+        if (ends) io.ss = ss;
+        long long blocks = 0;
+        for (int a : {1, 2}) blocks = fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassEl : kPassPlain, io);
         if (ends) {
+            if (ss && nss) *nss = blocks;
             TMI_HIP_CHECK(hipGetLastError());
             return;
         }
@@ -1425,9 +1495,9 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
     TMI_HIP_CHECK(hipGetLastError());
 }
 template void fft_el<double>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
-                             hipStream_t);
+                             hipStream_t, double *, long long *);
 template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
-                            hipStream_t);
+                            hipStream_t, double *, long long *);
 
 // Times the plain axis passes (k_fft_axis without fused ends) of one
 // residual fill in FFT form on the plan's work buffers, in the fill's order

@@ -169,6 +169,8 @@ enum Mode { kResidual = 0, kLinear = 1 };
 
 // F = residual (all rows); sumsq optional (host)
 void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev = nullptr);
+// F(x) and sum F^2 (from the fused last passes' partial sums where every row comes from them)
+double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s);
 // Jv for all rows, precision T for the stencil sums
 template <typename T>
 void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s);
@@ -201,9 +203,11 @@ bool fft_fused_available(const Problem &P);   // the fused x pass fits the LDS b
 bool fft_fused(const Problem &P);             // products through the fused x pass
 void fft_set_fused(Problem &P, bool on);      // products through the fused x pass
 // density products; rho_nodes (node arrays, fft_node_input) or, if null, workspace<T>(P).rho_pad
+// ss (optional): partial sums of the squares of the rows written by a fused last pass, *nss of them
 template <typename T>
 void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *in_rb, const double *sub_rb,
-               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s);
+               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s,
+               double *ss = nullptr, long long *nss = nullptr);
 // pair buffer of a wrapping FFT grid: field f of node (x, y, z) at element 2 ((f / 2) ns + x + y S1 + z S2) + f % 2
 struct FftPairs {
     void *buf;
@@ -213,7 +217,8 @@ FftPairs fft_pairs(const Problem &P);
 bool fft_node_input(const Problem &P);   // every density product reads the node arrays (no density halo)
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
-            double *out_el, hipStream_t s);
+            double *out_el, hipStream_t s, double *ss = nullptr, long long *nss = nullptr);
+double *fft_ss_buffer(const Problem &P, long long *cap);   // partial sums of F^2 of the fused last passes
 void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s);
 // bulk-circulant Schur preconditioner on wrapping FFT grids (tmi_fft.hip)
 void precond_setup(Problem &P);

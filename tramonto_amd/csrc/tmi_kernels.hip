@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <stdexcept>
 #include <string>
 
 #include "tmi_device.hpp"
@@ -1246,13 +1247,14 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
 template <typename T>
 void rb_mf_rows(Problem &P, const double *rho, bool linear, const double *in_rb, const double *sub_rb, double *out_rb,
                 const double *in_mf, const double *sub_mf, double *out_mf, hipStream_t s, hipEvent_t ev_pad = nullptr,
-                hipEvent_t ev_mid = nullptr)
+                hipEvent_t ev_mid = nullptr, double *ss = nullptr, long long *nss = nullptr)
 {
+    if (nss) *nss = 0;
     const bool nodes = fft_node_input(P);
     if (!nodes) pad_rho<T>(P, rho, linear, s);
     if (ev_pad) TMI_HIP_CHECK(hipEventRecord(ev_pad, s));
     if (fft_fmt(P)) {
-        fft_rb_mf<T>(P, nodes ? rho : nullptr, true, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf, s);
+        fft_rb_mf<T>(P, nodes ? rho : nullptr, true, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf, s, ss, nss);
         if (ev_mid) TMI_HIP_CHECK(hipEventRecord(ev_mid, s));
         if (!fft_mf(P)) mf_rows<T>(P, in_mf, sub_mf, out_mf, s);
         return;
@@ -1297,10 +1299,11 @@ void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear,
 
 template <typename T>
 void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
-             double *out_el, hipStream_t s)
+             double *out_el, hipStream_t s, double *ss = nullptr, long long *nss = nullptr)
 {
+    if (nss) *nss = 0;
     if (fft_fmt(P)) {
-        fft_el<T>(P, linear, x_rho, in_rho, in_mf, add, out_el, s);
+        fft_el<T>(P, linear, x_rho, in_rho, in_mf, add, out_el, s, ss, nss);
         return;
     }
     Geo gf = make_geo(P, P.pad_f);
@@ -1477,20 +1480,46 @@ void choose_fmt_form(Problem &P)
 // ---------------------------------------------------------------- public launchers
 // ev (optional): 6 events recorded between the stages pad_rho | rb_rows |
 // mf_rows | fmt_point+pad_adj | el_rows
-void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev)
+namespace {
+// ss (optional): partial sums of F^2 from the fused last passes; *nss their number when they cover
+// every row, else 0
+void residual_impl(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev, double *ss, long long *nss)
 {
     const size_t N = P.N;
+    long long n_rb = 0, n_el = 0;
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
     rb_mf_rows<double>(P, x + P.iu_rho * N, false, x + P.iu_rb * N, nullptr, F + P.iu_rb * N,
                        P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s, ev ? ev[1] : nullptr,
-                       ev ? ev[2] : nullptr);
+                       ev ? ev[2] : nullptr, ss, &n_rb);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
     el_rows<double>(P, false, x + P.iu_rho * N, x + P.iu_rho * N, P.nmf ? x + P.iu_mf * N : nullptr, nullptr,
-                    F + P.iu_rho * N, s);
+                    F + P.iu_rho * N, s, ss ? ss + n_rb : nullptr, &n_el);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[5], s));
     TMI_HIP_CHECK(hipGetLastError());
+    if (nss) *nss = (n_rb > 0 && n_el > 0) ? n_rb + n_el : 0;
+}
+}  // namespace
+
+void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev)
+{
+    residual_impl(P, x, F, s, ev, nullptr, nullptr);
+}
+
+double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s)
+{
+    long long cap = 0, cnt = 0;
+    double *ss = fft_ss_buffer(P, &cap);
+    residual_impl(P, x, F, s, nullptr, ss, &cnt);
+    if (cnt > cap) throw std::logic_error("partial sums past their buffer");
+    if (cnt > 0) {
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, ss, (int)cnt, 1, P.d_red + 1024);
+        TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, P.d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+        TMI_HIP_CHECK(hipStreamSynchronize(s));
+        return P.h_red[0];
+    }
+    return dev_dot(P, F, F, (long long)P.nunk * P.N, s);
 }
 
 template <typename T>
