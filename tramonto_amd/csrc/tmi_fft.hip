@@ -314,6 +314,92 @@ __device__ inline double fft_block_sum(double v, double *sh)
     return t;
 }
 
+// Element j of line i of a strided-axis group (lines = consecutive x from x0
+// at the other strided coordinate `other`): storage index base0 + i + j step;
+// on a wrapping grid also node (x0 + i, y, z).
+struct StridedGeo {
+    long long base0, step;
+    int x0, other, axis;
+};
+
+__device__ inline int strided_node(const StridedGeo &g, const PassIO &io, int i, int j)
+{
+    const int x = g.x0 + i, y = g.axis == 1 ? j : g.other, z = g.axis == 1 ? g.other : j;
+    return x + io.nx0 * (y + io.nx1 * z);
+}
+
+template <typename T, int MODE>
+__device__ inline Cx<T> strided_load(const Cx<T> *__restrict__ data, const StridedGeo &g, const PassIO &io, int field,
+                                     int i, int j)
+{
+    if constexpr (MODE == kPassGather) {
+        const size_t a = (size_t)(2 * field) * io.N + strided_node(g, io, i, j), b = a + io.N;
+        const T re = io.zero[a] ? (T)0 : (T)io.rho[a];
+        const T im = (2 * field + 1 < io.C && !io.zero[b]) ? (T)io.rho[b] : (T)0;
+        return Cx<T>{re, im};
+    } else {
+        return data[g.base0 + i + (long long)j * g.step];
+    }
+}
+
+// store element (i, j) of pair field `field`; kPassRows / kPassEl add the squares of the rows written to ssq
+template <typename T, int MODE>
+__device__ inline void strided_store(Cx<T> *__restrict__ data, const StridedGeo &g, const PassIO &io, int field, int i,
+                                     int j, Cx<T> v, double &ssq)
+{
+    if constexpr (MODE == kPassEl) {
+        const int nd = strided_node(g, io, i, j);
+        for (int h = 0; h < 2; ++h) {
+            const int c = 2 * field + h;
+            if (c >= io.C) break;
+            const size_t row = (size_t)c * io.N + nd;
+            double r;
+            if (io.elzero[row]) {
+                r = io.el_in_rho ? io.el_in_rho[row] : 0.0;
+            } else {
+                r = (double)(h ? v.y : v.x);
+                const double xv = io.x_rho[row];
+                if (!io.linear) {
+                    r += log(xv) + io.mu[c];
+                    if (io.nwall > 0) r += io.vext[row];
+                    if (io.el_in_mf) r += io.el_in_mf[row];
+                } else {
+                    if (io.el_in_rho) r += io.el_in_rho[row] / xv;
+                    if (io.el_in_mf) r += io.el_in_mf[row];
+                }
+            }
+            if (io.add) r += io.add[row];
+            io.out_el[row] = r;
+            ssq += r * r;
+        }
+    } else if constexpr (MODE == kPassRows) {
+        const int nd = strided_node(g, io, i, j);
+        for (int h = 0; h < 2; ++h) {
+            const int o = 2 * field + h;
+            if (o >= io.nout) break;
+            const double val = (double)(h ? v.y : v.x);
+            for (int r = 0; r < io.nrow[o]; ++r) {
+                const int row = io.row[o][r];
+                double out = val * io.scale[o][r];
+                if (row < io.nrb) {
+                    const size_t at = (size_t)row * io.N + nd;
+                    if (io.in_rb) out -= io.in_rb[at];
+                    if (io.sub_rb) out -= io.sub_rb[at];
+                    io.out_rb[at] = out;
+                } else {
+                    const size_t at = (size_t)(row - io.nrb) * io.N + nd;
+                    if (io.in_mf) out -= io.in_mf[at];
+                    if (io.sub_mf) out -= io.sub_mf[at];
+                    io.out_mf[at] = out;
+                }
+                ssq += out * out;
+            }
+        }
+    } else {
+        data[g.base0 + i + (long long)j * g.step] = v;
+    }
+}
+
 // One axis of a 3D transform of a batch of fields, in place.  A workgroup
 // takes lpb lines of the axis (consecutive lines along x for the strided
 // axes, so that the loads are coalesced), runs every Stockham stage in LDS and
@@ -361,11 +447,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
         const int l = l0 + i, z = small_div(l, M1, inv_m1);
         return (long long)(l - z * M1) * S1 + (long long)z * S2;
     };
-    // node of element j of line i of a strided group (wrapping grids)
-    auto node = [&](int i, int j) {
-        const int x = x0 + i, y = axis == 1 ? j : other, z = axis == 1 ? other : j;
-        return x + io.nx0 * (y + io.nx1 * z);
-    };
+    const StridedGeo sg{base0, step, x0, other, axis};
     // load
     if (axis == 0) {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
@@ -375,14 +457,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
             const int j = small_div(e, nl, inv_nl), i = e - j * nl;
-            if constexpr (MODE == kPassGather) {
-                const size_t a = (size_t)(2 * field) * io.N + node(i, j), b = a + io.N;
-                const T re = io.zero[a] ? (T)0 : (T)io.rho[a];
-                const T im = (2 * field + 1 < io.C && !io.zero[b]) ? (T)io.rho[b] : (T)0;
-                bufA[i * ld + j] = Cx<T>{re, im};
-            } else {
-                bufA[i * ld + j] = data[base0 + i + (long long)j * step];
-            }
+            bufA[i * ld + j] = strided_load<T, MODE>(data, sg, io, field, i, j);
         }
     }
     __syncthreads();
@@ -397,59 +472,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
     } else {
         for (int e = threadIdx.x; e < nl * M; e += kFftBlock) {
             const int j = small_div(e, nl, inv_nl), i = e - j * nl;
-            if constexpr (MODE == kPassEl) {
-                const Cx<T> v = bufA[i * ld + j];
-                const int nd = node(i, j);
-                for (int h = 0; h < 2; ++h) {
-                    const int c = 2 * field + h;
-                    if (c >= io.C) break;
-                    const size_t row = (size_t)c * io.N + nd;
-                    double r;
-                    if (io.elzero[row]) {
-                        r = io.el_in_rho ? io.el_in_rho[row] : 0.0;
-                    } else {
-                        r = (double)(h ? v.y : v.x);
-                        const double xv = io.x_rho[row];
-                        if (!io.linear) {
-                            r += log(xv) + io.mu[c];
-                            if (io.nwall > 0) r += io.vext[row];
-                            if (io.el_in_mf) r += io.el_in_mf[row];
-                        } else {
-                            if (io.el_in_rho) r += io.el_in_rho[row] / xv;
-                            if (io.el_in_mf) r += io.el_in_mf[row];
-                        }
-                    }
-                    if (io.add) r += io.add[row];
-                    io.out_el[row] = r;
-                    ssq += r * r;
-                }
-            } else if constexpr (MODE == kPassRows) {
-                const Cx<T> v = bufA[i * ld + j];
-                const int nd = node(i, j);
-                for (int h = 0; h < 2; ++h) {
-                    const int o = 2 * field + h;
-                    if (o >= io.nout) break;
-                    const double val = (double)(h ? v.y : v.x);
-                    for (int r = 0; r < io.nrow[o]; ++r) {
-                        const int row = io.row[o][r];
-                        double out = val * io.scale[o][r];
-                        if (row < io.nrb) {
-                            const size_t at = (size_t)row * io.N + nd;
-                            if (io.in_rb) out -= io.in_rb[at];
-                            if (io.sub_rb) out -= io.sub_rb[at];
-                            io.out_rb[at] = out;
-                        } else {
-                            const size_t at = (size_t)(row - io.nrb) * io.N + nd;
-                            if (io.in_mf) out -= io.in_mf[at];
-                            if (io.sub_mf) out -= io.sub_mf[at];
-                            io.out_mf[at] = out;
-                        }
-                        ssq += out * out;
-                    }
-                }
-            } else {
-                data[base0 + i + (long long)j * step] = bufA[i * ld + j];
-            }
+            strided_store<T, MODE>(data, sg, io, field, i, j, bufA[i * ld + j], ssq);
         }
     }
     if constexpr (MODE == kPassRows || MODE == kPassEl) {
@@ -457,6 +480,98 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_axis(Cx<T> *__restrict__ data
             __shared__ double sh[kFftBlock / 64];
             const double t = fft_block_sum(ssq, sh);
             if (threadIdx.x == 0) io.ss[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+        }
+    }
+}
+
+// exp(SIGN 2 pi i k / R) for R dividing 16 (k a compile-time index after unrolling)
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ Cx<T> unit_root(int k)
+{
+    constexpr double c16[16] = {1.0,  0.92387953251128675613,  0.70710678118654752440,  0.38268343236508977173,
+                                0.0,  -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                                -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                                0.0,  0.38268343236508977173,  0.70710678118654752440,  0.92387953251128675613};
+    const int q = (k * (16 / R)) & 15;
+    return Cx<T>{(T)c16[q], (T)(SIGN * c16[(q + 12) & 15])};   // sin(a) = cos(a - pi/2)
+}
+
+// X_p = sum_q a_q exp(SIGN 2 pi i p q / R), R = 2, 4, 8, 16, in registers (radix-2 splits down to 4)
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft_reg(Cx<T> *a)
+{
+    if constexpr (R <= 4) {
+        dft_small<R, SIGN>(a);
+    } else {
+        constexpr int H = R / 2;
+        Cx<T> e[H], o[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            e[k] = a[2 * k];
+            o[k] = a[2 * k + 1];
+        }
+        dft_reg<H, SIGN>(e);
+        dft_reg<H, SIGN>(o);
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const Cx<T> t = cmul(o[k], unit_root<R, SIGN, T>(k));
+            a[k] = cadd(e[k], t);
+            a[k + H] = csub(e[k], t);
+        }
+    }
+}
+
+// Strided axis pass of length M = R1 R2 (R2 <= R1 <= 16, both powers of two) with one wave per
+// workgroup: lane (i, n2) takes line i of L = 64 / R2 consecutive x and loads elements
+// R2 n1 + n2 straight into registers; an R1-point DFT, the twiddles W_M^(n2 k1), one transpose
+// through LDS, R1/R2 R2-point DFTs per lane and the store.  One LDS round trip per pass instead
+// of one per Stockham stage.
+template <typename T, int SIGN, int MODE, int R1, int R2>
+__global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, long long fstride, int M0, int M1, int M2,
+                                                    long long S1, long long S2, int axis,
+                                                    const Cx<T> *__restrict__ twtab, PassIO io)
+{
+    constexpr int M = R1 * R2, L = 64 / R2, LD = M + 1;
+    __shared__ Cx<T> tw[M];
+    __shared__ Cx<T> buf[L * LD];
+    const int field = blockIdx.y;
+    data += (long long)field * fstride;
+    for (int j = threadIdx.x; j < M; j += 64) tw[j] = twtab[j];
+    const int xg = (M0 + L - 1) / L;
+    const int other = blockIdx.x / xg, x0 = (blockIdx.x % xg) * L;
+    const StridedGeo sg{axis == 1 ? x0 + S2 * other : x0 + S1 * other, axis == 1 ? S1 : S2, x0, other, axis};
+    const int i = threadIdx.x % L, n2 = threadIdx.x / L;
+    const bool live = x0 + i < M0;
+    __syncthreads();
+    Cx<T> v[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1)
+        v[n1] = live ? strided_load<T, MODE>(data, sg, io, field, i, R2 * n1 + n2) : Cx<T>{(T)0, (T)0};
+    dft_reg<R1, SIGN>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        Cx<T> w = tw[n2 * k1];
+        if constexpr (SIGN > 0) w.y = -w.y;
+        buf[i * LD + k1 * R2 + n2] = cmul(v[k1], w);
+    }
+    __syncthreads();
+    double ssq = 0.0;
+#pragma unroll
+    for (int q = 0; q < R1 / R2; ++q) {
+        const int k1 = n2 + R2 * q;
+        Cx<T> u[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) u[m] = buf[i * LD + k1 * R2 + m];
+        dft_reg<R2, SIGN>(u);
+        if (live) {
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) strided_store<T, MODE>(data, sg, io, field, i, k1 + R1 * k2, u[k2], ssq);
+        }
+    }
+    if constexpr (MODE == kPassRows || MODE == kPassEl) {
+        if (io.ss) {
+            for (int o = 32; o > 0; o >>= 1) ssq += __shfl_down(ssq, o, 64);
+            if (threadIdx.x == 0) io.ss[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = ssq;
         }
     }
 }
@@ -926,6 +1041,66 @@ PassShape pass_shape(const FftPlan &F, int axis, int nfield)
     return PassShape{lpb, groups};
 }
 
+// register radices (R1, R2) of the one-wave strided pass for length M, or 0 when M has none
+// (TMI_FFT_REGPASS=0 keeps the LDS Stockham pass everywhere)
+inline int reg_radices(int M, int &R1, int &R2)
+{
+    static const bool off = [] {
+        const char *e = std::getenv("TMI_FFT_REGPASS");
+        return e && std::string(e) == "0";
+    }();
+    if (off) return 0;
+    switch (M) {
+    case 32: R1 = 8, R2 = 4; return 1;
+    case 64: R1 = 8, R2 = 8; return 1;
+    case 128: R1 = 16, R2 = 8; return 1;
+    case 256: R1 = 16, R2 = 16; return 1;
+    default: return 0;
+    }
+}
+
+// The one-wave register pass where it has radices and at least kRegPassWaves waves (one per
+// workgroup); fewer would leave SIMDs idle, and the LDS pass runs 4 waves per workgroup.
+constexpr long long kRegPassWaves = 1024;
+inline bool use_reg_pass(const FftPlan &F, int axis, int nfield, int &R1, int &R2)
+{
+    if (axis == 0 || !reg_radices(F.M[axis], R1, R2)) return false;
+    const int L = 64 / R2;
+    return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]) * nfield >= kRegPassWaves;
+}
+
+// workgroups of a pass over nfield fields
+template <typename T>
+long long pass_blocks(const FftPlan &F, int axis, int nfield)
+{
+    int R1 = 0, R2 = 0;
+    if (use_reg_pass(F, axis, nfield, R1, R2)) {
+        const int L = 64 / R2;
+        return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]) * nfield;
+    }
+    return pass_shape<T>(F, axis, nfield).groups * nfield;
+}
+
+template <typename T, int SIGN, int MODE>
+void launch_strided_reg(const FftPlan &F, Cx<T> *data, int nfield, int axis, int R1, int R2, hipStream_t s,
+                        const PassIO &io)
+{
+    const int L = 64 / R2;
+    const dim3 grid((unsigned)((long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1])), (unsigned)nfield);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1, F.S2, axis,
+                           twiddles<T>(F, axis), io);
+    };
+    if (R1 == 8 && R2 == 4)
+        go(k_fft_strided<T, SIGN, MODE, 8, 4>);
+    else if (R1 == 8 && R2 == 8)
+        go(k_fft_strided<T, SIGN, MODE, 8, 8>);
+    else if (R1 == 16 && R2 == 8)
+        go(k_fft_strided<T, SIGN, MODE, 16, 8>);
+    else
+        go(k_fft_strided<T, SIGN, MODE, 16, 16>);
+}
+
 // returns the number of workgroups launched
 template <typename T>
 long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis, hipStream_t s, int mode = kPassPlain,
@@ -933,12 +1108,29 @@ long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis
 {
     const int M = F.M[axis];
     if (M == 1 || nfield <= 0) return 0;
+    if (mode != kPassPlain && (axis == 0 || !F.wraps)) throw std::logic_error("fused pass ends need a strided axis of a wrapping grid");
+    int R1 = 0, R2 = 0;
+    if (use_reg_pass(F, axis, nfield, R1, R2)) {
+        if (sign < 0) {
+            if (mode == kPassGather)
+                launch_strided_reg<T, -1, kPassGather>(F, data, nfield, axis, R1, R2, s, io);
+            else
+                launch_strided_reg<T, -1, kPassPlain>(F, data, nfield, axis, R1, R2, s, io);
+        } else {
+            if (mode == kPassRows)
+                launch_strided_reg<T, 1, kPassRows>(F, data, nfield, axis, R1, R2, s, io);
+            else if (mode == kPassEl)
+                launch_strided_reg<T, 1, kPassEl>(F, data, nfield, axis, R1, R2, s, io);
+            else
+                launch_strided_reg<T, 1, kPassPlain>(F, data, nfield, axis, R1, R2, s, io);
+        }
+        return pass_blocks<T>(F, axis, nfield);
+    }
     const PassShape ps = pass_shape<T>(F, axis, nfield);
     const long long lpb = ps.lpb, groups = ps.groups;
     const RadixCounts rc{F.rc[axis][0], F.rc[axis][1], F.rc[axis][2], F.rc[axis][3]};
     const size_t lds = sizeof(Cx<T>) * ((size_t)M + 2 * (size_t)lpb * (M + 1));
     const dim3 grid((unsigned)groups, (unsigned)nfield);
-    if (mode != kPassPlain && (axis == 0 || !F.wraps)) throw std::logic_error("fused pass ends need a strided axis of a wrapping grid");
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(kFftBlock), lds, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1, F.S2, axis, rc,
                            (int)lpb, twiddles<T>(F, axis), io);
@@ -1221,7 +1413,7 @@ void fft_setup(Problem &P, bool mf, bool fmt)
                 if (F->M[a] > 1) last = a;
             auto most = [&](int pairs) {   // workgroups of the pass over at most `pairs` fields
                 long long m = 0;
-                for (int nf = 1; nf <= pairs; ++nf) m = std::max(m, pass_shape<double>(*F, last, nf).groups * nf);
+                for (int nf = 1; nf <= pairs; ++nf) m = std::max(m, pass_blocks<double>(*F, last, nf));
                 return m;
             };
             F->ss_cap = most((std::max(F->fwd[0].nout, F->fwd[1].nout) + 1) / 2) + most((C + 1) / 2);
