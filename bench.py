@@ -39,10 +39,12 @@ COPY_PROBE_BYTES = 512 << 20
 # the FMT reach touches 3.1 MiB) and writes ten rows (20 MiB).
 STAGE_TRAFFIC_MIB_64 = {"el_rows": 33.98 + 2.00, "rb_rows": 26.52 + 20.00}
 STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
-# Mean HBM bytes per k_fft_axis launch of one 64^3 fill in FFT form (fetch + write, MiB): counter runs of
-# this script with --fill-only --steps 50 --warmup 2 --copy-probe under TMI_FMT_FFT=1, mean over the last
-# 720 axis-pass dispatches (60 fills' worth), scaled as above.  Algorithmic: 16 MiB per launch.
-FFT_TRAFFIC_MIB_64 = 8.121 + 8.000
+# Mean HBM bytes per launch of the FFT-form kernels of a 64^3 fill (fetch + write, MiB): counter runs of
+# this script with --fill-only --steps 50 --warmup 2 --copy-probe under TMI_FMT_FFT=1, scaled as above,
+# means over the last 120 fused-x-pass and 300 plain-axis-pass dispatches (fills' launches).
+# Algorithmic: 27.0 MiB per fused x pass, 17.6 MiB per plain axis pass on average.
+FFT_TRAFFIC_MIB_64 = 8.886 + 8.800
+XPROD_TRAFFIC_MIB_64 = 26.701 + 8.000
 
 
 def stencil_flops_per_node(dp, spec):
@@ -313,20 +315,31 @@ def main():
     C = spec.ncomp
     bytes_node = 8.0 * (2 * spec.nunk + C) + C
     if fmt_fft:
-        # every stencil sum is an FFT convolution: the axis passes (k_fft_axis) dominate; each pass
-        # reads and writes every grid point of every field once (32 B per complex-double point)
+        # every stencil sum is an FFT convolution.  Dominant kernel: the fused x pass (k_fft_xprod: forward x
+        # transform, spectral products, inverse x transform) when it is in use, else the axis passes; both
+        # timed with HIP events around their launches in a fill's order (tmi_fft_timed)
         stage_names = ["pad_rho", "rb_mf_rows_fft", "mf_rows_direct", "fmt_point", "el_rows_fft"]
-        ft = dp.fft_timed(reps=max(5, args.steps // 5))
+        fused = dp.info.fmt_fft == 2
+        reps = max(5, args.steps // 5)
+        fp = dp.fft_timed(0, reps)
+        ft = dp.fft_timed(1, reps) if fused else fp
         achieved = ft["bytes"] / (ft["ms"] * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": "k_fft_axis", "achieved": achieved, "peak": PEAK_HBM_GBS,
+        traffic = XPROD_TRAFFIC_MIB_64 if fused else FFT_TRAFFIC_MIB_64
+        roofline = {"bound": "hbm", "kernel": "k_fft_xprod" if fused else "k_fft_axis / k_fft_strided",
+                    "achieved": achieved, "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                    "traffic": (FFT_TRAFFIC_MIB_64 * (1 << 20) if (args.n == 64 and FFT_TRAFFIC_MIB_64) else None),
+                    "traffic": (traffic * (1 << 20) if (args.n == 64 and traffic) else None),
                     "launch_ms": ft["ms"] / ft["launches"], "launches_per_fill": ft["launches"],
                     "algorithmic_bytes_per_launch": ft["bytes"] / ft["launches"],
                     "share_of_fill": ft["ms"] / ms_fill,
-                    "traffic_note": ("mean HBM bytes per k_fft_axis launch of a fill from rocprofv3 --pmc "
-                                     "FETCH_SIZE/WRITE_SIZE, scaled by a 512 MiB copy"),
-                    "note": "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"}
+                    "traffic_note": ("mean HBM bytes per launch of that kernel in a fill, from rocprofv3 --pmc "
+                                     "FETCH_SIZE/WRITE_SIZE scaled by a 512 MiB copy"),
+                    "note": ("fused x passes of one fill: input pairs and spectrum tables read, output pairs written "
+                             "(algorithmic), HIP events on the launch stream" if fused else
+                             "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"),
+                    "axis_passes": {"ms_per_fill": fp["ms"], "launches_per_fill": fp["launches"],
+                                    "achieved_gb_s": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9,
+                                    "frac": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}}
     else:
         # direct stencil sums; the FFT form of the mean-field sums is HBM-bound passes, not
         # direct-form flops, so it is not priced against the FP64 roof (its time is in fill_stage_ms)

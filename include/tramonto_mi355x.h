@@ -158,13 +158,15 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
  * rho-bar rows, mean-field rows, pointwise FMT + adjoint halo, EL rows. */
 int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int reps, double *stage_ms, void *stream);
 /* Measurement hook, no counterpart in the reference: when the FMT sums run in
- * FFT form (tmi_info.fmt_fft), times the FFT axis passes of one residual fill
- * on the problem's work buffers with HIP events on `stream`.  *ms receives
- * their mean time per fill over `reps` fills, *bytes their algorithmic
- * traffic per fill (each pass reads and writes every grid point of every
- * field once, 16 B complex doubles) and *launches the passes per fill.
- * Fails when the FMT sums run as direct stencil sums. */
-int tmi_fft_timed(void *handle, int reps, double *ms, double *bytes, int *launches, void *stream);
+ * FFT form (tmi_info.fmt_fft), times one kernel family of a residual fill on
+ * the problem's work buffers, in the fill's order and batch sizes, with HIP
+ * events on `stream`: which = 0 the plain FFT axis passes, which = 1 the
+ * fused x passes (tmi_info.fmt_fft == 2).  *ms receives their mean time per
+ * fill over `reps` fills, *bytes their algorithmic traffic per fill (axis
+ * pass: every point of every pair field read and written once, 16 B each way;
+ * fused x pass: input pairs and distinct spectrum tables read once, output
+ * pairs written once) and *launches their launches per fill. */
+int tmi_fft_timed(void *handle, int which, int reps, double *ms, double *bytes, int *launches, void *stream);
 /* Jv = J(x) v, the assembled-Jacobian product, src/dft_Schur_Epetra_Operator.cpp:149 (ApplyGlobal) */
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
 /* same product with FP32 stencil sums and FP32 accumulation (FP64 pointwise

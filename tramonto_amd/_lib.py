@@ -107,7 +107,7 @@ def lib() -> ct.CDLL:
     L.tmi_fill_residual.argtypes = [vp, vp, vp, ct.POINTER(d), vp]
     L.tmi_apply_jacobian.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_fill_residual_timed.argtypes = [vp, vp, vp, i, vp, vp]
-    L.tmi_fft_timed.argtypes = [vp, i, ct.POINTER(d), ct.POINTER(d), ct.POINTER(i), vp]
+    L.tmi_fft_timed.argtypes = [vp, i, i, ct.POINTER(d), ct.POINTER(d), ct.POINTER(i), vp]
     L.tmi_apply_jacobian_fp32.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_apply_jacobian_transpose.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_init_fields.argtypes = [vp, vp, vp]

@@ -903,12 +903,12 @@ int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int re
     });
 }
 
-int tmi_fft_timed(void *handle, int reps, double *ms, double *bytes, int *launches, void *stream)
+int tmi_fft_timed(void *handle, int which, int reps, double *ms, double *bytes, int *launches, void *stream)
 {
     return guarded([&] {
         Problem *P = as_problem(handle);
-        if (!ms || !bytes || !launches || reps < 1) throw std::invalid_argument("bad argument");
-        fft_time_passes(*P, reps, ms, bytes, launches, (hipStream_t)stream);
+        if (!ms || !bytes || !launches || reps < 1 || which < 0 || which > 1) throw std::invalid_argument("bad argument");
+        fft_time_kernels(*P, which, reps, ms, bytes, launches, (hipStream_t)stream);
         return 0;
     });
 }

@@ -34,6 +34,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdlib>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1691,53 +1692,77 @@ template void fft_el<double>(Problem &, bool, const double *, const double *, co
 template void fft_el<float>(Problem &, bool, const double *, const double *, const double *, const double *, double *,
                             hipStream_t, double *, long long *);
 
-// Times the plain axis passes (k_fft_axis without fused ends) of one
-// residual fill in FFT form on the plan's work buffers, in the fill's order
-// and batch sizes, with HIP events on s: *ms is their mean time per fill over
-// reps fills, *bytes their algorithmic traffic (every axis pass reads and
-// writes each grid point of each field once) and *launches the passes per
-// fill.  The buffers' contents are scratch between products.
-void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
+// Times one kernel family of a residual fill in FFT form on the plan's work
+// buffers, in the fill's order and batch sizes, with HIP events on s.
+// which = 0: the plain axis passes (without fused ends); which = 1: the fused
+// x passes (density and Euler-Lagrange products).  *ms is their mean time per
+// fill over reps fills, *bytes their algorithmic traffic per fill and
+// *launches their launches per fill.  Algorithmic bytes: an axis pass reads
+// and writes each grid point of each pair field once (16 B each way); a fused
+// x pass reads each input pair and each distinct spectrum table once (16 B,
+// 8 B per point) and writes each output pair once (16 B per point).  The
+// buffers' contents are scratch between products.
+void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
 {
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
     const FftPlan &F = *P.fft;
+    if (which == 1 && !(F.xlpb[0] && F.xlpb[1])) throw std::logic_error("the fused x pass is not in use");
     const int C = P.p.ncomp, nadj = C * (1 + P.p.ndim) + 1;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
-    // which passes run with fused ends (those are other kernels): the density product's first and last
-    // strided passes when it reads the node arrays, the Euler-Lagrange product's last one on wrapping grids
-    const bool rb_ends = F.xlpb[0] && fft_node_input(P), el_ends = F.xlpb[1] && F.wraps;
-    int first_strided = -1, last_strided = -1;
-    for (int a : {2, 1})
-        if (F.M[a] > 1 && first_strided < 0) first_strided = a;
-    for (int a : {1, 2})
-        if (F.M[a] > 1) last_strided = a;
-    struct Set {
-        Cx<double> *buf;
-        int nfield, sign;
-        bool fused;
-        int skip;   // axis run as a fused-end pass, -1 if none
-    };
-    const Set sets[4] = {{Z, (C + 1) / 2, -1, F.xlpb[0] > 0, rb_ends ? first_strided : -1},
-                         {O, (F.fwd[1].nout + 1) / 2, +1, F.xlpb[0] > 0, rb_ends ? last_strided : -1},
-                         {Z, (nadj + 1) / 2, -1, F.xlpb[1] > 0, -1},
-                         {O, (C + 1) / 2, +1, F.xlpb[1] > 0, el_ends ? last_strided : -1}};
-    auto runs = [&](const Set &t, int axis) { return F.M[axis] > 1 && !(t.fused && axis == 0) && axis != t.skip; };
+    std::function<void()> fill;
     int count = 0;
     double b = 0.0;
-    for (const Set &t : sets)
-        for (int axis = 0; axis < 3; ++axis)
-            if (runs(t, axis)) {
-                ++count;
-                b += 2.0 * sizeof(Cx<double>) * (double)F.n * t.nfield;
-            }
-    auto fill = [&] {
+    if (which == 1) {
+        const TermList *tls[2] = {&F.tfwd[1], &F.tel};
+        for (const TermList *tl : tls) {
+            int tables = 0;
+            for (int f = 0; f < tl->nin; ++f)
+                for (int t = tl->first[f]; t < tl->first[f + 1]; ++t)
+                    if (t == tl->first[f] || tl->term[t].t != tl->term[t - 1].t) ++tables;
+            b += (double)F.n * (16.0 * ((tl->nin + 1) / 2) + 16.0 * ((tl->nout + 1) / 2) + 8.0 * tables);
+            ++count;
+        }
+        fill = [&] {
+            fft_xprod<double>(F, Z, O, F.tfwd[1], F.xlpb[0], F.td, F.ud, s);
+            fft_xprod<double>(F, Z, O, F.tel, F.xlpb[1], F.td, F.ud, s);
+        };
+    } else {
+        // which passes run with fused ends (those are other kernels): the density product's first and last
+        // strided passes when it reads the node arrays, the Euler-Lagrange product's last one on wrapping grids
+        const bool rb_ends = F.xlpb[0] && fft_node_input(P), el_ends = F.xlpb[1] && F.wraps;
+        int first_strided = -1, last_strided = -1;
+        for (int a : {2, 1})
+            if (F.M[a] > 1 && first_strided < 0) first_strided = a;
+        for (int a : {1, 2})
+            if (F.M[a] > 1) last_strided = a;
+        struct Set {
+            Cx<double> *buf;
+            int nfield, sign;
+            bool fused;
+            int skip;   // axis run as a fused-end pass, -1 if none
+        };
+        const Set sets[4] = {{Z, (C + 1) / 2, -1, F.xlpb[0] > 0, rb_ends ? first_strided : -1},
+                             {O, (F.fwd[1].nout + 1) / 2, +1, F.xlpb[0] > 0, rb_ends ? last_strided : -1},
+                             {Z, (nadj + 1) / 2, -1, F.xlpb[1] > 0, -1},
+                             {O, (C + 1) / 2, +1, F.xlpb[1] > 0, el_ends ? last_strided : -1}};
+        auto runs = [&](const Set &t, int axis) {
+            return F.M[axis] > 1 && !(t.fused && axis == 0) && axis != t.skip;
+        };
         for (const Set &t : sets)
-            for (int a = 0; a < 3; ++a) {
-                const int axis = t.sign < 0 ? 2 - a : a;   // forward z, y, x; back x, y, z
-                if (runs(t, axis)) fft_pass<double>(F, t.buf, t.nfield, t.sign, axis, s);
-            }
-    };
+            for (int axis = 0; axis < 3; ++axis)
+                if (runs(t, axis)) {
+                    ++count;
+                    b += 2.0 * sizeof(Cx<double>) * (double)F.n * t.nfield;
+                }
+        fill = [&, sets, runs] {
+            for (const Set &t : sets)
+                for (int a = 0; a < 3; ++a) {
+                    const int axis = t.sign < 0 ? 2 - a : a;   // forward z, y, x; back x, y, z
+                    if (runs(t, axis)) fft_pass<double>(F, t.buf, t.nfield, t.sign, axis, s);
+                }
+        };
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float t = 0.f;
     try {

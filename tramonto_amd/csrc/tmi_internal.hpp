@@ -219,7 +219,7 @@ template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,
             double *out_el, hipStream_t s, double *ss = nullptr, long long *nss = nullptr);
 double *fft_ss_buffer(const Problem &P, long long *cap);   // partial sums of F^2 of the fused last passes
-void fft_time_passes(Problem &P, int reps, double *ms, double *bytes, int *launches, hipStream_t s);
+void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes, int *launches, hipStream_t s);
 // bulk-circulant Schur preconditioner on wrapping FFT grids (tmi_fft.hip)
 void precond_setup(Problem &P);
 bool precond_available(const Problem &P);

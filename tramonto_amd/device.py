@@ -133,12 +133,13 @@ class DeviceProblem:
                    "tmi_fill_residual_timed")
         return ms
 
-    def fft_timed(self, reps: int = 10) -> Dict:
-        """FFT axis passes of one residual fill in FFT form (HIP events on the launch
-        stream): mean ms per fill, algorithmic bytes per fill, launches per fill."""
+    def fft_timed(self, which: int = 0, reps: int = 10) -> Dict:
+        """One kernel family of a residual fill in FFT form (HIP events on the launch stream):
+        which = 0 the plain axis passes, 1 the fused x passes; mean ms per fill, algorithmic
+        bytes per fill, launches per fill."""
         ms, nbytes, launches = ct.c_double(), ct.c_double(), ct.c_int()
-        _lib.check(_lib.lib().tmi_fft_timed(self.handle, reps, ct.byref(ms), ct.byref(nbytes), ct.byref(launches),
-                                            _stream()), "tmi_fft_timed")
+        _lib.check(_lib.lib().tmi_fft_timed(self.handle, which, reps, ct.byref(ms), ct.byref(nbytes),
+                                            ct.byref(launches), _stream()), "tmi_fft_timed")
         return dict(ms=ms.value, bytes=nbytes.value, launches=launches.value)
 
     def apply_jacobian(self, x: torch.Tensor, v: torch.Tensor, out: Optional[torch.Tensor] = None,
