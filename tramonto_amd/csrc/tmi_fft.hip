@@ -848,6 +848,161 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
     }
 }
 
+// Fused x pass with the x transforms in registers, for x-lines of length M = R1 R2 (R1 = 8 or 16,
+// R2 = 8 or 4) and NOP output pairs whose NOP x R1 spectrum values fit in registers.  The
+// workgroup has 2 L R2 threads: lanes (i, n2) of the first half take line i of L consecutive
+// x-lines (y, z), the second half their mirror lines (-y, -z).  Per input pair every lane loads
+// elements R2 n1 + n2 of its line, runs an R1-point DFT, the twiddles W_M^(n2 k1), a transpose
+// through LDS and R1/R2 R2-point DFTs, leaving spectrum values k = n2 + R2 q + R1 k2; those go
+// back to LDS in natural order, and the first half reads the mirror spectrum at -k, splits the
+// pair and accumulates the terms into output pairs held in registers.  The outputs return
+// through LDS two pairs at a time (one per half), through the same two-stage transform with
+// SIGN +1, and are stored for the y and z passes.
+template <typename T, int R1, int R2, int NOP>
+__global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T> *__restrict__ Zin,
+                                                                    Cx<T> *__restrict__ Oout, long long ns, int M1,
+                                                                    int M2, long long S1, long long S2, int L,
+                                                                    const Cx<T> *__restrict__ twtab,
+                                                                    const T *__restrict__ tk, const T *__restrict__ uk,
+                                                                    TermList tl)
+{
+    constexpr int M = R1 * R2, LD = M + 1;
+    extern __shared__ char smem_raw[];
+    Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);   // [M]
+    Cx<T> *buf = tw + M;                               // [2 L][LD]: lines, then mirror lines
+    const int tid = threadIdx.x, half = tid / (L * R2), r0 = tid - half * L * R2;
+    const int i = r0 / R2, n2 = r0 - i * R2;
+    const int nth = 2 * L * R2;
+    for (int j = tid; j < M; j += nth) tw[j] = twtab[j];
+    const int l = blockIdx.x * L + i;
+    const bool live = l < M1 * M2;
+    int y = 0, z = 0;
+    if (live) {
+        z = l / M1;
+        y = l - z * M1;
+    }
+    const int ym = y ? M1 - y : 0, zm = z ? M2 - z : 0;
+    const long long row = (long long)y * S1 + (long long)z * S2;
+    const long long src_row = half ? (long long)ym * S1 + (long long)zm * S2 : row;
+    Cx<T> *mine = buf + (size_t)(half * L + i) * LD;   // this lane's line in LDS
+    const Cx<T> *mirror = buf + (size_t)(L + i) * LD;  // the mirror line of line i
+    // spectrum position of register r (= q R2 + k2)
+    auto kpos = [&](int r) { return n2 + R2 * (r / R2) + R1 * (r % R2); };
+    Cx<T> acc[NOP][R1];
+#pragma unroll
+    for (int p = 0; p < NOP; ++p)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) acc[p][r] = Cx<T>{(T)0, (T)0};
+    __syncthreads();
+    // forward transform of the lane's line from registers v (elements R2 n1 + n2) into X (positions kpos)
+    auto forward = [&](Cx<T> (&v)[R1], Cx<T> (&X)[R1]) {
+        dft_reg<R1, -1>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) mine[k1 * R2 + n2] = cmul(v[k1], tw[n2 * k1]);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < R1 / R2; ++q) {
+            const int k1 = n2 + R2 * q;
+            Cx<T> u[R2];
+#pragma unroll
+            for (int m = 0; m < R2; ++m) u[m] = mine[k1 * R2 + m];
+            dft_reg<R2, -1>(u);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) X[q * R2 + k2] = u[k2];
+        }
+        __syncthreads();
+    };
+    const int nip = (tl.nin + 1) / 2;
+    for (int jp = 0; jp < nip; ++jp) {
+        const Cx<T> *src = Zin + (size_t)jp * ns;
+        Cx<T> v[R1], X[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? src[src_row + R2 * n1 + n2] : Cx<T>{(T)0, (T)0};
+        forward(v, X);
+        // spectra in natural order: every half writes its own line
+#pragma unroll
+        for (int r = 0; r < R1; ++r) mine[kpos(r)] = X[r];
+        __syncthreads();
+        if (half == 0 && live) {
+            Cx<T> Yr[R1];   // mirror spectrum at -k
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const int k = kpos(r);
+                Yr[r] = mirror[k ? M - k : 0];
+            }
+            for (int h = 0; h < 2; ++h) {
+                const int f = 2 * jp + h;
+                if (f >= tl.nin) break;
+                int last = 0x7fffffff;
+                T kt[R1];
+                for (int t = tl.first[f]; t < tl.first[f + 1]; ++t) {
+                    const Term tm = tl.term[t];
+                    if (tm.t != last) {
+                        const T *tab = tm.t >= 0 ? tk + (size_t)tm.t * ns : uk + (size_t)(-tm.t - 1) * ns;
+#pragma unroll
+                        for (int r = 0; r < R1; ++r) kt[r] = tab[row + kpos(r)];
+                        last = tm.t;
+                    }
+                    const int p = tm.o >> 1, odd = tm.o & 1;
+                    const T cf = (T)tm.coef;
+#pragma unroll
+                    for (int r = 0; r < R1; ++r) {
+                        const T kv = cf * kt[r];
+                        const Cx<T> xh = unpack(X[r], Yr[r], h);
+                        const Cx<T> yv = tm.imag ? Cx<T>{-kv * xh.y, kv * xh.x} : Cx<T>{kv * xh.x, kv * xh.y};
+                        const Cx<T> add = odd ? Cx<T>{-yv.y, yv.x} : yv;
+#pragma unroll
+                        for (int pp = 0; pp < NOP; ++pp)
+                            if (pp == p) acc[pp][r] = cadd(acc[pp][r], add);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // output pairs back along x, two at a time: the first half writes both, each half transforms one
+    const int nop = (tl.nout + 1) / 2;
+#pragma unroll
+    for (int p0 = 0; p0 < NOP; p0 += 2) {
+        if (p0 >= nop) break;
+        if (half == 0) {
+#pragma unroll
+            for (int r = 0; r < R1; ++r) buf[(size_t)i * LD + kpos(r)] = acc[p0][r];
+            if (p0 + 1 < NOP) {
+#pragma unroll
+                for (int r = 0; r < R1; ++r) buf[(size_t)(L + i) * LD + kpos(r)] = acc[p0 + 1 < NOP ? p0 + 1 : p0][r];
+            }
+        }
+        __syncthreads();
+        const int p = p0 + half;
+        Cx<T> v[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = mine[R2 * n1 + n2];
+        __syncthreads();
+        dft_reg<R1, 1>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            Cx<T> w = tw[n2 * k1];
+            w.y = -w.y;
+            mine[k1 * R2 + n2] = cmul(v[k1], w);
+        }
+        __syncthreads();
+        if (p < nop && live) {
+#pragma unroll
+            for (int q = 0; q < R1 / R2; ++q) {
+                const int k1 = n2 + R2 * q;
+                Cx<T> u[R2];
+#pragma unroll
+                for (int m = 0; m < R2; ++m) u[m] = mine[k1 * R2 + m];
+                dft_reg<R2, 1>(u);
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) Oout[(size_t)p * ns + row + k1 + R1 * k2] = u[k2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Euler-Lagrange rows from the sums E_c, as k_el_rows:
 //  residual: E + ln x + mu [+ V_ext] [+ x_MF]      ; zero rows: x   (in_rho)
 //  linear  : E + v/x + v_MF                        ; zero rows: v
@@ -1173,11 +1328,59 @@ int xprod_lpb(const FftPlan &F, int nop)
     return (M * sizeof(Cx<double>) + lpb * per_line <= kXprodLds) ? lpb : 0;
 }
 
+// the register form of the fused x pass where the x length has register radices and the output
+// pairs' spectrum values fit in registers (NOP R1 <= 32); false otherwise
+template <typename T>
+bool xprod_reg(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, const T *tk, const T *uk, hipStream_t s)
+{
+    static const bool off = [] {   // TMI_FFT_XREG=0 keeps the LDS form of the fused x pass
+        const char *e = std::getenv("TMI_FFT_XREG");
+        return e && std::string(e) == "0";
+    }();
+    int R1 = 0, R2 = 0;
+    if (off || !reg_radices(F.M[0], R1, R2)) return false;
+    const int nop = (tl.nout + 1) / 2;
+    if (nop > 4 || nop * R1 > 32) return false;
+    const int L = 64 / R2;
+    const long long lines = F.n / F.M[0];
+    const dim3 grid((unsigned)((lines + L - 1) / L)), block(2 * L * R2);
+    const size_t lds = sizeof(Cx<T>) * ((size_t)F.M[0] + 2 * (size_t)L * (F.M[0] + 1));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, Z, O, F.ns, F.M[1], F.M[2], F.S1, F.S2, L, twiddles<T>(F, 0), tk,
+                           uk, tl);
+    };
+    auto by_nop = [&](auto r1, auto r2) {
+        constexpr int A = decltype(r1)::value, B = decltype(r2)::value;
+        if constexpr (A * 1 <= 32) {
+            if (nop == 1) go(k_fft_xprod_reg<T, A, B, 1>);
+        }
+        if constexpr (A * 2 <= 32) {
+            if (nop == 2) go(k_fft_xprod_reg<T, A, B, 2>);
+        }
+        if constexpr (A * 3 <= 32) {
+            if (nop == 3) go(k_fft_xprod_reg<T, A, B, 3>);
+        }
+        if constexpr (A * 4 <= 32) {
+            if (nop == 4) go(k_fft_xprod_reg<T, A, B, 4>);
+        }
+    };
+    if (R1 == 8 && R2 == 4)
+        by_nop(std::integral_constant<int, 8>{}, std::integral_constant<int, 4>{});
+    else if (R1 == 8 && R2 == 8)
+        by_nop(std::integral_constant<int, 8>{}, std::integral_constant<int, 8>{});
+    else if (R1 == 16 && R2 == 8)
+        by_nop(std::integral_constant<int, 16>{}, std::integral_constant<int, 8>{});
+    else
+        by_nop(std::integral_constant<int, 16>{}, std::integral_constant<int, 16>{});
+    return true;
+}
+
 // product of input pairs Z (y and z already transformed) into output pairs O (y and z still to invert)
 template <typename T>
 void fft_xprod(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, int lpb, const T *tk, const T *uk,
                hipStream_t s)
 {
+    if (xprod_reg<T>(F, Z, O, tl, tk, uk, s)) return;
     const int M = F.M[0], nop = (tl.nout + 1) / 2;
     const long long lines = F.n / M;
     const RadixCounts rc{F.rc[0][0], F.rc[0][1], F.rc[0][2], F.rc[0][3]};
