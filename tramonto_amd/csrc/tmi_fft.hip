@@ -1723,10 +1723,10 @@ bool fft_fused(const Problem &P) { return P.fft && (P.fft->xlpb[0] || P.fft->xlp
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
 bool fft_wraps(const Problem &P) { return P.fft && P.fft->wraps; }
 bool fft_fused_available(const Problem &P) { return P.fft && (P.fft->xfit[0] || P.fft->xfit[1]); }
-void fft_set_fused(Problem &P, bool on)
+void fft_set_fused(Problem &P, int mask)
 {
     if (!P.fft) return;
-    for (int w = 0; w < 2; ++w) P.fft->xlpb[w] = on ? P.fft->xfit[w] : 0;
+    for (int w = 0; w < 2; ++w) P.fft->xlpb[w] = ((mask >> w) & 1) ? P.fft->xfit[w] : 0;
 }
 void fft_set_fmt(Problem &P, bool on)
 {
@@ -1909,7 +1909,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
 {
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
     const FftPlan &F = *P.fft;
-    if (which == 1 && !(F.xlpb[0] && F.xlpb[1])) throw std::logic_error("the fused x pass is not in use");
+    if (which == 1 && !(F.xlpb[0] || F.xlpb[1])) throw std::logic_error("the fused x pass is not in use");
     const int C = P.p.ncomp, nadj = C * (1 + P.p.ndim) + 1;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
@@ -1918,7 +1918,9 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
     double b = 0.0;
     if (which == 1) {
         const TermList *tls[2] = {&F.tfwd[1], &F.tel};
-        for (const TermList *tl : tls) {
+        for (int w = 0; w < 2; ++w) {
+            if (!F.xlpb[w]) continue;
+            const TermList *tl = tls[w];
             int tables = 0;
             for (int f = 0; f < tl->nin; ++f)
                 for (int t = tl->first[f]; t < tl->first[f + 1]; ++t)
@@ -1927,8 +1929,8 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
             ++count;
         }
         fill = [&] {
-            fft_xprod<double>(F, Z, O, F.tfwd[1], F.xlpb[0], F.td, F.ud, s);
-            fft_xprod<double>(F, Z, O, F.tel, F.xlpb[1], F.td, F.ud, s);
+            if (F.xlpb[0]) fft_xprod<double>(F, Z, O, F.tfwd[1], F.xlpb[0], F.td, F.ud, s);
+            if (F.xlpb[1]) fft_xprod<double>(F, Z, O, F.tel, F.xlpb[1], F.td, F.ud, s);
         };
     } else {
         // which passes run with fused ends (those are other kernels): the density product's first and last

@@ -201,7 +201,7 @@ bool fft_wraps(const Problem &P);           // every axis periodic on the FFT gr
 void fft_set_fmt(Problem &P, bool on);
 bool fft_fused_available(const Problem &P);   // the fused x pass fits the LDS budget
 bool fft_fused(const Problem &P);             // products through the fused x pass
-void fft_set_fused(Problem &P, bool on);      // products through the fused x pass
+void fft_set_fused(Problem &P, int mask);     // fused x pass for the density (bit 0) / Euler-Lagrange (bit 1) product
 // density products; rho_nodes (node arrays, fft_node_input) or, if null, workspace<T>(P).rho_pad
 // ss (optional): partial sums of the squares of the rows written by a fused last pass, *nss of them
 template <typename T>

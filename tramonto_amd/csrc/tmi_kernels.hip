@@ -1424,26 +1424,28 @@ void autotune_rows1d(Problem &P)
 // ---------------------------------------------------------------- FMT form
 // Times residual fills with the FMT sums as direct stencil sums, as FFT
 // convolutions with separate x passes and product kernels, and with the fused
-// x pass (state: every unknown 4.8e-4, a positive byte pattern), and keeps
-// the fastest.  TMI_FMT_FFT=1 / 0 forces the FFT / direct form (the fused x
-// pass is then used where it fits, unless TMI_FFT_XFUSE=0).
+// x pass for the density product, the Euler-Lagrange product or both (state:
+// every unknown 4.8e-4, a positive byte pattern), and keeps the fastest.
+// TMI_FMT_FFT=1 / 0 forces the FFT / direct form (the fused x pass is then
+// used for both products where it fits, unless TMI_FFT_XFUSE=0).
 void choose_fmt_form(Problem &P)
 {
     if (!fft_fmt_available(P)) return;
     const char *env = std::getenv("TMI_FMT_FFT"), *xf = std::getenv("TMI_FFT_XFUSE");
-    if (env) {   // TMI_FFT_XFUSE=0 keeps separate x passes and product kernels
+    if (env) {
         fft_set_fmt(P, std::string(env) == "1");
-        fft_set_fused(P, !(xf && std::string(xf) == "0"));
+        fft_set_fused(P, (xf && std::string(xf) == "0") ? 0 : 3);
         return;
     }
     const size_t n = (size_t)P.nunk * P.N;
     double *x = nullptr, *F = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    float ms[3] = {0.f, 0.f, 0.f};
-    const int nform = fft_fused_available(P) ? 3 : 2;
+    // forms: 0 direct; 1 + mask: FFT with the fused x pass for the products in mask (bit 0 density, bit 1 EL)
+    const int nform = fft_fused_available(P) ? 5 : 2;
+    float ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     auto set_form = [&](int form) {
         fft_set_fmt(P, form > 0);
-        fft_set_fused(P, form == 2);
+        fft_set_fused(P, form > 0 ? form - 1 : 0);
     };
     try {
         TMI_HIP_CHECK(hipMalloc((void **)&x, n * sizeof(double)));
