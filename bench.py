@@ -44,7 +44,7 @@ STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
 # means over the last 120 fused-x-pass and 300 plain-axis-pass dispatches (fills' launches).
 # Algorithmic: 27.0 MiB per fused x pass, 17.6 MiB per plain axis pass on average.
 FFT_TRAFFIC_MIB_64 = 8.886 + 8.800
-XPROD_TRAFFIC_MIB_64 = 26.701 + 8.000
+XPROD_TRAFFIC_MIB_64 = 26.913 + 8.000
 
 
 def stencil_flops_per_node(dp, spec):
