@@ -80,6 +80,8 @@ typedef struct tmi_info {
     int fmt_fft;  /* FMT sums (rho-bar and Euler-Lagrange rows): 0 direct, 1 FFT convolutions, 2 FFT with the x pass
                      fused with the spectral products (2D/3D, whichever fills faster) */
     int precond;  /* preconditioner of the Schur GMRES: TMI_PRECOND_DIAG or TMI_PRECOND_BULK */
+    int fft_fold; /* 1: the stencil spectra of the FFT sums are stored for k_y, k_z >= 0 only (every stencil
+                     symmetric under the reflection of y and of z), 0: for the whole FFT grid or no FFT sums */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */

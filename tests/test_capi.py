@@ -95,3 +95,25 @@ def test_rejects_unsupported_problems():
     rc = _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None)
     assert rc == -2
     assert b"lhard_surf" in _lib.lib().tmi_last_error()
+
+
+@pytest.mark.parametrize("case", ["2d_periodic", "2d_wall_reflect", "2d_rough_wall", "3d_periodic_small",
+                                  "3d_periodic_wb", "3d_mix_bulk", "3d_solute_small"])
+def test_host_stencils_reflection_symmetric(case):
+    """The FFT sums store each stencil spectrum for k_y, k_z >= 0 only: that needs the
+    weights at o and at o with one component negated to agree (to 1e-13 of the largest weight)
+    for every axis, which the sphere and shell quadratures give."""
+    sp = small_case(case)
+    for t in range(3 if sp.type_attr >= 0 else 2):
+        for i in range(sp.ncomp):
+            for j in range(sp.ncomp if t == 2 else 1):
+                off, w = host_stencil(sp, t, i, j)
+                at = {}
+                for o, v in zip(map(tuple, off), w):
+                    at[o] = at.get(o, 0.0) + v
+                wmax = np.abs(w).max()
+                for o, v in at.items():
+                    for a in range(sp.ndim):
+                        r = list(o)
+                        r[a] = -r[a]
+                        assert abs(at.get(tuple(r), np.inf) - v) <= 1e-13 * wmax, (t, i, j, o, a)

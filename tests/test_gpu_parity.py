@@ -233,16 +233,20 @@ def fft_pair():
     import os
     from tramonto_amd.device import DeviceProblem
     cache = {}
-    forms = {"fft": ("1", "1", "1"), "fft_unfused": ("1", "1", "0"), "mf": ("1", "0", "1"), "direct": ("0", "0", "1")}
+    # (TMI_MF_FFT, TMI_FMT_FFT, TMI_FFT_XFUSE, TMI_FFT_FOLD)
+    forms = {"fft": ("1", "1", "1", "1"), "fft_unfused": ("1", "1", "0", "1"), "fft_whole_tables": ("1", "1", "1", "0"),
+             "mf": ("1", "0", "1", "1"), "direct": ("0", "0", "1", "1")}
+    keys = ("TMI_MF_FFT", "TMI_FMT_FFT", "TMI_FFT_XFUSE", "TMI_FFT_FOLD")
 
     def get(case):
         if case not in cache:
             spec = small_case(case)
-            old = {k: os.environ.get(k) for k in ("TMI_MF_FFT", "TMI_FMT_FFT", "TMI_FFT_XFUSE")}
+            old = {k: os.environ.get(k) for k in keys}
             probs = {}
             try:
-                for name, (mf, fmt, fuse) in forms.items():
-                    os.environ["TMI_MF_FFT"], os.environ["TMI_FMT_FFT"], os.environ["TMI_FFT_XFUSE"] = mf, fmt, fuse
+                for name, vals in forms.items():
+                    for k, v in zip(keys, vals):
+                        os.environ[k] = v
                     probs[name] = DeviceProblem(spec)
             finally:
                 for k, v in old.items():
@@ -250,16 +254,18 @@ def fft_pair():
                         os.environ.pop(k, None)
                     else:
                         os.environ[k] = v
-            for name, (mf, fmt, fuse) in forms.items():
+            for name, (mf, fmt, fuse, fold) in forms.items():
                 want_fmt = 0 if fmt == "0" else (2 if fuse == "1" else 1)
                 assert (probs[name].info.mf_fft, probs[name].info.fmt_fft) == (int(mf), want_fmt), name
+                # the stencils here are reflection-symmetric: their tables fold unless told not to
+                assert probs[name].info.fft_fold == (int(fold) if name != "direct" else 0), name
             cache[case] = (spec, Oracle.from_spec(spec), probs)
         return cache[case]
     return get
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["fft", "fft_unfused", "mf"])
+@pytest.mark.parametrize("form", ["fft", "fft_unfused", "fft_whole_tables", "mf"])
 @pytest.mark.parametrize("case", FFT_CASES)
 def test_fft_residual_matches_oracle(case, form, fft_pair):
     spec, o, probs = fft_pair(case)

@@ -31,7 +31,7 @@ class TmiInfo(ct.Structure):
     _fields_ = [
         ("nunk", ct.c_int), ("nnodes", ct.c_int), ("iu_rho", ct.c_int), ("iu_rb", ct.c_int), ("n_rb", ct.c_int),
         ("iu_mf", ct.c_int), ("n_mf", ct.c_int), ("sten_len", (ct.c_int * (NC * NC)) * 3), ("mf_fft", ct.c_int),
-        ("fmt_fft", ct.c_int), ("precond", ct.c_int),
+        ("fmt_fft", ct.c_int), ("precond", ct.c_int), ("fft_fold", ct.c_int),
     ]
 
 

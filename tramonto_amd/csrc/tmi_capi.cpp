@@ -565,8 +565,19 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     if (!bulk_pc) dev_a22_inverse_diag(P, x, dinv, s);
     TMI_HIP_CHECK(hipMemsetAsync(x2, 0, sizeof(double) * n, s));
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
-    // device slots of the Arnoldi coefficients (past the partial sums of the dot products)
-    double *dh = P.d_red + 1024 * 1024 / 2, *dh2 = dh + 512, *dhn = dh + 1024;
+    // device slot of the Arnoldi coefficients (past the partial sums of the dot products)
+    double *dh = P.d_red + 1024 * 1024 / 2;
+    // w = S M^-1 V_j, the products of an Arnoldi step
+    auto products = [&](int j) {
+        if (bulk_pc)
+            precond_apply(P, x_rho, V + (size_t)j * n, z, s);
+        else
+            dev_mul(dinv, V + (size_t)j * n, z, n, s);
+        if (prec == TMI_PREC_MIXED)
+            launch_schur_apply<float>(P, x, z, t1, w, s);
+        else
+            launch_schur_apply<double>(P, x, z, t1, w, s);
+    };
     const double bn0 = std::sqrt(dev_dot(P, bS, bS, n, s));
     const double bn = bn0 > 0.0 ? bn0 : 1.0;
     int its = 0;
@@ -592,28 +603,14 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
         int k = 0;
         bool done = false;
         for (; k < m && its < maxit; ++k) {
-            if (bulk_pc)
-                precond_apply(P, x_rho, V + (size_t)k * n, z, s);
-            else
-                dev_mul(dinv, V + (size_t)k * n, z, n, s);
-            if (prec == TMI_PREC_MIXED)
-                launch_schur_apply<float>(P, x, z, t1, w, s);
-            else
-                launch_schur_apply<double>(P, x, z, t1, w, s);
+            products(k);
             // classical Gram-Schmidt and one reorthogonalisation pass with the coefficients kept
             // on the device; one host synchronisation per step, for the Hessenberg column
-            dev_multi_dot_async(P, V, n, k + 1, w, n, dh, s);
-            dev_gemv_sub(V, n, k + 1, dh, w, n, s);
-            dev_multi_dot_async(P, V, n, k + 1, w, n, dh2, s);
-            dev_gemv_sub(V, n, k + 1, dh2, w, n, s);
-            dev_multi_dot_async(P, w, 0, 1, w, n, dhn, s);
-            dev_scale_by_norm(dhn, w, V + (size_t)(k + 1) * n, n, s);
-            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, dh, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red + 512, dh2, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red + 1023, dhn, sizeof(double), hipMemcpyDeviceToHost, s));
+            dev_cgs2(P, V, n, k + 1, w, n, V + (size_t)(k + 1) * n, dh, s);
+            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, dh, sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, s));
             TMI_HIP_CHECK(hipStreamSynchronize(s));
-            for (int j = 0; j <= k; ++j) h[j] = P.h_red[j] + P.h_red[512 + j];
-            const double hn = std::sqrt(P.h_red[1023]);
+            for (int j = 0; j <= k; ++j) h[j] = P.h_red[j] + P.h_red[k + 1 + j];
+            const double hn = std::sqrt(P.h_red[2 * (k + 1)]);
             for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = h[j];
             H[(size_t)(k + 1) * m + k] = hn;
             for (int j = 0; j < k; ++j) {
@@ -778,6 +775,7 @@ int tmi_problem_info(void *handle, tmi_info *info)
         info->mf_fft = fft_mf(*P) ? 1 : 0;
         info->fmt_fft = fft_fmt(*P) ? (fft_fused(*P) ? 2 : 1) : 0;
         info->precond = precond_active(*P) ? 1 : 0;
+        info->fft_fold = fft_folded(*P) ? 1 : 0;
         return 0;
     });
 }

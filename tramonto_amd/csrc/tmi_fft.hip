@@ -38,6 +38,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "tmi_device.hpp"
@@ -76,6 +77,7 @@ constexpr int kMaxTerms = 64;
 constexpr int kMaxIn = kNC * 4 + 1;   // adjoint fields C(1+D)+1
 struct Term {
     int o, t, imag;
+    int odd;   // axis in which the table is odd (vector delta d: d), -1 if even
     double coef;
 };
 struct TermList {
@@ -100,10 +102,15 @@ struct FftPlan {
     bool mf = false;            // mean-field sums in FFT form
     bool fmt_ok = false;        // FMT spectra built
     bool fmt = false;           // FMT sums (rho-bar, Euler-Lagrange) in FFT form
-    double *ud = nullptr;       // [C*C][ns] attraction spectra (real), index icomp + C*jcomp as the stencils
+    double *ud = nullptr;       // [C*C][tn] attraction spectra (real), index icomp + C*jcomp as the stencils
     float *uf = nullptr;
-    double *td = nullptr;       // [C][2+nd][ns] theta, delta (real) and vector-delta d (imaginary part) spectra
+    double *td = nullptr;       // [C][2+nd][tn] theta, delta (real) and vector-delta d (imaginary part) spectra
     float *tf = nullptr;
+    // layout of the spectrum tables (ud, uf, td, tf): with fold set (every stencil symmetric under the
+    // reflection of y and of z) only 0 <= k_y <= M_y/2, 0 <= k_z <= M_z/2 is stored, pitches tp1 and tp2;
+    // otherwise the storage layout of the transforms (pitches S1, S2)
+    bool fold = false;
+    long long tp1 = 1, tp2 = 1, tn = 1;   // pitch of y rows and z planes, elements per table
     Cx<double> *twd[3] = {nullptr, nullptr, nullptr};   // exp(-2 pi i j / M_d), j < M_d
     Cx<float> *twf[3] = {nullptr, nullptr, nullptr};
     void *zbuf = nullptr;       // transforms of the input pairs
@@ -577,15 +584,31 @@ __global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, lo
     }
 }
 
+// Layout of the spectrum tables.  Folded (fold = 1): the table of a stencil that is symmetric
+// under the reflection of each axis is even in every k_a (theta, delta, attraction) or odd in
+// k_d (vector delta d), so it is stored for 0 <= k_y <= M_y/2 and 0 <= k_z <= M_z/2 only (whole
+// x rows, which keep the x index affine in the fused x pass) and T(k) = s T(fold(k)) with
+// fold(k_a) = min(k_a, M_a - k_a) for a = y, z and s = -1 when the table is odd in y or z and
+// that k_a > M_a/2.  Unfolded: the transforms' layout.
+struct TabGeo {
+    int fold, M[3];
+    long long p1, p2, tn;   // pitches of y rows and z planes, elements per table
+};
+__device__ inline bool tab_neg(const TabGeo &t, int a, int k) { return a > 0 && t.fold && 2 * k > t.M[a]; }
+__device__ inline int tab_fold(const TabGeo &t, int a, int k) { return tab_neg(t, a, k) ? t.M[a] - k : k; }
+
 struct FftGeo {
     int M[3], off[3];
     int lo[3], lim[3], src[3];
     long long n, S1, S2, ns;
+    TabGeo tg;
 };
 
-// storage index of FFT-grid point m and of its negative -m
+// storage index of FFT-grid point m and of its negative -m; table index of m and the signs of
+// an odd table's value there (bit d of neg: the table odd in axis d changes sign)
 struct KIdx {
-    long long q, qn;
+    long long q, qn, t;
+    int neg;
 };
 __device__ inline KIdx k_index(const FftGeo &f, long long m)
 {
@@ -593,7 +616,10 @@ __device__ inline KIdx k_index(const FftGeo &f, long long m)
     const long long r = m / f.M[0];
     const int m1 = (int)(r % f.M[1]), m2 = (int)(r / f.M[1]);
     const int n0 = m0 ? f.M[0] - m0 : 0, n1 = m1 ? f.M[1] - m1 : 0, n2 = m2 ? f.M[2] - m2 : 0;
-    return KIdx{m0 + m1 * f.S1 + m2 * f.S2, n0 + n1 * f.S1 + n2 * f.S2};
+    const TabGeo &t = f.tg;
+    return KIdx{m0 + m1 * f.S1 + m2 * f.S2, n0 + n1 * f.S1 + n2 * f.S2,
+                tab_fold(t, 0, m0) + tab_fold(t, 1, m1) * t.p1 + tab_fold(t, 2, m2) * t.p2,
+                (tab_neg(t, 0, m0) ? 1 : 0) | (tab_neg(t, 1, m1) ? 2 : 0) | (tab_neg(t, 2, m2) ? 4 : 0)};
 }
 
 // spectrum of the real field carried as the real (odd = 0) or imaginary
@@ -673,7 +699,8 @@ __global__ void k_fwd_mul(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z, cons
 #pragma unroll
                 for (int c = 0; c < kNC; ++c) {
                     if (c < d.C) {
-                        const T w = (T)d.coef[o][c] * tk[((size_t)c * (2 + ND) + kind) * f.ns + ix.q];
+                        T w = (T)d.coef[o][c] * tk[((size_t)c * (2 + ND) + kind) * f.tg.tn + ix.t];
+                        if (kind >= 2 && ((ix.neg >> (kind - 2)) & 1)) w = -w;
                         if (kind < 2)
                             y = cadd(y, cscale(X[c], w));
                         else   // imaginary spectrum: (i w) X
@@ -684,7 +711,7 @@ __global__ void k_fwd_mul(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z, cons
                 const int row = kind - kKindMf;
 #pragma unroll
                 for (int c = 0; c < kNC; ++c)
-                    if (c < d.C) y = cadd(y, cscale(X[c], uk[(size_t)(row + d.C * c) * f.ns + ix.q]));
+                    if (c < d.C) y = cadd(y, cscale(X[c], uk[(size_t)(row + d.C * c) * f.tg.tn + ix.t]));
             }
             if ((o & 1) == 0) {
                 acc = y;
@@ -744,11 +771,12 @@ __global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *
 #pragma unroll
     for (int c = 0; c < kNC; ++c) {
         if (c < C) {
-            const T *t = tk + (size_t)c * (2 + ND) * f.ns + ix.q;
-            Cx<T> e = cadd(cscale(S3, t[0]), cscale(field(c * (1 + ND)), t[f.ns]));
+            const T *t = tk + (size_t)c * (2 + ND) * f.tg.tn + ix.t;
+            Cx<T> e = cadd(cscale(S3, t[0]), cscale(field(c * (1 + ND)), t[f.tg.tn]));
 #pragma unroll
             for (int dd = 0; dd < ND; ++dd) {
-                const T kv = t[(size_t)(2 + dd) * f.ns];
+                T kv = t[(size_t)(2 + dd) * f.tg.tn];
+                if ((ix.neg >> dd) & 1) kv = -kv;
                 const Cx<T> B = field(c * (1 + ND) + 1 + dd);
                 e = cadd(e, Cx<T>{kv * B.y, -kv * B.x});   // -(i kv) B
             }
@@ -772,7 +800,7 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
                                                          long long ns, int M0, int M1, int M2, long long S1,
                                                          long long S2, RadixCounts rc, int lpb,
                                                          const Cx<T> *__restrict__ twtab, const T *__restrict__ tk,
-                                                         const T *__restrict__ uk, TermList tl)
+                                                         const T *__restrict__ uk, TabGeo tg, TermList tl)
 {
     extern __shared__ char smem_raw[];
     const int M = M0, ld = M + 1;
@@ -810,7 +838,8 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
             const int i = small_div(e, M, inv_m), k = e - i * M;
             int y, z;
             yz(i, y, z);
-            const long long q = k + y * S1 + z * S2;
+            const long long q = tab_fold(tg, 0, k) + tab_fold(tg, 1, y) * tg.p1 + tab_fold(tg, 2, z) * tg.p2;
+            const int neg = (tab_neg(tg, 0, k) ? 1 : 0) | (tab_neg(tg, 1, y) ? 2 : 0) | (tab_neg(tg, 2, z) ? 4 : 0);
             const Cx<T> a = fa[i * ld + k], b = fb[i * ld + (k ? M - k : 0)];
             for (int h = 0; h < 2; ++h) {
                 const int f = 2 * jp + h;
@@ -821,7 +850,8 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
                 for (int t = tl.first[f]; t < tl.first[f + 1]; ++t) {
                     const Term tm = tl.term[t];
                     if (tm.t != last) {
-                        kt = tm.t >= 0 ? tk[(size_t)tm.t * ns + q] : uk[(size_t)(-tm.t - 1) * ns + q];
+                        kt = tm.t >= 0 ? tk[(size_t)tm.t * tg.tn + q] : uk[(size_t)(-tm.t - 1) * tg.tn + q];
+                        if (tm.odd >= 0 && ((neg >> tm.odd) & 1)) kt = -kt;
                         last = tm.t;
                     }
                     const T kv = (T)tm.coef * kt;
@@ -864,7 +894,7 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
                                                                     int M2, long long S1, long long S2, int L,
                                                                     const Cx<T> *__restrict__ twtab,
                                                                     const T *__restrict__ tk, const T *__restrict__ uk,
-                                                                    TermList tl)
+                                                                    TabGeo tg, TermList tl)
 {
     constexpr int M = R1 * R2, LD = M + 1;
     extern __shared__ char smem_raw[];
@@ -884,6 +914,8 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
     const int ym = y ? M1 - y : 0, zm = z ? M2 - z : 0;
     const long long row = (long long)y * S1 + (long long)z * S2;
     const long long src_row = half ? (long long)ym * S1 + (long long)zm * S2 : row;
+    const long long trow = tab_fold(tg, 1, y) * tg.p1 + tab_fold(tg, 2, z) * tg.p2;   // the line's table row
+    const bool neg_y = tab_neg(tg, 1, y), neg_z = tab_neg(tg, 2, z);
     Cx<T> *mine = buf + (size_t)(half * L + i) * LD;   // this lane's line in LDS
     const Cx<T> *mirror = buf + (size_t)(L + i) * LD;  // the mirror line of line i
     // spectrum position of register r (= q R2 + k2)
@@ -938,9 +970,13 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
                 for (int t = tl.first[f]; t < tl.first[f + 1]; ++t) {
                     const Term tm = tl.term[t];
                     if (tm.t != last) {
-                        const T *tab = tm.t >= 0 ? tk + (size_t)tm.t * ns : uk + (size_t)(-tm.t - 1) * ns;
+                        const T *tab = tm.t >= 0 ? tk + (size_t)tm.t * tg.tn : uk + (size_t)(-tm.t - 1) * tg.tn;
 #pragma unroll
-                        for (int r = 0; r < R1; ++r) kt[r] = tab[row + kpos(r)];
+                        for (int r = 0; r < R1; ++r) kt[r] = tab[trow + kpos(r)];
+                        if ((tm.odd == 1 && neg_y) || (tm.odd == 2 && neg_z)) {
+#pragma unroll
+                            for (int r = 0; r < R1; ++r) kt[r] = -kt[r];
+                        }
                         last = tm.t;
                     }
                     const int p = tm.o >> 1, odd = tm.o & 1;
@@ -1319,6 +1355,11 @@ constexpr size_t kXprodLds = 64 * 1024;
 
 // Lines per workgroup of the fused x pass for nop output pairs: the largest power of two up to 8
 // whose LDS fits kXprodLds and that leaves about kFftGroups workgroups; 0 when one line does not fit.
+TabGeo tab_geo(const FftPlan &F)
+{
+    return TabGeo{F.fold ? 1 : 0, {F.M[0], F.M[1], F.M[2]}, F.tp1, F.tp2, F.tn};
+}
+
 int xprod_lpb(const FftPlan &F, int nop)
 {
     const size_t M = (size_t)F.M[0], per_line = (4 + (size_t)nop) * (M + 1) * sizeof(Cx<double>);
@@ -1347,7 +1388,7 @@ bool xprod_reg(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, c
     const size_t lds = sizeof(Cx<T>) * ((size_t)F.M[0] + 2 * (size_t)L * (F.M[0] + 1));
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, block, lds, s, Z, O, F.ns, F.M[1], F.M[2], F.S1, F.S2, L, twiddles<T>(F, 0), tk,
-                           uk, tl);
+                           uk, tab_geo(F), tl);
     };
     auto by_nop = [&](auto r1, auto r2) {
         constexpr int A = decltype(r1)::value, B = decltype(r2)::value;
@@ -1386,12 +1427,13 @@ void fft_xprod(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, i
     const RadixCounts rc{F.rc[0][0], F.rc[0][1], F.rc[0][2], F.rc[0][3]};
     const size_t lds = sizeof(Cx<T>) * ((size_t)M + (4 + (size_t)nop) * lpb * (M + 1));
     hipLaunchKernelGGL(k_fft_xprod<T>, dim3((unsigned)((lines + lpb - 1) / lpb)), dim3(kFftBlock), lds, s, Z, O, F.ns,
-                       F.M[0], F.M[1], F.M[2], F.S1, F.S2, rc, lpb, twiddles<T>(F, 0), tk, uk, tl);
+                       F.M[0], F.M[1], F.M[2], F.S1, F.S2, rc, lpb, twiddles<T>(F, 0), tk, uk, tab_geo(F), tl);
 }
 
 FftGeo fft_geo(const FftPlan &F, int which)
 {
     FftGeo f;
+    f.tg = tab_geo(F);
     for (int d = 0; d < 3; ++d) {
         f.M[d] = F.M[d];
         f.off[d] = F.off[d];
@@ -1445,6 +1487,7 @@ void free_plan(FftPlan *F)
 // (imaginary for even stencils, real for odd ones) above which the FFT form
 // is refused.
 constexpr double kSpectrumParityTol = 1e-9;
+constexpr double kFoldTol = 1e-13;   // reflection symmetry of the stencil weights (relative)
 
 }  // namespace
 
@@ -1557,7 +1600,8 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         // the current field nin owns terms first[nin] .. first[nin+1]-1; first[nin+1] is the running end
         auto add_term = [&](TermList &tl, int o, int t, int imag, double coef) {
             if (tl.first[tl.nin + 1] >= kMaxTerms) throw std::logic_error("too many product terms");
-            tl.term[tl.first[tl.nin + 1]++] = Term{o, t, imag, coef};
+            const int odd = (t >= 0 && imag) ? t % (2 + ND) - 2 : -1;   // vector delta d is odd in axis d
+            tl.term[tl.first[tl.nin + 1]++] = Term{o, t, imag, odd, coef};
         };
         auto close_input = [&](TermList &tl) {   // sort the field's terms by table; the next field starts
             if (tl.nin + 2 > kMaxIn + 1) throw std::logic_error("too many product inputs");
@@ -1623,15 +1667,59 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             F->ss_cap = most((std::max(F->fwd[0].nout, F->fwd[1].nout) + 1) / 2) + most((C + 1) / 2);
             TMI_HIP_CHECK(hipMalloc((void **)&F->ss, (size_t)F->ss_cap * sizeof(double)));
         }
+        // the tables fold when every stencil in use is symmetric under the reflection of y and of z
+        // (weights at o and at o with o_a negated agree to kFoldTol of the largest weight)
+        auto reflective = [&](const HostStencil &hs) {
+            std::unordered_map<long long, double> at;
+            double wmax = 0.0;
+            auto key = [](int a, int b, int c) {
+                return (((long long)a + (1 << 19)) << 40) | (((long long)b + (1 << 19)) << 20) | ((long long)c + (1 << 19));
+            };
+            for (int k = 0; k < hs.L; ++k) {
+                const int *o = &hs.off[3 * (size_t)k];
+                at[key(o[0], o[1], o[2])] += hs.w[k];
+                wmax = std::max(wmax, std::fabs(hs.w[k]));
+            }
+            for (int k = 0; k < hs.L; ++k) {
+                const int *o = &hs.off[3 * (size_t)k];
+                const double w = at[key(o[0], o[1], o[2])];
+                for (int a = 1; a < nd; ++a) {
+                    int r[3] = {o[0], o[1], o[2]};
+                    r[a] = -r[a];
+                    const auto it = at.find(key(r[0], r[1], r[2]));
+                    if (it == at.end() || std::fabs(it->second - w) > kFoldTol * wmax) return false;
+                }
+            }
+            return true;
+        };
+        bool fold = true;
+        if (mf)
+            for (int q = 0; q < C * C; ++q) fold = fold && reflective(P.hs[TMI_UATT][q]);
+        if (fmt)
+            for (int c = 0; c < C; ++c) fold = fold && reflective(P.hs[TMI_THETA][c]) && reflective(P.hs[TMI_DELTA][c]);
+        const char *fold_env = std::getenv("TMI_FFT_FOLD");   // TMI_FFT_FOLD=0 keeps whole-grid tables
+        F->fold = fold && !(fold_env && std::string(fold_env) == "0");
+        if (F->fold) {
+            const long long H0 = F->M[0], H1 = F->M[1] / 2 + 1, H2 = F->M[2] / 2 + 1;
+            F->tp1 = H0;
+            F->tp2 = H0 * H1;
+            F->tn = H0 * H1 * H2;
+        } else {
+            F->tp1 = F->S1;
+            F->tp2 = F->S2;
+            F->tn = F->ns;
+        }
+        const size_t tn = (size_t)F->tn;
         // spectra: flipped stencil g(-o mod M) = w(o) on the FFT grid, transformed (in zbuf), the
-        // real (even) or imaginary (odd) part kept, scaled by 1/n
+        // real (even) or imaginary (odd) part kept, scaled by 1/n, folded or in the transforms' layout
         std::vector<Cx<double>> host(ns);
         std::vector<double> part(ns);
         Cx<double> *work = reinterpret_cast<Cx<double> *>(F->zbuf);
         bool parity_ok = true;
-        // vd < 0: the weights w; vd = d: the vector weights w o_d dx_d / R_comp (odd)
+        // vd < 0: the weights w; vd = d: the vector weights w o_d dx_d / R_comp (odd in axis d)
         auto spectrum = [&](const HostStencil &hs, int vd, int comp, double *dst_d, float *dst_f) {
             std::fill(host.begin(), host.end(), Cx<double>{0.0, 0.0});
+            std::fill(part.begin(), part.end(), 0.0);
             const long long pitch[3] = {1, F->S1, F->S2};
             for (int k = 0; k < hs.L; ++k) {
                 long long idx = 0;
@@ -1648,24 +1736,41 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             TMI_HIP_CHECK(hipMemcpy(work, host.data(), ns * sizeof(Cx<double>), hipMemcpyHostToDevice));
             fft3d<double>(*F, work, 1, -1, 0);
             TMI_HIP_CHECK(hipMemcpy(host.data(), work, ns * sizeof(Cx<double>), hipMemcpyDeviceToHost));
-            double keep = 0.0, drop = 0.0;
+            double keep = 0.0, drop = 0.0, asym = 0.0;
             const double inv = 1.0 / (double)F->n;
-            for (size_t q = 0; q < ns; ++q) {
-                const double a = vd < 0 ? host[q].x : host[q].y, z = vd < 0 ? host[q].y : host[q].x;
-                keep = std::max(keep, std::fabs(a));
-                drop = std::max(drop, std::fabs(z));
-                part[q] = a * inv;
-            }
-            if (drop > kSpectrumParityTol * keep) parity_ok = false;
-            TMI_HIP_CHECK(hipMemcpy(dst_d, part.data(), ns * sizeof(double), hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_to_float, dim3(grid_for((long long)ns)), dim3(kBlock), 0, 0, dst_d, dst_f,
-                               (long long)ns);
+            auto kept = [&](long long q) { return vd < 0 ? host[q].x : host[q].y; };
+            for (int m2 = 0; m2 < F->M[2]; ++m2)
+                for (int m1 = 0; m1 < F->M[1]; ++m1)
+                    for (int m0 = 0; m0 < F->M[0]; ++m0) {
+                        const long long q = m0 + m1 * F->S1 + m2 * F->S2;
+                        const double a = kept(q), z = vd < 0 ? host[q].y : host[q].x;
+                        keep = std::max(keep, std::fabs(a));
+                        drop = std::max(drop, std::fabs(z));
+                        if (!F->fold) {
+                            part[q] = a * inv;
+                            continue;
+                        }
+                        const int m[3] = {m0, m1, m2};
+                        int f[3];
+                        double sgn = 1.0;
+                        for (int d = 0; d < 3; ++d) {   // y and z fold; x rows are kept whole
+                            const bool hi = d > 0 && 2 * m[d] > F->M[d];
+                            f[d] = hi ? F->M[d] - m[d] : m[d];
+                            if (hi && d == vd) sgn = -1.0;
+                        }
+                        asym = std::max(asym, std::fabs(a - sgn * kept(f[0] + f[1] * F->S1 + f[2] * F->S2)));
+                        if (f[0] == m0 && f[1] == m1 && f[2] == m2) part[m0 + m1 * F->tp1 + m2 * F->tp2] = a * inv;
+                    }
+            if (drop > kSpectrumParityTol * keep || asym > kSpectrumParityTol * keep) parity_ok = false;
+            TMI_HIP_CHECK(hipMemcpy(dst_d, part.data(), tn * sizeof(double), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_to_float, dim3(grid_for((long long)tn)), dim3(kBlock), 0, 0, dst_d, dst_f,
+                               (long long)tn);
         };
         if (mf) {
-            TMI_HIP_CHECK(hipMalloc((void **)&F->ud, (size_t)C * C * ns * sizeof(double)));
-            TMI_HIP_CHECK(hipMalloc((void **)&F->uf, (size_t)C * C * ns * sizeof(float)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->ud, (size_t)C * C * tn * sizeof(double)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->uf, (size_t)C * C * tn * sizeof(float)));
             for (int q = 0; q < C * C; ++q)
-                spectrum(P.hs[TMI_UATT][q], -1, q % C, F->ud + (size_t)q * ns, F->uf + (size_t)q * ns);
+                spectrum(P.hs[TMI_UATT][q], -1, q % C, F->ud + (size_t)q * tn, F->uf + (size_t)q * tn);
             if (!parity_ok) {   // keep the direct sums
                 free_plan(F);
                 return;
@@ -1674,14 +1779,14 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         }
         if (fmt) {
             const size_t nk = (size_t)C * (2 + ND);
-            TMI_HIP_CHECK(hipMalloc((void **)&F->td, nk * ns * sizeof(double)));
-            TMI_HIP_CHECK(hipMalloc((void **)&F->tf, nk * ns * sizeof(float)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->td, nk * tn * sizeof(double)));
+            TMI_HIP_CHECK(hipMalloc((void **)&F->tf, nk * tn * sizeof(float)));
             for (int c = 0; c < C; ++c) {
                 const size_t base = (size_t)c * (2 + ND);
-                spectrum(P.hs[TMI_THETA][c], -1, c, F->td + base * ns, F->tf + base * ns);
-                spectrum(P.hs[TMI_DELTA][c], -1, c, F->td + (base + 1) * ns, F->tf + (base + 1) * ns);
+                spectrum(P.hs[TMI_THETA][c], -1, c, F->td + base * tn, F->tf + base * tn);
+                spectrum(P.hs[TMI_DELTA][c], -1, c, F->td + (base + 1) * tn, F->tf + (base + 1) * tn);
                 for (int d = 0; d < ND; ++d)
-                    spectrum(P.hs[TMI_DELTA][c], d, c, F->td + (base + 2 + d) * ns, F->tf + (base + 2 + d) * ns);
+                    spectrum(P.hs[TMI_DELTA][c], d, c, F->td + (base + 2 + d) * tn, F->tf + (base + 2 + d) * tn);
             }
             F->fmt_ok = parity_ok;
         }
@@ -1722,6 +1827,7 @@ bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
 bool fft_fused(const Problem &P) { return P.fft && (P.fft->xlpb[0] || P.fft->xlpb[1]); }
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
 bool fft_wraps(const Problem &P) { return P.fft && P.fft->wraps; }
+bool fft_folded(const Problem &P) { return P.fft && P.fft->fold; }
 bool fft_fused_available(const Problem &P) { return P.fft && (P.fft->xfit[0] || P.fft->xfit[1]); }
 void fft_set_fused(Problem &P, int mask)
 {
@@ -1902,9 +2008,10 @@ template void fft_el<float>(Problem &, bool, const double *, const double *, con
 // fill over reps fills, *bytes their algorithmic traffic per fill and
 // *launches their launches per fill.  Algorithmic bytes: an axis pass reads
 // and writes each grid point of each pair field once (16 B each way); a fused
-// x pass reads each input pair and each distinct spectrum table once (16 B,
-// 8 B per point) and writes each output pair once (16 B per point).  The
-// buffers' contents are scratch between products.
+// x pass reads each input pair once (16 B per point), each distinct spectrum
+// table once (8 B per stored table element: the folded quarter or the whole
+// grid) and writes each output pair once (16 B per point).  The buffers'
+// contents are scratch between products.
 void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes, int *launches, hipStream_t s)
 {
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
@@ -1925,7 +2032,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
             for (int f = 0; f < tl->nin; ++f)
                 for (int t = tl->first[f]; t < tl->first[f + 1]; ++t)
                     if (t == tl->first[f] || tl->term[t].t != tl->term[t - 1].t) ++tables;
-            b += (double)F.n * (16.0 * ((tl->nin + 1) / 2) + 16.0 * ((tl->nout + 1) / 2) + 8.0 * tables);
+            b += (double)F.n * (16.0 * ((tl->nin + 1) / 2) + 16.0 * ((tl->nout + 1) / 2)) + 8.0 * (double)F.tn * tables;
             ++count;
         }
         fill = [&] {

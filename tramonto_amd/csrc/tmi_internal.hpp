@@ -198,6 +198,7 @@ bool fft_mf(const Problem &P);              // mean-field sums in FFT form
 bool fft_fmt(const Problem &P);             // FMT sums (rho-bar, Euler-Lagrange) in FFT form
 bool fft_fmt_available(const Problem &P);   // FMT spectra built
 bool fft_wraps(const Problem &P);           // every axis periodic on the FFT grid (halos unused)
+bool fft_folded(const Problem &P);          // spectrum tables stored for k_y, k_z >= 0 only
 void fft_set_fmt(Problem &P, bool on);
 bool fft_fused_available(const Problem &P);   // the fused x pass fits the LDS budget
 bool fft_fused(const Problem &P);             // products through the fused x pass
@@ -239,6 +240,11 @@ double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStr
 void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out_d,
                          hipStream_t s);
 void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s);
+// classical Gram-Schmidt of w against V_0 .. V_k-1 with one reorthogonalisation pass, all on the
+// device: coef_d[0, k) and coef_d[k, 2k) the two passes' projections, coef_d[2k] the squared
+// norm of the result, next = the result normalised (0 if its norm is 0); w holds the result
+void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
+              hipStream_t s);
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);

@@ -1099,6 +1099,65 @@ __global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, co
     w[i] -= s;
 }
 
+// Second Gram-Schmidt pass fused with the first update (k <= KM columns): w -= V h, then
+// partial[j][blk] = sum over the block's elements of V_j . w (updated w), j < k.  Every V_j
+// element is read once for both: the update and the new projections.
+template <int KM>
+__global__ __launch_bounds__(kBlock) void k_gemv_sub_dots(const double *__restrict__ V, long long ld, int k,
+                                                          const double *__restrict__ h, double *__restrict__ w,
+                                                          long long n, double *__restrict__ partial)
+{
+    double acc[KM];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) acc[j] = 0.0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        double v[KM];
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) {
+            v[j] = j < k ? V[(size_t)j * ld + i] : 0.0;
+            if (j < k) s += v[j] * h[j];
+        }
+        const double wi = w[i] - s;
+        w[i] = wi;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) acc[j] += v[j] * wi;
+    }
+    __shared__ double sh[kBlock / 64][KM];   // per-wave sums
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+        if (j < k) {
+            double a = acc[j];
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+            if (lane == 0) sh[wid][j] = a;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double a = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) a += sh[q][threadIdx.x];
+        partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// w -= V h (k columns), partial[blk] = sum over the block's elements of w^2 (updated w)
+__global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const double *__restrict__ V, long long ld, int k,
+                                                          const double *__restrict__ h, double *__restrict__ w,
+                                                          long long n, double *__restrict__ partial)
+{
+    double q = 0.0;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        double s = 0.0;
+        for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * h[j];
+        const double wi = w[i] - s;
+        w[i] = wi;
+        q += wi * wi;
+    }
+    q = block_sum<kBlock>(q);
+    if (threadIdx.x == 0) partial[blockIdx.x] = q;
+}
+
 __global__ void k_gemv_add(const double *__restrict__ V, long long ld, int k, const double *__restrict__ y,
                            double *__restrict__ x, long long n)
 {
@@ -1601,6 +1660,32 @@ void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const
     unsigned nb = red_blocks(n);
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d);
+}
+
+void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
+              hipStream_t s)
+{
+    const unsigned nb = red_blocks(n);
+    double *h1 = coef_d, *h2 = coef_d + k, *nrm = coef_d + 2 * k;
+    dev_multi_dot_async(P, V, ld, k, w, n, h1, s);
+    auto fused = [&](auto km) {
+        constexpr int KM = decltype(km)::value;
+        hipLaunchKernelGGL(k_gemv_sub_dots<KM>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, P.d_red);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2);
+    };
+    if (k <= 8)
+        fused(std::integral_constant<int, 8>{});
+    else if (k <= 16)
+        fused(std::integral_constant<int, 16>{});
+    else if (k <= 32)
+        fused(std::integral_constant<int, 32>{});
+    else {
+        dev_gemv_sub(V, ld, k, h1, w, n, s);
+        dev_multi_dot_async(P, V, ld, k, w, n, h2, s);
+    }
+    hipLaunchKernelGGL(k_gemv_sub_norm, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb, 1, nrm);
+    dev_scale_by_norm(nrm, w, next, n, s);
 }
 
 void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s)
