@@ -467,6 +467,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
         autotune_rows1d(*P);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
+        TMI_HIP_CHECK(hipHostGetDevicePointer((void **)&P->h_red_dev, P->h_red, 0));
         // FFT forms of the stencil sums on 2D/3D grids: the mean-field sums when the attraction
         // stencils are wide (TMI_MF_FFT=1 forces it, 0 keeps the direct sums), the FMT sums when
         // that form fills faster (TMI_FMT_FFT=1 / 0 forces the FFT / direct form)
@@ -607,7 +608,6 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
             // classical Gram-Schmidt and one reorthogonalisation pass with the coefficients kept
             // on the device; one host synchronisation per step, for the Hessenberg column
             dev_cgs2(P, V, n, k + 1, w, n, V + (size_t)(k + 1) * n, dh, s);
-            TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, dh, sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, s));
             TMI_HIP_CHECK(hipStreamSynchronize(s));
             for (int j = 0; j <= k; ++j) h[j] = P.h_red[j] + P.h_red[k + 1 + j];
             const double hn = std::sqrt(P.h_red[2 * (k + 1)]);

@@ -161,7 +161,8 @@ struct Problem {
     double *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     double *d_red = nullptr;     // reduction buffer
-    double *h_red = nullptr;     // pinned host mirror
+    double *h_red = nullptr;     // pinned host mirror (1024 doubles), written by reduction kernels directly
+    double *h_red_dev = nullptr; // its device address
 };
 
 // ---------------------------------------------------------------- kernels
@@ -236,13 +237,15 @@ void autotune_rows1d(Problem &P);
 
 // vector helpers
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s);
-// out_d[j] = V_j . w on the device (partial sums in P.d_red[0, k*1024)), no host synchronisation
+// out_d[j] = V_j . w on the device (partial sums in P.d_red[0, k*1024)), no host synchronisation;
+// mirror (optional, the device address of pinned host memory such as P.h_red_dev) gets a copy
 void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out_d,
-                         hipStream_t s);
+                         hipStream_t s, double *mirror = nullptr);
 void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s);
 // classical Gram-Schmidt of w against V_0 .. V_k-1 with one reorthogonalisation pass, all on the
 // device: coef_d[0, k) and coef_d[k, 2k) the two passes' projections, coef_d[2k] the squared
-// norm of the result, next = the result normalised (0 if its norm is 0); w holds the result
+// norm of the result (copied to P.h_red[0, 2k] as well), next = the result normalised (0 if
+// its norm is 0); w holds the result
 void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
               hipStream_t s);
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,

@@ -1050,14 +1050,20 @@ __global__ void k_multidot(const double *__restrict__ V, long long ld, int k, co
     }
 }
 
-__global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k, double *__restrict__ out)
+// out[j] = sum_b partial[j*nb + b]; mirror (optional, pinned host memory) gets a copy, which the
+// host reads after synchronising with the stream: no separate transfer
+__global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k, double *__restrict__ out,
+                              double *__restrict__ mirror)
 {
     int j = blockIdx.x;
     if (j >= k) return;
     double s = 0.0;
     for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[(size_t)j * nb + b];
     s = block_sum<kBlock>(s);
-    if (threadIdx.x == 0) out[j] = s;
+    if (threadIdx.x == 0) {
+        if (out) out[j] = s;
+        if (mirror) mirror[j] = s;
+    }
 }
 
 __global__ void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y, long long n)
@@ -1575,8 +1581,7 @@ double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s)
     residual_impl(P, x, F, s, nullptr, ss, &cnt);
     if (cnt > cap) throw std::logic_error("partial sums past their buffer");
     if (cnt > 0) {
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, ss, (int)cnt, 1, P.d_red + 1024);
-        TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, P.d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, ss, (int)cnt, 1, nullptr, P.h_red_dev);
         TMI_HIP_CHECK(hipStreamSynchronize(s));
         return P.h_red[0];
     }
@@ -1646,20 +1651,18 @@ void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const doubl
 {
     unsigned nb = red_blocks(n);
     double *partial = P.d_red;            // [k][nb]
-    double *res = P.d_red + (size_t)k * 1024;
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, partial);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, partial, (int)nb, k, res);
-    TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, res, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, partial, (int)nb, k, nullptr, P.h_red_dev);
     TMI_HIP_CHECK(hipStreamSynchronize(s));
     for (int j = 0; j < k; ++j) out[j] = P.h_red[j];
 }
 
 void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out_d,
-                         hipStream_t s)
+                         hipStream_t s, double *mirror)
 {
     unsigned nb = red_blocks(n);
     hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d, mirror);
 }
 
 void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
@@ -1667,11 +1670,12 @@ void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long 
 {
     const unsigned nb = red_blocks(n);
     double *h1 = coef_d, *h2 = coef_d + k, *nrm = coef_d + 2 * k;
-    dev_multi_dot_async(P, V, ld, k, w, n, h1, s);
+    double *m1 = P.h_red_dev, *m2 = m1 + k, *mn = m1 + 2 * k;   // host copies (P.h_red)
+    dev_multi_dot_async(P, V, ld, k, w, n, h1, s, m1);
     auto fused = [&](auto km) {
         constexpr int KM = decltype(km)::value;
         hipLaunchKernelGGL(k_gemv_sub_dots<KM>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, P.d_red);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2, m2);
     };
     if (k <= 8)
         fused(std::integral_constant<int, 8>{});
@@ -1681,10 +1685,10 @@ void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long 
         fused(std::integral_constant<int, 32>{});
     else {
         dev_gemv_sub(V, ld, k, h1, w, n, s);
-        dev_multi_dot_async(P, V, ld, k, w, n, h2, s);
+        dev_multi_dot_async(P, V, ld, k, w, n, h2, s, m2);
     }
     hipLaunchKernelGGL(k_gemv_sub_norm, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb, 1, nrm);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb, 1, nrm, mn);
     dev_scale_by_norm(nrm, w, next, n, s);
 }
 
@@ -1739,8 +1743,7 @@ void dev_newton_update(Problem &P, double *x, const double *dx, const tmi_newton
     for (unsigned b = 0; b < nb; ++b) frac = std::min(frac, P.h_red[b]);
     unsigned nb2 = red_blocks(n);
     hipLaunchKernelGGL(k_update_norm, dim3(nb2), dim3(kBlock), 0, s, x, dx, n, frac, o.nl_rel_tol, o.nl_abs_tol, P.d_red);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb2, 1, P.d_red + 1024);
-    TMI_HIP_CHECK(hipMemcpyAsync(P.h_red, P.d_red + 1024, sizeof(double), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb2, 1, nullptr, P.h_red_dev);
     TMI_HIP_CHECK(hipStreamSynchronize(s));
     const double unorm = std::sqrt(P.h_red[0]);
     hipLaunchKernelGGL(k_apply_update, dim3(grid_for(n)), dim3(kBlock), 0, s, x, dx, n, n_rho, P.N, P.iu_rb, frac);
