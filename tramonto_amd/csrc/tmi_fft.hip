@@ -1859,7 +1859,7 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
     const Geo gu = make_geo(P, P.pad_u);
     const FftGeo fg = fft_geo(F, 0);
     const int nip = (C + 1) / 2, nop = (d.nout + 1) / 2;
-    if (F.xlpb[0]) {   // z, y passes; fused x pass with the product; y, z passes back
+    {   // z, y, x passes; the product (fused with the x passes or not); x, y, z passes back
         // on a wrapping grid the first pass reads the node arrays and the last one writes the rows
         const bool ends = F.wraps && rho_nodes;
         PassIO io;
@@ -1898,7 +1898,18 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
         for (int a : inv_axes)
             if (F.M[a] > 1) last = a;
         for (int a : fwd_axes) fft_pass<T>(F, Z, nip, -1, a, s, (ends && a == first) ? kPassGather : kPassPlain, io);
-        fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        if (F.xlpb[0]) {
+            fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        } else {
+            fft_pass<T>(F, Z, nip, -1, 0, s);
+            if (P.p.ndim == 2)
+                hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z,
+                                   fmt_spectra<T>(F), attr_spectra<T>(F), O);
+            else
+                hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z,
+                                   fmt_spectra<T>(F), attr_spectra<T>(F), O);
+            fft_pass<T>(F, O, nop, +1, 0, s);
+        }
         if (ends) io.ss = ss;
         long long blocks = 0;
         for (int a : inv_axes)
@@ -1908,20 +1919,6 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
             TMI_HIP_CHECK(hipGetLastError());
             return;
         }
-    } else {
-        if (rho_nodes)
-            hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
-        else
-            hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gu,
-                               workspace<T>(P).rho_pad, C, Z);
-        fft3d<T>(F, Z, nip, -1, s);
-        if (P.p.ndim == 2)
-            hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
-                               attr_spectra<T>(F), O);
-        else
-            hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
-                               attr_spectra<T>(F), O);
-        fft3d<T>(F, O, nop, +1, s);
     }
     hipLaunchKernelGGL(k_fwd_extract<T>, dim3(grid_for(P.N), rm.nrb + rm.nmf), dim3(kBlock), 0, s, fg, gu, rm, O, in_rb,
                        sub_rb, out_rb, in_mf, sub_mf, out_mf);
@@ -1949,7 +1946,7 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
         hipLaunchKernelGGL(k_gather_pairs<T>, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, gf, workspace<T>(P).adj_pad,
                            nadj, Z);
     const int nip = (nadj + 1) / 2, nop = (C + 1) / 2;
-    if (F.xlpb[1]) {
+    {   // z, y, x passes; the product (fused with the x passes or not); x, y, z passes back
         // on a wrapping grid the last pass writes the Euler-Lagrange rows
         const bool ends = F.wraps;
         PassIO io;
@@ -1975,7 +1972,18 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
             if (F.M[a] > 1) last = a;
         fft_pass<T>(F, Z, nip, -1, 2, s);
         fft_pass<T>(F, Z, nip, -1, 1, s);
-        fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        if (F.xlpb[1]) {
+            fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+        } else {
+            fft_pass<T>(F, Z, nip, -1, 0, s);
+            if (nd == 2)
+                hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z,
+                                   fmt_spectra<T>(F), O);
+            else
+                hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z,
+                                   fmt_spectra<T>(F), O);
+            fft_pass<T>(F, O, nop, +1, 0, s);
+        }
         if (ends) io.ss = ss;
         long long blocks = 0;
         for (int a : {1, 2}) blocks = fft_pass<T>(F, O, nop, +1, a, s, (ends && a == last) ? kPassEl : kPassPlain, io);
@@ -1984,13 +1992,6 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
             TMI_HIP_CHECK(hipGetLastError());
             return;
         }
-    } else {
-        fft3d<T>(F, Z, nip, -1, s);
-        if (nd == 2)
-            hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
-        else
-            hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, fmt_spectra<T>(F), O);
-        fft3d<T>(F, O, nop, +1, s);
     }
     hipLaunchKernelGGL(k_el_extract<T>, dim3(grid_for(P.N), C), dim3(kBlock), 0, s, fg, gf, make_consts(P),
                        linear ? 1 : 0, P.p.nwall, O, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
@@ -2042,7 +2043,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
     } else {
         // which passes run with fused ends (those are other kernels): the density product's first and last
         // strided passes when it reads the node arrays, the Euler-Lagrange product's last one on wrapping grids
-        const bool rb_ends = F.xlpb[0] && fft_node_input(P), el_ends = F.xlpb[1] && F.wraps;
+        const bool rb_ends = fft_node_input(P), el_ends = F.wraps;
         int first_strided = -1, last_strided = -1;
         for (int a : {2, 1})
             if (F.M[a] > 1 && first_strided < 0) first_strided = a;
