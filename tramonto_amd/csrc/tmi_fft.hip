@@ -531,15 +531,17 @@ __device__ __forceinline__ void dft_reg(Cx<T> *a)
 
 // Strided axis pass of length M = R1 R2 (R2 <= R1 <= 16, both powers of two) with one wave per
 // workgroup: lane (i, n2) takes line i of L = 64 / R2 consecutive x and loads elements
-// R2 n1 + n2 straight into registers; an R1-point DFT, the twiddles W_M^(n2 k1), one transpose
+// R2 n1 + n2 straight into registers; an R1-point DFT, the twiddles W_M^(n2 k1), a transpose
 // through LDS, R1/R2 R2-point DFTs per lane and the store.  One LDS round trip per pass instead
-// of one per Stockham stage.
+// of one per Stockham stage.  The transpose runs in R1/R2 phases of R2 values of k1 each
+// (phase q: k1 = R2 q .. R2 q + R2 - 1, read back by lane n2 as k1 = n2 + R2 q), so the LDS
+// buffer holds L R2^2 values whatever R1: more resident waves per CU.
 template <typename T, int SIGN, int MODE, int R1, int R2>
 __global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, long long fstride, int M0, int M1, int M2,
                                                     long long S1, long long S2, int axis,
                                                     const Cx<T> *__restrict__ twtab, PassIO io)
 {
-    constexpr int M = R1 * R2, L = 64 / R2, LD = M + 1;
+    constexpr int M = R1 * R2, L = 64 / R2, LD = R2 * R2 + 1;
     __shared__ Cx<T> tw[M];
     __shared__ Cx<T> buf[L * LD];
     const int field = blockIdx.y;
@@ -556,20 +558,22 @@ __global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, lo
     for (int n1 = 0; n1 < R1; ++n1)
         v[n1] = live ? strided_load<T, MODE>(data, sg, io, field, i, R2 * n1 + n2) : Cx<T>{(T)0, (T)0};
     dft_reg<R1, SIGN>(v);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-        Cx<T> w = tw[n2 * k1];
-        if constexpr (SIGN > 0) w.y = -w.y;
-        buf[i * LD + k1 * R2 + n2] = cmul(v[k1], w);
-    }
-    __syncthreads();
     double ssq = 0.0;
 #pragma unroll
     for (int q = 0; q < R1 / R2; ++q) {
+        if (q > 0) __syncthreads();   // the previous phase's reads are done
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {   // twiddles W_M^(n2 k1)
+            const int k1 = R2 * q + t;
+            Cx<T> w = tw[n2 * k1];
+            if constexpr (SIGN > 0) w.y = -w.y;
+            buf[i * LD + t * R2 + n2] = cmul(v[k1], w);
+        }
+        __syncthreads();
         const int k1 = n2 + R2 * q;
         Cx<T> u[R2];
 #pragma unroll
-        for (int m = 0; m < R2; ++m) u[m] = buf[i * LD + k1 * R2 + m];
+        for (int m = 0; m < R2; ++m) u[m] = buf[i * LD + n2 * R2 + m];
         dft_reg<R2, SIGN>(u);
         if (live) {
 #pragma unroll
