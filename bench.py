@@ -42,9 +42,10 @@ STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
 # Mean HBM bytes per launch of the FFT-form kernels of a 64^3 fill (fetch + write, MiB): counter runs of
 # this script with --fill-only --steps 50 --warmup 2 --copy-probe under TMI_FMT_FFT=1, scaled as above,
 # means over the last 120 fused-x-pass and 300 plain-axis-pass dispatches (fills' launches).
-# Algorithmic: 27.0 MiB per fused x pass, 17.6 MiB per plain axis pass on average.
-FFT_TRAFFIC_MIB_64 = 8.886 + 8.800
-XPROD_TRAFFIC_MIB_64 = 26.913 + 8.000
+# Algorithmic: 18.9 MiB per fused x pass (pairs and the spectrum tables, stored for k_y, k_z >= 0),
+# 17.6 MiB per plain axis pass on average.
+FFT_TRAFFIC_MIB_64 = 8.885 + 8.800
+XPROD_TRAFFIC_MIB_64 = 21.600 + 8.000
 
 
 def stencil_flops_per_node(dp, spec):
