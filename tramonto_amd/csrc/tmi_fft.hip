@@ -127,6 +127,12 @@ struct FftPlan {
     int xlpb[2] = {0, 0};       // in use (0: separate x passes and product kernel)
     // bulk-circulant Schur preconditioner (wrapping grids): Q(k) = S_b(k)^-1 diag(1/rho_b) / n
     Cx<double> *pq = nullptr;   // [C*C][ns], entry (i, j) at i + C j
+    // Q real and even in k_y, k_z (the bulk operator is real and reflection-symmetric): its real part
+    // in the spectrum-table layout, applied by the fused x pass between fused-end passes
+    double *pqr = nullptr;      // [C*C][tn]
+    TermList tpc;               // output i += Q_ij X_j
+    int pc_lpb = 0;             // lines per workgroup of its LDS form
+    bool pc_fused = false;
     bool pc_ok = false;         // Q built
     int pc_want = -1;           // wanted (TMI_PRECOND=diag: no; -1 until the first set-up reads it)
     bool pc_on = false;         // GMRES uses it: built and wanted
@@ -288,7 +294,7 @@ __device__ inline Cx<T> *fft_lines(Cx<T> *a, Cx<T> *b, int ld, int nl, int M, Ra
 // rows (< nrb) minus in_rb and sub_rb, mean-field rows minus in_mf and sub_mf.
 // kPassEl writes the Euler-Lagrange rows of components 2 field and 2 field + 1
 // with their local terms, as k_el_extract.
-constexpr int kPassPlain = 0, kPassGather = 1, kPassRows = 2, kPassEl = 3;
+constexpr int kPassPlain = 0, kPassGather = 1, kPassRows = 2, kPassEl = 3, kPassPc = 4;
 struct PassIO {
     const double *rho = nullptr;
     const unsigned char *zero = nullptr;
@@ -307,6 +313,8 @@ struct PassIO {
     int linear = 0, nwall = 0;
     // kPassRows, kPassEl: per-workgroup partial sums of the squares of the rows written
     double *ss = nullptr;
+    // kPassGather: optional factor of the gathered values; kPassPc: the rows kept where elzero is set
+    const double *mul = nullptr, *pv = nullptr;
 };
 
 // sum over the workgroup (kFftBlock threads), valid in thread 0
@@ -342,8 +350,10 @@ __device__ inline Cx<T> strided_load(const Cx<T> *__restrict__ data, const Strid
 {
     if constexpr (MODE == kPassGather) {
         const size_t a = (size_t)(2 * field) * io.N + strided_node(g, io, i, j), b = a + io.N;
-        const T re = io.zero[a] ? (T)0 : (T)io.rho[a];
-        const T im = (2 * field + 1 < io.C && !io.zero[b]) ? (T)io.rho[b] : (T)0;
+        const bool has_b = 2 * field + 1 < io.C;
+        const double fa = io.mul ? io.mul[a] : 1.0, fb = (io.mul && has_b) ? io.mul[b] : 1.0;
+        const T re = io.zero[a] ? (T)0 : (T)(io.rho[a] * fa);
+        const T im = (has_b && !io.zero[b]) ? (T)(io.rho[b] * fb) : (T)0;
         return Cx<T>{re, im};
     } else {
         return data[g.base0 + i + (long long)j * g.step];
@@ -379,6 +389,14 @@ __device__ inline void strided_store(Cx<T> *__restrict__ data, const StridedGeo 
             if (io.add) r += io.add[row];
             io.out_el[row] = r;
             ssq += r * r;
+        }
+    } else if constexpr (MODE == kPassPc) {
+        const int nd = strided_node(g, io, i, j);
+        for (int h = 0; h < 2; ++h) {
+            const int c = 2 * field + h;
+            if (c >= io.C) break;
+            const size_t row = (size_t)c * io.N + nd;
+            io.out_el[row] = io.elzero[row] ? io.pv[row] : (double)(h ? v.y : v.x);
         }
     } else if constexpr (MODE == kPassRows) {
         const int nd = strided_node(g, io, i, j);
@@ -1317,6 +1335,8 @@ long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis
                 launch_strided_reg<T, 1, kPassRows>(F, data, nfield, axis, R1, R2, s, io);
             else if (mode == kPassEl)
                 launch_strided_reg<T, 1, kPassEl>(F, data, nfield, axis, R1, R2, s, io);
+            else if (mode == kPassPc)
+                launch_strided_reg<T, 1, kPassPc>(F, data, nfield, axis, R1, R2, s, io);
             else
                 launch_strided_reg<T, 1, kPassPlain>(F, data, nfield, axis, R1, R2, s, io);
         }
@@ -1341,6 +1361,8 @@ long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis
             go(k_fft_axis<T, 1, kPassRows>);
         else if (mode == kPassEl)
             go(k_fft_axis<T, 1, kPassEl>);
+        else if (mode == kPassPc)
+            go(k_fft_axis<T, 1, kPassPc>);
         else
             go(k_fft_axis<T, 1, kPassPlain>);
     }
@@ -1479,6 +1501,7 @@ void free_plan(FftPlan *F)
     (void)hipFree(F->td);
     (void)hipFree(F->tf);
     (void)hipFree(F->pq);
+    (void)hipFree(F->pqr);
     (void)hipFree(F->ss);
     for (int d = 0; d < 3; ++d) {
         (void)hipFree(F->twd[d]);
@@ -2121,9 +2144,11 @@ void precond_setup(Problem &P)
         const char *env = std::getenv("TMI_PRECOND");
         F.pc_want = (env && std::string(env) == "diag") ? 0 : 1;
     }
-    F.pc_ok = F.pc_on = false;
+    F.pc_ok = F.pc_on = F.pc_fused = false;
     (void)hipFree(F.pq);
+    (void)hipFree(F.pqr);
     F.pq = nullptr;
+    F.pqr = nullptr;
     if (!F.wraps || P.npores > 0) return;
     const int C = P.p.ncomp;
     const size_t N = P.N, ns = (size_t)F.ns, nq = (size_t)P.nrb + P.nmf;
@@ -2247,6 +2272,41 @@ void precond_setup(Problem &P)
     TMI_HIP_CHECK(hipMemcpy(F.pq, hq.data(), hq.size() * sizeof(Cx<double>), hipMemcpyHostToDevice));
     F.pc_ok = true;
     F.pc_on = F.pc_want != 0;
+    // the fused form: Q real (imaginary parts below 1e-9 of the largest entry) and, with folded
+    // tables, even in k_y and k_z to the same tolerance; the x pass fused with the product fits
+    const size_t tn = (size_t)F.tn;
+    std::vector<double> qr((size_t)C * C * tn, 0.0);
+    double qmax = 0.0, bad = 0.0;
+    for (int e = 0; e < C * C; ++e)
+        for (int m2 = 0; m2 < F.M[2]; ++m2)
+            for (int m1 = 0; m1 < F.M[1]; ++m1)
+                for (int m0 = 0; m0 < F.M[0]; ++m0) {
+                    const Cx<double> v = hq[(size_t)e * ns + m0 + m1 * F.S1 + m2 * F.S2];
+                    qmax = std::max(qmax, std::fabs(v.x));
+                    bad = std::max(bad, std::fabs(v.y));
+                    if (!F.fold) {
+                        qr[(size_t)e * tn + m0 + m1 * F.S1 + m2 * F.S2] = v.x;
+                        continue;
+                    }
+                    const int f1 = 2 * m1 > F.M[1] ? F.M[1] - m1 : m1, f2 = 2 * m2 > F.M[2] ? F.M[2] - m2 : m2;
+                    bad = std::max(bad, std::fabs(v.x - hq[(size_t)e * ns + m0 + f1 * F.S1 + f2 * F.S2].x));
+                    if (f1 == m1 && f2 == m2) qr[(size_t)e * tn + m0 + m1 * F.tp1 + m2 * F.tp2] = v.x;
+                }
+    F.pc_lpb = xprod_lpb(F, (C + 1) / 2);
+    const char *pf = std::getenv("TMI_PRECOND_FUSED");   // 0: the separate gather, product and extract kernels
+    if (bad <= 1e-9 * qmax && F.pc_lpb > 0 && !(pf && std::string(pf) == "0")) {
+        TMI_HIP_CHECK(hipMalloc((void **)&F.pqr, qr.size() * sizeof(double)));
+        TMI_HIP_CHECK(hipMemcpy(F.pqr, qr.data(), qr.size() * sizeof(double), hipMemcpyHostToDevice));
+        TermList &tl = F.tpc;
+        tl = TermList();
+        tl.nout = C;
+        for (int j = 0; j < C; ++j) {   // input component j feeds every output i through table i + C j
+            for (int i = 0; i < C; ++i) tl.term[tl.first[tl.nin + 1]++] = Term{i, i + C * j, 0, -1, 1.0};
+            ++tl.nin;
+            tl.first[tl.nin + 1] = tl.first[tl.nin];
+        }
+        F.pc_fused = true;
+    }
 }
 
 bool precond_available(const Problem &P) { return P.fft && P.fft->pc_ok; }
@@ -2268,6 +2328,30 @@ void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, 
     const int C = P.p.ncomp;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
+    if (F.pc_fused) {   // z, y passes (the first gathers rho v); x pass fused with Q; y, z passes (the last writes z)
+        const int np = (C + 1) / 2;
+        PassIO io;
+        io.C = C;
+        io.nx0 = P.p.nx[0];
+        io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
+        io.N = P.N;
+        io.rho = x_rho;
+        io.mul = v;
+        io.zero = P.d_elzero;
+        io.pv = v;
+        io.elzero = P.d_elzero;
+        io.out_el = z;
+        int first = -1, last = -1;
+        for (int a : {2, 1})
+            if (F.M[a] > 1 && first < 0) first = a;
+        for (int a : {1, 2})
+            if (F.M[a] > 1) last = a;
+        for (int a : {2, 1}) fft_pass<double>(F, Z, np, -1, a, s, a == first ? kPassGather : kPassPlain, io);
+        fft_xprod<double>(F, Z, O, F.tpc, F.pc_lpb, F.pqr, nullptr, s);
+        for (int a : {1, 2}) fft_pass<double>(F, O, np, +1, a, s, a == last ? kPassPc : kPassPlain, io);
+        TMI_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const Geo g = make_geo(P, P.pad_u);
     const FftGeo fg = fft_geo(F, 0);
     hipLaunchKernelGGL(k_pc_gather, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, g, x_rho, v, P.d_elzero, Z);
