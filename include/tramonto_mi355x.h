@@ -151,6 +151,27 @@ int tmi_problem_set_vext(void *handle, const double *vext);
  * selects the diagonal.  Fails with TMI_PRECOND_BULK where unavailable. */
 int tmi_problem_set_preconditioner(void *handle, int kind);
 
+/* Integrated observables of a profile, in the units of the reference's output. */
+typedef struct tmi_observables {
+    double omega;    /* grand potential, omega_sum of calc_free_energy */
+    double omega_s;  /* its surface excess, omega_s_sum: the reference's "energy" key output */
+    double ideal, ideal_b, mu, mu_b, vext, hs, hs_b, att, att_b;   /* terms and their bulk values */
+    double ads[TMI_NCOMP_MAX], ads_b[TMI_NCOMP_MAX];              /* adsorption, its bulk value */
+} tmi_observables;
+
+/* The free-energy terms of calc_free_energy (src/dft_energy.c:30-391: ideal gas, chemical
+ * potential, neutral external field, hard spheres, mean-field attraction; the functionals of this
+ * library) and the adsorption of calc_adsorption (src/dft_out_ads.c:33-110), summed on the
+ * device at the profile x_d (device pointer, x[iunk][inode] rows).  nel_hit, nel_hit2: host
+ * arrays [ncomp][nnodes], the reference's Nel_hit and Nel_hit2 (setup_integrals,
+ * src/dft_out_utils.c:182-270), the elements counted per node for the bulk and hard-sphere
+ * terms and for the profile terms.  scale: the factor integrateInSpace applies
+ * (src/dft_out_utils.c:55-59: Fac_vol, or Fac_vol / (Fac_area Area) per unit area).  No
+ * counterpart in the reference's fill; it replaces the post-processing integrals for this path.
+ * Not for batches of pores. */
+int tmi_compute_observables(void *handle, const double *x_d, const int *nel_hit, const int *nel_hit2, double scale,
+                            tmi_observables *out, void *stream);
+
 /* F(x): the residual of every equation (the reference inserts -F into the
  * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
  * fill_resid_and_matrix_control returns. */

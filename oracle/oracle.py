@@ -73,6 +73,7 @@ def lib() -> ct.CDLL:
         L.tmo_jacobian_coo.restype = ct.c_long
         L.tmo_init_fields.argtypes = [vp, vp]
         L.tmo_newton.argtypes = [vp, vp, ct.POINTER(TmoNewtonOpts), ct.POINTER(TmoNewtonStats)]
+        L.tmo_observables.argtypes = [vp, vp, vp, vp, d, vp]
         _L = L
     return _L
 
@@ -178,6 +179,21 @@ class Oracle:
         lib().tmo_newton(self.h, _p(x), ct.byref(o), ct.byref(st))
         return x, dict(iters=st.iters, converged=bool(st.converged), resid_norm=st.resid_norm,
                        gmres_iters=st.gmres_iters_total)
+
+    def observables(self, x: np.ndarray, nel_hit: np.ndarray, nel_hit2: np.ndarray, scale: float = 1.0) -> Dict:
+        """Free-energy terms and adsorption of the profile x (calc_free_energy, calc_adsorption):
+        nel_hit, nel_hit2 are the [ncomp][N] element counts of the reference's setup_integrals."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h1 = np.ascontiguousarray(nel_hit, dtype=np.int32)
+        h2 = np.ascontiguousarray(nel_hit2, dtype=np.int32)
+        out = np.zeros(19)
+        lib().tmo_observables(self.h, _p(x), _p(h1), _p(h2), float(scale), _p(out))
+        C = self.params.ncomp
+        keys = ("omega", "omega_s", "ideal", "ideal_b", "mu", "mu_b", "vext", "hs", "hs_b", "att", "att_b")
+        r = {k: float(v) for k, v in zip(keys, out[:11])}
+        r["ads"] = out[11:11 + C].copy()
+        r["ads_b"] = out[15:15 + C].copy()
+        return r
 
     # pointwise FMT pieces (n order)
     def dphi(self, n: np.ndarray) -> np.ndarray:

@@ -1566,3 +1566,88 @@ int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton
     free(S.u);
     return converged ? iter : -iter;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Integrated observables of a profile x: calc_free_energy (dft_energy.c:30-391)
+ * with the integrands of dft_energy_id.c:22-56, dft_energy_mu.c:22-96,
+ * dft_energy_vext.c:22-37, dft_energy_hs.c:22-58 and dft_energy_att.c:22-65
+ * (int_stencil, dft_energy_CLSintsten.c:73-129), and calc_adsorption
+ * (dft_out_ads.c:33-110, integrands :141-157), each integrated as
+ * integrateInSpace (dft_out_utils.c:38-98): the sum over nodes of
+ * integrand * nelhit[comp][node] * Vol_el / 2^D, times scale (Fac_vol, or
+ * Fac_vol / (Fac_area Area) per unit area).  Profile terms take Nel_hit2, bulk
+ * terms and the hard-sphere term Nel_hit (of component 0 for the latter).
+ * out[0] omega_sum, [1] omega_s_sum (the reference's "energy"), [2..10] ideal,
+ * ideal_b, mu, mu_b, vext, hs, hs_b, att, att_b, [11..14] ads, [15..18] ads_b.
+ * Pinned by: the README "energy" and "adsorption" key outputs of the 1D
+ * Examples (tests/test_oracle_golden.py).                                     */
+void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, const int *nel_hit2, double scale,
+                     double *out)
+{
+    const int N = c->N, C = c->p.ncomp, nd = c->p.ndim;
+    const int att_on = c->p.type_attr >= 0;
+    double vol = 1.0;
+    for (int d = 0; d < nd; d++) vol *= c->p.dx[d];
+    const double f = vol / (double)(1 << nd) * scale;
+    double betamu[NC];
+    for (int i = 0; i < C; i++)
+        betamu[i] = log(c->p.rho_b[i]) + c->betamu_hs_ex[i] + (att_on ? c->betamu_att[i] : 0.0);
+    double id = 0.0, mu = 0.0, vx = 0.0, hs = 0.0, att = 0.0, ads[NC] = {0}, sw1[NC] = {0};
+    for (int node = 0; node < N; node++) {
+        int ijk[3], refl[3];
+        node_ijk(c, node, ijk);
+        for (int i = 0; i < C; i++) {
+            const double rho = X(c->iu_rho + i, node);
+            const double w2 = nel_hit2[(size_t)i * N + node], w1 = nel_hit[(size_t)i * N + node];
+            if (rho > 1.e-20) { /* DENSITY_MIN */
+                id += rho * (log(rho) - 1.0) * w2;
+                mu += -rho * betamu[i] * w2;
+                if (c->p.nwall > 0) vx += rho * c->vext[(size_t)node * C + i] * w2;
+            }
+            if (att_on && rho > 0.0) {
+                double mf = 0.0;
+                for (int jc = 0; jc < C; jc++) {
+                    const sten_t *s = &c->sten[TMO_UATT][i + C * jc];
+                    for (int k = 0; k < s->L; k++) {
+                        int jn = o2n(c, ijk, &s->off[3 * k], refl);
+                        mf += s->w[k] * rho_val(c, x, jc, jn);
+                    }
+                }
+                att += 0.5 * rho * mf * w2;
+            }
+            ads[i] += rho * w2;
+            sw1[i] += w1;
+        }
+        double rb[10], n[10];
+        for (int q = 0; q < c->nrb; q++) rb[q] = X(c->iu_rb + q, node);
+        sv_to_n(nd, rb, n);
+        /* White Bear's third term is 0/0 at n3 = 0 (the reference would add a NaN): taken as 0 for
+           n3 <= 1e-10, the derivatives' cut-off (dft_physics_FMT3.c:68) */
+        if (c->p.type_func != TMO_FMT1 && !(n[3] > 1.e-10)) {
+            double d12 = 0.0;
+            for (int d = 0; d < nd; d++) d12 += n[4 + d] * n[4 + nd + d];
+            hs += (-n[0] * log(1.0 - n[3]) + (n[1] * n[2] - d12) / (1.0 - n[3])) * nel_hit[node];
+        } else
+            hs += tmo_fmt_phi(c, n) * nel_hit[node];
+    }
+    double nb[10] = {0};
+    sv_to_n(nd, c->rhobar_b, nb);
+    for (int d = 0; d < 2 * nd; d++) nb[4 + d] = 0.0;
+    double id_b = 0.0, mu_b = 0.0, att_b = 0.0;
+    for (int i = 0; i < C; i++) {
+        const double rb = c->p.rho_b[i];
+        id_b += rb * (log(rb) - 1.0) * sw1[i];
+        mu_b += -rb * betamu[i] * sw1[i];
+        if (att_on) att_b += 0.5 * rb * c->betamu_att[i] * sw1[i];
+    }
+    const double hs_b = tmo_fmt_phi(c, nb) * sw1[0];
+    double t[9] = {id, id_b, mu, mu_b, vx, hs, hs_b, att, att_b};
+    for (int q = 0; q < 9; q++) t[q] *= f;
+    out[0] = t[0] + t[2] + t[4] + t[5] + t[7];
+    out[1] = (t[0] - t[1]) + (t[2] - t[3]) + t[4] + (t[5] - t[6]) + (t[7] - t[8]);
+    for (int q = 0; q < 9; q++) out[2 + q] = t[q];
+    for (int i = 0; i < NC; i++) {
+        out[11 + i] = i < C ? ads[i] * f : 0.0;
+        out[15 + i] = i < C ? c->p.rho_b[i] * sw1[i] * f : 0.0;
+    }
+}

@@ -110,6 +110,10 @@ void tmo_init_fields(const tmo_ctx *c, double *x);
 /* built-in Newton with the reference's damping and a Schur-complement GMRES */
 int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton_stats *st);
 
+/* integrated observables (free-energy terms, adsorption); 19 doubles out */
+void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, const int *nel_hit2, double scale,
+                     double *out);
+
 int tmo_num_threads(void);
 
 #ifdef __cplusplus

@@ -2,7 +2,9 @@
 
 1. The device Newton-Krylov solve reproduces each 1D Example's README
    adsorption (absolute 1e-3, the reference harness' rule) and the oracle's
-   converged profile to 1e-8.
+   converged profile to 1e-8; the device observables (tmi_compute_observables)
+   of that profile reproduce the README energy and adsorption to 2e-6 and the
+   oracle's observables of the same profile to 1e-12.
 2. The reference's own Newton call sequence (src/dft_newton.c:131-196) driven
    through the LinProbMgr C ABI and fill_resid_and_matrix_control, with
    host double** arrays exactly as the reference passes them, converges to
@@ -19,7 +21,7 @@ torch = pytest.importorskip("torch")
 
 from oracle.oracle import Oracle  # noqa: E402
 from tramonto_amd import _lib  # noqa: E402
-from tramonto_amd.examples import adsorption, spec_from_tramonto  # noqa: E402
+from tramonto_amd.examples import adsorption, integration_counts, spec_from_tramonto  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 with open(os.path.join(HERE, "golden", "examples_1d.json")) as f:
@@ -46,6 +48,14 @@ def test_device_newton_reproduces_example(name):
     xo, sto = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10, update_scaling=g["nl_update_scaling"])
     assert sto["converged"]
     assert np.abs(xg[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+    h = integration_counts(spec, g)
+    obs = dp.observables(x, h)
+    assert abs(obs["omega_s"] - g["key_outputs"]["energy"]) < 2e-6, (obs, g["key_outputs"])
+    for c in range(spec.ncomp):
+        assert abs(obs["ads"][c] - g["key_outputs"]["adsorption"][c]) < 2e-6, (obs, g["key_outputs"])
+    ref = o.observables(xg, h, h)
+    for k in ("omega", "omega_s", "ideal", "mu", "vext", "hs", "att"):
+        assert abs(obs[k] - ref[k]) <= 1e-12 * max(1.0, abs(ref[k])), (k, obs[k], ref[k])
 
 
 def rows(a):

@@ -6,7 +6,8 @@ Examples/check_results/main.cpp:120-139).  The fixture
 tests/golden/examples_1d.json (written by make_fixtures.py) records the
 parameters and key outputs of the 1D soft-wall Rosenfeld + WCA examples; the
 oracle solves each to convergence with the reference's Newton rules and must
-reproduce the adsorption and the iteration count.
+reproduce the adsorption, the free energy ("energy": the surface excess grand
+potential of calc_free_energy) and the iteration count.
 """
 import json
 import os
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from tramonto_amd.examples import adsorption, spec_from_tramonto
+from tramonto_amd.examples import adsorption, integration_counts, spec_from_tramonto
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 with open(os.path.join(HERE, "golden", "examples_1d.json")) as f:
@@ -43,6 +44,20 @@ def test_example_key_outputs(name):
     for c in range(spec.ncomp):
         assert abs(ads[c] - ref["adsorption"][c]) < 1e-3, (c, ads[c], ref["adsorption"][c])
     assert abs(st["iters"] - ref["niter"]) <= 4, (st["iters"], ref["niter"])
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_example_free_energy(name):
+    """The oracle's observables at the converged profile: the README energy and adsorption."""
+    g = EXAMPLES[name]
+    spec, o, x, st = solve_oracle(g, name)
+    h = integration_counts(spec, g)
+    obs = o.observables(x, h, h)
+    ref = g["key_outputs"]
+    # the reference's rule is 1e-3 absolute; the README prints 6 decimals and the oracle agrees to 5e-7
+    assert abs(obs["omega_s"] - ref["energy"]) < 2e-6, (obs["omega_s"], ref["energy"], obs)
+    for c in range(spec.ncomp):
+        assert abs(obs["ads"][c] - ref["adsorption"][c]) < 2e-6, (c, obs["ads"][c], ref["adsorption"][c])
 
 
 def test_example_converged_residual_is_zero():

@@ -35,6 +35,13 @@ class TmiInfo(ct.Structure):
     ]
 
 
+class TmiObservables(ct.Structure):
+    _fields_ = [("omega", ct.c_double), ("omega_s", ct.c_double), ("ideal", ct.c_double), ("ideal_b", ct.c_double),
+                ("mu", ct.c_double), ("mu_b", ct.c_double), ("vext", ct.c_double), ("hs", ct.c_double),
+                ("hs_b", ct.c_double), ("att", ct.c_double), ("att_b", ct.c_double),
+                ("ads", ct.c_double * NC), ("ads_b", ct.c_double * NC)]
+
+
 class TmiNewtonOpts(ct.Structure):
     _fields_ = [
         ("max_iter", ct.c_int), ("nl_rel_tol", ct.c_double), ("nl_abs_tol", ct.c_double),
@@ -72,7 +79,7 @@ DECLARED_SYMBOLS = [
     "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning",
-    "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner",
+    "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_compute_observables",
     "fill_resid_and_matrix_control",
     "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
     "dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids", "dft_hardsphere_lin_prob_mgr_setdensityequationids",
@@ -118,6 +125,7 @@ def lib() -> ct.CDLL:
     L.tmi_problem_set_rho_b.argtypes = [vp, vp]
     L.tmi_problem_set_vext.argtypes = [vp, vp]
     L.tmi_problem_set_preconditioner.argtypes = [vp, i]
+    L.tmi_compute_observables.argtypes = [vp, vp, vp, vp, d, ct.POINTER(TmiObservables), vp]
     L.tmi_batch_create.argtypes = [ct.POINTER(TmiParams), i, vp, vp, vp, vp, ct.POINTER(vp)]
     L.tmi_batch_info.argtypes = [vp, ct.POINTER(i), vp]
     L.tmi_batch_newton_solve.argtypes = [vp, vp, ct.POINTER(TmiNewtonOpts), ct.POINTER(TmiNewtonStats), vp, vp, vp,

@@ -13,6 +13,7 @@
 #include <map>
 #include <vector>
 
+#include "tmi_fmt.hpp"
 #include "tmi_internal.hpp"
 
 namespace tmi {
@@ -988,6 +989,60 @@ int tmi_problem_set_preconditioner(void *handle, int kind)
         if (kind == TMI_PRECOND_BULK && !precond_available(*P))
             throw std::domain_error("the bulk preconditioner needs a periodic FFT grid and a stable bulk");
         precond_enable(*P, kind == TMI_PRECOND_BULK);
+        return 0;
+    });
+}
+
+int tmi_compute_observables(void *handle, const double *x_d, const int *nel_hit, const int *nel_hit2, double scale,
+                            tmi_observables *out, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !nel_hit || !nel_hit2 || !out) throw std::invalid_argument("null pointer");
+        hipStream_t s = (hipStream_t)stream;
+        const int C = P->p.ncomp, nd = P->p.ndim;
+        const size_t N = P->N, cn = (size_t)C * N;
+        // scratch: u * rho [C][N], then the two count arrays (ints, two to a double)
+        double *mf = scratch(*P, cn + cn);
+        int *nh = reinterpret_cast<int *>(mf + cn), *nh2 = nh + cn;
+        TMI_HIP_CHECK(hipMemcpyAsync(nh, nel_hit, cn * sizeof(int), hipMemcpyHostToDevice, s));
+        TMI_HIP_CHECK(hipMemcpyAsync(nh2, nel_hit2, cn * sizeof(int), hipMemcpyHostToDevice, s));
+        double sums[5 + 2 * TMI_NCOMP_MAX];
+        observables_sums(*P, x_d, nh, nh2, mf, sums, s);
+        double vol = 1.0;
+        for (int d = 0; d < nd; ++d) vol *= P->p.dx[d];
+        const double f = vol / (double)(1 << nd) * scale;
+        const bool att_on = P->p.type_attr >= 0 && P->nmf > 0;
+        double nb[10] = {0.0};   // bulk rho-bar values in n order, vector parts 0
+        nb[0] = P->bulk.rhobar_b[3];
+        nb[1] = P->bulk.rhobar_b[2];
+        nb[2] = P->bulk.rhobar_b[1];
+        nb[3] = P->bulk.rhobar_b[0];
+        const double phi_b = nd == 1 ? fmt_phi<1>(P->p.type_func, nb)
+                                     : (nd == 2 ? fmt_phi<2>(P->p.type_func, nb) : fmt_phi<3>(P->p.type_func, nb));
+        tmi_observables o{};
+        double id_b = 0.0, mu_b = 0.0, att_b = 0.0;
+        for (int c = 0; c < C; ++c) {
+            const double rb = P->p.rho_b[c], sw1 = sums[5 + TMI_NCOMP_MAX + c];
+            const double betamu = std::log(rb) + P->bulk.betamu_hs_ex[c] + (P->p.type_attr >= 0 ? P->bulk.betamu_att[c] : 0.0);
+            id_b += rb * (std::log(rb) - 1.0) * sw1;
+            mu_b += -rb * betamu * sw1;
+            if (att_on) att_b += 0.5 * rb * P->bulk.betamu_att[c] * sw1;
+            o.ads[c] = sums[5 + c] * f;
+            o.ads_b[c] = rb * sw1 * f;
+        }
+        o.ideal = sums[0] * f;
+        o.ideal_b = id_b * f;
+        o.mu = sums[1] * f;
+        o.mu_b = mu_b * f;
+        o.vext = sums[2] * f;
+        o.hs = sums[3] * f;
+        o.hs_b = phi_b * sums[5 + TMI_NCOMP_MAX] * f;
+        o.att = sums[4] * f;
+        o.att_b = att_b * f;
+        o.omega = o.ideal + o.mu + o.vext + o.hs + o.att;
+        o.omega_s = (o.ideal - o.ideal_b) + (o.mu - o.mu_b) + o.vext + (o.hs - o.hs_b) + (o.att - o.att_b);
+        *out = o;
         return 0;
     });
 }

@@ -58,6 +58,22 @@ __host__ __device__ __attribute__((always_inline)) inline FmtScalars fmt_scalars
     return s;
 }
 
+// Phi(n) (FMT1_energy_density / FMT3_energy_density, src/dft_physics_FMT1.c:39-57,
+// src/dft_physics_FMT3.c:39-59); White Bear's third term, 0/0 at n3 = 0, is 0 below the cut-off
+template <int ND>
+__host__ __device__ __attribute__((always_inline)) inline double fmt_phi(int func, const double *n)
+{
+    constexpr double inv_pi = 0.318309886183790671537767526745028724;
+    const FmtScalars s = fmt_scalars(func, n[3]);
+    double d12 = 0.0, d22 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        d12 += n[4 + d] * n[4 + ND + d];
+        d22 += n[4 + ND + d] * n[4 + ND + d];
+    }
+    return n[0] * s.lnA + (n[1] * n[2] - d12) * s.a + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f0 * (inv_pi / 36.0);
+}
+
 // dPhi/dn in n order
 template <int ND>
 __host__ __device__ __attribute__((always_inline)) inline void fmt_dphi(int func, const double *n, double *out)

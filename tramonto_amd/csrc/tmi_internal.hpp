@@ -172,6 +172,11 @@ enum Mode { kResidual = 0, kLinear = 1 };
 void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev = nullptr);
 // F(x) and sum F^2 (from the fused last passes' partial sums where every row comes from them)
 double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s);
+// weighted node sums of the free-energy and adsorption integrands of x (k_observables): sums[0..4]
+// ideal, mu, vext, hs, att; sums[5 + c] adsorption; sums[5 + kNC + c] sum of Nel_hit of c.
+// nh_d, nh2_d: device [C][N] element counts; mf_tmp: device [C][N] scratch
+void observables_sums(Problem &P, const double *x, const int *nh_d, const int *nh2_d, double *mf_tmp, double *sums,
+                      hipStream_t s);
 // Jv for all rows, precision T for the stencil sums
 template <typename T>
 void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStream_t s);

@@ -1574,6 +1574,108 @@ void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipE
     residual_impl(P, x, F, s, ev, nullptr, nullptr);
 }
 
+// ---------------------------------------------------------------- integrated observables
+// Integrands of calc_free_energy (src/dft_energy.c:30-391) and calc_adsorption
+// (src/dft_out_ads.c:33-110) at each node, weighted by the element counts as integrateInSpace
+// (src/dft_out_utils.c:38-98) weights them: partial[v][blk], v = 0 ideal gas
+// rho (ln rho - 1) (src/dft_energy_id.c:22-34), 1 chemical potential -rho beta mu
+// (src/dft_energy_mu.c:22-45), 2 external field rho V_ext (src/dft_energy_vext.c:22-37), 3 Phi(n)
+// of the rho-bar unknowns (src/dft_energy_hs.c:22-42), 4 attraction rho (u * rho) / 2
+// (src/dft_energy_att.c:22-31, int_stencil src/dft_energy_CLSintsten.c:73-129), 5 + c adsorption
+// of component c, 5 + C + c the sum of its Nel_hit counts (the bulk terms' integrals).  The first
+// three count only rho > 1e-20 (DENSITY_MIN), the attraction rho > 0.
+constexpr int kObsVals = 5 + 2 * kNC;
+struct ObsConsts {
+    double betamu[kNC];
+};
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_observables(int N, int C, int func, int nwall, int att_on,
+                                                        const double *__restrict__ x_rho,
+                                                        const double *__restrict__ x_rb,
+                                                        const double *__restrict__ mf,
+                                                        const double *__restrict__ vext, const int *__restrict__ nh,
+                                                        const int *__restrict__ nh2, ObsConsts oc,
+                                                        double *__restrict__ partial)
+{
+    double acc[kObsVals];
+#pragma unroll
+    for (int v = 0; v < kObsVals; ++v) acc[v] = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            if (c >= C) break;
+            const size_t row = (size_t)c * N + i;
+            const double rho = x_rho[row], w2 = (double)nh2[row], w1 = (double)nh[row];
+            if (rho > 1.e-20) {
+                acc[0] += rho * (log(rho) - 1.0) * w2;
+                acc[1] += -rho * oc.betamu[c] * w2;
+                if (nwall > 0) acc[2] += rho * vext[row] * w2;
+            }
+            if (att_on && rho > 0.0) acc[4] += 0.5 * rho * mf[row] * w2;
+            acc[5 + c] += rho * w2;
+            acc[5 + kNC + c] += w1;
+        }
+        double n[4 + 2 * ND];   // n order from the solution order n3 n2 n1 n0 nv2[D] nv1[D]
+        n[0] = x_rb[(size_t)3 * N + i];
+        n[1] = x_rb[(size_t)2 * N + i];
+        n[2] = x_rb[(size_t)1 * N + i];
+        n[3] = x_rb[i];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            n[4 + d] = x_rb[(size_t)(4 + ND + d) * N + i];
+            n[4 + ND + d] = x_rb[(size_t)(4 + d) * N + i];
+        }
+        acc[3] += fmt_phi<ND>(func, n) * (double)nh[i];
+    }
+    __shared__ double sh[kBlock / 64][kObsVals];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < kObsVals; ++v) {
+        double a = acc[v];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if (lane == 0) sh[wid][v] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kObsVals) {
+        double a = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) a += sh[q][threadIdx.x];
+        partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+void observables_sums(Problem &P, const double *x, const int *nh_d, const int *nh2_d, double *mf_tmp, double *sums,
+                      hipStream_t s)
+{
+    if (P.npores > 0) throw std::invalid_argument("observables of a batch of pores are not supported");
+    const size_t N = P.N;
+    const int C = P.p.ncomp, nd = P.p.ndim;
+    const bool att_on = P.p.type_attr >= 0 && P.nmf > 0;
+    if (att_on) {   // u * rho with the boundary values of the fill (int_stencil)
+        pad_rho<double>(P, x + (size_t)P.iu_rho * N, false, s);
+        mf_rows<double>(P, nullptr, nullptr, mf_tmp, s);
+    }
+    ObsConsts oc{};
+    for (int c = 0; c < C; ++c)
+        oc.betamu[c] = std::log(P.p.rho_b[c]) + P.bulk.betamu_hs_ex[c] + (P.p.type_attr >= 0 ? P.bulk.betamu_att[c] : 0.0);
+    const unsigned nb = (unsigned)std::min<long long>(1024, ((long long)N + kBlock - 1) / kBlock);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlock), 0, s, (int)N, C, P.p.type_func, P.p.nwall, att_on ? 1 : 0,
+                           x + (size_t)P.iu_rho * N, x + (size_t)P.iu_rb * N, mf_tmp, P.d_vext, nh_d, nh2_d, oc,
+                           P.d_red);
+    };
+    if (nd == 1)
+        go(k_observables<1>);
+    else if (nd == 2)
+        go(k_observables<2>);
+    else
+        go(k_observables<3>);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(kObsVals), dim3(kBlock), 0, s, P.d_red, (int)nb, kObsVals, nullptr,
+                       P.h_red_dev);
+    TMI_HIP_CHECK(hipGetLastError());
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    for (int v = 0; v < kObsVals; ++v) sums[v] = P.h_red[v];
+}
+
 double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s)
 {
     long long cap = 0, cnt = 0;

@@ -113,6 +113,26 @@ class DeviceProblem:
         _lib.check(_lib.lib().tmi_problem_set_preconditioner(self.handle, int(kind)), "tmi_problem_set_preconditioner")
         self.info.precond = int(kind)
 
+    def observables(self, x: torch.Tensor, nel_hit: np.ndarray, nel_hit2: Optional[np.ndarray] = None,
+                    scale: float = 1.0) -> Dict:
+        """Free-energy terms and adsorption of the profile x, summed on the device
+        (tmi_compute_observables): nel_hit / nel_hit2 are the [ncomp][N] element counts of the
+        reference's setup_integrals (nel_hit2 defaults to nel_hit, as without hard surfaces)."""
+        h1 = np.ascontiguousarray(nel_hit, dtype=np.int32)
+        h2 = h1 if nel_hit2 is None else np.ascontiguousarray(nel_hit2, dtype=np.int32)
+        if h1.shape != (self.spec.ncomp, self.nnodes) or h2.shape != h1.shape:
+            raise ValueError("element counts must be [ncomp][nnodes]")
+        o = _lib.TmiObservables()
+        _lib.check(_lib.lib().tmi_compute_observables(self.handle, _ptr(x), h1.ctypes.data_as(ct.c_void_p),
+                                                      h2.ctypes.data_as(ct.c_void_p), float(scale), ct.byref(o),
+                                                      _stream()), "tmi_compute_observables")
+        C = self.spec.ncomp
+        r = {k: float(getattr(o, k)) for k in ("omega", "omega_s", "ideal", "ideal_b", "mu", "mu_b", "vext", "hs",
+                                                "hs_b", "att", "att_b")}
+        r["ads"] = np.array(o.ads[:C])
+        r["ads_b"] = np.array(o.ads_b[:C])
+        return r
+
     # ------------------------------------------------------------ device ops
     def zeros(self) -> torch.Tensor:
         return torch.zeros((self.nunk, self.nnodes), dtype=torch.float64, device="cuda")

@@ -126,8 +126,10 @@ def node_weights_1d(spec: ProblemSpec, wall_elem: np.ndarray) -> np.ndarray:
     return w
 
 
-def adsorption(spec: ProblemSpec, g: Dict, rho: np.ndarray) -> np.ndarray:
-    """Integral of each component's density over the fluid (calc_adsorption)."""
+def integration_counts(spec: ProblemSpec, g: Dict) -> np.ndarray:
+    """The [ncomp][N] element counts Nel_hit = Nel_hit2 of a 1D example without hard surfaces
+    (setup_integrals, src/dft_out_utils.c:182-270): an element is a wall element when its centre
+    lies within the wall's half-width of a planar wall."""
     nx, dx = spec.nx[0], spec.dx[0]
     x = -0.5 * g["size_x"][0] + dx * np.arange(nx)
     mid = x[:-1] + 0.5 * dx
@@ -135,4 +137,10 @@ def adsorption(spec: ProblemSpec, g: Dict, rho: np.ndarray) -> np.ndarray:
     for w in g["walls"]:
         wall_elem |= np.abs(mid - w["pos"][0]) < g["wall_param"]
     wt = node_weights_1d(spec, wall_elem)
-    return np.array([(rho[c] * wt * dx / 2.0).sum() for c in range(spec.ncomp)])
+    return np.tile(wt.astype(np.int32), (spec.ncomp, 1))
+
+
+def adsorption(spec: ProblemSpec, g: Dict, rho: np.ndarray) -> np.ndarray:
+    """Integral of each component's density over the fluid (calc_adsorption)."""
+    wt = integration_counts(spec, g)[0]
+    return np.array([(rho[c] * wt * spec.dx[0] / 2.0).sum() for c in range(spec.ncomp)])
