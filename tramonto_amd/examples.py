@@ -1,7 +1,8 @@
 """State points described with Tramonto input parameters (1D planar walls).
 
-`spec_from_tramonto` turns the parameters of a Tramonto 1D example (as
-recorded in tests/golden/examples_1d.json) into a ProblemSpec:
+`read_tramonto_input` reads those parameters from a Tramonto input directory
+(dft_input.dat, dft_surfaces.dat); `spec_from_tramonto` turns them (or the
+records of tests/golden/examples_1d.json) into a ProblemSpec:
 
 * node positions x_i = -Size_x/2 + i Esize_x (node_to_position,
   src/dft_mesh_lib.c:579-620);
@@ -23,6 +24,8 @@ of src/dft_out_utils.c:225-270.
 from __future__ import annotations
 
 import math
+import os
+import re
 from typing import Dict
 
 import numpy as np
@@ -144,3 +147,79 @@ def adsorption(spec: ProblemSpec, g: Dict, rho: np.ndarray) -> np.ndarray:
     """Integral of each component's density over the fluid (calc_adsorption)."""
     wt = integration_counts(spec, g)[0]
     return np.array([(rho[c] * wt * spec.dx[0] / 2.0).sum() for c in range(spec.ncomp)])
+
+
+# ---------------------------------------------------------------- input files
+_NUM = re.compile(r"^[-+]?(\d+\.?\d*|\.\d+)([eE][-+]?\d+)?$")
+
+
+def _at_lines(path):
+    out = []
+    with open(path) as f:
+        for line in f:
+            if not line.startswith("@"):
+                continue
+            toks = line[1:].split()
+            vals = []
+            for t in toks:
+                if _NUM.match(t):
+                    vals.append(float(t))
+                else:
+                    break
+            label = " ".join(toks[len(vals):])
+            out.append((vals, label))
+    return out
+
+
+def _find(lines, prefix, start=0):
+    for k in range(start, len(lines)):
+        if lines[k][1].startswith(prefix):
+            return lines[k][0], k
+    raise KeyError(prefix)
+
+
+def read_tramonto_input(d: str) -> Dict:
+    """The parameters of a Tramonto input directory (dft_input.dat, dft_surfaces.dat) that the
+    nonlocal-density fill and its Newton solve consume, read from the positional `@` lines of
+    dft_input.dat (read_input_file, src/dft_input.c:47) and the wall rows of dft_surfaces.dat."""
+    L = _at_lines(os.path.join(d, "dft_input.dat"))
+    g = {}
+    v, _ = _find(L, "Length_ref")
+    g["vext_max"] = v[4]
+    g["ndim"] = int(_find(L, "Ndim")[0][0])
+    g["size_x"] = _find(L, "Size_x")[0][: g["ndim"]]
+    g["esize_x"] = _find(L, "Esize_x")[0][: g["ndim"]]
+    g["type_bc_x"] = [int(t) for t in _find(L, "Type_bc(x0")[0][:2]]
+    v, _ = _find(L, "Type_func")
+    g["type_func"] = int(v[0])
+    v, _ = _find(L, "Type_attr")
+    g["type_attr"], g["type_pairpot"] = int(v[0]), int(v[1])
+    v, _ = _find(L, "Nwall_types")
+    g["nwall_type"], g["nwall"] = int(v[0]), int(v[1])
+    g["surf_type"] = int(_find(L, "Surf_type")[0][0])
+    g["orientation"] = int(_find(L, "Orientation[iwall_type]")[0][0])
+    g["wall_param"] = _find(L, "WallParam[iwall_type]")[0][0]
+    g["ipot_wf_n"] = int(_find(L, "Type of neutral surface-fluid")[0][0])
+    g["lhard_surf"] = int(_find(L, "Lhard_surf")[0][0])
+    g["type_vext"] = int(_find(L, "Type_vext")[0][0])
+    g["vext_potential_id"] = int(_find(L, "Vext_PotentialID")[0][0])
+    v, _ = _find(L, "Ncomp Mix_type")
+    g["ncomp"], g["mix_type"] = int(v[0]), int(v[1])
+    C = g["ncomp"]
+    for key, lab in (("sigma_ff", "Sigma_ff"), ("eps_ff", "Eps_ff"), ("cut_ff", "Cut_ff")):
+        g[key] = _find(L, lab)[0][: C * C]
+    for key, lab in (("rho_w", "Rho_w"), ("sigma_ww", "Sigma_ww"), ("eps_ww", "Eps_ww"), ("cut_ww", "Cut_ww")):
+        g[key] = _find(L, lab)[0][0]
+    for key, lab in (("sigma_wf", "Sigma_wf"), ("eps_wf", "Eps_wf"), ("cut_wf", "Cut_wf")):
+        g[key] = _find(L, lab)[0][:C]
+    g["rho_b"] = _find(L, "Rho_b_0")[0][:C]
+    v, _ = _find(L, "Iguess")
+    g["iguess"] = int(v[0])
+    v, _ = _find(L, "NL_Solver")
+    g["nl_solver"], g["max_nl_iter"], g["att_in_a22"] = int(v[0]), int(v[1]), int(v[3])
+    v, _ = _find(L, "NL_rel_tol NL_abs_tol")
+    g["nl_rel_tol"], g["nl_abs_tol"], g["nl_update_scaling"] = v[0], v[1], v[2]
+    with open(os.path.join(d, "dft_surfaces.dat")) as f:
+        rows = [r.split() for r in f if r.strip()]
+    g["walls"] = [{"type": int(r[0]), "pos": [float(t) for t in r[2:2 + g["ndim"]]]} for r in rows]
+    return g
