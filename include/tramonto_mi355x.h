@@ -70,6 +70,9 @@ typedef struct tmi_params {
     double eps_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
     double cut_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
     double rho_b[TMI_NCOMP_MAX];
+    int att_in_a22;     /* 1: the reference's ATTInA22Block = TRUE layout: no mean-field unknowns, the
+                           attraction sums enter the Euler-Lagrange rows directly (A22 not diagonal);
+                           0: one mean-field unknown per component (ATTInA22Block = FALSE) */
 } tmi_params;
 
 typedef struct tmi_info {

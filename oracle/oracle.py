@@ -79,7 +79,10 @@ def lib() -> ct.CDLL:
 
 
 def _fill(struct, d: Dict):
+    names = {f[0] for f in struct._fields_}
     for k, v in d.items():
+        if k not in names:   # layout options of the device library (att_in_a22): the oracle keeps MF rows
+            continue
         f = getattr(struct, k)
         if isinstance(v, (list, tuple)):
             if isinstance(v[0], (list, tuple)):

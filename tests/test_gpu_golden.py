@@ -85,13 +85,17 @@ def update_solution(x, dx, ncomp, scaling):
 
 
 @pytest.mark.gpu
-def test_reference_abi_newton_sequence():
+@pytest.mark.parametrize("att_in_a22", [0, 1])
+def test_reference_abi_newton_sequence(att_in_a22):
     """initializeproblemvalues -> fill(x, iter, FALSE) -> finalizeproblemvalues
-    -> setupsolver -> solve -> getlhs -> x += dx, as dft_newton.c does."""
+    -> setupsolver -> solve -> getlhs -> x += dx, as dft_newton.c does; in the
+    mean-field layout and in the example's own ATTInA22Block = 1 layout."""
+    import dataclasses
     from tramonto_amd.device import DeviceProblem
     L = _lib.lib()
     g = EXAMPLES["LJ1_FMT1_WCA_1D"]
-    spec = spec_from_tramonto(g, "LJ1")
+    spec0 = spec_from_tramonto(g, "LJ1")
+    spec = dataclasses.replace(spec0, att_in_a22=att_in_a22)
     dp = DeviceProblem(spec)
     nunk, N = spec.nunk, spec.nnodes
     mgr = L.dft_hardsphere_lin_prob_mgr_create(nunk, None, None)
@@ -124,8 +128,8 @@ def test_reference_abi_newton_sequence():
         assert L.dft_linprobmgr_getlhs(mgr, ct.cast(dxr, ct.c_void_p)) == 0
         update_solution(x, dx, spec.ncomp, g["nl_update_scaling"])
     assert converged
-    o = Oracle.from_spec(spec)
-    xo, _ = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10, update_scaling=1.0)
+    o = Oracle.from_spec(spec0)
+    xo, _ = o.newton(o.init_fields(spec0.initial_x()), resid_tol=1e-10, update_scaling=1.0)
     assert np.abs(x[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
     # applymatrix: J dx against the device product
     v = np.ascontiguousarray(np.random.default_rng(1).uniform(-1, 1, x.shape))

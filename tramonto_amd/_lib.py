@@ -24,6 +24,7 @@ class TmiParams(ct.Structure):
         ("hs_diam", ct.c_double * NC),
         ("sigma_ff", (ct.c_double * NC) * NC), ("eps_ff", (ct.c_double * NC) * NC), ("cut_ff", (ct.c_double * NC) * NC),
         ("rho_b", ct.c_double * NC),
+        ("att_in_a22", ct.c_int),
     ]
 
 

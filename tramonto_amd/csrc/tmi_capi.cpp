@@ -324,6 +324,9 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
     P->iu_rb = C;
     P->iu_mf = C + P->nrb;
     P->nunk = C + P->nrb + P->nmf;
+    P->ext_mf = p.att_in_a22 != 0 && P->nmf > 0;
+    P->nunk_ext = P->ext_mf ? C + P->nrb : P->nunk;
+    if (P->ext_mf && batch) throw std::domain_error("att_in_a22 is not supported for batches of pores");
     P->nadj = C * (1 + nd) + 1;
     compute_bulk_constants(p, P->bulk);
     if (p.type_attr >= 0)
@@ -525,6 +528,10 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->wsf.corr_f);
     (void)hipFree(P->d_red);
     (void)hipFree(P->d_scratch);
+    (void)hipFree(P->d_xi);
+    (void)hipFree(P->d_vi);
+    (void)hipFree(P->d_fi);
+    (void)hipFree(P->d_gi);
     if (P->h_red) (void)hipHostFree(P->h_red);
     delete P;
 }
@@ -533,6 +540,30 @@ Problem *as_problem(void *h)
 {
     if (!h) throw std::invalid_argument("null problem handle");
     return (Problem *)h;
+}
+
+// ---- external layout without mean-field rows (att_in_a22): adapters of the entry points
+double *ibuf(Problem &P, double *&b)
+{
+    if (!b) b = dev_alloc<double>((size_t)P.nunk * P.N);
+    return b;
+}
+// x in the internal layout: the external rows, then u * rho of its densities (linear: a direction,
+// zero boundary values); x itself when the layouts agree
+const double *expand(Problem &P, const double *x, double *&buf, bool linear, hipStream_t s)
+{
+    if (!P.ext_mf) return x;
+    double *b = ibuf(P, buf);
+    const size_t N = P.N;
+    if (b != x) TMI_HIP_CHECK(hipMemcpyAsync(b, x, sizeof(double) * P.nunk_ext * N, hipMemcpyDeviceToDevice, s));
+    launch_mf_conv(P, b + (size_t)P.iu_rho * N, linear, b + (size_t)P.iu_mf * N, s);
+    return b;
+}
+// the external rows of an internal-layout result
+void shrink(Problem &P, const double *in, double *out, hipStream_t s)
+{
+    if (in != out)
+        TMI_HIP_CHECK(hipMemcpyAsync(out, in, sizeof(double) * P.nunk_ext * P.N, hipMemcpyDeviceToDevice, s));
 }
 
 double *scratch(Problem &P, size_t doubles)
@@ -764,13 +795,13 @@ int tmi_problem_info(void *handle, tmi_info *info)
     return guarded([&] {
         Problem *P = as_problem(handle);
         std::memset(info, 0, sizeof(*info));
-        info->nunk = P->nunk;
+        info->nunk = P->nunk_ext;
         info->nnodes = P->N;
         info->iu_rho = P->iu_rho;
         info->iu_rb = P->iu_rb;
         info->n_rb = P->nrb;
-        info->iu_mf = P->iu_mf;
-        info->n_mf = P->nmf;
+        info->iu_mf = P->ext_mf ? P->nunk_ext : P->iu_mf;   // no mean-field rows in the att_in_a22 layout
+        info->n_mf = P->ext_mf ? 0 : P->nmf;
         for (int t = 0; t < 3; ++t)
             for (int q = 0; q < kNC * kNC; ++q) info->sten_len[t][q] = P->hs[t][q].L;
         info->mf_fft = fft_mf(*P) ? 1 : 0;
@@ -871,10 +902,13 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
         Problem *P = as_problem(handle);
         if (!x_d || !F_d) throw std::invalid_argument("null device pointer");
         hipStream_t s = (hipStream_t)stream;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s);
+        double *Fi = P->ext_mf ? ibuf(*P, P->d_fi) : F_d;
         if (sumsq)
-            *sumsq = residual_sumsq(*P, x_d, F_d, s);
+            *sumsq = residual_sumsq(*P, xi, Fi, s);
         else
-            launch_residual(*P, x_d, F_d, s);
+            launch_residual(*P, xi, Fi, s);
+        if (P->ext_mf) shrink(*P, Fi, F_d, s);
         return 0;
     });
 }
@@ -885,11 +919,13 @@ int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int re
         Problem *P = as_problem(handle);
         if (!x_d || !F_d || !stage_ms || reps < 1) throw std::invalid_argument("bad argument");
         hipStream_t s = (hipStream_t)stream;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s);
+        double *Fi = P->ext_mf ? ibuf(*P, P->d_fi) : F_d;
         hipEvent_t ev[6];
         for (auto &e : ev) TMI_HIP_CHECK(hipEventCreate(&e));
         for (int q = 0; q < 5; ++q) stage_ms[q] = 0.0;
         for (int r = 0; r < reps; ++r) {
-            launch_residual(*P, x_d, F_d, s, ev);
+            launch_residual(*P, xi, Fi, s, ev);
             TMI_HIP_CHECK(hipEventSynchronize(ev[5]));
             for (int q = 0; q < 5; ++q) {
                 float ms = 0.f;
@@ -917,7 +953,11 @@ int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, doubl
     return guarded([&] {
         Problem *P = as_problem(handle);
         if (!x_d || !v_d || !Jv_d) throw std::invalid_argument("null device pointer");
-        launch_jvp<double>(*P, x_d, v_d, Jv_d, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s), *vi = expand(*P, v_d, P->d_vi, true, s);
+        double *Ji = P->ext_mf ? ibuf(*P, P->d_fi) : Jv_d;
+        launch_jvp<double>(*P, xi, vi, Ji, s);
+        if (P->ext_mf) shrink(*P, Ji, Jv_d, s);
         return 0;
     });
 }
@@ -927,7 +967,11 @@ int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, 
     return guarded([&] {
         Problem *P = as_problem(handle);
         if (!x_d || !v_d || !Jv_d) throw std::invalid_argument("null device pointer");
-        launch_jvp<float>(*P, x_d, v_d, Jv_d, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s), *vi = expand(*P, v_d, P->d_vi, true, s);
+        double *Ji = P->ext_mf ? ibuf(*P, P->d_fi) : Jv_d;
+        launch_jvp<float>(*P, xi, vi, Ji, s);
+        if (P->ext_mf) shrink(*P, Ji, Jv_d, s);
         return 0;
     });
 }
@@ -938,6 +982,7 @@ int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *
         Problem *P = as_problem(handle);
         if (!x_d || !w_d || !JTw_d) throw std::invalid_argument("null device pointer");
         if (P->npores > 0) throw std::domain_error("transpose products of batched problems are not implemented");
+        if (P->ext_mf) throw std::domain_error("transpose products in the att_in_a22 layout are not implemented");
         launch_vjp<double>(*P, x_d, w_d, JTw_d, (hipStream_t)stream);
         return 0;
     });
@@ -1051,7 +1096,15 @@ int tmi_init_fields(void *handle, double *x_d, void *stream)
 {
     return guarded([&] {
         Problem *P = as_problem(handle);
-        launch_init_fields(*P, x_d, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        if (!P->ext_mf) {
+            launch_init_fields(*P, x_d, s);
+            return 0;
+        }
+        double *xi = ibuf(*P, P->d_xi);
+        TMI_HIP_CHECK(hipMemcpyAsync(xi, x_d, sizeof(double) * P->nunk_ext * P->N, hipMemcpyDeviceToDevice, s));
+        launch_init_fields(*P, xi, s);
+        shrink(*P, xi, x_d, s);
         return 0;
     });
 }
@@ -1061,7 +1114,19 @@ int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *
 {
     return guarded([&] {
         Problem *P = as_problem(handle);
-        return schur_solve(*P, x_d, F_d, dx_d, rtol, restart, maxit, precision, iters, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        if (!P->ext_mf) return schur_solve(*P, x_d, F_d, dx_d, rtol, restart, maxit, precision, iters, s);
+        // internal F: the external rows, mean-field rows 0 (x_mf = u * rho holds); dx's mean-field rows
+        // (u * drho) are dropped
+        const double *xi = expand(*P, x_d, P->d_xi, false, s);
+        double *Fi = ibuf(*P, P->d_fi), *dxi = ibuf(*P, P->d_gi);
+        const size_t ne = (size_t)P->nunk_ext * P->N;
+        TMI_HIP_CHECK(hipMemcpyAsync(Fi, F_d, sizeof(double) * ne, hipMemcpyDeviceToDevice, s));
+        TMI_HIP_CHECK(hipMemsetAsync(Fi + ne, 0, sizeof(double) * ((size_t)P->nunk * P->N - ne), s));
+        const int rc = schur_solve(*P, xi, Fi, dxi, rtol, restart, maxit, precision, iters, s);
+        shrink(*P, dxi, dx_d, s);
+        TMI_HIP_CHECK(hipStreamSynchronize(s));
+        return rc;
     });
 }
 
@@ -1072,7 +1137,13 @@ int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi
         if (!opts || !stats) throw std::invalid_argument("null argument");
         if (P->npores > 0)
             return batch_newton_solve(*P, x_d, *opts, *stats, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
-        return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
+        if (!P->ext_mf) return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        double *xi = const_cast<double *>(expand(*P, x_d, P->d_xi, false, s));
+        const int rc = newton_solve(*P, xi, *opts, *stats, s);
+        shrink(*P, xi, x_d, s);
+        TMI_HIP_CHECK(hipStreamSynchronize(s));
+        return rc;
     });
 }
 

@@ -163,6 +163,12 @@ struct Problem {
     double *d_red = nullptr;     // reduction buffer
     double *h_red = nullptr;     // pinned host mirror (1024 doubles), written by reduction kernels directly
     double *h_red_dev = nullptr; // its device address
+    // external layout without the mean-field rows (attraction in the A22 block): the entry points
+    // expand x to the internal layout, whose mean-field rows (the last nmf rows) are recomputed
+    // from the densities, and return the first nunk_ext rows
+    bool ext_mf = false;
+    int nunk_ext = 0;
+    double *d_xi = nullptr, *d_vi = nullptr, *d_fi = nullptr, *d_gi = nullptr;   // [nunk][N] each
 };
 
 // ---------------------------------------------------------------- kernels
@@ -172,6 +178,9 @@ enum Mode { kResidual = 0, kLinear = 1 };
 void launch_residual(Problem &P, const double *x, double *F, hipStream_t s, hipEvent_t *ev = nullptr);
 // F(x) and sum F^2 (from the fused last passes' partial sums where every row comes from them)
 double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s);
+// out = u * rho (the mean-field sums, [C][N]) of the densities rho ([C][N] node rows; linear: zero
+// boundary values, else the bulk values of the fill)
+void launch_mf_conv(Problem &P, const double *rho, bool linear, double *out, hipStream_t s);
 // weighted node sums of the free-energy and adsorption integrands of x (k_observables): sums[0..4]
 // ideal, mu, vext, hs, att; sums[5 + c] adsorption; sums[5 + kNC + c] sum of Nel_hit of c.
 // nh_d, nh2_d: device [C][N] element counts; mf_tmp: device [C][N] scratch

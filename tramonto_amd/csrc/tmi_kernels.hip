@@ -1643,6 +1643,13 @@ __global__ __launch_bounds__(kBlock) void k_observables(int N, int C, int func, 
     }
 }
 
+void launch_mf_conv(Problem &P, const double *rho, bool linear, double *out, hipStream_t s)
+{
+    if (P.nmf == 0) return;
+    pad_rho<double>(P, rho, linear, s);
+    mf_rows<double>(P, nullptr, nullptr, out, s);
+}
+
 void observables_sums(Problem &P, const double *x, const int *nh_d, const int *nh2_d, double *mf_tmp, double *sums,
                       hipStream_t s)
 {

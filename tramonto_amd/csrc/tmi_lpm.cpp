@@ -70,18 +70,21 @@ void ensure_device(LinProbMgr &M)
 
 void check_layout(const LinProbMgr &M)
 {
-    if (M.numUnks != M.prob->nunk) throw std::logic_error("numUnks differs from the registered problem");
+    if (M.numUnks != M.prob->nunk_ext) throw std::logic_error("numUnks differs from the registered problem");
 }
 
-// host double** [nunk][N] <-> contiguous device buffer
-void to_device(const Problem &P, double **x, double *d)
+// host double** [nunk_ext][N] <-> contiguous device buffer in the internal layout; with attraction in
+// A22 (no mean-field unknowns) the device buffer's mean-field rows are u * rho of its densities
+// (linear: a direction, zero boundary values)
+void to_device(Problem &P, double **x, double *d, bool linear)
 {
-    for (int u = 0; u < P.nunk; ++u)
+    for (int u = 0; u < P.nunk_ext; ++u)
         TMI_HIP_CHECK(hipMemcpy(d + (size_t)u * P.N, x[u], sizeof(double) * P.N, hipMemcpyHostToDevice));
+    if (P.ext_mf) launch_mf_conv(P, d + (size_t)P.iu_rho * P.N, linear, d + (size_t)P.iu_mf * P.N, nullptr);
 }
 void to_host(const Problem &P, const double *d, double **x)
 {
-    for (int u = 0; u < P.nunk; ++u)
+    for (int u = 0; u < P.nunk_ext; ++u)
         TMI_HIP_CHECK(hipMemcpy(x[u], d + (size_t)u * P.N, sizeof(double) * P.N, hipMemcpyDeviceToHost));
 }
 
@@ -134,9 +137,9 @@ double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag)
         LinProbMgr &M = g_active ? *g_active : tmp;
         M.prob = &P;
         ensure_device(M);
-        to_device(P, x, M.d_x);
+        to_device(P, x, M.d_x, false);
         launch_residual(P, M.d_x, M.d_F, nullptr);
-        const size_t n = (size_t)P.nunk * P.N;
+        const size_t n = (size_t)P.nunk_ext * P.N;   // the caller's rows
         std::vector<double> F(n);
         TMI_HIP_CHECK(hipMemcpy(F.data(), M.d_F, n * sizeof(double), hipMemcpyDeviceToHost));
         l2 = 0.0;
@@ -365,10 +368,10 @@ int dft_linprobmgr_solve(void *h)
         if (!M->have_x) throw std::logic_error("solve requires a full fill first");
         Problem &P = *M->prob;
         ensure_device(*M);
-        const size_t n = (size_t)P.nunk * P.N;
-        std::vector<double> negb(n);
+        const size_t n = (size_t)P.nunk_ext * P.N, ni = (size_t)P.nunk * P.N;
+        std::vector<double> negb(ni, 0.0);   // internal mean-field rows (if any) stay 0: x_mf = u * rho holds
         for (size_t i = 0; i < n; ++i) negb[i] = -M->rhs[i];   // F such that rhs = -F
-        TMI_HIP_CHECK(hipMemcpy(M->d_F, negb.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        TMI_HIP_CHECK(hipMemcpy(M->d_F, negb.data(), ni * sizeof(double), hipMemcpyHostToDevice));
         schur_solve_export(P, M->d_x, M->d_F, M->d_dx, M->gmres_tol, M->gmres_restart, M->gmres_maxit, TMI_PREC_FP64,
                            &M->last_iters, nullptr);
         M->lhs.assign(n, 0.0);
@@ -384,7 +387,7 @@ int dft_linprobmgr_applymatrix(void *h, double **x, double **b)
         LinProbMgr *M = lpm(h);
         if (!M->have_x) throw std::logic_error("applymatrix requires a full fill first");
         Problem &P = *M->prob;
-        to_device(P, x, M->d_v);
+        to_device(P, x, M->d_v, true);
         launch_jvp<double>(P, M->d_x, M->d_v, M->d_dx, nullptr);
         TMI_HIP_CHECK(hipDeviceSynchronize());
         to_host(P, M->d_dx, b);
@@ -435,9 +438,11 @@ double dft_linprobmgr_getmatrixvalue(void *h, int iunk, int ownednode, int junk,
         if (!M->have_x) throw std::logic_error("getmatrixvalue requires a full fill first");
         Problem &P = *M->prob;
         const size_t n = (size_t)P.nunk * P.N;
+        if (junk < 0 || junk >= P.nunk_ext || iunk < 0 || iunk >= P.nunk_ext) throw std::invalid_argument("unknown index");
         TMI_HIP_CHECK(hipMemset(M->d_v, 0, n * sizeof(double)));
         double one = 1.0;
         TMI_HIP_CHECK(hipMemcpy(M->d_v + (size_t)junk * P.N + boxnode, &one, sizeof(double), hipMemcpyHostToDevice));
+        if (P.ext_mf) launch_mf_conv(P, M->d_v + (size_t)P.iu_rho * P.N, true, M->d_v + (size_t)P.iu_mf * P.N, nullptr);
         launch_jvp<double>(P, M->d_x, M->d_v, M->d_dx, nullptr);
         TMI_HIP_CHECK(hipMemcpy(&val, M->d_dx + (size_t)iunk * P.N + ownednode, sizeof(double), hipMemcpyDeviceToHost));
         return 0;
@@ -451,7 +456,7 @@ int dft_linprobmgr_writeMatrix(void *h, char *filename, char *matrixName, char *
         LinProbMgr *M = lpm(h);
         if (!M->have_x) throw std::logic_error("writeMatrix requires a full fill first");
         Problem &P = *M->prob;
-        const size_t n = (size_t)P.nunk * P.N;
+        const size_t n = (size_t)P.nunk_ext * P.N, ni = (size_t)P.nunk * P.N;
         if (n > 20000) throw std::domain_error("writeMatrix is limited to small problems");
         FILE *f = std::fopen(filename, "w");
         if (!f) throw std::invalid_argument("cannot open matrix file");
@@ -461,9 +466,11 @@ int dft_linprobmgr_writeMatrix(void *h, char *filename, char *matrixName, char *
         std::vector<std::vector<std::pair<size_t, double>>> cols(n);
         size_t nnz = 0;
         for (size_t j = 0; j < n; ++j) {
-            TMI_HIP_CHECK(hipMemset(M->d_v, 0, n * sizeof(double)));
+            TMI_HIP_CHECK(hipMemset(M->d_v, 0, ni * sizeof(double)));
             double one = 1.0;
             TMI_HIP_CHECK(hipMemcpy(M->d_v + j, &one, sizeof(double), hipMemcpyHostToDevice));
+            if (P.ext_mf)
+                launch_mf_conv(P, M->d_v + (size_t)P.iu_rho * P.N, true, M->d_v + (size_t)P.iu_mf * P.N, nullptr);
             launch_jvp<double>(P, M->d_x, M->d_v, M->d_dx, nullptr);
             TMI_HIP_CHECK(hipMemcpy(col.data(), M->d_dx, n * sizeof(double), hipMemcpyDeviceToHost));
             for (size_t i = 0; i < n; ++i)

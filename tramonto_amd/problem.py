@@ -74,6 +74,7 @@ class ProblemSpec:
     vext: Optional[np.ndarray] = None       # [N, C]
     zero_tf: Optional[np.ndarray] = None    # [N, C] uint8
     rho0: Optional[np.ndarray] = None       # [C, N] initial densities
+    att_in_a22: int = 0                     # 1: no mean-field unknowns (the reference's ATTInA22Block)
 
     @property
     def nnodes(self) -> int:
@@ -85,7 +86,8 @@ class ProblemSpec:
 
     @property
     def nmf(self) -> int:
-        return self.ncomp if self.type_attr >= 0 else 0
+        """Mean-field unknowns of the layout (none with the attraction in the A22 block)."""
+        return self.ncomp if (self.type_attr >= 0 and not self.att_in_a22) else 0
 
     @property
     def nunk(self) -> int:
@@ -105,7 +107,7 @@ class ProblemSpec:
             bc=[list(b) for b in self.bc] + [[IN_BULK, IN_BULK]] * (3 - len(self.bc)),
             type_func=self.type_func, type_attr=self.type_attr, nwall=self.nwall, lhard_surf=0,
             vext_max=self.vext_max, hs_diam=pad(self.hs_diam), sigma_ff=mat(sig), eps_ff=mat(eps),
-            cut_ff=mat(cut), rho_b=pad(self.rho_b),
+            cut_ff=mat(cut), rho_b=pad(self.rho_b), att_in_a22=int(self.att_in_a22),
         )
 
     def initial_x(self) -> np.ndarray:
