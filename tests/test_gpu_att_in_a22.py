@@ -140,3 +140,39 @@ def test_examples_in_their_own_layout(name):
     assert abs(obs["omega_s"] - g["key_outputs"]["energy"]) < 2e-6, (obs["omega_s"], g["key_outputs"])
     for c in range(spec.ncomp):
         assert abs(obs["ads"][c] - g["key_outputs"]["adsorption"][c]) < 2e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["1d_reflect_wca", "1d_pore_mix"])
+def test_transpose_product_eliminates_the_mean_field_rows(case):
+    from tramonto_amd.device import DeviceProblem
+    spec, spec_a = pair(case)
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec_a)
+    x = state(spec, o, 8)
+    N, ne = spec.nnodes, spec_a.nunk
+    J = o.jacobian_dense(x)
+    e = np.arange(ne * N)
+    mf = np.arange(ne * N, spec.nunk * N)
+    Je = J[np.ix_(e, e)] + J[np.ix_(e, mf)] @ J[np.ix_(mf, e)]
+    w = np.random.default_rng(9).uniform(-1.0, 1.0, ne * N)
+    got = dp.apply_jacobian_transpose(dev(x[:ne]), dev(w.reshape(ne, N))).cpu().numpy().ravel()
+    ref = Je.T @ w
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2d_rough_wall", "3d_periodic_small"])
+def test_adjoint_identity_in_the_att_in_a22_layout(case):
+    from tramonto_amd.device import DeviceProblem
+    spec, spec_a = pair(case)
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec_a)
+    ne = spec_a.nunk
+    x = dev(state(spec, o, 10)[:ne])
+    g = torch.Generator(device="cpu").manual_seed(3)
+    v = (torch.rand(x.shape, generator=g, dtype=torch.float64) - 0.5).cuda()
+    w = (torch.rand(x.shape, generator=g, dtype=torch.float64) - 0.5).cuda()
+    a = float((w * dp.apply_jacobian(x, v)).sum())
+    b = float((dp.apply_jacobian_transpose(x, w) * v).sum())
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)), (a, b)

@@ -596,7 +596,7 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     const bool bulk_pc = precond_active(P);
     const double *x_rho = x + (size_t)P.iu_rho * P.N;
     if (!bulk_pc) dev_a22_inverse_diag(P, x, dinv, s);
-    TMI_HIP_CHECK(hipMemsetAsync(x2, 0, sizeof(double) * n, s));
+    dev_fill(0.0, x2, (long long)n, s);
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), h(m + 1);
     // device slot of the Arnoldi coefficients (past the partial sums of the dot products)
     double *dh = P.d_red + 1024 * 1024 / 2;
@@ -671,7 +671,7 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
             y[i] = sum / H[(size_t)i * m + i];
         }
         TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        TMI_HIP_CHECK(hipMemsetAsync(z, 0, sizeof(double) * n, s));
+        dev_fill(0.0, z, (long long)n, s);
         dev_gemv_add(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
         if (bulk_pc)
             precond_apply(P, x_rho, z, w, s);
@@ -982,8 +982,24 @@ int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *
         Problem *P = as_problem(handle);
         if (!x_d || !w_d || !JTw_d) throw std::invalid_argument("null device pointer");
         if (P->npores > 0) throw std::domain_error("transpose products of batched problems are not implemented");
-        if (P->ext_mf) throw std::domain_error("transpose products in the att_in_a22 layout are not implemented");
-        launch_vjp<double>(*P, x_d, w_d, JTw_d, (hipStream_t)stream);
+        hipStream_t s = (hipStream_t)stream;
+        if (!P->ext_mf) {
+            launch_vjp<double>(*P, x_d, w_d, JTw_d, s);
+            return 0;
+        }
+        // J_ext = J[e,e] + J[e,mf] J[mf,e]: J_ext^T w = (J^T [w; 0])_e + (J^T [0; (J^T [w; 0])_mf])_e, since
+        // the mean-field rows are x_mf - u * rho (J[mf,mf] = -I): two internal transpose products
+        const size_t N = P->N, ne = (size_t)P->nunk_ext * N, ni = (size_t)P->nunk * N;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s);
+        double *wi = ibuf(*P, P->d_vi), *y = ibuf(*P, P->d_fi), *z = ibuf(*P, P->d_gi);
+        TMI_HIP_CHECK(hipMemcpyAsync(wi, w_d, sizeof(double) * ne, hipMemcpyDeviceToDevice, s));
+        dev_fill(0.0, wi + ne, (long long)(ni - ne), s);
+        launch_vjp<double>(*P, xi, wi, y, s);
+        dev_fill(0.0, wi, (long long)ne, s);
+        TMI_HIP_CHECK(hipMemcpyAsync(wi + ne, y + ne, sizeof(double) * (ni - ne), hipMemcpyDeviceToDevice, s));
+        launch_vjp<double>(*P, xi, wi, z, s);
+        dev_axpy(1.0, z, y, (long long)ne, s);
+        shrink(*P, y, JTw_d, s);
         return 0;
     });
 }
@@ -1122,7 +1138,7 @@ int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *
         double *Fi = ibuf(*P, P->d_fi), *dxi = ibuf(*P, P->d_gi);
         const size_t ne = (size_t)P->nunk_ext * P->N;
         TMI_HIP_CHECK(hipMemcpyAsync(Fi, F_d, sizeof(double) * ne, hipMemcpyDeviceToDevice, s));
-        TMI_HIP_CHECK(hipMemsetAsync(Fi + ne, 0, sizeof(double) * ((size_t)P->nunk * P->N - ne), s));
+        dev_fill(0.0, Fi + ne, (long long)((size_t)P->nunk * P->N - ne), s);
         const int rc = schur_solve(*P, xi, Fi, dxi, rtol, restart, maxit, precision, iters, s);
         shrink(*P, dxi, dx_d, s);
         TMI_HIP_CHECK(hipStreamSynchronize(s));

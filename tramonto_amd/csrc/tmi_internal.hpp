@@ -265,6 +265,8 @@ void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long 
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);
+// y[0..n) = v with a kernel (no memset node, so a captured graph replays it as launched)
+void dev_fill(double v, double *y, long long n, hipStream_t s);
 void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s);
 void dev_mul(const double *a, const double *b, double *y, long long n, hipStream_t s);
 void dev_gemv_sub(const double *V, long long ld, int k, const double *h, double *w, long long n, hipStream_t s);

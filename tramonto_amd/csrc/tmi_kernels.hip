@@ -1072,6 +1072,11 @@ __global__ void k_axpy(double a, const double *__restrict__ x, double *__restric
     if (i < n) y[i] += a * x[i];
 }
 
+__global__ void k_fill(double v, double *__restrict__ y, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = v;
+}
 __global__ void k_scale_copy(double a, const double *__restrict__ x, double *__restrict__ y, long long n)
 {
     long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -1816,6 +1821,10 @@ double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStr
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+void dev_fill(double v, double *y, long long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, v, y, n);
 }
 void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s)
 {
