@@ -63,7 +63,8 @@ typedef struct tmi_params {
     int type_func;
     int type_attr;      /* -1 none; 0 strict mean field with u(r<r_min)=u(r_min) */
     int nwall;          /* > 0: external field present */
-    int lhard_surf;     /* must be 0 */
+    int lhard_surf;     /* 1: hard surfaces (Lhard_surf), only without attraction (type_attr = -1);
+                           the boundary-node weights come from tmi_set_hard_surface_masks */
     double vext_max;
     double hs_diam[TMI_NCOMP_MAX];
     double sigma_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
@@ -162,6 +163,13 @@ typedef struct tmi_observables {
     double ads[TMI_NCOMP_MAX], ads_b[TMI_NCOMP_MAX];              /* adsorption, its bulk value */
 } tmi_observables;
 
+/* Solvation force on the walls: out[k] = sum over components c and nodes n of
+ * w_d[k][c][n] * rho_c(n), the reference's contact sums of sum_rho_wall
+ * (src/dft_out_force.c:223-280, collated per linked wall and divided by the area per unit area at
+ * :84-167) with the surface-element weights (-sign of the surface normal x Area_surf_el /
+ * Nnodes_per_el_S per boundary-node surface element, src/dft_mesh.c:1760-2230) given by the
+ * caller; w_d: device [nsum][ncomp][N], nsum 1..64 (links x directions); out: host [nsum]. */
+int tmi_compute_force(void *handle, const double *x_d, const double *w_d, int nsum, double *out, void *stream);
 /* The free-energy terms of calc_free_energy (src/dft_energy.c:30-391: ideal gas, chemical
  * potential, neutral external field, hard spheres, mean-field attraction; the functionals of this
  * library) and the adsorption of calc_adsorption (src/dft_out_ads.c:33-110), summed on the
@@ -200,6 +208,15 @@ int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, doubl
 int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);
 /* JTw = J(x)^T w, adjoint (transpose-Jacobian-vector) product */
 int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *w_d, double *JTw_d, void *stream);
+/* Hard surfaces (lhard_surf = 1): the wall-fluid boundary nodes of each component's element list
+ * and which of their elements are fluid, the inputs of HW_boundary_weight (src/dft_utils.c:41-62)
+ * that resid_and_Jac_sten_fill_sum_Ncomp applies to the rho-bar rows (src/dft_fill_CLSmf.c:140-160).
+ * mask: host [ncomp][N]; 0 for a node that is not a boundary node of the component's list
+ * (Nodes_2_boundary_wall == -1), else bit ln set when the element that has the node as its local
+ * node ln (x bit 0, y bit 1, z bit 2 of the element's corner) is a fluid element (Wall_elems == -1)
+ * inside the domain or by reflection.  NULL clears.  Every later fill, JVP, transpose product and
+ * Schur solve uses the weights; calc_init_rho_bar (tmi_init_fields) does not, as in the reference. */
+int tmi_set_hard_surface_masks(void *handle, const unsigned char *mask);
 /* rho-bar and mean-field unknowns from the densities, src/dft_guess_HSFMT.c:70, src/dft_guess_MFATT.c:66 */
 int tmi_init_fields(void *handle, double *x_d, void *stream);
 /* Newton-Krylov solve in place on the device (Schur GMRES with A22 preconditioning) */

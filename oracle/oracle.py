@@ -55,6 +55,7 @@ def lib() -> ct.CDLL:
         L.tmo_create.restype = vp
         L.tmo_create.argtypes = [ct.POINTER(TmoParams), vp, vp]
         L.tmo_destroy.argtypes = [vp]
+        L.tmo_set_hw_masks.argtypes = [vp, vp]
         L.tmo_last_error.restype = ct.c_char_p
         L.tmo_nunk.argtypes = [vp]
         L.tmo_nnodes.argtypes = [vp]
@@ -121,7 +122,11 @@ class Oracle:
 
     @classmethod
     def from_spec(cls, spec):
-        return cls(spec.as_params(), spec.vext, spec.zero_tf)
+        o = cls(spec.as_params(), spec.vext, spec.zero_tf)
+        if getattr(spec, "hw_mask", None) is not None:
+            o.hw_mask = np.ascontiguousarray(spec.hw_mask, dtype=np.uint8)
+            lib().tmo_set_hw_masks(o.h, _p(o.hw_mask))
+        return o
 
     def __del__(self):
         try:

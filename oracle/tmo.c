@@ -27,6 +27,7 @@ typedef struct {
     int L;
     int *off;     /* [L][3] */
     double *w;    /* [L] */
+    double *hw;   /* [L][8]: the part of w from elements where the point is local node ln (HW_Weight) */
 } sten_t;
 
 struct tmo_ctx {
@@ -34,6 +35,7 @@ struct tmo_ctx {
     int N, nrb, nrbs, nunk, iu_rho, iu_rb, iu_mf, nmf;
     double *vext;            /* [N][C] or NULL */
     unsigned char *zero;     /* [N][C] */
+    unsigned char *hwmask;   /* [C][N] or NULL: fluid elements of boundary nodes (0: not a boundary node) */
     double inv_4pi, inv_rad[NC], inv_4pir[NC], inv_4pirsq[NC];
     double rhobar_b[10], dphi_b[10], betamu_hs_ex[NC], betamu_att[NC], avdw[NC][NC], rmin[NC][NC];
     sten_t sten[3][NC * NC];
@@ -517,6 +519,7 @@ static void build_stencil(tmo_ctx *c, int isten, int icomp, int jcomp)
     s->L = 0;
     s->off = (int *)calloc((size_t)maxn * 3, sizeof(int));
     s->w = (double *)calloc((size_t)maxn, sizeof(double));
+    s->hw = (double *)calloc((size_t)maxn * 8, sizeof(double));
     int *index = (int *)malloc((size_t)maxn * sizeof(int));
     for (int q = 0; q < maxn; q++) index[q] = -1;
     int nnode_el = 1 << nd;
@@ -558,11 +561,13 @@ static void build_stencil(tmo_ctx *c, int isten, int icomp, int jcomp)
                     }
                     if (index[id] == -1) {
                         s->w[s->L] = elw[ln];
+                        s->hw[8 * s->L + ln] = elw[ln];
                         for (int d = 0; d < 3; d++) s->off[3 * s->L + d] = (d < nd) ? elo[ln][d] : 0;
                         index[id] = s->L;
                         s->L++;
                     } else {
                         s->w[index[id]] += elw[ln];
+                        s->hw[8 * index[id] + ln] += elw[ln];
                     }
                 }
             }
@@ -575,7 +580,31 @@ static void build_stencil(tmo_ctx *c, int isten, int icomp, int jcomp)
     if (sum != vol) {
         double ratio = vol / sum;
         for (int q = 0; q < s->L; q++) s->w[q] *= ratio;
+        for (int q = 0; q < 8 * s->L; q++) s->hw[q] *= ratio;   /* dft_stencil.c:700-706 */
     }
+}
+
+/* HW_boundary_weight (dft_utils.c:41-62): the weight of stencil point k at a wall-fluid boundary
+ * node is the sum of its parts from the local nodes whose element is fluid */
+static double hw_weight(const tmo_ctx *c, const sten_t *s, int k, int jcomp, int jnode, double w)
+{
+    if (!c->hwmask || jnode < 0) return w;
+    const unsigned m = c->hwmask[(size_t)jcomp * c->N + jnode];
+    if (!m) return w;
+    double v = 0.0;
+    for (int ln = 0; ln < (1 << c->p.ndim); ln++)
+        if (m & (1u << ln)) v += s->hw[8 * k + ln];
+    return v;
+}
+
+void tmo_set_hw_masks(tmo_ctx *c, const unsigned char *mask)
+{
+    if (!c->p.lhard_surf) die("hard-surface masks need lhard_surf");
+    free(c->hwmask);
+    c->hwmask = NULL;
+    if (!mask) return;
+    c->hwmask = (unsigned char *)malloc((size_t)c->p.ncomp * c->N);
+    memcpy(c->hwmask, mask, (size_t)c->p.ncomp * c->N);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -861,7 +890,8 @@ tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char
 
 static tmo_ctx *create_impl(const tmo_params *p, const double *vext, const unsigned char *zero_tf)
 {
-    if (p->lhard_surf) die("lhard_surf not supported");
+    /* hard surfaces weight the mean-field stencils at the walls (dft_fill_CLSmf.c:140-160): not restated */
+    if (p->lhard_surf && p->type_attr >= 0) die("lhard_surf with attraction not supported");
     if (p->type_func != TMO_FMT1 && p->type_func != TMO_FMT3) die("type_func");
     if (p->ncomp < 1 || p->ncomp > NC || p->ndim < 1 || p->ndim > 3) die("dims");
     tmo_ctx *c = (tmo_ctx *)calloc(1, sizeof(tmo_ctx));
@@ -915,7 +945,9 @@ void tmo_destroy(tmo_ctx *c)
         for (int q = 0; q < NC * NC; q++) {
             free(c->sten[t][q].off);
             free(c->sten[t][q].w);
+            free(c->sten[t][q].hw);
         }
+    free(c->hwmask);
     free(c->vext);
     free(c->zero);
     free(c);
@@ -1166,6 +1198,7 @@ static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int
                         fac *= 1.0 * vec;
                     }
                     int jn = o2n(c, ijk, off, refl);
+                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, w);   /* dft_fill_CLSmf.c:140-160 */
                     r += fac * (w * rho_val(c, x, jc, jn) - 0.0);
                     if (J && jn >= 0 && !c->zero[(size_t)jn * ncomp + jc])
                         coo_add(J, row, (c->iu_rho + jc) * N + jn, w * fac * 1.0);

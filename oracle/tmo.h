@@ -42,7 +42,7 @@ typedef struct {
     int type_func;      /* TMO_FMT1 / TMO_FMT3 */
     int type_attr;      /* -1: none; 0: strict MF, u = u(r_min) for r < r_min */
     int nwall;          /* > 0: external field added to the Euler-Lagrange rows */
-    int lhard_surf;     /* must be 0 (hard-surface boundary weights not restated) */
+    int lhard_surf;     /* 1 only without attraction (hard-surface mean-field weights not restated) */
     double vext_max;
     double hs_diam[TMO_NC];
     double sigma_ff[TMO_NC][TMO_NC];
@@ -72,6 +72,11 @@ typedef struct {
  * Zero_density_TF[inode_box][icomp]).  Both may be NULL (no walls). */
 tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char *zero_tf);
 void tmo_destroy(tmo_ctx *c);
+/* hard surfaces (lhard_surf): per component and node, 0 for nodes that are not wall-fluid boundary
+ * nodes of the component's element list, else bit ln set when the element that has the node as
+ * its local node ln is a fluid element (Nodes_2_boundary_wall / Wall_elems, dft_mesh.c:1452-1470);
+ * [ncomp][N], NULL to clear */
+void tmo_set_hw_masks(tmo_ctx *c, const unsigned char *mask);
 /* message of the last failed tmo_create (NULL return) */
 const char *tmo_last_error(void);
 

@@ -90,8 +90,11 @@ def test_rejects_unsupported_problems():
     sp = small_case("1d_hw")
     d = sp.as_params()
     d["lhard_surf"] = 1
-    p = _lib.fill_params(_lib.TmiParams(), d)
     n = ct.c_int()
+    p = _lib.fill_params(_lib.TmiParams(), d)   # hard surfaces without attraction are served
+    assert _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None) == 0
+    d["type_attr"] = 0   # with attraction the mean-field sums would need the boundary weights too
+    p = _lib.fill_params(_lib.TmiParams(), d)
     rc = _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None)
     assert rc == -2
     assert b"lhard_surf" in _lib.lib().tmi_last_error()

@@ -1,0 +1,100 @@
+"""The oracle on the reference's hard-wall Examples (Lhard_surf = 1, no attraction).
+
+tests/golden/examples_hs_1d.json (written by make_fixtures.py) holds the inputs
+and README key outputs of HS1-HS5_FMT1_1D and MIX1_FMT1_1D: hard spheres
+(Rosenfeld FMT) between hard planar walls, where the rho-bar stencils take the
+hard-surface weights at the contact nodes (HW_boundary_weight,
+src/dft_utils.c:41-62, in src/dft_fill_CLSmf.c:140-160).  The oracle's Newton
+solve must reproduce the README adsorption, energy and wall force to the digits
+printed there, and niter within the reference harness' +-4
+(Examples/check_results/main.cpp:120-139).  Without the boundary-node weights
+the same solves miss the README values by far more than the harness' 1e-3.
+"""
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tramonto_amd.examples import force_weights, integration_weights, spec_from_tramonto
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "examples_hs_1d.json")) as f:
+    EXAMPLES = json.load(f)
+
+
+def printed_tol(text):
+    """half a unit in the last digit the README prints"""
+    dec = len(text.split(".")[1]) if "." in text else 0
+    return 0.5 * 10.0 ** (-dec) + 1e-9
+
+
+def solve(spec, g):
+    o = Oracle.from_spec(spec)
+    x, st = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10, update_scaling=g["nl_update_scaling"])
+    assert st["converged"], st
+    return o, x, st
+
+
+def outputs(spec, g, o, x):
+    h1, h2, scale = integration_weights(spec, g)
+    obs = o.observables(x, h1, h2, scale)
+    W = force_weights(spec, g)
+    force = [float((W[k] * x[: spec.ncomp]).sum()) for k in range(W.shape[0])]
+    return obs, force
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_hard_wall_example_readme_outputs(name):
+    g = EXAMPLES[name]
+    ref = g["key_outputs"]
+    spec = spec_from_tramonto(g, name)
+    assert spec.lhard_surf == 1 and spec.hw_mask is not None
+    o, x, st = solve(spec, g)
+    assert abs(st["iters"] - ref["niter"]) <= 4
+    obs, force = outputs(spec, g, o, x)
+    assert abs(obs["omega_s"] - ref["energy"]) < printed_tol(ref["energy_text"]), (obs["omega_s"], ref)
+    for c in range(spec.ncomp):
+        assert abs(obs["ads"][c] - ref["adsorption"][c]) < 2e-6, (c, obs["ads"], ref)
+    # the README quotes one of the walls' values (ptilde: contact sums per unit area)
+    tol = printed_tol(ref["force_text"])
+    assert min(abs(f - ref["force"]) for f in force) < tol, (force, ref)
+
+
+@pytest.mark.parametrize("name", ["HS1_FMT1_1D", "HS5_FMT1_1D"])
+def test_boundary_node_weights_are_needed(name):
+    """the same solve with full stencil weights at the contact nodes misses the README values"""
+    g = EXAMPLES[name]
+    spec = spec_from_tramonto(g, name)
+    o, x, _ = solve(dataclasses.replace(spec, hw_mask=None), g)
+    obs, force = outputs(spec, g, o, x)
+    assert abs(obs["omega_s"] - g["key_outputs"]["energy"]) > 0.1
+    assert min(abs(f - g["key_outputs"]["force"]) for f in force) > 0.1
+
+
+def test_contact_theorem_single_wall():
+    """One hard wall in contact with bulk: the contact density (the wall force per unit area)
+    approaches the bulk pressure of the functional, the Percus-Yevick compressibility pressure for
+    Rosenfeld FMT, as the mesh is refined (contact theorem); HS5 (dx = 0.1) is within 5 %."""
+    g = EXAMPLES["HS5_FMT1_1D"]
+    spec = spec_from_tramonto(g, "HS5")
+    o, x, _ = solve(spec, g)
+    _, force = outputs(spec, g, o, x)
+    rho = g["rho_b"][0]
+    eta = np.pi / 6.0 * rho
+    p_py = rho * (1 + eta + eta ** 2) / (1 - eta) ** 3
+    assert abs(abs(force[0]) - p_py) < 0.05 * p_py, (force, p_py)
+
+
+def test_reflective_example_equals_half_slit():
+    """HS3 (one wall, reflecting edge at the slit's middle) and HS4 (its mirror image) are half
+    of the symmetric slit: same profile mirrored, same integrals, opposite forces."""
+    s3, s4 = (spec_from_tramonto(EXAMPLES[n], n) for n in ("HS3_FMT1_1D", "HS4_FMT1_1D"))
+    o3, x3, _ = solve(s3, EXAMPLES["HS3_FMT1_1D"])
+    o4, x4, _ = solve(s4, EXAMPLES["HS4_FMT1_1D"])
+    assert np.abs(x3[0] - x4[0][::-1]).max() < 1e-9
+    a3, f3 = outputs(s3, EXAMPLES["HS3_FMT1_1D"], o3, x3)
+    a4, f4 = outputs(s4, EXAMPLES["HS4_FMT1_1D"], o4, x4)
+    assert abs(a3["omega_s"] - a4["omega_s"]) < 1e-9 and abs(f3[0] + f4[0]) < 1e-9

@@ -78,7 +78,7 @@ _lib: Optional[ct.CDLL] = None
 DECLARED_SYMBOLS = [
     "tmi_last_error", "tmi_version", "tmi_problem_create", "tmi_problem_destroy", "tmi_problem_info",
     "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
-    "tmi_init_fields", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
+    "tmi_init_fields", "tmi_set_hard_surface_masks", "tmi_compute_force", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning",
     "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_compute_observables",
     "fill_resid_and_matrix_control",
@@ -119,6 +119,8 @@ def lib() -> ct.CDLL:
     L.tmi_apply_jacobian_fp32.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_apply_jacobian_transpose.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_init_fields.argtypes = [vp, vp, vp]
+    L.tmi_set_hard_surface_masks.argtypes = [vp, vp]
+    L.tmi_compute_force.argtypes = [vp, vp, vp, ct.c_int, vp, vp]
     L.tmi_newton_solve.argtypes = [vp, vp, ct.POINTER(TmiNewtonOpts), ct.POINTER(TmiNewtonStats), vp]
     L.tmi_schur_solve.argtypes = [vp, vp, vp, vp, d, i, i, i, ct.POINTER(i), vp]
     L.tmi_set_active_problem.argtypes = [vp, vp]

@@ -354,6 +354,7 @@ void launch_vjp(Problem &P, const double *x, const double *w, double *y, hipStre
     case 2: vjp_nd<T, 2>(P, x, w, y, s); break;
     default: vjp_nd<T, 3>(P, x, w, y, s); break;
     }
+    hw_apply(P.hw_tr, w + (size_t)P.iu_rb * P.N, y + (size_t)P.iu_rho * P.N, s);   // hard-surface weights
 }
 template void launch_vjp<double>(Problem &, const double *, const double *, double *, hipStream_t);
 template void launch_vjp<float>(Problem &, const double *, const double *, double *, hipStream_t);

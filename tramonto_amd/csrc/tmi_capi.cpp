@@ -64,7 +64,9 @@ void validate(const tmi_params &p)
     if (p.type_func != TMI_FMT1 && p.type_func != TMI_FMT3)
         throw std::domain_error("type_func: only Rosenfeld (0) and White Bear (2) are implemented");
     if (p.type_attr != -1 && p.type_attr != 0) throw std::domain_error("type_attr: only -1 and 0 are implemented");
-    if (p.lhard_surf) throw std::domain_error("lhard_surf: hard-surface boundary weights are not implemented");
+    if (p.lhard_surf && p.type_attr >= 0)
+        throw std::domain_error("lhard_surf with attraction: the hard-surface weights of the mean-field sums are not "
+                                "implemented");
     for (int d = 0; d < p.ndim; ++d) {
         if (p.nx[d] < 1) throw std::invalid_argument("nx must be positive");
         if (!(p.dx[d] > 0.0)) throw std::invalid_argument("dx must be positive");
@@ -497,6 +499,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
 
 void destroy_problem(Problem *P)
 {
+    hw_free(*P);
     for (int t = 0; t < 3; ++t)
         for (int q = 0; q < kNC * kNC; ++q) {
             free_stencil(P->dsd[t][q]);
@@ -1054,6 +1057,19 @@ int tmi_problem_set_preconditioner(void *handle, int kind)
     });
 }
 
+int tmi_compute_force(void *handle, const double *x_d, const double *w_d, int nsum, double *out, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !w_d || !out) throw std::invalid_argument("null pointer");
+        if (nsum < 1 || nsum > 64) throw std::invalid_argument("nsum must be 1..64");
+        if (P->npores > 0) throw std::invalid_argument("surface sums of a batch of pores are not supported");
+        const long long n = (long long)P->p.ncomp * P->N;
+        dev_multi_dot(*P, w_d, n, nsum, x_d + (size_t)P->iu_rho * P->N, n, out, (hipStream_t)stream);
+        return 0;
+    });
+}
+
 int tmi_compute_observables(void *handle, const double *x_d, const int *nel_hit, const int *nel_hit2, double scale,
                             tmi_observables *out, void *stream)
 {
@@ -1104,6 +1120,17 @@ int tmi_compute_observables(void *handle, const double *x_d, const int *nel_hit,
         o.omega = o.ideal + o.mu + o.vext + o.hs + o.att;
         o.omega_s = (o.ideal - o.ideal_b) + (o.mu - o.mu_b) + o.vext + (o.hs - o.hs_b) + (o.att - o.att_b);
         *out = o;
+        return 0;
+    });
+}
+
+int tmi_set_hard_surface_masks(void *handle, const unsigned char *mask)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!P->p.lhard_surf) throw std::invalid_argument("hard-surface masks need lhard_surf = 1");
+        TMI_HIP_CHECK(hipDeviceSynchronize());
+        hw_build(*P, mask);
         return 0;
     });
 }

@@ -20,6 +20,15 @@ struct HostStencil {
     int L = 0;
     std::vector<int> off;     // [L][3]
     std::vector<double> w;    // [L]
+    std::vector<double> hw;   // [L][8]: the part of w from elements where the point is local node ln
+};
+
+// Sparse corrections of rows by sums over a few columns: out[rowid[r]] += sum val[k] in[col[k]],
+// k in [rowptr[r], rowptr[r + 1]); device arrays, one thread per row (rows distinct).
+struct HwCsr {
+    int nrows = 0;
+    int *rowptr = nullptr, *rowid = nullptr, *col = nullptr;
+    double *val = nullptr;
 };
 
 struct BulkConstants {
@@ -167,6 +176,9 @@ struct Problem {
     // expand x to the internal layout, whose mean-field rows (the last nmf rows) are recomputed
     // from the densities, and return the first nunk_ext rows
     bool ext_mf = false;
+    // hard surfaces: the rho-bar rows' weights at wall-fluid boundary nodes (hw_fwd: rows q*N + i
+    // of the rho-bar block, columns c*N + j of the densities; hw_tr its transpose)
+    HwCsr hw_fwd, hw_tr;
     int nunk_ext = 0;
     double *d_xi = nullptr, *d_vi = nullptr, *d_fi = nullptr, *d_gi = nullptr;   // [nunk][N] each
 };
@@ -265,6 +277,11 @@ void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long 
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);
+// out += the rows of a sparse correction applied to in (no-op when empty)
+void hw_apply(const HwCsr &h, const double *in, double *out, hipStream_t s);
+// build / free the hard-surface corrections from per-component boundary-node masks [C][N]
+void hw_build(Problem &P, const unsigned char *mask);
+void hw_free(Problem &P);
 // y[0..n) = v with a kernel (no memset node, so a captured graph replays it as launched)
 void dev_fill(double v, double *y, long long n, hipStream_t s);
 void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s);

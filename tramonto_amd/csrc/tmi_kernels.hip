@@ -1072,6 +1072,16 @@ __global__ void k_axpy(double a, const double *__restrict__ x, double *__restric
     if (i < n) y[i] += a * x[i];
 }
 
+__global__ void k_hw_gather(int nrows, const int *__restrict__ rowptr, const int *__restrict__ rowid,
+                            const int *__restrict__ col, const double *__restrict__ val, const double *__restrict__ in,
+                            double *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double sum = 0.0;
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) sum += val[k] * in[col[k]];
+    out[rowid[r]] += sum;
+}
 __global__ void k_fill(double v, double *__restrict__ y, long long n)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -1559,10 +1569,12 @@ void residual_impl(Problem &P, const double *x, double *F, hipStream_t s, hipEve
 {
     const size_t N = P.N;
     long long n_rb = 0, n_el = 0;
+    if (P.hw_fwd.nrows) ss = nullptr;   // the rho-bar rows change after the fused passes
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
     rb_mf_rows<double>(P, x + P.iu_rho * N, false, x + P.iu_rb * N, nullptr, F + P.iu_rb * N,
                        P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s, ev ? ev[1] : nullptr,
                        ev ? ev[2] : nullptr, ss, &n_rb);
+    hw_apply(P.hw_fwd, x + P.iu_rho * N, F + P.iu_rb * N, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
@@ -1708,6 +1720,7 @@ void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStr
     const size_t N = P.N;
     rb_mf_rows<T>(P, v + P.iu_rho * N, true, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N,
                   P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_mf * N, s);
+    hw_apply(P.hw_fwd, v + P.iu_rho * N, Jv + P.iu_rb * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, v + P.iu_rb * N, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, v + P.iu_rho * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_rho * N,
                s);
@@ -1731,6 +1744,7 @@ void launch_schur_apply(Problem &P, const double *x, const double *x2, double *t
 {
     const size_t N = P.N;
     rb_mf_rows<T>(P, x2, true, nullptr, nullptr, t1, nullptr, nullptr, t1 + (size_t)P.nrb * N, s);
+    hw_apply(P.hw_fwd, x2, t1, s);
     fmt_point<T>(P, x + P.iu_rb * N, t1, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, x2, P.nmf ? t1 + (size_t)P.nrb * N : nullptr, nullptr, y2, s);
     TMI_HIP_CHECK(hipGetLastError());
@@ -1754,6 +1768,7 @@ void launch_schur_backsub(Problem &P, const double *x, const double *b, double *
     const size_t N = P.N;
     rb_mf_rows<double>(P, dx + P.iu_rho * N, true, nullptr, b + P.iu_rb * N, dx + P.iu_rb * N, nullptr,
                        P.nmf ? b + P.iu_mf * N : nullptr, P.nmf ? dx + P.iu_mf * N : nullptr, s);
+    hw_apply(P.hw_fwd, dx + P.iu_rho * N, dx + P.iu_rb * N, s);
     TMI_HIP_CHECK(hipGetLastError());
 }
 
@@ -1821,6 +1836,12 @@ double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStr
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+void hw_apply(const HwCsr &h, const double *in, double *out, hipStream_t s)
+{
+    if (h.nrows == 0) return;
+    hipLaunchKernelGGL(k_hw_gather, dim3((h.nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, h.nrows, h.rowptr,
+                       h.rowid, h.col, h.val, in, out);
 }
 void dev_fill(double v, double *y, long long n, hipStream_t s)
 {

@@ -229,9 +229,12 @@ class StencilBuilder {
                         if (slot[key] < 0) {
                             slot[key] = s.L++;
                             s.w.push_back(nodal[c]);
+                            s.hw.resize(8 * (size_t)s.L, 0.0);
+                            s.hw[8 * (size_t)slot[key] + c] = nodal[c];
                             for (int d = 0; d < 3; ++d) s.off.push_back(d < nd_ ? o[d] : 0);
                         } else {
                             s.w[slot[key]] += nodal[c];
+                            s.hw[8 * (size_t)slot[key] + c] += nodal[c];
                         }
                     }
                 }
@@ -242,6 +245,7 @@ class StencilBuilder {
         if (total != volume_) {
             double ratio = volume_ / total;
             for (double &v : s.w) v *= ratio;
+            for (double &v : s.hw) v *= ratio;   // src/dft_stencil.c:700-706
         }
         return s;
     }

@@ -51,6 +51,10 @@ class DeviceProblem:
                                         None if zero is None else zero.ctypes.data_as(ct.c_void_p),
                                         ct.byref(h)), "tmi_problem_create")
         self.handle = h
+        if getattr(spec, "hw_mask", None) is not None:
+            self._hw_mask = np.ascontiguousarray(spec.hw_mask, dtype=np.uint8).reshape(C, N)
+            _lib.check(L.tmi_set_hard_surface_masks(self.handle, self._hw_mask.ctypes.data_as(ct.c_void_p)),
+                       "tmi_set_hard_surface_masks")
         info = _lib.TmiInfo()
         _lib.check(L.tmi_problem_info(h, ct.byref(info)), "tmi_problem_info")
         self.info = info
@@ -134,6 +138,18 @@ class DeviceProblem:
         return r
 
     # ------------------------------------------------------------ device ops
+    def force(self, x: torch.Tensor, weights: np.ndarray) -> np.ndarray:
+        """Wall forces of the profile x, summed on the device (tmi_compute_force): weights
+        [nsum][ncomp][nnodes], e.g. examples.force_weights."""
+        W = np.ascontiguousarray(weights, dtype=np.float64)
+        if W.ndim != 3 or W.shape[1:] != (self.spec.ncomp, self.nnodes):
+            raise ValueError("weights must be [nsum][ncomp][nnodes]")
+        Wd = torch.from_numpy(W).to(x.device)
+        out = np.zeros(W.shape[0])
+        _lib.check(_lib.lib().tmi_compute_force(self.handle, _ptr(x), _ptr(Wd), W.shape[0],
+                                                out.ctypes.data_as(ct.c_void_p), _stream()), "tmi_compute_force")
+        return out
+
     def zeros(self) -> torch.Tensor:
         return torch.zeros((self.nunk, self.nnodes), dtype=torch.float64, device="cuda")
 

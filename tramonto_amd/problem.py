@@ -75,6 +75,8 @@ class ProblemSpec:
     zero_tf: Optional[np.ndarray] = None    # [N, C] uint8
     rho0: Optional[np.ndarray] = None       # [C, N] initial densities
     att_in_a22: int = 0                     # 1: no mean-field unknowns (the reference's ATTInA22Block)
+    lhard_surf: int = 0                     # 1: hard surfaces (Lhard_surf); needs type_attr = -1
+    hw_mask: Optional[np.ndarray] = None    # [ncomp][N] uint8 fluid-element bits of boundary nodes
 
     @property
     def nnodes(self) -> int:
@@ -105,7 +107,7 @@ class ProblemSpec:
             nx=list(self.nx) + [1] * (3 - len(self.nx)),
             dx=list(self.dx) + [1.0] * (3 - len(self.dx)),
             bc=[list(b) for b in self.bc] + [[IN_BULK, IN_BULK]] * (3 - len(self.bc)),
-            type_func=self.type_func, type_attr=self.type_attr, nwall=self.nwall, lhard_surf=0,
+            type_func=self.type_func, type_attr=self.type_attr, nwall=self.nwall, lhard_surf=int(self.lhard_surf),
             vext_max=self.vext_max, hs_diam=pad(self.hs_diam), sigma_ff=mat(sig), eps_ff=mat(eps),
             cut_ff=mat(cut), rho_b=pad(self.rho_b), att_in_a22=int(self.att_in_a22),
         )
