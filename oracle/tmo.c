@@ -1082,6 +1082,7 @@ static void coo_add(coo_t *J, int row, int col, double v)
  * following fill_resid_and_matrix / load_standard_node (dft_fill_main.c:38-145,
  * :451-560) with dphi precomputed at every node (FMT1stDeriv_switch,
  * dft_switch_FMT.c:68-96). */
+static int one_size(const tmo_ctx *c);
 static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int inode, double *F, coo_t *J)
 {
     const int N = c->N, nd = c->p.ndim, ncomp = c->p.ncomp, nrb = c->nrb;
@@ -1174,6 +1175,15 @@ static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int
             int st = (irb == 0) ? TMO_THETA : TMO_DELTA;
             r += -X(iunk, inode);
             if (J) coo_add(J, row, row, -1.0);
+            if (one_size(c) && (irb == 2 || irb == 3 || irb >= 4 + nd)) {
+                /* dft_fill_FMT.c:215-236 (n1, n0), :305-320 (nv1): multiples of n2 and nv2 */
+                const int ind = (irb < 4) ? 1 : irb - nd;
+                const double f = (irb == 3) ? c->inv_4pirsq[0] : c->inv_4pir[0];
+                r += X(c->iu_rb + ind, inode) * f;
+                if (J) coo_add(J, row, (c->iu_rb + ind) * N + inode, f);
+                if (F) F[row] = r;
+                continue;
+            }
             /* resid_and_Jac_sten_fill_sum_Ncomp, dft_fill_CLSmf.c:38-171 */
             for (int jc = 0; jc < ncomp; jc++) {
                 const sten_t *s = &c->sten[st][jc];
@@ -1405,7 +1415,26 @@ typedef struct {
     const csr_t *A;
     int n1, n2; /* block sizes: block 2 = densities (first ncomp*N GIDs) */
     double *t, *u;
+    const tmo_ctx *c;
 } schur_t;
+
+/* One particle size with hard surfaces (Lhard_surf with Nlists_HW == 2, one component and walls):
+ * the reference fills n1, n0 and nv1 as multiples of n2 and nv2 (dft_fill_FMT.c:215-236, 305-320)
+ * instead of stencil sums, so A11 = [-I 0; B -I] with those multiples in B, and its A11 operator
+ * applies the exact inverse [-I 0; -B -I] (dft_HardSphereA11_Epetra_Operator.cpp:127-172). */
+static int one_size(const tmo_ctx *c) { return c->p.lhard_surf && c->p.nwall > 0 && c->p.ncomp == 1; }
+
+/* v_dep += B v_ind over the rho-bar block v[nrb][N] */
+static void one_size_lift(const tmo_ctx *c, double *v)
+{
+    if (!one_size(c)) return;
+    const int N = c->N, nd = c->p.ndim;
+    for (int i = 0; i < N; i++) {
+        v[2 * N + i] += c->inv_4pir[0] * v[N + i];
+        v[3 * N + i] += c->inv_4pirsq[0] * v[N + i];
+        for (int d = 0; d < nd; d++) v[(size_t)(4 + nd + d) * N + i] += c->inv_4pir[0] * v[(size_t)(4 + d) * N + i];
+    }
+}
 
 /* S x2 = A22 x2 + A21 (A12 x2) */
 static void schur_apply(schur_t *S, const double *x2, double *y2)
@@ -1416,7 +1445,8 @@ static void schur_apply(schur_t *S, const double *x2, double *y2)
     spmv(S->A, S->t, S->u);                /* u = J [x2; 0] */
     for (int i = 0; i < n2; i++) y2[i] = S->u[i];   /* A22 x2 */
     memset(S->t, 0, sizeof(double) * n2);
-    memcpy(S->t + n2, S->u + n2, sizeof(double) * (n - n2));  /* t = [0; A12 x2] */
+    memcpy(S->t + n2, S->u + n2, sizeof(double) * (n - n2));  /* t = [0; -A11^{-1} A12 x2] */
+    one_size_lift(S->c, S->t + n2);
     spmv(S->A, S->t, S->u);
     for (int i = 0; i < n2; i++) y2[i] += S->u[i];
 }
@@ -1511,6 +1541,7 @@ int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton
     double *b = (double *)malloc(sizeof(double) * n), *dinv = (double *)malloc(sizeof(double) * n2);
     double *bs = (double *)malloc(sizeof(double) * n2), *tmp = (double *)malloc(sizeof(double) * n);
     schur_t S;
+    S.c = c;
     S.n2 = n2;
     S.n1 = n - n2;
     S.t = (double *)malloc(sizeof(double) * n);
@@ -1535,9 +1566,10 @@ int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton
                 if (A.ci[q] == i) d = A.v[q];
             dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
         }
-        /* b_S = b2 + A21 b1 */
+        /* b_S = b2 - A21 A11^{-1} b1 */
         memset(tmp, 0, sizeof(double) * n2);
         memcpy(tmp + n2, b + n2, sizeof(double) * (n - n2));
+        one_size_lift(c, tmp + n2);
         spmv(&A, tmp, S.u);
         for (int i = 0; i < n2; i++) bs[i] = b[i] + S.u[i];
         st->gmres_iters_total += gmres(&S, dinv, bs, dx, o->gmres_restart, o->gmres_max_iter, o->gmres_tol);
@@ -1545,7 +1577,8 @@ int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton
         memset(tmp, 0, sizeof(double) * n);
         memcpy(tmp, dx, sizeof(double) * n2);
         spmv(&A, tmp, S.u);
-        for (int i = n2; i < n; i++) dx[i] = S.u[i] - b[i];
+        for (int i = n2; i < n; i++) dx[i] = S.u[i] - b[i];   /* x1 = A11^{-1} (b1 - A12 x2) */
+        one_size_lift(c, dx + n2);
         csr_free(&A);
         /* update_solution, dft_newton.c:266-380 */
         double frac_min = 1.0;

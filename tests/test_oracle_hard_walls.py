@@ -98,3 +98,23 @@ def test_reflective_example_equals_half_slit():
     a3, f3 = outputs(s3, EXAMPLES["HS3_FMT1_1D"], o3, x3)
     a4, f4 = outputs(s4, EXAMPLES["HS4_FMT1_1D"], o4, x4)
     assert abs(a3["omega_s"] - a4["omega_s"]) < 1e-9 and abs(f3[0] + f4[0]) < 1e-9
+
+
+def test_one_size_rows_give_the_newton_steps_of_two_identical_components():
+    """With one component the reference fills n1, n0 and nv1 as multiples of n2 and nv2 and
+    inverts A11 = [-I 0; B -I] exactly (src/dft_fill_FMT.c:215-236, 305-320;
+    src/dft_HardSphereA11_Epetra_Operator.cpp:127-172); with two components (MIX1) it fills their
+    stencil sums.  HS5 and MIX1 are the same fluid (two identical species at half the density), so
+    every Newton iterate of HS5 equals the sum of MIX1's two densities: a wrong factor in B, in the
+    Schur right-hand side or in the back-substitution changes the steps."""
+    g5, g1 = EXAMPLES["HS5_FMT1_1D"], EXAMPLES["MIX1_FMT1_1D"]
+    s5, s1 = spec_from_tramonto(g5, "HS5"), spec_from_tramonto(g1, "MIX1")
+    o5, o1 = Oracle.from_spec(s5), Oracle.from_spec(s1)
+    x5, x1 = o5.init_fields(s5.initial_x()), o1.init_fields(s1.initial_x())
+    r5 = o5.residual(x5).reshape(o5.nunk, -1)
+    # n1, n0 (unknowns 3, 4): the algebraic rows hold for the rho-bars built by stencil sums
+    assert np.abs(r5[o5.iu_rb + 2: o5.iu_rb + 4]).max() < 1e-14
+    for k in (1, 2, 3):
+        y5, _ = o5.newton(x5.copy(), max_iter=k, resid_tol=0.0, update_scaling=g5["nl_update_scaling"])
+        y1, _ = o1.newton(x1.copy(), max_iter=k, resid_tol=0.0, update_scaling=g1["nl_update_scaling"])
+        assert np.abs(y5[0] - (y1[0] + y1[1])).max() < 1e-10 * np.abs(y5[0]).max(), k

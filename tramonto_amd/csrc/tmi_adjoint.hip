@@ -347,14 +347,24 @@ void vjp_nd(Problem &P, const double *x, const double *w, double *y, hipStream_t
 }  // namespace
 
 template <typename T>
-void launch_vjp(Problem &P, const double *x, const double *w, double *y, hipStream_t s)
+void launch_vjp(Problem &P, const double *x, const double *w_in, double *y, hipStream_t s)
 {
+    const double *w = w_in;
+    if (P.one_size) {   // the stencil transpose over w without the n1, n0, nv1 rows, which are algebraic
+        const size_t N = P.N, nd = P.p.ndim;
+        TMI_HIP_CHECK(hipMemcpyAsync(P.d_w0, w_in, sizeof(double) * P.nunk * N, hipMemcpyDeviceToDevice, s));
+        double *rb = P.d_w0 + (size_t)P.iu_rb * N;
+        dev_fill(0.0, rb + 2 * N, (long long)(2 * N), s);
+        dev_fill(0.0, rb + (4 + nd) * N, (long long)(nd * N), s);
+        w = P.d_w0;
+    }
     switch (P.p.ndim) {
     case 1: vjp_nd<T, 1>(P, x, w, y, s); break;
     case 2: vjp_nd<T, 2>(P, x, w, y, s); break;
     default: vjp_nd<T, 3>(P, x, w, y, s); break;
     }
     hw_apply(P.hw_tr, w + (size_t)P.iu_rb * P.N, y + (size_t)P.iu_rho * P.N, s);   // hard-surface weights
+    one_size_transpose(P, w_in + (size_t)P.iu_rb * P.N, y + (size_t)P.iu_rb * P.N, s);
 }
 template void launch_vjp<double>(Problem &, const double *, const double *, double *, hipStream_t);
 template void launch_vjp<float>(Problem &, const double *, const double *, double *, hipStream_t);

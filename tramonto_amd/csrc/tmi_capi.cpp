@@ -327,6 +327,10 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
     P->iu_mf = C + P->nrb;
     P->nunk = C + P->nrb + P->nmf;
     P->ext_mf = p.att_in_a22 != 0 && P->nmf > 0;
+    if (p.lhard_surf && P->npores > 0) throw std::domain_error("lhard_surf in a batch of pores is not supported");
+    // Nlists_HW == 2 with hard surfaces (src/dft_mesh.c:330-345): one component and walls
+    P->one_size = p.lhard_surf && p.nwall > 0 && p.ncomp == 1;
+    if (P->one_size) TMI_HIP_CHECK(hipMalloc((void **)&P->d_w0, sizeof(double) * (size_t)P->nunk * P->N));
     P->nunk_ext = P->ext_mf ? C + P->nrb : P->nunk;
     if (P->ext_mf && batch) throw std::domain_error("att_in_a22 is not supported for batches of pores");
     P->nadj = C * (1 + nd) + 1;
@@ -500,6 +504,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
 void destroy_problem(Problem *P)
 {
     hw_free(*P);
+    (void)hipFree(P->d_w0);
     for (int t = 0; t < 3; ++t)
         for (int q = 0; q < kNC * kNC; ++q) {
             free_stencil(P->dsd[t][q]);
@@ -701,6 +706,10 @@ int schur_solve(Problem &P, const double *x, const double *F, double *dx, double
     double *work = scratch(P, 2 * ntot + n2 + gmres_work_doubles(P, restart));
     double *b = work, *bS = b + ntot, *gw = bS + n2;
     dev_scale_copy(-1.0, F, b, (long long)ntot, s);
+    // with algebraic n1, n0, nv1 rows the reference's A11 is [-I 0; B -I]: -A11^{-1} b1 is the lifted
+    // b1 under A11 = -I, and the back-substitution A11^{-1} (b1 - A12 x2) is the same lift of
+    // A12 x2 - b1 with the stencil rows of A12 (n1 = n2 / (4 pi R) holds for the stencil sums too)
+    one_size_lift(P, b + (size_t)P.iu_rb * N, s);
     launch_schur_rhs(P, x, b, bS, s);
     int its = gmres_schur(P, x, bS, dx, rtol, restart, maxit, prec, gw, s);
     launch_schur_backsub(P, x, b, dx, s);

@@ -179,6 +179,10 @@ struct Problem {
     // hard surfaces: the rho-bar rows' weights at wall-fluid boundary nodes (hw_fwd: rows q*N + i
     // of the rho-bar block, columns c*N + j of the densities; hw_tr its transpose)
     HwCsr hw_fwd, hw_tr;
+    // one particle size with hard surfaces (one component, walls): n1, n0, nv1 rows are multiples
+    // of n2, nv2 as in the reference; d_w0 is the transpose product's copy of w
+    bool one_size = false;
+    double *d_w0 = nullptr;
     int nunk_ext = 0;
     double *d_xi = nullptr, *d_vi = nullptr, *d_fi = nullptr, *d_gi = nullptr;   // [nunk][N] each
 };
@@ -277,6 +281,12 @@ void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long 
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);
+// one particle size: out's n1, n0, nv1 rows = -in_dep + B in_ind (rows of F or of J v)
+void one_size_rows(const Problem &P, const double *in_rb, double *out_rb, hipStream_t s);
+// v_dep += B v_ind: -A11^{-1} of the reference's A11 = [-I 0; B -I] relative to A11 = -I
+void one_size_lift(const Problem &P, double *v_rb, hipStream_t s);
+// y += (rows n1, n0, nv1 of J)^T w_dep: -w_dep on the dependent, B^T w_dep on n2, nv2
+void one_size_transpose(const Problem &P, const double *w_rb, double *y_rb, hipStream_t s);
 // out += the rows of a sparse correction applied to in (no-op when empty)
 void hw_apply(const HwCsr &h, const double *in, double *out, hipStream_t s);
 // build / free the hard-surface corrections from per-component boundary-node masks [C][N]

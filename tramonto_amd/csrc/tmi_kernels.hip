@@ -1082,6 +1082,35 @@ __global__ void k_hw_gather(int nrows, const int *__restrict__ rowptr, const int
     for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) sum += val[k] * in[col[k]];
     out[rowid[r]] += sum;
 }
+// one particle size (dft_fill_FMT.c:215-236, 305-320): rows n1 = 2, n0 = 3, nv1 = 4 + D + d of the
+// rho-bar block depend on n2 = 1 and nv2 = 4 + d with factors c1 = 1/(4 pi R), c0 = 1/(4 pi R^2).
+// mode 0: out_dep = -in_dep + B in_ind; 1: out_dep += B out_ind (in unused); 2: out_dep -= in_dep,
+// out_ind += B^T in_dep
+__global__ void k_one_size(int mode, int N, int nd, double c1, double c0, const double *__restrict__ in,
+                           double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t n = (size_t)N;
+    if (mode == 0) {
+        out[2 * n + i] = -in[2 * n + i] + in[n + i] * c1;
+        out[3 * n + i] = -in[3 * n + i] + in[n + i] * c0;
+        for (int d = 0; d < nd; ++d)
+            out[(4 + nd + d) * n + i] = -in[(4 + nd + d) * n + i] + in[(4 + d) * n + i] * c1;
+    } else if (mode == 1) {
+        out[2 * n + i] += c1 * out[n + i];
+        out[3 * n + i] += c0 * out[n + i];
+        for (int d = 0; d < nd; ++d) out[(4 + nd + d) * n + i] += c1 * out[(4 + d) * n + i];
+    } else {
+        out[2 * n + i] -= in[2 * n + i];
+        out[3 * n + i] -= in[3 * n + i];
+        out[n + i] += c1 * in[2 * n + i] + c0 * in[3 * n + i];
+        for (int d = 0; d < nd; ++d) {
+            out[(4 + nd + d) * n + i] -= in[(4 + nd + d) * n + i];
+            out[(4 + d) * n + i] += c1 * in[(4 + nd + d) * n + i];
+        }
+    }
+}
 __global__ void k_fill(double v, double *__restrict__ y, long long n)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -1569,12 +1598,13 @@ void residual_impl(Problem &P, const double *x, double *F, hipStream_t s, hipEve
 {
     const size_t N = P.N;
     long long n_rb = 0, n_el = 0;
-    if (P.hw_fwd.nrows) ss = nullptr;   // the rho-bar rows change after the fused passes
+    if (P.hw_fwd.nrows || P.one_size) ss = nullptr;   // the rho-bar rows change after the fused passes
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[0], s));
     rb_mf_rows<double>(P, x + P.iu_rho * N, false, x + P.iu_rb * N, nullptr, F + P.iu_rb * N,
                        P.nmf ? x + P.iu_mf * N : nullptr, nullptr, F + P.iu_mf * N, s, ev ? ev[1] : nullptr,
                        ev ? ev[2] : nullptr, ss, &n_rb);
     hw_apply(P.hw_fwd, x + P.iu_rho * N, F + P.iu_rb * N, s);
+    one_size_rows(P, x + P.iu_rb * N, F + P.iu_rb * N, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[3], s));
     fmt_point<double>(P, x + P.iu_rb * N, nullptr, false, s);
     if (ev) TMI_HIP_CHECK(hipEventRecord(ev[4], s));
@@ -1721,6 +1751,7 @@ void launch_jvp(Problem &P, const double *x, const double *v, double *Jv, hipStr
     rb_mf_rows<T>(P, v + P.iu_rho * N, true, v + P.iu_rb * N, nullptr, Jv + P.iu_rb * N,
                   P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_mf * N, s);
     hw_apply(P.hw_fwd, v + P.iu_rho * N, Jv + P.iu_rb * N, s);
+    one_size_rows(P, v + P.iu_rb * N, Jv + P.iu_rb * N, s);
     fmt_point<T>(P, x + P.iu_rb * N, v + P.iu_rb * N, true, s);
     el_rows<T>(P, true, x + P.iu_rho * N, v + P.iu_rho * N, P.nmf ? v + P.iu_mf * N : nullptr, nullptr, Jv + P.iu_rho * N,
                s);
@@ -1836,6 +1867,21 @@ double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStr
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+static void one_size_launch(const Problem &P, int mode, const double *in, double *out, hipStream_t s)
+{
+    if (!P.one_size) return;
+    hipLaunchKernelGGL(k_one_size, dim3(grid_for(P.N)), dim3(kBlock), 0, s, mode, (int)P.N, P.p.ndim,
+                       P.bulk.inv_4pir[0], P.bulk.inv_4pirsq[0], in, out);
+}
+void one_size_rows(const Problem &P, const double *in_rb, double *out_rb, hipStream_t s)
+{
+    one_size_launch(P, 0, in_rb, out_rb, s);
+}
+void one_size_lift(const Problem &P, double *v_rb, hipStream_t s) { one_size_launch(P, 1, nullptr, v_rb, s); }
+void one_size_transpose(const Problem &P, const double *w_rb, double *y_rb, hipStream_t s)
+{
+    one_size_launch(P, 2, w_rb, y_rb, s);
 }
 void hw_apply(const HwCsr &h, const double *in, double *out, hipStream_t s)
 {
