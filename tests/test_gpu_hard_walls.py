@@ -92,3 +92,72 @@ def test_device_newton_reproduces_hard_wall_example(name):
     assert min(abs(f - ref["force"]) for f in force) < printed_tol(ref["force_text"]), (force, ref)
     fo = [(W[k] * xg[: spec.ncomp]).sum() for k in range(W.shape[0])]
     assert np.abs(np.array(fo) - force).max() < 1e-12 * max(1.0, np.abs(fo).max())
+
+
+def synthetic_masks(spec):
+    """Boundary-node masks for a 2D grid from its zero-density nodes: an element is a wall element
+    when one of its corners has zero density; a nonzero node touching a wall element is a boundary
+    node, with the bits of its fluid elements (inside the domain or by reflection)."""
+    nx, ny = spec.nx
+    zero = np.asarray(spec.zero_tf).reshape(ny, nx, -1)[:, :, 0].astype(bool)
+
+    def corner(i, j):
+        bi, bj = spec.bc[0], spec.bc[1]
+        for v, n, b in ((i, nx, bi), (j, ny, bj)):
+            if v < 0 or v >= n:
+                side = b[0] if v < 0 else b[1]
+                if side != 2 and side != 1:
+                    return None
+        i = i % nx if spec.bc[0][0] == 1 else (-i if i < 0 else (2 * (nx - 1) - i if i >= nx else i))
+        j = j % ny if spec.bc[1][0] == 1 else (-j if j < 0 else (2 * (ny - 1) - j if j >= ny else j))
+        return i, j
+
+    def element(i0, j0):   # lower corner (i0, j0): None outside the domain, else wall flag
+        cs = [corner(i0 + a, j0 + b) for a in (0, 1) for b in (0, 1)]
+        if any(c is None for c in cs):
+            return None
+        return any(zero[c[1], c[0]] for c in cs)
+
+    m = np.zeros((spec.ncomp, ny * nx), dtype=np.uint8)
+    for j in range(ny):
+        for i in range(nx):
+            if zero[j, i]:
+                continue
+            els = [element(i - (ln & 1), j - ((ln >> 1) & 1)) for ln in range(4)]
+            if not any(e is True for e in els):
+                continue
+            bits = sum(1 << ln for ln, e in enumerate(els) if e is False)
+            m[:, j * nx + i] = bits
+    return m
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2d_wall_reflect", "2d_rough_wall"])
+def test_hard_surface_weights_in_2d_match_oracle(case):
+    import dataclasses
+
+    import scipy.sparse as sp
+    from tramonto_amd.device import DeviceProblem
+    from tramonto_amd.problem import small_case
+    base = small_case(case)
+    spec = dataclasses.replace(base, type_attr=-1, lhard_surf=1)
+    spec = dataclasses.replace(spec, hw_mask=synthetic_masks(spec))
+    assert spec.hw_mask.any()
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec)
+    x = perturbed(spec, o, 11)
+    F, _ = dp.fill_residual(dev(x))
+    ref = o.residual(x)
+    assert np.abs(F.cpu().numpy().ravel() - ref.ravel()).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    r, c, v = o.jacobian_coo(x)
+    n = o.nunk * o.N
+    J = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-1.0, 1.0, x.shape)
+    Ju = dp.apply_jacobian(dev(x), dev(u)).cpu().numpy().ravel()
+    ref = J @ u.ravel()
+    assert np.abs(Ju - ref).max() <= 1e-12 * np.abs(ref).max()
+    w = rng.uniform(-1.0, 1.0, x.shape)
+    JTw = dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy().ravel()
+    ref = J.T @ w.ravel()
+    assert np.abs(JTw - ref).max() <= 1e-12 * np.abs(ref).max()
