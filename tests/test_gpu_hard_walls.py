@@ -161,3 +161,30 @@ def test_hard_surface_weights_in_2d_match_oracle(case):
     JTw = dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy().ravel()
     ref = J.T @ w.ravel()
     assert np.abs(JTw - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["HS1_FMT1_1D", "HS3_FMT1_1D", "MIX1_FMT1_1D"])
+def test_hard_walls_with_white_bear_match_oracle(name):
+    """the same walls and boundary-node weights under the White Bear functional (FMT3)"""
+    import dataclasses
+    from tramonto_amd.device import DeviceProblem
+    spec = dataclasses.replace(spec_from_tramonto(EXAMPLES[name], name), type_func=2)
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec)
+    x = perturbed(spec, o, 13)
+    F, _ = dp.fill_residual(dev(x))
+    ref = o.residual(x)
+    assert np.abs(F.cpu().numpy().ravel() - ref.ravel()).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    J = o.jacobian_dense(x)
+    rng = np.random.default_rng(17)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    r = J @ v.ravel()
+    assert np.abs(dp.apply_jacobian(dev(x), dev(v)).cpu().numpy().ravel() - r).max() <= 1e-12 * np.abs(r).max()
+    r = J.T @ v.ravel()
+    assert np.abs(dp.apply_jacobian_transpose(dev(x), dev(v)).cpu().numpy().ravel() - r).max() <= 1e-12 * np.abs(r).max()
+    xo, st = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10)
+    xd = dp.initial_state()
+    std = dp.newton(xd, resid_tol=1e-10, gmres_tol=1e-10)
+    assert st["converged"] and std["converged"]
+    assert np.abs(xd.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
