@@ -63,8 +63,8 @@ typedef struct tmi_params {
     int type_func;
     int type_attr;      /* -1 none; 0 strict mean field with u(r<r_min)=u(r_min) */
     int nwall;          /* > 0: external field present */
-    int lhard_surf;     /* 1: hard surfaces (Lhard_surf), only without attraction (type_attr = -1);
-                           the boundary-node weights come from tmi_set_hard_surface_masks */
+    int lhard_surf;     /* 1: hard surfaces (Lhard_surf); the boundary-node weights of the rho-bar
+                           and mean-field sums come from tmi_set_hard_surface_masks */
     double vext_max;
     double hs_diam[TMI_NCOMP_MAX];
     double sigma_ff[TMI_NCOMP_MAX][TMI_NCOMP_MAX];
@@ -210,7 +210,9 @@ int tmi_apply_jacobian_fp32(void *handle, const double *x_d, const double *v_d, 
 int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *w_d, double *JTw_d, void *stream);
 /* Hard surfaces (lhard_surf = 1): the wall-fluid boundary nodes of each component's element list
  * and which of their elements are fluid, the inputs of HW_boundary_weight (src/dft_utils.c:41-62)
- * that resid_and_Jac_sten_fill_sum_Ncomp applies to the rho-bar rows (src/dft_fill_CLSmf.c:140-160).
+ * that resid_and_Jac_sten_fill_sum_Ncomp applies to the rho-bar and mean-field rows
+ * (src/dft_fill_CLSmf.c:140-160) and int_stencil to the attraction energy
+ * (src/dft_energy_CLSintsten.c:110-117).
  * mask: host [ncomp][N]; 0 for a node that is not a boundary node of the component's list
  * (Nodes_2_boundary_wall == -1), else bit ln set when the element that has the node as its local
  * node ln (x bit 0, y bit 1, z bit 2 of the element's corner) is a fluid element (Wall_elems == -1)

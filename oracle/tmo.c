@@ -890,8 +890,7 @@ tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char
 
 static tmo_ctx *create_impl(const tmo_params *p, const double *vext, const unsigned char *zero_tf)
 {
-    /* hard surfaces weight the mean-field stencils at the walls (dft_fill_CLSmf.c:140-160): not restated */
-    if (p->lhard_surf && p->type_attr >= 0) die("lhard_surf with attraction not supported");
+
     if (p->type_func != TMO_FMT1 && p->type_func != TMO_FMT3) die("type_func");
     if (p->ncomp < 1 || p->ncomp > NC || p->ndim < 1 || p->ndim > 3) die("dims");
     tmo_ctx *c = (tmo_ctx *)calloc(1, sizeof(tmo_ctx));
@@ -1226,6 +1225,7 @@ static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int
                     const int *off = &s->off[3 * k];
                     double w = s->w[k];
                     int jn = o2n(c, ijk, off, refl);
+                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, w);   /* dft_fill_CLSmf.c:140-160 */
                     r += 1.0 * (w * rho_val(c, x, jc, jn) - 0.0);
                     if (J && jn >= 0 && !c->zero[(size_t)jn * ncomp + jc])
                         coo_add(J, row, (c->iu_rho + jc) * N + jn, w * 1.0 * 1.0);
@@ -1676,7 +1676,9 @@ void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, cons
                     const sten_t *s = &c->sten[TMO_UATT][i + C * jc];
                     for (int k = 0; k < s->L; k++) {
                         int jn = o2n(c, ijk, &s->off[3 * k], refl);
-                        mf += s->w[k] * rho_val(c, x, jc, jn);
+                        /* int_stencil with hard surfaces, dft_energy_CLSintsten.c:110-117 */
+                        const double w = c->p.lhard_surf ? hw_weight(c, s, k, jc, jn, s->w[k]) : s->w[k];
+                        mf += w * rho_val(c, x, jc, jn);
                     }
                 }
                 att += 0.5 * rho * mf * w2;

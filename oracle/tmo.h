@@ -42,7 +42,7 @@ typedef struct {
     int type_func;      /* TMO_FMT1 / TMO_FMT3 */
     int type_attr;      /* -1: none; 0: strict MF, u = u(r_min) for r < r_min */
     int nwall;          /* > 0: external field added to the Euler-Lagrange rows */
-    int lhard_surf;     /* 1 only without attraction (hard-surface mean-field weights not restated) */
+    int lhard_surf;     /* hard surfaces: boundary-node weights from tmo_set_hw_masks */
     double vext_max;
     double hs_diam[TMO_NC];
     double sigma_ff[TMO_NC][TMO_NC];

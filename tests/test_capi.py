@@ -91,13 +91,16 @@ def test_rejects_unsupported_problems():
     d = sp.as_params()
     d["lhard_surf"] = 1
     n = ct.c_int()
-    p = _lib.fill_params(_lib.TmiParams(), d)   # hard surfaces without attraction are served
-    assert _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None) == 0
-    d["type_attr"] = 0   # with attraction the mean-field sums would need the boundary weights too
+    for attr in (-1, 0):   # hard surfaces are served with and without attraction
+        d["type_attr"] = attr
+        p = _lib.fill_params(_lib.TmiParams(), d)
+        assert _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None) == 0
+    d["lhard_surf"] = 0
+    d["type_func"] = 1   # FMT2
     p = _lib.fill_params(_lib.TmiParams(), d)
     rc = _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None)
     assert rc == -2
-    assert b"lhard_surf" in _lib.lib().tmi_last_error()
+    assert b"type_func" in _lib.lib().tmi_last_error()
 
 
 @pytest.mark.parametrize("case", ["2d_periodic", "2d_wall_reflect", "2d_rough_wall", "3d_periodic_small",

@@ -188,3 +188,53 @@ def test_hard_walls_with_white_bear_match_oracle(name):
     std = dp.newton(xd, resid_tol=1e-10, gmres_tol=1e-10)
     assert st["converged"] and std["converged"]
     assert np.abs(xd.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+
+
+def with_attraction(name, att_in_a22=0):
+    """a hard-wall Example with a WCA mean-field attraction added (eps/kT = 1, cut-off 2.5 sigma),
+    whose sums take the same boundary-node weights (src/dft_fill_CLSmf.c:140-160) and whose
+    energy's int_stencil too (src/dft_energy_CLSintsten.c:110-117)"""
+    import dataclasses
+    g = EXAMPLES[name]
+    spec = spec_from_tramonto(g, name)
+    C = spec.ncomp
+    return g, dataclasses.replace(spec, type_attr=0, eps_ff=np.ones((C, C)), cut_ff=np.full((C, C), 2.5),
+                                  att_in_a22=att_in_a22)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["HS1_FMT1_1D", "HS3_FMT1_1D", "MIX1_FMT1_1D"])
+def test_hard_walls_with_attraction_match_oracle(name):
+    from tramonto_amd.device import DeviceProblem
+    g, spec = with_attraction(name)
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec)
+    x = perturbed(spec, o, 19)
+    F, _ = dp.fill_residual(dev(x))
+    ref = o.residual(x)
+    assert np.abs(F.cpu().numpy().ravel() - ref.ravel()).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    J = o.jacobian_dense(x)
+    rng = np.random.default_rng(23)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    r = J @ v.ravel()
+    assert np.abs(dp.apply_jacobian(dev(x), dev(v)).cpu().numpy().ravel() - r).max() <= 1e-12 * np.abs(r).max()
+    r = J.T @ v.ravel()
+    assert np.abs(dp.apply_jacobian_transpose(dev(x), dev(v)).cpu().numpy().ravel() - r).max() <= 1e-12 * np.abs(r).max()
+    xo, st = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10, update_scaling=g["nl_update_scaling"])
+    xd = dp.initial_state()
+    std = dp.newton(xd, resid_tol=1e-10, gmres_tol=1e-10, update_scaling=g["nl_update_scaling"])
+    assert st["converged"] and std["converged"]
+    xg = xd.cpu().numpy()
+    assert np.abs(xg[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+    h1, h2, scale = integration_weights(spec, g)
+    obs = dp.observables(xd, h1, h2, scale)
+    ref = o.observables(xg, h1, h2, scale)
+    for k in ("omega", "omega_s", "ideal", "mu", "hs", "att"):
+        assert abs(obs[k] - ref[k]) <= 1e-12 * max(1.0, abs(ref[k])), (k, obs[k], ref[k])
+    # the att_in_a22 layout (no mean-field unknowns) converges to the same densities
+    _, spec_a = with_attraction(name, att_in_a22=1)
+    da = DeviceProblem(spec_a)
+    xa = da.initial_state()
+    sta = da.newton(xa, resid_tol=1e-10, gmres_tol=1e-10, update_scaling=g["nl_update_scaling"])
+    assert sta["converged"]
+    assert np.abs(xa.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8

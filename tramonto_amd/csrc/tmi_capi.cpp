@@ -64,9 +64,7 @@ void validate(const tmi_params &p)
     if (p.type_func != TMI_FMT1 && p.type_func != TMI_FMT3)
         throw std::domain_error("type_func: only Rosenfeld (0) and White Bear (2) are implemented");
     if (p.type_attr != -1 && p.type_attr != 0) throw std::domain_error("type_attr: only -1 and 0 are implemented");
-    if (p.lhard_surf && p.type_attr >= 0)
-        throw std::domain_error("lhard_surf with attraction: the hard-surface weights of the mean-field sums are not "
-                                "implemented");
+
     for (int d = 0; d < p.ndim; ++d) {
         if (p.nx[d] < 1) throw std::invalid_argument("nx must be positive");
         if (!(p.dx[d] > 0.0)) throw std::invalid_argument("dx must be positive");

@@ -112,6 +112,7 @@ void hw_free(Problem &P)
 {
     free_csr(P.hw_fwd);
     free_csr(P.hw_tr);
+    free_csr(P.hw_mf);
 }
 
 void hw_build(Problem &P, const unsigned char *mask)
@@ -131,8 +132,8 @@ void hw_build(Problem &P, const unsigned char *mask)
     // a row can reach a boundary node only if, in every dimension, its coordinate reaches one of
     // the boundary nodes' coordinates (a conservative filter before the exact stencil walk)
     int reach[3] = {0, 0, 0};
-    for (int t = TMI_DELTA; t <= TMI_THETA; ++t)
-        for (int c = 0; c < C; ++c) {
+    for (int t = TMI_DELTA; t <= TMI_UATT; ++t)
+        for (int c = 0; c < (t == TMI_UATT ? (P.nmf ? C * C : 0) : C); ++c) {
             const HostStencil &hs = P.hs[t][c];
             for (int k = 0; k < hs.L; ++k)
                 for (int d = 0; d < nd; ++d) reach[d] = std::max(reach[d], std::abs(hs.off[3 * k + d]));
@@ -164,7 +165,7 @@ void hw_build(Problem &P, const unsigned char *mask)
                     near[d][i] = hit[d][cd % p.nx[d]];
                 }
             }
-    std::vector<std::tuple<int, int, double>> fwd, tr;
+    std::vector<std::tuple<int, int, double>> fwd, tr, mf;
     for (int i = 0; i < N; ++i) {
         const int ijk[3] = {i % p.nx[0], (i / p.nx[0]) % p.nx[1], i / (p.nx[0] * p.nx[1])};
         bool cand = true;
@@ -189,9 +190,30 @@ void hw_build(Problem &P, const unsigned char *mask)
                 }
             }
         }
+        // mean-field rows (load_mean_field through the same stencil-sum fill), right after the
+        // rho-bar block: rows (nrb + ic) N + i
+        for (int ic = 0; ic < P.nmf; ++ic)
+            for (int c = 0; c < C; ++c) {
+                const HostStencil &hs = P.hs[TMI_UATT][ic + C * c];
+                for (int k = 0; k < hs.L; ++k) {
+                    const int *off = &hs.off[3 * k];
+                    const int j = node_of(p, ijk, off);
+                    if (j < 0) continue;
+                    const unsigned m = mask[(size_t)c * N + j];
+                    if (!m) continue;
+                    double wf = 0.0;
+                    for (int ln = 0; ln < (1 << nd); ++ln)
+                        if (m & (1u << ln)) wf += hs.hw[8 * (size_t)k + ln];
+                    const double v = wf - hs.w[k];
+                    fwd.emplace_back((P.nrb + ic) * N + i, c * N + j, v);
+                    tr.emplace_back(c * N + j, (P.nrb + ic) * N + i, v);
+                    mf.emplace_back(ic * N + i, c * N + j, v);
+                }
+            }
     }
     P.hw_fwd = make_csr(fwd);
     P.hw_tr = make_csr(tr);
+    if (!mf.empty()) P.hw_mf = make_csr(mf);
 }
 
 }  // namespace tmi

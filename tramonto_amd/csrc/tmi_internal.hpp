@@ -178,7 +178,7 @@ struct Problem {
     bool ext_mf = false;
     // hard surfaces: the rho-bar rows' weights at wall-fluid boundary nodes (hw_fwd: rows q*N + i
     // of the rho-bar block, columns c*N + j of the densities; hw_tr its transpose)
-    HwCsr hw_fwd, hw_tr;
+    HwCsr hw_fwd, hw_tr, hw_mf;   // hw_fwd rows cover the mean-field block too; hw_mf those rows alone
     // one particle size with hard surfaces (one component, walls): n1, n0, nv1 rows are multiples
     // of n2, nv2 as in the reference; d_w0 is the transpose product's copy of w
     bool one_size = false;

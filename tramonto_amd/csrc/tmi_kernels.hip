@@ -1695,6 +1695,7 @@ void launch_mf_conv(Problem &P, const double *rho, bool linear, double *out, hip
     if (P.nmf == 0) return;
     pad_rho<double>(P, rho, linear, s);
     mf_rows<double>(P, nullptr, nullptr, out, s);
+    hw_apply(P.hw_mf, rho, out, s);
 }
 
 void observables_sums(Problem &P, const double *x, const int *nh_d, const int *nh2_d, double *mf_tmp, double *sums,
@@ -1704,10 +1705,8 @@ void observables_sums(Problem &P, const double *x, const int *nh_d, const int *n
     const size_t N = P.N;
     const int C = P.p.ncomp, nd = P.p.ndim;
     const bool att_on = P.p.type_attr >= 0 && P.nmf > 0;
-    if (att_on) {   // u * rho with the boundary values of the fill (int_stencil)
-        pad_rho<double>(P, x + (size_t)P.iu_rho * N, false, s);
-        mf_rows<double>(P, nullptr, nullptr, mf_tmp, s);
-    }
+    if (att_on)   // u * rho with the boundary values and hard-surface weights of the fill (int_stencil)
+        launch_mf_conv(P, x + (size_t)P.iu_rho * N, false, mf_tmp, s);
     ObsConsts oc{};
     for (int c = 0; c < C; ++c)
         oc.betamu[c] = std::log(P.p.rho_b[c]) + P.bulk.betamu_hs_ex[c] + (P.p.type_attr >= 0 ? P.bulk.betamu_att[c] : 0.0);

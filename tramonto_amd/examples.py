@@ -146,11 +146,11 @@ def _pair_matrix(g: Dict, key: str) -> np.ndarray:
 
 
 def spec_from_tramonto(g: Dict, name: str = "example") -> ProblemSpec:
-    hard = g["ipot_wf_n"] == VEXT_HARD and g["lhard_surf"] == 1 and g["type_attr"] < 0
+    hard = g["ipot_wf_n"] == VEXT_HARD and g["lhard_surf"] == 1
     lj = g["ipot_wf_n"] == VEXT_DIST_TO_SURF and g["lhard_surf"] == 0
     if g["ndim"] != 1 or g["surf_type"] != 0 or not (hard or lj):
         raise ValueError("only 1D planar walls are described here: LJ 9-3 walls without hard-surface "
-                         "weights, or hard walls with them and no attraction")
+                         "weights, or hard walls with them")
     C = g["ncomp"]
     nx, dx, x = _grid(g)
     vmax = g["vext_max"]

@@ -75,7 +75,7 @@ class ProblemSpec:
     zero_tf: Optional[np.ndarray] = None    # [N, C] uint8
     rho0: Optional[np.ndarray] = None       # [C, N] initial densities
     att_in_a22: int = 0                     # 1: no mean-field unknowns (the reference's ATTInA22Block)
-    lhard_surf: int = 0                     # 1: hard surfaces (Lhard_surf); needs type_attr = -1
+    lhard_surf: int = 0                     # 1: hard surfaces (Lhard_surf)
     hw_mask: Optional[np.ndarray] = None    # [ncomp][N] uint8 fluid-element bits of boundary nodes
 
     @property
