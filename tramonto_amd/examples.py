@@ -12,7 +12,7 @@ records of tests/golden/examples_1d.json) into a ProblemSpec:
   src/dft_vext_1D.c:40-56; setup_vext_XRSurf, src/dft_vext.c:364-434):
   VEXT_MAX inside the wall and within 0.001 of it, otherwise min(V, VEXT_MAX);
   Zero_density_TF where V_ext >= VEXT_MAX (src/dft_vext.c:155-166);
-* or hard walls (Ipot_wf_n = VEXT_HARD with Lhard_surf = 1, no attraction):
+* or hard walls (Ipot_wf_n = VEXT_HARD with Lhard_surf = 1):
   V_ext = 0 and Zero_density_TF from the element lists of the mesh
   (src/dft_mesh.c:1340-1470): component i's list flags the elements whose
   centre lies within WallParam + Sigma_ff[i][i]/2 of a wall
