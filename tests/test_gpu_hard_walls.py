@@ -238,3 +238,63 @@ def test_hard_walls_with_attraction_match_oracle(name):
     sta = da.newton(xa, resid_tol=1e-10, gmres_tol=1e-10, update_scaling=g["nl_update_scaling"])
     assert sta["converged"]
     assert np.abs(xa.cpu().numpy()[: spec.ncomp] - xo[: spec.ncomp]).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_reference_abi_newton_sequence_with_hard_walls():
+    """dft_newton.c's call sequence through the LinProbMgr C ABI on HS1 (one particle size: the
+    reference registers n1, n0, nv1 as dependent nonlocal unknowns, src/dft_linsolver_setup.c:196-230)
+    converges to the oracle's profile and the README adsorption."""
+    import ctypes as ct
+
+    from test_gpu_golden import rows, update_solution
+    from tramonto_amd import _lib
+    from tramonto_amd.device import DeviceProblem
+    L = _lib.lib()
+    g = EXAMPLES["HS1_FMT1_1D"]
+    spec = spec_from_tramonto(g, "HS1")
+    dp = DeviceProblem(spec)
+    nunk, N, C, nd = spec.nunk, spec.nnodes, spec.ncomp, spec.ndim
+    mgr = L.dft_hardsphere_lin_prob_mgr_create(nunk, None, None)
+    ind = np.array([C, C + 1] + [C + 4 + d for d in range(nd)], dtype=np.int32)
+    dep = np.array([C + 2, C + 3] + [C + 4 + nd + d for d in range(nd)], dtype=np.int32)
+    dens = np.arange(0, C, dtype=np.int32)
+    gids = np.arange(N, dtype=np.int32)
+    vp = lambda a: a.ctypes.data_as(ct.c_void_p)
+    assert L.dft_hardsphere_lin_prob_mgr_setindnonlocalequationids(mgr, len(ind), vp(ind)) == 0
+    assert L.dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids(mgr, len(dep), vp(dep)) == 0
+    assert L.dft_hardsphere_lin_prob_mgr_setdensityequationids(mgr, len(dens), vp(dens)) == 0
+    assert L.dft_linprobmgr_setnodalrowmap(mgr, N, vp(gids)) == 0
+    assert L.dft_linprobmgr_setnodalcolmap(mgr, N, vp(gids)) == 0
+    assert L.dft_linprobmgr_finalizeblockstructure(mgr) == 0
+    assert L.tmi_set_active_problem(dp.handle, mgr) == 0
+    x = np.ascontiguousarray(dp.initial_state().cpu().numpy())
+    dx = np.zeros_like(x)
+    xr, dxr = rows(x), rows(dx)
+    converged = False
+    for it in range(1, 30):
+        assert L.dft_linprobmgr_initializeproblemvalues(mgr) == 0
+        l2 = L.fill_resid_and_matrix_control(ct.cast(xr, ct.c_void_p), it, 0)
+        assert l2 >= 0.0
+        assert L.dft_linprobmgr_finalizeproblemvalues(mgr) == 0
+        if np.sqrt(l2) < 1e-10:
+            converged = True
+            break
+        assert L.dft_linprobmgr_setupsolver(mgr) == 0
+        assert L.dft_linprobmgr_solve(mgr) == 0
+        assert L.dft_linprobmgr_getlhs(mgr, ct.cast(dxr, ct.c_void_p)) == 0
+        update_solution(x, dx, C, g["nl_update_scaling"])
+    assert converged and it - 1 <= g["key_outputs"]["niter"] + 4
+    o = Oracle.from_spec(spec)
+    xo, _ = o.newton(o.init_fields(spec.initial_x()), resid_tol=1e-10, update_scaling=g["nl_update_scaling"])
+    assert np.abs(x[:C] - xo[:C]).max() < 1e-8
+    h1, h2, scale = integration_weights(spec, g)
+    obs = dp.observables(torch.from_numpy(x).cuda(), h1, h2, scale)
+    assert abs(obs["ads"][0] - g["key_outputs"]["adsorption"][0]) < 2e-6
+    v = np.ascontiguousarray(np.random.default_rng(1).uniform(-1, 1, x.shape))
+    b = np.zeros_like(v)
+    assert L.dft_linprobmgr_applymatrix(mgr, ct.cast(rows(v), ct.c_void_p), ct.cast(rows(b), ct.c_void_p)) == 0
+    r = o.jacobian_dense(x) @ v.ravel()
+    assert np.abs(b.ravel() - r).max() <= 1e-12 * np.abs(r).max()
+    assert L.dft_linprobmgr_destruct(mgr) == 0
+    L.tmi_set_active_problem(None, None)
