@@ -86,6 +86,11 @@ typedef struct tmi_info {
     int precond;  /* preconditioner of the Schur GMRES: TMI_PRECOND_DIAG or TMI_PRECOND_BULK */
     int fft_fold; /* 1: the stencil spectra of the FFT sums are stored for k_y, k_z >= 0 only (every stencil
                      symmetric under the reflection of y and of z), 0: for the whole FFT grid or no FFT sums */
+    int fft_xreg; /* the fused x pass runs in registers (x length 32-256, few enough outputs): bit 0 for the
+                     density product, bit 1 for the Euler-Lagrange product; 0 in LDS or not fused */
+    int fft_regpass; /* strided FFT axes of length 32-256: 0 LDS Stockham passes (TMI_FFT_REGPASS=0), 1 the
+                        one-wave register pass where a pass has at least 1024 waves, 2 wherever the length
+                        allows (TMI_FFT_REGPASS=2); read at creation */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */
@@ -216,7 +221,11 @@ int tmi_apply_jacobian_transpose(void *handle, const double *x_d, const double *
  * mask: host [ncomp][N]; 0 for a node that is not a boundary node of the component's list
  * (Nodes_2_boundary_wall == -1), else bit ln set when the element that has the node as its local
  * node ln (x bit 0, y bit 1, z bit 2 of the element's corner) is a fluid element (Wall_elems == -1)
- * inside the domain or by reflection.  NULL clears.  Every later fill, JVP, transpose product and
+ * inside the domain or by reflection.  A stencil point that reaches the node through a REFLECT
+ * boundary in dimension d reads the mirror-side element, bit ln ^ (1 << d), as the reference's
+ * node_box_to_elem_box_reflect does (src/dft_mesh_lib.c:896-937).  Entries at a component's
+ * Zero_density_TF nodes are ignored: those columns carry 0 in the reference's sums
+ * (src/dft_fill_CLSmf.c:156).  NULL clears.  Every later fill, JVP, transpose product and
  * Schur solve uses the weights; calc_init_rho_bar (tmi_init_fields) does not, as in the reference. */
 int tmi_set_hard_surface_masks(void *handle, const unsigned char *mask);
 /* rho-bar and mean-field unknowns from the densities, src/dft_guess_HSFMT.c:70, src/dft_guess_MFATT.c:66 */
@@ -233,6 +242,10 @@ int tmi_schur_solve(void *handle, const double *x_d, const double *F_d, double *
  * FP64 / 1 FP32, kind 0 rho-bar rows / 1 mean-field rows / 2 Euler-Lagrange
  * rows.  Zeros when the problem does not use the row kernels. */
 int tmi_row_tuning(void *handle, int *shape, double *ms);
+/* Sets one of those launch shapes (outputs per lane 4, 8 or 16; lanes per
+ * workgroup 64, 128 or 256), replacing the autotuner's choice; every shape
+ * computes the same sums.  TMI_EINVAL when the problem has no row kernels. */
+int tmi_row_tuning_set(void *handle, int prec, int kind, int per_lane, int lanes);
 
 /* --------------------------------------------------------------------------
  * Batched 1D problems.  No counterpart in the reference, which solves one
@@ -267,14 +280,51 @@ int tmi_batch_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opt
  * the problem registered with tmi_set_active_problem (the reference reads
  * its problem from globals).  Arrays are host double** in box layout,
  * exactly as in the reference.
+ *
+ * The communicator argument is the reference's MPI_Comm.  Where mpi.h is
+ * included first it is declared as MPI_Comm; otherwise as int, the width of
+ * MPICH's MPI_Comm (the MPI the reference builds with here).  Both travel in
+ * one integer register of the x86-64 calling convention, and the library
+ * never reads it: one GPU holds the whole domain.
  * ------------------------------------------------------------------------ */
+#ifdef MPI_VERSION
+typedef MPI_Comm tmi_comm_t;
+#else
+typedef int tmi_comm_t;
+#endif
+
+/* Registers the problem the fill entry point and the LinProbMgr serve.  When
+ * the manager's block structure is already finalised, its equation IDs and
+ * maps are checked against the problem's layout (TMI_EINVAL on mismatch). */
 int tmi_set_active_problem(void *problem_handle, void *linprobmgr_handle);
 
-/* src/dft_fill_control.c:39 */
+/* src/dft_fill_control.c:39.  resid_only_flag 0 adds -F to the manager's
+ * right-hand side and records x as the Jacobian's linearisation point; 1 and
+ * 4 add -F only; 2 and 3 insert nothing.  Returns sum F^2 (-1 on error). */
 double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag);
 
-/* src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58 */
-void *dft_hardsphere_lin_prob_mgr_create(int numUnks, void *ParameterList, void *comm);
+/* src/dft_parameterlist_wrapper.h:40-52.  A list of named solver options;
+ * the managers read "Max_iter" and "Tol" at solve time
+ * (src/dft_HardSphereLinProbMgr.cpp:401) and the basic manager "Kspace".
+ * set_all_aztec stores the Aztec options/params arrays under the reference's
+ * names (src/dft_parameterlist_wrapper.cpp:74-108). */
+void *dft_parameterlist_create(void);
+void dft_parameterlist_destruct(void *parameterlistptr);
+void dft_parameterlist_set_int(void *parameterlistptr, char *name, int value);
+void dft_parameterlist_set_double(void *parameterlistptr, char *name, double value);
+void dft_parameterlist_set_int_array(void *parameterlistptr, char *name, int *values);
+void dft_parameterlist_set_double_array(void *parameterlistptr, char *name, double *values);
+void dft_parameterlist_set_all_aztec(void *parameterlistptr, int *options, double *params);
+
+/* src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58.  finalizeblockstructure
+ * checks the three equation-ID lists against the registered problem's
+ * layout (linsolver_setup_HSTYPE, src/dft_linsolver_setup.c:182-239).  The
+ * debug manager prints ||J dx - b||_2 after each solve, as the reference's
+ * debug solve does (src/dft_HardSphereLinProbMgr.cpp:405-414), without
+ * writing matrix files or aborting. */
+void *dft_hardsphere_lin_prob_mgr_create(int numUnks, void *ParameterList, tmi_comm_t comm);
+void *dft_hardsphere_lin_prob_mgr_create_debug(int numUnks, void *ParameterList, tmi_comm_t comm);
+void dft_hardsphere_lin_prob_mgr_destruct(void *linprobmgr);
 int dft_hardsphere_lin_prob_mgr_setindnonlocalequationids(void *linprobmgr, int numIndNonLocalEquations,
                                                           int *indNonLocalEquationIDs);
 int dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids(void *linprobmgr, int numDepNonLocalEquations,
@@ -283,8 +333,12 @@ int dft_hardsphere_lin_prob_mgr_setdensityequationids(void *linprobmgr, int numD
                                                       int *densityEquationIDs);
 int dft_hardsphere_lin_prob_mgr_seta22blockisdiagonal(void *linprobmgr, int isA22Diagonal);
 
-/* src/dft_basic_lin_prob_mgr_wrapper.h:44-92 */
-int dft_linprobmgr_destruct(void *linprobmgr);
+/* src/dft_basic_lin_prob_mgr_wrapper.h:44-92.  The basic manager
+ * (dft_linsolver_setup.c:48, problems without a Schur solver) takes no
+ * equation IDs; its solve returns the solution of the same system J dx = b
+ * through the device Schur complement. */
+void *dft_basic_lin_prob_mgr_create(int numUnks, void *ParameterList, tmi_comm_t comm);
+void dft_linprobmgr_destruct(void *linprobmgr);
 int dft_linprobmgr_setnodalrowmap(void *linprobmgr, int numgids, int *gids);
 int dft_linprobmgr_setnodalcolmap(void *linprobmgr, int numgids, int *gids);
 int dft_linprobmgr_setcoarsenednodeslist(void *linprobmgr, int numgids, int *gids);
@@ -309,6 +363,11 @@ int dft_linprobmgr_importnodalr2c(void *linprobmgr, double *xOwned, double *xBox
 int dft_linprobmgr_importsingleunknownr2c(void *linprobmgr, double *xOwned, double *xBox);
 double dft_linprobmgr_getmatrixvalue(void *linprobmgr, int iunk, int ownednode, int junk, int boxnode);
 int dft_linprobmgr_writeMatrix(void *linprobmgr, char *filename, char *matrixName, char *matrixDescription);
+
+/* No counterpart in the reference (whose solver reports nothing through the
+ * ABI): GMRES iterations of the manager's last solve and, for the debug
+ * manager, its ||J dx - b||_2 (-1 otherwise). */
+int tmi_linprobmgr_last_solve(void *linprobmgr, int *iters, double *debug_resid);
 
 #ifdef __cplusplus
 }

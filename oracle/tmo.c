@@ -585,15 +585,21 @@ static void build_stencil(tmo_ctx *c, int isten, int icomp, int jcomp)
 }
 
 /* HW_boundary_weight (dft_utils.c:41-62): the weight of stencil point k at a wall-fluid boundary
- * node is the sum of its parts from the local nodes whose element is fluid */
-static double hw_weight(const tmo_ctx *c, const sten_t *s, int k, int jcomp, int jnode, double w)
+ * node is the sum of its parts hw[ln] over the local nodes ln whose element is fluid.  The element
+ * is node_box_to_elem_box_reflect's (dft_mesh_lib.c:896-937): where offset_to_node_box reflected the
+ * point in dimension d (refl[d]), the element on the mirror side, i.e. local node ln with bit d
+ * flipped in the node's own element numbering. */
+static double hw_weight(const tmo_ctx *c, const sten_t *s, int k, int jcomp, int jnode, const int *refl, double w)
 {
     if (!c->hwmask || jnode < 0) return w;
     const unsigned m = c->hwmask[(size_t)jcomp * c->N + jnode];
     if (!m) return w;
+    unsigned flip = 0;
+    for (int d = 0; d < c->p.ndim; d++)
+        if (refl[d]) flip |= 1u << d;
     double v = 0.0;
     for (int ln = 0; ln < (1 << c->p.ndim); ln++)
-        if (m & (1u << ln)) v += s->hw[8 * k + ln];
+        if (m & (1u << (ln ^ flip))) v += s->hw[8 * k + ln];
     return v;
 }
 
@@ -1207,7 +1213,7 @@ static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int
                         fac *= 1.0 * vec;
                     }
                     int jn = o2n(c, ijk, off, refl);
-                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, w);   /* dft_fill_CLSmf.c:140-160 */
+                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, refl, w);   /* dft_fill_CLSmf.c:140-160 */
                     r += fac * (w * rho_val(c, x, jc, jn) - 0.0);
                     if (J && jn >= 0 && !c->zero[(size_t)jn * ncomp + jc])
                         coo_add(J, row, (c->iu_rho + jc) * N + jn, w * fac * 1.0);
@@ -1225,7 +1231,7 @@ static void fill_node(const tmo_ctx *c, const double *x, const double *dphi, int
                     const int *off = &s->off[3 * k];
                     double w = s->w[k];
                     int jn = o2n(c, ijk, off, refl);
-                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, w);   /* dft_fill_CLSmf.c:140-160 */
+                    if (c->p.lhard_surf) w = hw_weight(c, s, k, jc, jn, refl, w);   /* dft_fill_CLSmf.c:140-160 */
                     r += 1.0 * (w * rho_val(c, x, jc, jn) - 0.0);
                     if (J && jn >= 0 && !c->zero[(size_t)jn * ncomp + jc])
                         coo_add(J, row, (c->iu_rho + jc) * N + jn, w * 1.0 * 1.0);
@@ -1677,7 +1683,7 @@ void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, cons
                     for (int k = 0; k < s->L; k++) {
                         int jn = o2n(c, ijk, &s->off[3 * k], refl);
                         /* int_stencil with hard surfaces, dft_energy_CLSintsten.c:110-117 */
-                        const double w = c->p.lhard_surf ? hw_weight(c, s, k, jc, jn, s->w[k]) : s->w[k];
+                        const double w = c->p.lhard_surf ? hw_weight(c, s, k, jc, jn, refl, s->w[k]) : s->w[k];
                         mf += w * rho_val(c, x, jc, jn);
                     }
                 }

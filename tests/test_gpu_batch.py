@@ -130,3 +130,36 @@ def test_batch_newton_matches_single_pore_solves(small):
         xs = dp.initial_state()
         assert dp.newton(xs, resid_tol=1e-11, gmres_tol=1e-10, max_iter=60)["converged"]
         assert np.abs(xp - xs.cpu().numpy()).max() < 1e-9 * max(1.0, np.abs(xp).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_lane", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [64, 128, 256])
+def test_every_row_kernel_shape_matches_oracle(small, per_lane, lanes):
+    """Every launch shape the 1D autotuner can choose (outputs per lane x lanes per workgroup, for the
+    rho-bar, mean-field and Euler-Lagrange row kernels, FP64 and FP32), forced with
+    tmi_row_tuning_set: residual and J v per pore against the oracle to 1e-12, the FP32 product
+    against FP64 to 1e-5."""
+    b, oracles, db = small
+    db.set_row_tuning(per_lane, lanes)
+    tun = db.row_tuning()
+    assert all(tun[p][k][:2] == (per_lane, lanes) for p in tun for k in tun[p] if tun[p][k][0])
+    try:
+        x = generic_state(b, oracles, 30 + per_lane + lanes)
+        Fg, _ = db.fill_residual(dev(x))
+        for Fp, xp, o in zip(b.split(Fg.cpu().numpy()), b.split(x), oracles):
+            Fo = o.residual(np.ascontiguousarray(xp))
+            assert np.abs(Fp - Fo).max() < RTOL * np.abs(Fo).max()
+        rng = np.random.default_rng(per_lane * lanes)
+        v = rng.uniform(-1.0, 1.0, x.shape)
+        Jv = db.apply_jacobian(dev(x), dev(v)).cpu().numpy()
+        for p, o, xp, vp, Jp in zip(b.pores, oracles, b.split(x), b.split(v), b.split(Jv)):
+            r, c, val = o.jacobian_coo(np.ascontiguousarray(xp))
+            n = p.nunk * p.nnodes
+            J = sps.coo_matrix((val, (r, c)), shape=(n, n)).tocsr()
+            ref = (J @ np.ascontiguousarray(vp).reshape(-1)).reshape(p.nunk, p.nnodes)
+            assert np.abs(Jp - ref).max() < RTOL * np.abs(ref).max()
+        J32 = db.apply_jacobian(dev(x), dev(v), fp32=True).cpu().numpy()
+        assert np.abs(J32 - Jv).max() < 1e-5 * np.abs(Jv).max()
+    finally:
+        db.set_row_tuning(8, 256)

@@ -255,7 +255,7 @@ def test_reference_abi_newton_sequence_with_hard_walls():
     spec = spec_from_tramonto(g, "HS1")
     dp = DeviceProblem(spec)
     nunk, N, C, nd = spec.nunk, spec.nnodes, spec.ncomp, spec.ndim
-    mgr = L.dft_hardsphere_lin_prob_mgr_create(nunk, None, None)
+    mgr = L.dft_hardsphere_lin_prob_mgr_create(nunk, None, 0)
     ind = np.array([C, C + 1] + [C + 4 + d for d in range(nd)], dtype=np.int32)
     dep = np.array([C + 2, C + 3] + [C + 4 + nd + d for d in range(nd)], dtype=np.int32)
     dens = np.arange(0, C, dtype=np.int32)
@@ -296,7 +296,7 @@ def test_reference_abi_newton_sequence_with_hard_walls():
     assert L.dft_linprobmgr_applymatrix(mgr, ct.cast(rows(v), ct.c_void_p), ct.cast(rows(b), ct.c_void_p)) == 0
     r = o.jacobian_dense(x) @ v.ravel()
     assert np.abs(b.ravel() - r).max() <= 1e-12 * np.abs(r).max()
-    assert L.dft_linprobmgr_destruct(mgr) == 0
+    L.dft_linprobmgr_destruct(mgr)
     L.tmi_set_active_problem(None, None)
 
 
@@ -319,3 +319,78 @@ def test_mixed_precision_newton_with_hard_walls(attraction):
         assert st["converged"], (prec, st)
         out.append(x.cpu().numpy())
     assert np.abs(out[0][: spec.ncomp] - out[1][: spec.ncomp]).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_reflected_boundary_node_weights_match_oracle():
+    """A hard slab on a REFLECT plane (tramonto_amd.problem.reflect_slab_1d): the stencil points that
+    reach the slab's surface node through the reflection take the mirror-side element's part
+    (node_box_to_elem_box_reflect, src/dft_mesh_lib.c:896-937).  Device = oracle to 1e-12 for F,
+    J v against the assembled J and J^T w, with the mean-field sums on."""
+    import scipy.sparse as sp
+    from tramonto_amd.device import DeviceProblem
+    from tramonto_amd.problem import small_case
+    spec = small_case("1d_reflect_slab")
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec)
+    x = perturbed(spec, o, 19)
+    F, _ = dp.fill_residual(dev(x))
+    ref = o.residual(x)
+    assert np.abs(F.cpu().numpy().ravel() - ref.ravel()).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    r, c, v = o.jacobian_coo(x)
+    n = o.nunk * o.N
+    J = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    rng = np.random.default_rng(23)
+    u = rng.uniform(-1.0, 1.0, x.shape)
+    ref = J @ u.ravel()
+    assert np.abs(dp.apply_jacobian(dev(x), dev(u)).cpu().numpy().ravel() - ref).max() <= 1e-12 * np.abs(ref).max()
+    ref = J.T @ u.ravel()
+    got = dp.apply_jacobian_transpose(dev(x), dev(u)).cpu().numpy().ravel()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["2d_wall_reflect", "2d_rough_wall"])
+@pytest.mark.parametrize("mf_fft", ["0", "1"])
+def test_hard_surface_weights_in_2d_with_attraction(case, mf_fft, monkeypatch):
+    """2D hard surfaces with the WCA attraction on: the mean-field sums take the boundary-node
+    weights too (src/dft_fill_CLSmf.c:140-160), in the direct and the FFT form of those sums and in
+    both unknown layouts.  Mean-field layout: F, J v and J^T w against the oracle to 1e-12.
+    ATTInA22Block layout: F against the oracle's residual at x extended by u * rho (the oracle's own
+    mean-field rows, with the hard-surface weights), without the mean-field rows."""
+    import dataclasses
+
+    import scipy.sparse as sp
+    from tramonto_amd.device import DeviceProblem
+    from tramonto_amd.problem import small_case
+    monkeypatch.setenv("TMI_MF_FFT", mf_fft)
+    base = small_case(case)
+    spec = dataclasses.replace(base, type_attr=0, lhard_surf=1)
+    spec = dataclasses.replace(spec, hw_mask=synthetic_masks(spec))
+    o = Oracle.from_spec(spec)
+    dp = DeviceProblem(spec)
+    assert dp.info.mf_fft == int(mf_fft)
+    x = perturbed(spec, o, 29)
+    F, _ = dp.fill_residual(dev(x))
+    ref = o.residual(x)
+    assert np.abs(F.cpu().numpy().ravel() - ref.ravel()).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    r, c, v = o.jacobian_coo(x)
+    n = o.nunk * o.N
+    J = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    rng = np.random.default_rng(31)
+    u = rng.uniform(-1.0, 1.0, x.shape)
+    ref = J @ u.ravel()
+    assert np.abs(dp.apply_jacobian(dev(x), dev(u)).cpu().numpy().ravel() - ref).max() <= 1e-12 * np.abs(ref).max()
+    ref = J.T @ u.ravel()
+    got = dp.apply_jacobian_transpose(dev(x), dev(u)).cpu().numpy().ravel()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the same state in the ATTInA22Block layout
+    sa = dataclasses.replace(spec, att_in_a22=1)
+    dpa = DeviceProblem(sa)
+    ne = sa.nunk
+    xi = x.copy()
+    xi[ne:] = 0.0
+    xi[ne:] = o.residual(xi)[ne:]            # u * rho with the boundary-node weights
+    Fa, _ = dpa.fill_residual(dev(np.ascontiguousarray(x[:ne])))
+    ref = o.residual(xi)[:ne]
+    assert np.abs(Fa.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())

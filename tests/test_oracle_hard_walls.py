@@ -118,3 +118,33 @@ def test_one_size_rows_give_the_newton_steps_of_two_identical_components():
         y5, _ = o5.newton(x5.copy(), max_iter=k, resid_tol=0.0, update_scaling=g5["nl_update_scaling"])
         y1, _ = o1.newton(x1.copy(), max_iter=k, resid_tol=0.0, update_scaling=g1["nl_update_scaling"])
         assert np.abs(y5[0] - (y1[0] + y1[1])).max() < 1e-10 * np.abs(y5[0]).max(), k
+
+
+@pytest.mark.parametrize("attraction", [False, True])
+def test_reflected_boundary_node_weights_match_full_line(attraction):
+    """HW_boundary_weight through a REFLECT edge (src/dft_utils.c:41-62 with the reflect flags of
+    offset_to_node_box, src/dft_mesh_lib.c:203-297 and 896-937): a hard slab centred on a REFLECT
+    plane is half of the full line holding the whole slab.  In the half line the stencil points of
+    the rows near the plane reach the slab's surface node through the reflection, where the fluid
+    element is on the mirror side; the full line reaches the mirror node directly.  Residual rows
+    (rho-bar, mean field, Euler-Lagrange) must agree.  Reading the unflipped element bit for the
+    reflected points changes the rho-bar rows near the plane by ~1e-2 and fails this."""
+    from tramonto_amd.problem import reflect_slab_1d
+    full, half = reflect_slab_1d(False, attraction=attraction), reflect_slab_1d(True, attraction=attraction)
+    of, oh = Oracle.from_spec(full), Oracle.from_spec(half)
+    xf = of.init_fields(full.initial_x())
+    M = half.nnodes - 1
+    xh = np.ascontiguousarray(xf[:, M:])
+    # exact mirror symmetry of the state (odd vector rows), so both lines hold the same fluid
+    nd, C = 1, 1
+    for u in range(xf.shape[0]):
+        odd = C + 4 <= u < C + 4 + 2 * nd
+        xh[u] = 0.5 * (xf[u, M:] + (-1 if odd else 1) * xf[u, M::-1])
+        xf[u, M:] = xh[u]
+        xf[u, :M + 1] = ((-1 if odd else 1) * xh[u])[::-1]
+    Ff = of.residual(xf).reshape(xf.shape[0], -1)[:, M:]
+    Fh = oh.residual(xh).reshape(xh.shape[0], -1)
+    assert np.abs(Fh - Ff).max() <= 1e-13 * np.abs(Ff).max(), np.abs(Fh - Ff).max()
+    # the hard-surface weights are in play: full stencil weights at the surface node change the rows
+    o0 = Oracle.from_spec(dataclasses.replace(half, hw_mask=None))
+    assert np.abs(o0.residual(xh).reshape(xh.shape[0], -1) - Fh).max() > 1e-3

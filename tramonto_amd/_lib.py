@@ -32,7 +32,8 @@ class TmiInfo(ct.Structure):
     _fields_ = [
         ("nunk", ct.c_int), ("nnodes", ct.c_int), ("iu_rho", ct.c_int), ("iu_rb", ct.c_int), ("n_rb", ct.c_int),
         ("iu_mf", ct.c_int), ("n_mf", ct.c_int), ("sten_len", (ct.c_int * (NC * NC)) * 3), ("mf_fft", ct.c_int),
-        ("fmt_fft", ct.c_int), ("precond", ct.c_int), ("fft_fold", ct.c_int),
+        ("fmt_fft", ct.c_int), ("precond", ct.c_int), ("fft_fold", ct.c_int), ("fft_xreg", ct.c_int),
+        ("fft_regpass", ct.c_int),
     ]
 
 
@@ -79,10 +80,14 @@ DECLARED_SYMBOLS = [
     "tmi_last_error", "tmi_version", "tmi_problem_create", "tmi_problem_destroy", "tmi_problem_info",
     "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_set_hard_surface_masks", "tmi_compute_force", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
-    "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning",
+    "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning", "tmi_row_tuning_set",
     "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_compute_observables",
-    "fill_resid_and_matrix_control",
-    "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
+    "fill_resid_and_matrix_control", "tmi_linprobmgr_last_solve",
+    "dft_parameterlist_create", "dft_parameterlist_destruct", "dft_parameterlist_set_int", "dft_parameterlist_set_double",
+    "dft_parameterlist_set_int_array", "dft_parameterlist_set_double_array", "dft_parameterlist_set_all_aztec",
+    "dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_create_debug",
+    "dft_hardsphere_lin_prob_mgr_destruct", "dft_basic_lin_prob_mgr_create",
+    "dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
     "dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids", "dft_hardsphere_lin_prob_mgr_setdensityequationids",
     "dft_hardsphere_lin_prob_mgr_seta22blockisdiagonal",
     "dft_linprobmgr_destruct", "dft_linprobmgr_setnodalrowmap", "dft_linprobmgr_setnodalcolmap",
@@ -125,6 +130,7 @@ def lib() -> ct.CDLL:
     L.tmi_schur_solve.argtypes = [vp, vp, vp, vp, d, i, i, i, ct.POINTER(i), vp]
     L.tmi_set_active_problem.argtypes = [vp, vp]
     L.tmi_row_tuning.argtypes = [vp, vp, vp]
+    L.tmi_row_tuning_set.argtypes = [vp, i, i, i, i]
     L.tmi_problem_set_rho_b.argtypes = [vp, vp]
     L.tmi_problem_set_vext.argtypes = [vp, vp]
     L.tmi_problem_set_preconditioner.argtypes = [vp, i]
@@ -135,15 +141,29 @@ def lib() -> ct.CDLL:
                                          vp, vp]
     L.fill_resid_and_matrix_control.argtypes = [vp, i, i]
     L.fill_resid_and_matrix_control.restype = d
-    L.dft_hardsphere_lin_prob_mgr_create.argtypes = [i, vp, vp]
-    L.dft_hardsphere_lin_prob_mgr_create.restype = vp
+    for name in ("dft_hardsphere_lin_prob_mgr_create", "dft_hardsphere_lin_prob_mgr_create_debug",
+                 "dft_basic_lin_prob_mgr_create"):
+        getattr(L, name).argtypes = [i, vp, i]   # the communicator is MPICH's int MPI_Comm, never read
+        getattr(L, name).restype = vp
+    for name in ("dft_linprobmgr_destruct", "dft_hardsphere_lin_prob_mgr_destruct", "dft_parameterlist_destruct"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = None
+    L.dft_parameterlist_create.argtypes = []
+    L.dft_parameterlist_create.restype = vp
+    L.dft_parameterlist_set_int.argtypes = [vp, ct.c_char_p, i]
+    L.dft_parameterlist_set_int.restype = None
+    L.dft_parameterlist_set_double.argtypes = [vp, ct.c_char_p, d]
+    L.dft_parameterlist_set_double.restype = None
+    L.dft_parameterlist_set_all_aztec.argtypes = [vp, vp, vp]
+    L.dft_parameterlist_set_all_aztec.restype = None
+    L.tmi_linprobmgr_last_solve.argtypes = [vp, ct.POINTER(i), ct.POINTER(d)]
     for name in ("dft_hardsphere_lin_prob_mgr_setindnonlocalequationids",
                  "dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids",
                  "dft_hardsphere_lin_prob_mgr_setdensityequationids", "dft_linprobmgr_setnodalrowmap",
                  "dft_linprobmgr_setnodalcolmap", "dft_linprobmgr_setcoarsenednodeslist"):
         getattr(L, name).argtypes = [vp, i, vp]
     L.dft_hardsphere_lin_prob_mgr_seta22blockisdiagonal.argtypes = [vp, i]
-    for name in ("dft_linprobmgr_destruct", "dft_linprobmgr_finalizeblockstructure",
+    for name in ("dft_linprobmgr_finalizeblockstructure",
                  "dft_linprobmgr_initializeproblemvalues", "dft_linprobmgr_finalizeproblemvalues",
                  "dft_linprobmgr_setupsolver", "dft_linprobmgr_solve"):
         getattr(L, name).argtypes = [vp]
@@ -154,6 +174,11 @@ def lib() -> ct.CDLL:
     L.dft_linprobmgr_getlhs.argtypes = [vp, vp]
     L.dft_linprobmgr_applymatrix.argtypes = [vp, vp, vp]
     L.dft_linprobmgr_importr2c.argtypes = [vp, vp, vp]
+    L.dft_linprobmgr_insertmultinodematrixvalues.argtypes = [vp, i, i, i, vp, vp, i]
+    L.dft_linprobmgr_insertmultiphysicsmatrixvalues.argtypes = [vp, i, i, vp, i, vp, i]
+    L.dft_linprobmgr_importnodalr2c.argtypes = [vp, vp, vp]
+    L.dft_linprobmgr_importsingleunknownr2c.argtypes = [vp, vp, vp]
+    L.dft_linprobmgr_writeMatrix.argtypes = [vp, ct.c_char_p, ct.c_char_p, ct.c_char_p]
     L.dft_linprobmgr_getmatrixvalue.argtypes = [vp, i, i, i, i]
     L.dft_linprobmgr_getmatrixvalue.restype = d
     _lib = L

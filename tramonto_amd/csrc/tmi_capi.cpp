@@ -818,6 +818,8 @@ int tmi_problem_info(void *handle, tmi_info *info)
         info->fmt_fft = fft_fmt(*P) ? (fft_fused(*P) ? 2 : 1) : 0;
         info->precond = precond_active(*P) ? 1 : 0;
         info->fft_fold = fft_folded(*P) ? 1 : 0;
+        info->fft_xreg = fft_xreg_mask(*P);
+        info->fft_regpass = fft_regpass_mode(*P);
         return 0;
     });
 }
@@ -1210,6 +1212,22 @@ int tmi_row_tuning(void *handle, int *shape, double *ms)
                 }
                 if (ms) ms[prec * 3 + k] = P->use_rows1d ? t.ms : 0.0;
             }
+        return 0;
+    });
+}
+
+int tmi_row_tuning_set(void *handle, int prec, int kind, int per_lane, int lanes)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!P->use_rows1d) throw std::invalid_argument("the problem does not use the 1D row kernels");
+        if (prec < 0 || prec > 1 || kind < 0 || kind > 2) throw std::invalid_argument("prec must be 0..1, kind 0..2");
+        if (per_lane != 4 && per_lane != 8 && per_lane != 16) throw std::invalid_argument("outputs per lane: 4, 8 or 16");
+        if (lanes != 64 && lanes != 128 && lanes != 256) throw std::invalid_argument("lanes per workgroup: 64, 128 or 256");
+        TMI_HIP_CHECK(hipDeviceSynchronize());
+        P->tune[prec][kind].B = per_lane;
+        P->tune[prec][kind].wg = lanes;
+        P->tune[prec][kind].ms = 0.0;
         return 0;
     });
 }

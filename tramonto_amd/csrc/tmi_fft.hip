@@ -136,6 +136,12 @@ struct FftPlan {
     bool pc_ok = false;         // Q built
     int pc_want = -1;           // wanted (TMI_PRECOND=diag: no; -1 until the first set-up reads it)
     bool pc_on = false;         // GMRES uses it: built and wanted
+    // kernel-family switches, read from the environment when the plan is built: regpass 0 keeps the
+    // LDS Stockham pass on strided axes (TMI_FFT_REGPASS=0), 1 uses the one-wave register pass where it
+    // has enough waves, 2 wherever it has radices (TMI_FFT_REGPASS=2); xreg false keeps the LDS form of
+    // the fused x pass (TMI_FFT_XREG=0)
+    int regpass = 1;
+    bool xreg = true;
 };
 
 namespace {
@@ -1255,15 +1261,9 @@ PassShape pass_shape(const FftPlan &F, int axis, int nfield)
     return PassShape{lpb, groups};
 }
 
-// register radices (R1, R2) of the one-wave strided pass for length M, or 0 when M has none
-// (TMI_FFT_REGPASS=0 keeps the LDS Stockham pass everywhere)
+// register radices (R1, R2) of the register passes for length M, or 0 when M has none
 inline int reg_radices(int M, int &R1, int &R2)
 {
-    static const bool off = [] {
-        const char *e = std::getenv("TMI_FFT_REGPASS");
-        return e && std::string(e) == "0";
-    }();
-    if (off) return 0;
     switch (M) {
     case 32: R1 = 8, R2 = 4; return 1;
     case 64: R1 = 8, R2 = 8; return 1;
@@ -1278,7 +1278,8 @@ inline int reg_radices(int M, int &R1, int &R2)
 constexpr long long kRegPassWaves = 1024;
 inline bool use_reg_pass(const FftPlan &F, int axis, int nfield, int &R1, int &R2)
 {
-    if (axis == 0 || !reg_radices(F.M[axis], R1, R2)) return false;
+    if (axis == 0 || F.regpass == 0 || !reg_radices(F.M[axis], R1, R2)) return false;
+    if (F.regpass == 2) return true;
     const int L = 64 / R2;
     return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]) * nfield >= kRegPassWaves;
 }
@@ -1397,17 +1398,19 @@ int xprod_lpb(const FftPlan &F, int nop)
 
 // the register form of the fused x pass where the x length has register radices and the output
 // pairs' spectrum values fit in registers (NOP R1 <= 32); false otherwise
+inline bool xreg_fits(const FftPlan &F, const TermList &tl, int &R1, int &R2)
+{
+    if (!F.xreg || !reg_radices(F.M[0], R1, R2)) return false;
+    const int nop = (tl.nout + 1) / 2;
+    return nop <= 4 && nop * R1 <= 32;
+}
+
 template <typename T>
 bool xprod_reg(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, const T *tk, const T *uk, hipStream_t s)
 {
-    static const bool off = [] {   // TMI_FFT_XREG=0 keeps the LDS form of the fused x pass
-        const char *e = std::getenv("TMI_FFT_XREG");
-        return e && std::string(e) == "0";
-    }();
     int R1 = 0, R2 = 0;
-    if (off || !reg_radices(F.M[0], R1, R2)) return false;
+    if (!xreg_fits(F, tl, R1, R2)) return false;
     const int nop = (tl.nout + 1) / 2;
-    if (nop > 4 || nop * R1 > 32) return false;
     const int L = 64 / R2;
     const long long lines = F.n / F.M[0];
     const dim3 grid((unsigned)((lines + L - 1) / L)), block(2 * L * R2);
@@ -1531,6 +1534,8 @@ void fft_setup(Problem &P, bool mf, bool fmt)
     try {
         F->nd = nd;
         F->C = C;
+        if (const char *e = std::getenv("TMI_FFT_REGPASS")) F->regpass = std::string(e) == "0" ? 0 : (std::string(e) == "2" ? 2 : 1);
+        if (const char *e = std::getenv("TMI_FFT_XREG")) F->xreg = std::string(e) != "0";
         for (int d = 0; d < nd; ++d) {
             const int n = P.p.nx[d], h = P.pad_u.h[d], hf = P.pad_f.h[d];
             const bool per = P.p.bc[d][0] == TMI_PERIODIC;
@@ -1852,6 +1857,16 @@ double *fft_ss_buffer(const Problem &P, long long *cap)
 }
 bool fft_fmt(const Problem &P) { return P.fft && P.fft->fmt; }
 bool fft_fused(const Problem &P) { return P.fft && (P.fft->xlpb[0] || P.fft->xlpb[1]); }
+int fft_xreg_mask(const Problem &P)
+{
+    if (!P.fft || !P.fft->fmt) return 0;
+    const FftPlan &F = *P.fft;
+    int R1 = 0, R2 = 0, m = 0;
+    if (F.xlpb[0] && xreg_fits(F, F.tfwd[1], R1, R2)) m |= 1;
+    if (F.xlpb[1] && xreg_fits(F, F.tel, R1, R2)) m |= 2;
+    return m;
+}
+int fft_regpass_mode(const Problem &P) { return P.fft ? P.fft->regpass : 0; }
 bool fft_fmt_available(const Problem &P) { return P.fft && P.fft->fmt_ok; }
 bool fft_wraps(const Problem &P) { return P.fft && P.fft->wraps; }
 bool fft_folded(const Problem &P) { return P.fft && P.fft->fold; }

@@ -20,10 +20,12 @@ namespace tmi {
 namespace {
 
 // node reached from ijk by off with the boundary rules of offset_to_node_box
-// (src/dft_mesh_lib.c:203-297); -1 bulk, -2 wall
-int node_of(const tmi_params &p, const int *ijk, const int *off)
+// (src/dft_mesh_lib.c:203-297); -1 bulk, -2 wall.  refl (optional) receives the reflect flags as
+// bits, bit d when the point was mirrored in dimension d.
+int node_of(const tmi_params &p, const int *ijk, const int *off, unsigned *refl = nullptr)
 {
     int s[3] = {0, 0, 0};
+    if (refl) *refl = 0;
     for (int d = 0; d < p.ndim; ++d) {
         int v = ijk[d] + off[d];
         const int n = p.nx[d];
@@ -35,6 +37,7 @@ int node_of(const tmi_params &p, const int *ijk, const int *off)
                 v += n;
             } else {
                 v = -v;
+                if (refl) *refl |= 1u << d;
                 if (v >= n) return p.bc[d][1] == TMI_IN_BULK ? -1 : -2;
             }
         } else if (v >= n) {
@@ -45,6 +48,7 @@ int node_of(const tmi_params &p, const int *ijk, const int *off)
                 v -= n;
             } else {
                 v = 2 * (n - 1) - v;
+                if (refl) *refl |= 1u << d;
                 if (v < 0) return p.bc[d][0] == TMI_IN_BULK ? -1 : -2;
             }
         }
@@ -62,6 +66,18 @@ double rb_prefactor(const Problem &P, int q, int c, const int *off)
     const int idim = q < 4 + nd ? q - 4 : q - 4 - nd;
     double fac = q < 4 + nd ? 1.0 : P.bulk.inv_4pir[c];
     return fac * ((double)off[idim] * P.p.dx[idim] * P.bulk.inv_rad[c]);
+}
+
+// HW_boundary_weight (src/dft_utils.c:41-62) of stencil point k at a boundary node with fluid-element
+// bits m: the parts hw[ln] of the local nodes ln whose element is fluid.  The element is
+// node_box_to_elem_box_reflect's (src/dft_mesh_lib.c:896-937): for a point mirrored in dimension d it
+// lies on the mirror side, local node ln with bit d flipped in the node's own numbering.
+double fluid_part(const HostStencil &hs, int k, unsigned m, unsigned flip, int nd)
+{
+    double wf = 0.0;
+    for (int ln = 0; ln < (1 << nd); ++ln)
+        if (m & (1u << (ln ^ flip))) wf += hs.hw[8 * (size_t)k + ln];
+    return wf;
 }
 
 template <typename T>
@@ -177,13 +193,12 @@ void hw_build(Problem &P, const unsigned char *mask)
                 const HostStencil &hs = P.hs[type][c];
                 for (int k = 0; k < hs.L; ++k) {
                     const int *off = &hs.off[3 * k];
-                    const int j = node_of(p, ijk, off);
-                    if (j < 0) continue;
+                    unsigned flip = 0;
+                    const int j = node_of(p, ijk, off, &flip);
+                    if (j < 0 || P.h_zero[(size_t)c * N + j]) continue;   // zero-density columns carry 0
                     const unsigned m = mask[(size_t)c * N + j];
                     if (!m) continue;
-                    double wf = 0.0;
-                    for (int ln = 0; ln < (1 << nd); ++ln)
-                        if (m & (1u << ln)) wf += hs.hw[8 * (size_t)k + ln];
+                    const double wf = fluid_part(hs, k, m, flip, nd);
                     const double v = rb_prefactor(P, q, c, off) * (wf - hs.w[k]);
                     fwd.emplace_back(q * N + i, c * N + j, v);
                     tr.emplace_back(c * N + j, q * N + i, v);
@@ -197,13 +212,12 @@ void hw_build(Problem &P, const unsigned char *mask)
                 const HostStencil &hs = P.hs[TMI_UATT][ic + C * c];
                 for (int k = 0; k < hs.L; ++k) {
                     const int *off = &hs.off[3 * k];
-                    const int j = node_of(p, ijk, off);
-                    if (j < 0) continue;
+                    unsigned flip = 0;
+                    const int j = node_of(p, ijk, off, &flip);
+                    if (j < 0 || P.h_zero[(size_t)c * N + j]) continue;
                     const unsigned m = mask[(size_t)c * N + j];
                     if (!m) continue;
-                    double wf = 0.0;
-                    for (int ln = 0; ln < (1 << nd); ++ln)
-                        if (m & (1u << ln)) wf += hs.hw[8 * (size_t)k + ln];
+                    const double wf = fluid_part(hs, k, m, flip, nd);
                     const double v = wf - hs.w[k];
                     fwd.emplace_back((P.nrb + ic) * N + i, c * N + j, v);
                     tr.emplace_back(c * N + j, (P.nrb + ic) * N + i, v);

@@ -233,6 +233,8 @@ bool fft_folded(const Problem &P);          // spectrum tables stored for k_y, k
 void fft_set_fmt(Problem &P, bool on);
 bool fft_fused_available(const Problem &P);   // the fused x pass fits the LDS budget
 bool fft_fused(const Problem &P);             // products through the fused x pass
+int fft_xreg_mask(const Problem &P);          // fused x pass in registers: bit 0 density, bit 1 Euler-Lagrange product
+int fft_regpass_mode(const Problem &P);       // strided register pass: 0 off, 1 where it has enough waves, 2 always
 void fft_set_fused(Problem &P, int mask);     // fused x pass for the density (bit 0) / Euler-Lagrange (bit 1) product
 // density products; rho_nodes (node arrays, fft_node_input) or, if null, workspace<T>(P).rho_pad
 // ss (optional): partial sums of the squares of the rows written by a fused last pass, *nss of them

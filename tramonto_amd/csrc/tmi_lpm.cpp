@@ -1,25 +1,33 @@
 // The reference's LinProbMgr C ABI (src/dft_basic_lin_prob_mgr_wrapper.h:44-92,
-// src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58) and the fill entry point
+// src/dft_hardsphere_lin_prob_mgr_wrapper.h:43-58), its ParameterList wrapper
+// (src/dft_parameterlist_wrapper.h:40-52) and the fill entry point
 // fill_resid_and_matrix_control (src/dft_fill_control.c:39), backed by the
 // device fill and the matrix-free Schur solve.
 //
-// Call sequence served (src/dft_newton.c:131-196 and
+// Call sequences served (src/dft_newton.c:131-196 and
 // src/NOXLOCA_Tramonto_Group.cpp:176-313):
-//   initializeproblemvalues -> fill_resid_and_matrix_control(x, iter, 0)
-//   -> finalizeproblemvalues -> setupsolver -> solve -> getlhs
-// The fill adds -F to the right-hand side (the reference's insertrhsvalue
-// calls) and, for a full fill, records x as the Jacobian's linearisation
-// point; solve() then runs GMRES on the Schur complement with the device JVP.
+//   Newton:  initializeproblemvalues -> fill_resid_and_matrix_control(x, iter, 0)
+//            -> finalizeproblemvalues -> setupsolver -> solve -> getlhs
+//   NOX computeF:  setrhs(0) -> importr2c -> fill(x, 0, 1) -> getrhs (= -F)
+//   NOX applyJacobianInverse:  setrhs(b) -> setupsolver -> solve -> getlhs
+// A fill adds -F to the right-hand side (the reference's insertrhsvalue
+// calls).  Only a full fill (resid_only_flag 0) records x as the Jacobian's
+// linearisation point, as only that fill changes the reference's matrix;
+// solve(), applymatrix and getmatrixvalue use the Jacobian at that point.
 //
 // Entries inserted through insert*matrixvalue* are kept (summed, as the
 // reference's += semantics) for getmatrixvalue/writeMatrix only; solve()
 // always uses the matrix-free Jacobian of the registered problem.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "tmi_internal.hpp"
@@ -31,21 +39,35 @@ Problem *problem_from_handle(void *h);
 
 namespace {
 
+// The solver options of a Teuchos::ParameterList as the reference's managers read them
+// (src/dft_parameterlist_wrapper.cpp:30-120): named ints and doubles.
+struct ParamList {
+    std::map<std::string, int> ints;
+    std::map<std::string, double> dbls;
+    std::map<std::string, const void *> arrays;   // kept by address, as the reference keeps them
+};
+std::set<ParamList *> g_live_lists;
+
+enum class Kind { HardSphere, Basic };
+
 struct LinProbMgr {
+    Kind kind = Kind::HardSphere;
+    bool debug = false;
     int numUnks = 0;
     std::vector<int> indNonLocal, depNonLocal, density;
+    bool ids_set[3] = {false, false, false};
     bool a22diag = true;
     std::vector<int> rowmap, colmap;
     bool finalized = false;
     Problem *prob = nullptr;
+    ParamList *plist = nullptr;
     std::vector<double> rhs;   // [numUnks][Nowned]
     std::vector<double> lhs;
     std::map<std::pair<long long, long long>, double> inserted;
     double *d_x = nullptr, *d_F = nullptr, *d_dx = nullptr, *d_v = nullptr;
     bool have_x = false;
-    double gmres_tol = 1e-8;
-    int gmres_restart = 100, gmres_maxit = 500;
     int last_iters = 0;
+    double last_debug_resid = -1.0;
 };
 
 LinProbMgr *g_active = nullptr;
@@ -68,9 +90,75 @@ void ensure_device(LinProbMgr &M)
     TMI_HIP_CHECK(hipMalloc((void **)&M.d_v, n * sizeof(double)));
 }
 
-void check_layout(const LinProbMgr &M)
+void free_device(LinProbMgr &M)
 {
-    if (M.numUnks != M.prob->nunk_ext) throw std::logic_error("numUnks differs from the registered problem");
+    (void)hipFree(M.d_x);
+    (void)hipFree(M.d_F);
+    (void)hipFree(M.d_dx);
+    (void)hipFree(M.d_v);
+    M.d_x = M.d_F = M.d_dx = M.d_v = nullptr;
+}
+
+// The block IDs the reference's linsolver_setup_HSTYPE (src/dft_linsolver_setup.c:182-239) registers
+// for this problem: densities; dependent rho-bar rows n1, n0, nv1 when one particle size has hard
+// surfaces (one_particle_size, :197, with Nlists_HW == 2); every other rho-bar and mean-field row
+// independent.  The caller's lists must hold exactly these (in any order) for the Schur blocks of
+// the device solve to be the reference's.
+void check_block_ids(const LinProbMgr &M, const Problem &P)
+{
+    if (M.numUnks != P.nunk_ext)
+        throw std::invalid_argument("numUnks (" + std::to_string(M.numUnks) + ") differs from the problem's " +
+                                    std::to_string(P.nunk_ext) + " unknowns per node");
+    if (M.kind == Kind::Basic) return;   // the basic manager takes no block IDs
+    const int C = P.p.ncomp, D = P.p.ndim;
+    std::set<int> dens, dep, ind;
+    for (int c = 0; c < C; ++c) dens.insert(P.iu_rho + c);
+    for (int q = 0; q < P.nrb; ++q) {
+        const int u = P.iu_rb + q;
+        const bool dependent = P.one_size && ((q >= 2 && q < 4) || q >= 4 + D);
+        (dependent ? dep : ind).insert(u);
+    }
+    for (int u = P.iu_rb + P.nrb; u < P.nunk_ext; ++u) ind.insert(u);   // mean-field rows of the external layout
+    auto same = [](const std::vector<int> &v, const std::set<int> &s) {
+        std::set<int> a(v.begin(), v.end());
+        return a.size() == v.size() && a == s;
+    };
+    if (!same(M.density, dens)) throw std::invalid_argument("density equation IDs differ from the problem's density rows");
+    if (!same(M.depNonLocal, dep))
+        throw std::invalid_argument("dependent nonlocal equation IDs differ from the problem's (n1, n0, nv1 only for one "
+                                    "particle size with hard surfaces)");
+    if (!same(M.indNonLocal, ind))
+        throw std::invalid_argument("independent nonlocal equation IDs differ from the problem's rho-bar/mean-field rows");
+    // seta22blockisdiagonal only selects the reference's A22 preconditioner operator; the solution does
+    // not depend on it, so it is recorded and not checked
+}
+
+// rows and columns of the registered problem: one GPU owns the whole domain (owned == box == N nodes)
+void check_maps(const LinProbMgr &M, const Problem &P)
+{
+    if (M.rowmap.size() != (size_t)P.N || M.colmap.size() != (size_t)P.N)
+        throw std::invalid_argument("nodal row/column maps must list the problem's " + std::to_string(P.N) + " nodes");
+}
+
+// solver parameters: Max_iter and Tol from the ParameterList, read at solve time as the reference's
+// solve() does (src/dft_HardSphereLinProbMgr.cpp:401, src/dft_BasicLinProbMgr.cpp:429); the Krylov
+// space is 500 for the HardSphere manager (setupSolver, :368-371) and the list's Kspace for the
+// basic one.  A missing list or key keeps the defaults (500 iterations, tolerance 1e-8).
+void solver_params(const LinProbMgr &M, double *tol, int *maxit, int *kspace)
+{
+    *tol = 1e-8;
+    *maxit = 500;
+    *kspace = 500;
+    const ParamList *L = (M.plist && g_live_lists.count(M.plist)) ? M.plist : nullptr;
+    if (!L) return;
+    auto it = L->ints.find("Max_iter");
+    if (it != L->ints.end() && it->second > 0) *maxit = it->second;
+    auto dt = L->dbls.find("Tol");
+    if (dt != L->dbls.end() && dt->second > 0.0) *tol = dt->second;
+    if (M.kind == Kind::Basic) {
+        auto kt = L->ints.find("Kspace");
+        if (kt != L->ints.end() && kt->second > 0) *kspace = kt->second;
+    }
 }
 
 // host double** [nunk_ext][N] <-> contiguous device buffer in the internal layout; with attraction in
@@ -95,6 +183,12 @@ int guarded(F &&fn)
         return fn();
     } catch (const HipFailure &) {
         return TMI_EHIP;
+    } catch (const std::invalid_argument &e) {
+        set_error(e.what());
+        return TMI_EINVAL;
+    } catch (const std::domain_error &e) {
+        set_error(e.what());
+        return TMI_EUNSUP;
     } catch (const std::logic_error &e) {
         set_error(e.what());
         return TMI_ESTATE;
@@ -102,6 +196,26 @@ int guarded(F &&fn)
         set_error(e.what());
         return TMI_EINVAL;
     }
+}
+
+LinProbMgr *create(Kind kind, bool debug, int numUnks, void *ParameterList)
+{
+    auto M = new LinProbMgr();
+    M->kind = kind;
+    M->debug = debug;
+    M->numUnks = numUnks;
+    M->plist = (ParamList *)ParameterList;
+    if (kind == Kind::Basic) M->ids_set[0] = M->ids_set[1] = M->ids_set[2] = true;
+    return M;
+}
+
+void destroy(void *h)
+{
+    if (!h) return;
+    LinProbMgr *M = (LinProbMgr *)h;
+    if (g_active == M) g_active = nullptr;
+    free_device(*M);
+    delete M;
 }
 
 }  // namespace
@@ -117,28 +231,41 @@ int tmi_set_active_problem(void *problem_handle, void *linprobmgr_handle)
         Problem *P = problem_handle ? problem_from_handle(problem_handle) : nullptr;
         if (P && P->npores > 0)
             throw std::invalid_argument("a batched problem has no single-problem LinProbMgr counterpart");
+        LinProbMgr *M = (LinProbMgr *)linprobmgr_handle;
+        if (P && M && M->finalized) {
+            check_block_ids(*M, *P);
+            check_maps(*M, *P);
+        }
+        if (M && M->prob != P) {   // buffers are sized for one problem
+            free_device(*M);
+            M->have_x = false;
+        }
         g_active_problem = P;
-        g_active = (LinProbMgr *)linprobmgr_handle;
+        g_active = M;
         if (g_active) g_active->prob = g_active_problem;
         return 0;
     });
 }
 
-// src/dft_fill_control.c:39.  resid_only_flag: 0 full fill (records the
-// linearisation point), 1 residual only, 2/3 compute without inserting.
+// src/dft_fill_control.c:39.  resid_only_flag (src/dft_globals_const.h:422-424): 0 full fill (adds -F
+// to the right-hand side and records x as the linearisation point), 1 and 4 residual only (add -F),
+// 2 and 3 compute without inserting.  Returns sum F^2, -1 on error.
 double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag)
 {
     (void)iter;
     double l2 = -1.0;
     int rc = guarded([&] {
         if (!g_active_problem) throw std::logic_error("no active problem");
+        if (resid_only_flag < 0 || resid_only_flag > 4) throw std::invalid_argument("resid_only_flag must be 0..4");
         Problem &P = *g_active_problem;
         LinProbMgr tmp;
         LinProbMgr &M = g_active ? *g_active : tmp;
         M.prob = &P;
         ensure_device(M);
-        to_device(P, x, M.d_x, false);
-        launch_residual(P, M.d_x, M.d_F, nullptr);
+        // only a full fill moves the linearisation point; other fills evaluate F at x in d_v
+        double *xd = (g_active && resid_only_flag == 0) ? M.d_x : M.d_v;
+        to_device(P, x, xd, false);
+        launch_residual(P, xd, M.d_F, nullptr);
         const size_t n = (size_t)P.nunk_ext * P.N;   // the caller's rows
         std::vector<double> F(n);
         TMI_HIP_CHECK(hipMemcpy(F.data(), M.d_F, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -149,12 +276,7 @@ double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag)
             for (size_t i = 0; i < n; ++i) M.rhs[i] += -F[i];
         }
         if (g_active && resid_only_flag == 0) M.have_x = true;
-        if (!g_active) {
-            (void)hipFree(M.d_x);
-            (void)hipFree(M.d_F);
-            (void)hipFree(M.d_dx);
-            (void)hipFree(M.d_v);
-        }
+        if (!g_active) free_device(M);
         return 0;
     });
     if (rc != 0) {
@@ -164,33 +286,107 @@ double fill_resid_and_matrix_control(double **x, int iter, int resid_only_flag)
     return l2;
 }
 
-void *dft_hardsphere_lin_prob_mgr_create(int numUnks, void *ParameterList, void *comm)
+// ---- ParameterList (src/dft_parameterlist_wrapper.h:40-52)
+void *dft_parameterlist_create(void)
 {
-    (void)ParameterList;
-    (void)comm;
-    auto M = new LinProbMgr();
-    M->numUnks = numUnks;
-    return M;
+    auto L = new ParamList();
+    g_live_lists.insert(L);
+    return L;
 }
+
+void dft_parameterlist_destruct(void *parameterlistptr)
+{
+    auto L = (ParamList *)parameterlistptr;
+    if (!L) return;
+    g_live_lists.erase(L);
+    delete L;
+}
+
+void dft_parameterlist_set_int(void *parameterlistptr, char *name, int value)
+{
+    if (parameterlistptr && name) ((ParamList *)parameterlistptr)->ints[name] = value;
+}
+
+void dft_parameterlist_set_double(void *parameterlistptr, char *name, double value)
+{
+    if (parameterlistptr && name) ((ParamList *)parameterlistptr)->dbls[name] = value;
+}
+
+void dft_parameterlist_set_int_array(void *parameterlistptr, char *name, int *values)
+{
+    if (parameterlistptr && name) ((ParamList *)parameterlistptr)->arrays[name] = values;
+}
+
+void dft_parameterlist_set_double_array(void *parameterlistptr, char *name, double *values)
+{
+    if (parameterlistptr && name) ((ParamList *)parameterlistptr)->arrays[name] = values;
+}
+
+// src/dft_parameterlist_wrapper.cpp:74-108: the Aztec options and params arrays under the names the
+// managers read.  Indices are AztecOO's (az_aztec_defs.h of Trilinos 11.x, the version the reference
+// builds against): options AZ_solver 0 ... AZ_orthog 11, AZ_aux_vec 12, AZ_reorder 13, AZ_keep_info 14,
+// AZ_graph_fill 17, AZ_subdomain_solve 18, AZ_init_guess 19; params AZ_tol 0, AZ_drop 1,
+// AZ_ilut_fill 2, AZ_omega 3, AZ_rthresh 4, AZ_athresh 5.
+void dft_parameterlist_set_all_aztec(void *parameterlistptr, int *options, double *params)
+{
+    auto L = (ParamList *)parameterlistptr;
+    if (!L || !options || !params) return;
+    static const char *opt_names[] = {"Solver", "Scaling", "Precond", "Conv", "Output", "Pre_calc", "Max_iter",
+                                      "Poly_ord", "Overlap", "Type_overlap", "Kspace", "Orthog", "Aux_vec",
+                                      "Reorder", "Keep_info"};
+    for (int i = 0; i < 15; ++i) L->ints[opt_names[i]] = options[i];
+    L->ints["Graph_fill"] = options[17];
+    L->ints["Subdomain_solve"] = options[18];
+    L->ints["Init_guess"] = options[19];
+    static const char *par_names[] = {"Tol", "Drop", "Ilut_fill", "Omega", "Rthresh", "Athresh"};
+    for (int i = 0; i < 6; ++i) L->dbls[par_names[i]] = params[i];
+}
+
+// ---- managers
+void *dft_hardsphere_lin_prob_mgr_create(int numUnks, void *ParameterList, tmi_comm_t comm)
+{
+    (void)comm;
+    return create(Kind::HardSphere, false, numUnks, ParameterList);
+}
+
+void *dft_hardsphere_lin_prob_mgr_create_debug(int numUnks, void *ParameterList, tmi_comm_t comm)
+{
+    (void)comm;
+    return create(Kind::HardSphere, true, numUnks, ParameterList);
+}
+
+void *dft_basic_lin_prob_mgr_create(int numUnks, void *ParameterList, tmi_comm_t comm)
+{
+    (void)comm;
+    return create(Kind::Basic, false, numUnks, ParameterList);
+}
+
+void dft_hardsphere_lin_prob_mgr_destruct(void *h) { destroy(h); }
 
 int dft_hardsphere_lin_prob_mgr_setindnonlocalequationids(void *h, int n, int *ids)
 {
     return guarded([&] {
+        if (n < 0 || (n > 0 && !ids)) throw std::invalid_argument("equation ID list");
         lpm(h)->indNonLocal.assign(ids, ids + n);
+        lpm(h)->ids_set[0] = true;
         return 0;
     });
 }
 int dft_hardsphere_lin_prob_mgr_setdepnonlocalequationids(void *h, int n, int *ids)
 {
     return guarded([&] {
+        if (n < 0 || (n > 0 && !ids)) throw std::invalid_argument("equation ID list");
         lpm(h)->depNonLocal.assign(ids, ids + n);
+        lpm(h)->ids_set[1] = true;
         return 0;
     });
 }
 int dft_hardsphere_lin_prob_mgr_setdensityequationids(void *h, int n, int *ids)
 {
     return guarded([&] {
+        if (n < 0 || (n > 0 && !ids)) throw std::invalid_argument("equation ID list");
         lpm(h)->density.assign(ids, ids + n);
+        lpm(h)->ids_set[2] = true;
         return 0;
     });
 }
@@ -202,23 +398,12 @@ int dft_hardsphere_lin_prob_mgr_seta22blockisdiagonal(void *h, int isA22Diagonal
     });
 }
 
-int dft_linprobmgr_destruct(void *h)
-{
-    return guarded([&] {
-        LinProbMgr *M = lpm(h);
-        if (g_active == M) g_active = nullptr;
-        (void)hipFree(M->d_x);
-        (void)hipFree(M->d_F);
-        (void)hipFree(M->d_dx);
-        (void)hipFree(M->d_v);
-        delete M;
-        return 0;
-    });
-}
+void dft_linprobmgr_destruct(void *h) { destroy(h); }
 
 int dft_linprobmgr_setnodalrowmap(void *h, int n, int *gids)
 {
     return guarded([&] {
+        if (n < 0 || (n > 0 && !gids)) throw std::invalid_argument("node map");
         lpm(h)->rowmap.assign(gids, gids + n);
         return 0;
     });
@@ -226,6 +411,7 @@ int dft_linprobmgr_setnodalrowmap(void *h, int n, int *gids)
 int dft_linprobmgr_setnodalcolmap(void *h, int n, int *gids)
 {
     return guarded([&] {
+        if (n < 0 || (n > 0 && !gids)) throw std::invalid_argument("node map");
         lpm(h)->colmap.assign(gids, gids + n);
         return 0;
     });
@@ -240,13 +426,25 @@ int dft_linprobmgr_setcoarsenednodeslist(void *h, int n, int *gids)
     });
 }
 
-// 0 ok, 1 already finalised, -1 a set call is missing (dft_HardSphereLinProbMgr.cpp:71-90)
+// 0 ok, 1 already finalised, -1 a set call is missing (dft_HardSphereLinProbMgr.cpp:71-90) or, with a
+// problem registered, the block IDs or maps do not match it (tmi_last_error says which)
 int dft_linprobmgr_finalizeblockstructure(void *h)
 {
     LinProbMgr *M = (LinProbMgr *)h;
     if (!M) return -1;
     if (M->finalized) return 1;
-    if (M->rowmap.empty() || M->colmap.empty() || M->indNonLocal.empty() || M->density.empty()) return -1;
+    if (M->rowmap.empty() || M->colmap.empty()) return -1;
+    if (M->kind == Kind::HardSphere &&
+        (!M->ids_set[0] || !M->ids_set[2] || M->indNonLocal.empty() || M->density.empty()))
+        return -1;
+    if (M->prob) {
+        int rc = guarded([&] {
+            check_block_ids(*M, *M->prob);
+            check_maps(*M, *M->prob);
+            return 0;
+        });
+        if (rc) return -1;
+    }
     M->finalized = true;
     return 0;
 }
@@ -284,6 +482,9 @@ int dft_linprobmgr_insertonematrixvalue(void *h, int iunk, int ownednode, int bo
     return guarded([&] {
         LinProbMgr *M = lpm(h);
         long long nown = (long long)M->rowmap.size(), nbox = (long long)M->colmap.size();
+        if (iunk < 0 || iunk >= M->numUnks || boxphysicsID < 0 || boxphysicsID >= M->numUnks || ownednode < 0 ||
+            ownednode >= nown || boxnode < 0 || boxnode >= nbox)
+            throw std::invalid_argument("matrix entry index");
         M->inserted[{(long long)iunk * nown + ownednode, (long long)boxphysicsID * nbox + boxnode}] += value;
         return 0;
     });
@@ -355,27 +556,53 @@ int dft_linprobmgr_setupsolver(void *h)
     return guarded([&] {
         LinProbMgr *M = lpm(h);
         if (!M->prob) throw std::logic_error("no problem registered");
-        check_layout(*M);
+        if (!M->finalized) throw std::logic_error("finalizeblockstructure not called");
+        check_block_ids(*M, *M->prob);
+        check_maps(*M, *M->prob);
         return 0;
     });
 }
 
-// J dx = rhs with the Jacobian at the last full fill
+// J dx = rhs with the Jacobian at the last full fill.  Both managers solve through the device Schur
+// complement: the basic manager's AztecOO solve of the whole system (src/dft_BasicLinProbMgr.cpp:414-440)
+// and the HardSphere manager's Schur GMRES (src/dft_HardSphereLinProbMgr.cpp:392-428) have the same
+// solution, to the solver tolerance.  The debug manager then prints ||J dx - rhs||_2 as the reference's
+// debug solve does (:405-414); it neither writes matrix files nor aborts.
 int dft_linprobmgr_solve(void *h)
 {
     return guarded([&] {
         LinProbMgr *M = lpm(h);
         if (!M->have_x) throw std::logic_error("solve requires a full fill first");
         Problem &P = *M->prob;
+        check_maps(*M, P);
         ensure_device(*M);
         const size_t n = (size_t)P.nunk_ext * P.N, ni = (size_t)P.nunk * P.N;
+        if (M->rhs.size() != n) throw std::logic_error("right-hand side not set (fill or setrhs first)");
+        double tol;
+        int maxit, kspace;
+        solver_params(*M, &tol, &maxit, &kspace);
+        // keep the Krylov basis within 32 GiB of HBM
+        const double nS = (double)P.p.ncomp * P.N;
+        kspace = std::max(1, std::min(kspace, (int)std::min(500.0, 32.0 * 1073741824.0 / (8.0 * nS) - 1.0)));
         std::vector<double> negb(ni, 0.0);   // internal mean-field rows (if any) stay 0: x_mf = u * rho holds
         for (size_t i = 0; i < n; ++i) negb[i] = -M->rhs[i];   // F such that rhs = -F
         TMI_HIP_CHECK(hipMemcpy(M->d_F, negb.data(), ni * sizeof(double), hipMemcpyHostToDevice));
-        schur_solve_export(P, M->d_x, M->d_F, M->d_dx, M->gmres_tol, M->gmres_restart, M->gmres_maxit, TMI_PREC_FP64,
-                           &M->last_iters, nullptr);
+        schur_solve_export(P, M->d_x, M->d_F, M->d_dx, tol, kspace, maxit, TMI_PREC_FP64, &M->last_iters, nullptr);
         M->lhs.assign(n, 0.0);
         TMI_HIP_CHECK(hipMemcpy(M->lhs.data(), M->d_dx, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (M->debug) {
+            // internal-layout direction: its mean-field rows are u * d rho, as to_device forms them
+            TMI_HIP_CHECK(hipMemcpy(M->d_v, M->d_dx, ni * sizeof(double), hipMemcpyDeviceToDevice));
+            if (P.ext_mf)
+                launch_mf_conv(P, M->d_v + (size_t)P.iu_rho * P.N, true, M->d_v + (size_t)P.iu_mf * P.N, nullptr);
+            launch_jvp<double>(P, M->d_x, M->d_v, M->d_F, nullptr);
+            std::vector<double> Jdx(n);
+            TMI_HIP_CHECK(hipMemcpy(Jdx.data(), M->d_F, n * sizeof(double), hipMemcpyDeviceToHost));
+            double r2 = 0.0;
+            for (size_t i = 0; i < n; ++i) r2 += (Jdx[i] - M->rhs[i]) * (Jdx[i] - M->rhs[i]);
+            M->last_debug_resid = std::sqrt(r2);
+            std::printf("Global Residual for solution = %.17g\n", M->last_debug_resid);
+        }
         return 0;
     });
 }
@@ -412,6 +639,7 @@ int dft_linprobmgr_importnodalr2c(void *h, double *xOwned, double *xBox)
 {
     return guarded([&] {
         LinProbMgr *M = lpm(h);
+        if (M->rowmap.size() != M->colmap.size()) throw std::domain_error("owned and box maps differ");
         if (xBox != xOwned) std::memcpy(xBox, xOwned, sizeof(double) * M->colmap.size());
         return 0;
     });
@@ -430,6 +658,9 @@ double dft_linprobmgr_getmatrixvalue(void *h, int iunk, int ownednode, int junk,
     int rc = guarded([&] {
         LinProbMgr *M = lpm(h);
         long long nown = (long long)M->rowmap.size(), nbox = (long long)M->colmap.size();
+        if (iunk < 0 || iunk >= M->numUnks || junk < 0 || junk >= M->numUnks || ownednode < 0 || ownednode >= nown ||
+            boxnode < 0 || boxnode >= nbox)
+            throw std::invalid_argument("matrix entry index");
         if (!M->inserted.empty()) {
             auto it = M->inserted.find({(long long)iunk * nown + ownednode, (long long)junk * nbox + boxnode});
             val = (it == M->inserted.end()) ? 0.0 : it->second;
@@ -437,9 +668,9 @@ double dft_linprobmgr_getmatrixvalue(void *h, int iunk, int ownednode, int junk,
         }
         if (!M->have_x) throw std::logic_error("getmatrixvalue requires a full fill first");
         Problem &P = *M->prob;
+        check_maps(*M, P);
         const size_t n = (size_t)P.nunk * P.N;
-        if (junk < 0 || junk >= P.nunk_ext || iunk < 0 || iunk >= P.nunk_ext) throw std::invalid_argument("unknown index");
-        TMI_HIP_CHECK(hipMemset(M->d_v, 0, n * sizeof(double)));
+        dev_fill(0.0, M->d_v, (long long)n, nullptr);
         double one = 1.0;
         TMI_HIP_CHECK(hipMemcpy(M->d_v + (size_t)junk * P.N + boxnode, &one, sizeof(double), hipMemcpyHostToDevice));
         if (P.ext_mf) launch_mf_conv(P, M->d_v + (size_t)P.iu_rho * P.N, true, M->d_v + (size_t)P.iu_mf * P.N, nullptr);
@@ -455,6 +686,7 @@ int dft_linprobmgr_writeMatrix(void *h, char *filename, char *matrixName, char *
     return guarded([&] {
         LinProbMgr *M = lpm(h);
         if (!M->have_x) throw std::logic_error("writeMatrix requires a full fill first");
+        if (!filename) throw std::invalid_argument("null file name");
         Problem &P = *M->prob;
         const size_t n = (size_t)P.nunk_ext * P.N, ni = (size_t)P.nunk * P.N;
         if (n > 20000) throw std::domain_error("writeMatrix is limited to small problems");
@@ -466,7 +698,7 @@ int dft_linprobmgr_writeMatrix(void *h, char *filename, char *matrixName, char *
         std::vector<std::vector<std::pair<size_t, double>>> cols(n);
         size_t nnz = 0;
         for (size_t j = 0; j < n; ++j) {
-            TMI_HIP_CHECK(hipMemset(M->d_v, 0, ni * sizeof(double)));
+            dev_fill(0.0, M->d_v, (long long)ni, nullptr);
             double one = 1.0;
             TMI_HIP_CHECK(hipMemcpy(M->d_v + j, &one, sizeof(double), hipMemcpyHostToDevice));
             if (P.ext_mf)
@@ -483,6 +715,17 @@ int dft_linprobmgr_writeMatrix(void *h, char *filename, char *matrixName, char *
         for (size_t j = 0; j < n; ++j)
             for (auto &e : cols[j]) std::fprintf(f, "%zu %zu %.17g\n", e.first + 1, j + 1, e.second);
         std::fclose(f);
+        return 0;
+    });
+}
+
+// number of GMRES iterations and, for the debug manager, ||J dx - rhs||_2 of the last solve
+int tmi_linprobmgr_last_solve(void *h, int *iters, double *debug_resid)
+{
+    return guarded([&] {
+        LinProbMgr *M = lpm(h);
+        if (iters) *iters = M->last_iters;
+        if (debug_resid) *debug_resid = M->last_debug_resid;
         return 0;
     });
 }

@@ -99,6 +99,12 @@ class DeviceProblem:
                           for i, k in enumerate(("rb_rows", "mf_rows", "el_rows"))}
         return out
 
+    def set_row_tuning(self, per_lane: int, lanes: int, prec: Optional[int] = None, kind: Optional[int] = None):
+        """Forces a launch shape of the 1D row kernels (all precisions / kinds when None)."""
+        for p in ((0, 1) if prec is None else (prec,)):
+            for k in ((0, 1, 2) if kind is None else (kind,)):
+                _lib.check(_lib.lib().tmi_row_tuning_set(self.handle, p, k, per_lane, lanes), "tmi_row_tuning_set")
+
     def set_rho_b(self, rho_b) -> None:
         """New bulk densities, in place (chemical potentials and bulk boundary values follow)."""
         a = np.ascontiguousarray(np.asarray(rho_b, dtype=np.float64)[: self.spec.ncomp])

@@ -171,42 +171,65 @@ def hard_wall_1d(seed: int = 1, size: float = 20.0, dx: float = 0.05, rho_b: flo
                        sigma_ff=np.ones((1, 1)), rho_b=[rho_b], vext=vext, zero_tf=zero, rho0=rho[None, :])
 
 
-def periodic_3d(seed: int = 2, n: int = 64, dx: float = 0.1, rho_b: float = 0.6, cut: float = 2.5,
+def periodic_3d(seed: int = 2, n=64, dx: float = 0.1, rho_b: float = 0.6, cut: float = 2.5,
                 eps: float = 1.0, kT: float = 1.0, func: int = FMT1, modes: int = 8) -> ProblemSpec:
-    """3D periodic Rosenfeld + WCA mean-field fluid (LJ 12-6, eps, cut-off `cut`).
-    Seeded field rho = clip(rho_b + 0.2 f, 0.05, 0.9) with
-    f = (1/modes) sum_m cos(k_m . r + phi_m); each k_m = 2 pi n_m / L is a lattice
-    vector of the periodic box with wavelength in [2, 6] sigma."""
-    L = n * dx
-    if L < 2.0:
+    """3D periodic Rosenfeld + WCA mean-field fluid (LJ 12-6, eps, cut-off `cut`) on n^3 nodes, or
+    on n = (nx, ny, nz) nodes.  Seeded field rho = clip(rho_b + 0.2 f, 0.05, 0.9) with the plain sum
+    f = sum_m cos(k_m . r + phi_m) of `modes` random-phase modes; each k_m = 2 pi (n_m / L) is a
+    lattice vector of the periodic box with wavelength 2 pi / |k_m| in [2, 6] sigma.  With 8 modes
+    f spans [-8, 8], so the clip bounds are reached over much of the box."""
+    ns = [n, n, n] if np.isscalar(n) else [int(v) for v in n]
+    L = np.asarray(ns, dtype=float) * dx
+    if L.max() < 2.0:
         raise ValueError("periodic_3d: the box (n dx) must be at least 2 sigma for modes of wavelength 2-6 sigma")
     rng = SplitMix64(seed)
     ks = []
+    tries = 0
     while len(ks) < modes:
+        tries += 1
+        if tries > 100000:
+            raise ValueError("periodic_3d: no lattice modes of wavelength 2-6 sigma in this box")
         v = np.round(rng.uniform(3, -4.0, 4.0)).astype(int)
-        nrm = float(np.linalg.norm(v))
+        nrm = float(np.linalg.norm(v / L))
         if nrm == 0:
             continue
-        lam = L / nrm
+        lam = 1.0 / nrm
         if 2.0 <= lam <= 6.0:
             ks.append(v)
     phis = rng.uniform(modes, 0.0, 2 * math.pi)
-    g = np.arange(n) * dx
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    gx, gy, gz = (np.arange(m) * dx for m in ns)
+    X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
     f = np.zeros_like(X)
     for v, ph in zip(ks, phis):
         k = 2 * math.pi * np.asarray(v, dtype=float) / L
         f += np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)
-    f /= modes
     rho = np.clip(rho_b + 0.2 * f, 0.05, 0.9)
     # node index = i + j nx + k nx ny: x fastest
     rho = np.transpose(rho, (2, 1, 0)).reshape(-1)
     attr = 0 if cut > 0 else -1
-    return ProblemSpec(name="periodic_3d", ndim=3, ncomp=1, nx=[n, n, n], dx=[dx, dx, dx],
+    return ProblemSpec(name="periodic_3d", ndim=3, ncomp=1, nx=ns, dx=[dx, dx, dx],
                        bc=[[PERIODIC, PERIODIC]] * 3, type_func=func, type_attr=attr, nwall=0,
                        hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.full((1, 1), eps / kT),
                        cut_ff=np.full((1, 1), cut), rho_b=[rho_b], vext=None, zero_tf=None,
                        rho0=rho[None, :])
+
+
+def periodic_mix3(seed: int = 9, n=(128, 12, 12), dx: float = 0.25, sig=(1.0, 0.9, 0.7), eta_b: float = 0.35,
+                  cut_fac: float = 1.2) -> ProblemSpec:
+    """Three-component White Bear + WCA fluid (sigma_i = sig, eps = 1, cut-off cut_fac sigma_i,
+    Lorentz-Berthelot cross terms) on a periodic box: bulk packing fraction eta_b split over the
+    species by seeded Dirichlet weights (as solute_3d), and each species' density the bulk value
+    times (1 + 0.5 f), f the seeded mode sum of periodic_3d scaled into [-1, 1]."""
+    C = len(sig)
+    eta = dirichlet_split(eta_b, C, seed)
+    rho_b = [float(eta[c] / (math.pi * sig[c] ** 3 / 6.0)) for c in range(C)]
+    base = periodic_3d(seed=seed, n=n, dx=dx, rho_b=0.0, cut=1.0)
+    f = np.clip(base.rho0[0] / 0.2, -8.0, 8.0) / 8.0
+    S, E, R = lorentz_berthelot(list(sig), [1.0] * C, [cut_fac * s for s in sig])
+    rho0 = np.stack([rho_b[c] * (1.0 + 0.5 * f) for c in range(C)])
+    return ProblemSpec(name="periodic_mix3", ndim=3, ncomp=C, nx=list(base.nx), dx=[dx] * 3,
+                       bc=[[PERIODIC, PERIODIC]] * 3, type_func=FMT3, type_attr=0, nwall=0, hs_diam=list(sig),
+                       sigma_ff=S, eps_ff=E, cut_ff=R, rho_b=rho_b, rho0=rho0)
 
 
 def slit_pore_1d(H: float, dx: float, sig=(1.0, 0.8), rho_b=(0.2, 0.2), eps_w: float = 1.0,
@@ -327,8 +350,47 @@ def solute_3d(seed: int = 5, n: int = 128, dx: float = 0.1, sig=(1.0, 0.9, 0.7),
                        vext=vext, zero_tf=zero, rho0=rho0)
 
 
+def reflect_slab_1d(half: bool, seed: int = 7, M: int = 60, dx: float = 0.05, a: int = 2,
+                    rho_b: float = 0.6, attraction: bool = True) -> ProblemSpec:
+    """A hard slab of 2a elements centred on node M of a 1D line of 2M + 1 nodes (IN_BULK at both
+    ends), with hard surfaces (Lhard_surf) and a symmetric seeded density; `half` keeps nodes
+    M..2M with a REFLECT boundary at the slab's centre instead.  The two are the same system:
+    every stencil point that crosses the REFLECT plane in the half line lands on the mirror node of
+    the full line, and at the slab's surface node a the fluid element lies on the other side of the
+    mirror image (mask bit 0 in the half line, bit 1 at node M - a of the full line)."""
+    n_full = 2 * M + 1
+    own = np.zeros(n_full - 1, dtype=bool)            # wall elements
+    centre = (np.arange(n_full - 1) + 0.5) - M
+    own[np.abs(centre) < a] = True
+    rng = SplitMix64(seed)
+    r = rho_b * (1.0 + 0.1 * rng.uniform(M + 1, -1.0, 1.0))
+    rho = np.concatenate([r[:0:-1], r])               # symmetric about node M
+    zero = np.zeros(n_full, dtype=np.uint8)
+    mask = np.zeros(n_full, dtype=np.uint8)
+    for n in range(n_full):
+        left = own[n - 1] if n >= 1 else False
+        right = own[n] if n < n_full - 1 else False
+        if left and right:
+            zero[n] = 1
+        elif left != right:
+            mask[n] = 1 if not right else 2           # bit 0: element to the right is fluid
+    rho[zero == 1] = 0.0
+    vext = np.where(zero == 1, 10.0, 0.0)
+    sl = slice(M, None) if half else slice(None)
+    nx = M + 1 if half else n_full
+    bc = [[REFLECT, IN_BULK]] if half else [[IN_BULK, IN_BULK]]
+    return ProblemSpec(name="reflect_slab_1d" + ("_half" if half else ""), ndim=1, ncomp=1, nx=[nx], dx=[dx],
+                       bc=bc, type_func=FMT1, type_attr=0 if attraction else -1, nwall=1, vext_max=10.0,
+                       hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.full((1, 1), 0.8),
+                       cut_ff=np.full((1, 1), 1.5), rho_b=[rho_b], vext=vext[sl, None].copy(),
+                       zero_tf=zero[sl, None].copy(), rho0=rho[None, sl].copy(), lhard_surf=1,
+                       hw_mask=mask[None, sl].copy())
+
+
 def small_case(kind: str, seed: int = 7) -> ProblemSpec:
     """Small state points the oracle evaluates in seconds (parity tests)."""
+    if kind == "1d_reflect_slab":
+        return reflect_slab_1d(True, seed=seed)
     if kind == "1d_hw":
         return hard_wall_1d(seed=seed, size=6.0, dx=0.05)
     if kind == "1d_hw_full":
@@ -398,4 +460,17 @@ def small_case(kind: str, seed: int = 7) -> ProblemSpec:
         return rough_wall_2d(seed=seed, nx=40, ny=32, dx=0.1, cut=1.5, wall_cut=2.0)
     if kind == "3d_solute_small":
         return solute_3d(seed=seed, n=20, dx=0.25, sig=(1.0, 0.8), radius=1.2, cut_fac=1.2, wall_cut=2.0)
+    if kind == "3d_solute3_small":   # the 128^3 solute's three species, small box
+        return solute_3d(seed=seed, n=20, dx=0.25, sig=(1.0, 0.9, 0.7), radius=1.2, cut_fac=1.2, wall_cut=2.0)
+    # periodic boxes whose axis lengths select the register FFT kernels (x 64 / 128, strided y, z)
+    if kind == "3d_x64":
+        return periodic_3d(seed=seed, n=(64, 12, 12), dx=0.25, cut=1.2)
+    if kind == "3d_x128":
+        return periodic_3d(seed=seed, n=(128, 12, 12), dx=0.25, cut=1.2)
+    if kind == "3d_yz64":
+        return periodic_3d(seed=seed, n=(12, 64, 64), dx=0.25, cut=1.2)
+    if kind == "3d_y128_z32":
+        return periodic_3d(seed=seed, n=(12, 128, 32), dx=0.25, cut=1.2, func=FMT3)
+    if kind == "3d_mix3_x128":
+        return periodic_mix3(seed=seed, n=(128, 12, 12), dx=0.25)
     raise KeyError(kind)
