@@ -1,23 +1,23 @@
 """Benchmark of the nonlocal-density fill and Newton-Krylov solve on one MI355X.
 
-Workload: the 3D periodic Rosenfeld + WCA mean-field fluid on a 64^3 grid at
-dx = 0.1 sigma (eps = 1, cut-off 2.5 sigma, kT = 1), seeded smooth density
-field (see tramonto_amd.problem.periodic_3d).  Inputs are resident in HBM
-before the timed region starts.
+Headline workload: the 3D 128^3 grid at dx = 0.1 sigma, three-component White Bear FMT + WCA
+mean-field mixture (sigma 1.0/0.9/0.7, cut-off 3 sigma_i, kT = 1) around a spherical solute of
+radius 4 sigma with an LJ 9-3 surface potential, bulk packing fraction 0.35 split by seeded
+Dirichlet weights (tramonto_amd.problem.solute_3d): 2,097,152 nodes, 16 unknowns per node, the
+largest single-GPU configuration of the benchmark set.  Inputs are resident in HBM before the
+timed region starts.
 
-A step is one FP64 residual fill over the whole grid; `value` is the fill
-throughput in Gnode/s over K timed steps.  The same JSON line carries the
-Jacobian-vector product throughput (FP64 and FP32), the Newton
-time-to-solve to ||F||_2 < 1e-8 for the plain FP64 path and the mixed path
-(FP32 inner Krylov products, FP64 residual/update), the per-stage kernel
-times of one fill measured with HIP events, the roofline of the dominant
-kernel (the FFT axis passes when the stencil sums run in FFT form, else the
-direct stencil sum), and the CPU baseline (the oracle's OpenMP fill on the
-host cores), followed by the batched slit-pore, 2D adjoint and (--solute)
-128^3 solute workloads.
+A step is one FP64 residual fill over the whole grid; `value` is the fill throughput in Gnode/s
+over K timed steps.  The same JSON line carries the Jacobian-vector product throughput (FP64 and
+FP32), the Newton time-to-solve to ||F||_2 < 1e-8 for the plain FP64 path and the mixed path
+(FP32 inner Krylov products, FP64 residual/update), the per-stage kernel times of one fill (HIP
+events), the roofline of the dominant kernel (the fused x pass of the FFT form), and the CPU
+baseline (the oracle's OpenMP fill on the host cores, on a bounded sample of nodes).  Extra keys:
+the 64^3 periodic Rosenfeld + WCA workload, the batched slit pores and the 2D adjoint workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-newton] [--no-cpu]
-                       [--no-batch] [--no-adjoint] [--solute] [--fill-only] [--copy-probe]
+                       [--no-batch] [--no-adjoint] [--no-periodic] [--fill-only] [--copy-probe]
+                       [--workload solute128|periodic64]
 """
 import argparse
 import json
@@ -31,21 +31,12 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 COPY_PROBE_BYTES = 512 << 20
-# HBM bytes per launch of the stencil kernels on the 64^3 workload: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE runs of this script with --copy-probe --no-batch --no-adjoint, scaled by the counters'
-# readings on the 512 MiB copy (x2.0 and x1.0), in MiB (fetch + write).  Algorithmic bytes per launch:
-# el_rows reads the five adjoint fields with their FMT halo (5 x 74^3 x 8 B = 15.5 MiB) and x, writes
-# one row (2 MiB each); rb_rows reads the density copy (11.3 MiB with the attraction halo, of which
-# the FMT reach touches 3.1 MiB) and writes ten rows (20 MiB).
-STAGE_TRAFFIC_MIB_64 = {"el_rows": 33.98 + 2.00, "rb_rows": 26.52 + 20.00}
-STAGE_ALGO_MIB_64 = {"el_rows": 15.5 + 2.0 + 2.0, "rb_rows": 3.1 + 20.0}
-# Mean HBM bytes per launch of the FFT-form kernels of a 64^3 fill (fetch + write, MiB): counter runs of
-# this script with --fill-only --steps 50 --warmup 2 --copy-probe under TMI_FMT_FFT=1, scaled as above,
-# means over the last 120 fused-x-pass and 300 plain-axis-pass dispatches (fills' launches).
-# Algorithmic: 18.9 MiB per fused x pass (pairs and the spectrum tables, stored for k_y, k_z >= 0),
-# 17.6 MiB per plain axis pass on average.
-FFT_TRAFFIC_MIB_64 = 8.885 + 8.800
-XPROD_TRAFFIC_MIB_64 = 21.600 + 8.000
+# Mean HBM bytes per launch of the fused x pass (k_fft_xprod_reg) in a fill, fetch + write in MiB: rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with --fill-only --copy-probe, scaled by the
+# counters' readings on the 512 MiB copy (x2.0 and x1.0).  Keys: workload.
+XPROD_TRAFFIC_MIB = {"periodic64": 21.600 + 8.000, "solute128": None}
+# Mean HBM bytes per launch of the plain FFT axis passes (same runs), MiB
+AXIS_TRAFFIC_MIB = {"periodic64": 8.885 + 8.800, "solute128": None}
 
 
 def stencil_flops_per_node(dp, spec):
@@ -58,20 +49,35 @@ def stencil_flops_per_node(dp, spec):
     return 2.0 * (fwd + adj + mf), 2.0 * fwd, 2.0 * adj, 2.0 * mf
 
 
-def cpu_baseline(spec, x_np):
-    from oracle.oracle import Oracle, lib
+def cpu_baseline(spec, x_np, label, target_s=10.0):
+    """The oracle's OpenMP FP64 fill on the host cores, on a bounded sample: the pointwise FMT
+    derivatives of every node (as the reference's FMT1stDeriv_switch over all box nodes) plus the
+    residual rows of a random node sample sized to about target_s seconds, scaled to the grid."""
+    import numpy as np
+
+    from oracle.oracle import Oracle
     o = Oracle.from_spec(spec)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    reps, t_total = 0, 0.0
-    while t_total < 10.0 and reps < 20:
+    rng = np.random.default_rng(17)
+    N = spec.nnodes
+    t0 = time.perf_counter()
+    o.residual_nodes(x_np, np.array([0], dtype=np.int32))          # derivatives of every node + one row
+    t_deriv = time.perf_counter() - t0
+    n = min(N, 256)
+    while True:
+        nodes = rng.choice(N, n, replace=False).astype(np.int32)
         t0 = time.perf_counter()
-        o.residual(x_np)
-        t_total += time.perf_counter() - t0
-        reps += 1
-    t = t_total / reps
-    return dict(value=spec.nnodes / t / 1e9, unit="Gnode/s", cores=threads, kind="port",
-                sample=f"{reps} FP64 residual fills of the {spec.nx[0]}^3 workload by the OpenMP C oracle "
-                       f"({t * 1e3:.1f} ms each)")
+        o.residual_nodes(x_np, nodes)
+        t = time.perf_counter() - t0
+        rows = max(t - t_deriv, 1e-9)
+        if n == N or t >= 0.5 * target_s:
+            break
+        n = int(min(N, max(2 * n, n * 0.9 * target_s / max(t, 1e-3))))
+    t_full = t_deriv + rows * N / n
+    return dict(value=N / t_full / 1e9, unit="Gnode/s", cores=threads, kind="port",
+                sample=f"{label}: pointwise derivatives of all {N} nodes ({t_deriv:.2f} s) plus the residual rows of "
+                       f"{n} random nodes ({rows:.2f} s) by the OpenMP C oracle, scaled to the grid: "
+                       f"{t_full:.1f} s per FP64 fill")
 
 
 def bench_batched_pores(npores=256):
@@ -190,93 +196,18 @@ def bench_adjoint_2d():
             "dgamma_drho_b": float(theta.grad[1]), "dgamma_dbetamu": float(theta.grad[1]) / dmu}
 
 
-def bench_solute_3d(n=128):
-    """The 3-component White Bear solute workload: mixed-precision and FP64
-    Newton-Krylov to ||F|| < 1e-8."""
+def bench_workload(spec, label, args, newton_tol=1e-8, newton=True):
+    """Fill throughput, per-stage times, JVP throughput, the dominant kernel's roofline and the FP64 and
+    mixed Newton time-to-solve of one workload.  Set-up is outside every timed region."""
     import torch
 
     from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
-    from tramonto_amd.problem import solute_3d
 
-    spec = solute_3d(n=n)
-    dp = DeviceProblem(spec)
-    x0 = dp.initial_state()
-    F = dp.zeros()
-    dp.fill_residual(x0, F)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(3):
-        dp.fill_residual(x0, F)
-    e1.record()
-    torch.cuda.synchronize()
-    fill_ms = e0.elapsed_time(e1) / 3
-    out = {"workload": f"3D {n}^3 at dx = 0.1 sigma, 3-component White Bear FMT + WCA (sigma 1.0/0.9/0.7, "
-                       "cut-off 3 sigma_i), spherical solute radius 4 sigma with LJ 9-3 surface, eta_b = 0.35 "
-                       "split by seeded Dirichlet weights, periodic box",
-           "nodes": spec.nnodes, "unknowns_per_node": spec.nunk, "fill_ms": fill_ms,
-           "fmt_form": ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft],
-           "precond": ["A22 diagonal", "bulk symbol"][dp.info.precond],
-           "fill_gnode_s": spec.nnodes / (fill_ms * 1e-3) / 1e9,
-           "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
-                                     [float(v) for v in dp.fill_residual_timed(x0, 2)]))}
-    sols = {}
-    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
-        x = x0.clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st = dp.newton(x, max_iter=50, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)
-        torch.cuda.synchronize()
-        st["wall_s"] = time.perf_counter() - t0
-        out[name] = st
-        sols[name] = x
-    out["max_profile_diff_mixed_vs_fp64"] = float((sols["mixed"] - sols["fp64"])[: spec.ncomp].abs().max())
-    return out
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--no-newton", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched slit-pore workload")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D rough-wall adjoint workload")
-    ap.add_argument("--solute", action="store_true", help="also run the 128^3 three-component solute workload")
-    ap.add_argument("--fill-only", action="store_true",
-                    help="only the fill throughput, its stages and the roofline (profiling runs)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="first run a plain 512 MiB device copy three times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-    import torch
-
-    from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
-    from tramonto_amd.problem import periodic_3d
-
-    if args.copy_probe:
-        a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
-
-    spec = periodic_3d(seed=2, n=args.n)
     dp = DeviceProblem(spec)
     x0 = dp.initial_state()
     torch.cuda.synchronize()
     N = spec.nnodes
     F = dp.zeros()
-
-    # ---------------------------------------------------------------- fill throughput
     for _ in range(args.warmup):
         dp.fill_residual(x0, F)
     torch.cuda.synchronize()
@@ -289,15 +220,12 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_fill = e0.elapsed_time(e1) / args.steps
-    fill_gnodes = N / (ms_fill * 1e-3) / 1e9
-
-    # per-stage kernel times (HIP events on the launch stream)
-    stages = dp.fill_residual_timed(x0, reps=max(5, args.steps // 5))
-
-    # ---------------------------------------------------------------- JVP throughput
+    out = {"nodes": N, "unknowns_per_node": spec.nunk, "fill_ms": ms_fill, "fill_gnode_s": N / (ms_fill * 1e-3) / 1e9,
+           "wall_s_timed": wall}
+    stages = dp.fill_residual_timed(x0, reps=max(3, args.steps // 5))
+    jvp = {}
     v = torch.rand_like(x0)
     Jv = dp.zeros()
-    jvp = {}
     for fp32 in (() if args.fill_only else (False, True)):
         for _ in range(args.warmup):
             dp.apply_jacobian(x0, v, Jv, fp32=fp32)
@@ -309,118 +237,165 @@ def main():
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.steps
         jvp["fp32" if fp32 else "fp64"] = dict(ms=ms, gnode_s=N / (ms * 1e-3) / 1e9)
-
-    # ---------------------------------------------------------------- roofline of the dominant kernel
+    out["jvp"] = jvp
     flops_node, f_fwd, f_adj, f_mf = stencil_flops_per_node(dp, spec)
-    mf_fft, fmt_fft = bool(dp.info.mf_fft), bool(dp.info.fmt_fft)
     C = spec.ncomp
     bytes_node = 8.0 * (2 * spec.nunk + C) + C
-    if fmt_fft:
-        # every stencil sum is an FFT convolution.  Dominant kernel: the fused x pass (k_fft_xprod: forward x
-        # transform, spectral products, inverse x transform) when it is in use, else the axis passes; both
-        # timed with HIP events around their launches in a fill's order (tmi_fft_timed)
-        stage_names = ["pad_rho", "rb_mf_rows_fft", "mf_rows_direct", "fmt_point", "el_rows_fft"]
+    out["flops_per_node_direct"] = flops_node
+    out["bytes_per_node_residual"] = bytes_node
+    out["fill_hbm_frac"] = (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9)
+    out["mean_field_form"] = "fft" if dp.info.mf_fft else "direct"
+    out["fmt_form"] = ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft]
+    out["fft_xreg"] = dp.info.fft_xreg
+    out["precond"] = ["A22 diagonal", "bulk symbol"][dp.info.precond]
+    out["stencil_lengths"] = {"delta": [dp.info.sten_len[0][c] for c in range(C)],
+                              "theta": [dp.info.sten_len[1][c] for c in range(C)],
+                              "uatt": [dp.info.sten_len[2][q] for q in range(C * C)]}
+    if dp.info.fmt_fft:
+        # every stencil sum is an FFT convolution.  Dominant kernel: the fused x pass (forward x transform,
+        # spectral products, inverse x transform; the longest launch of a fill) when it is in use, else the
+        # axis passes; both timed with HIP events around their launches in a fill's order (tmi_fft_timed)
+        out["fill_stage_ms"] = dict(zip(["pad_rho", "rb_mf_rows_fft", "mf_rows_direct", "fmt_point", "el_rows_fft"],
+                                        [float(v) for v in stages]))
         fused = dp.info.fmt_fft == 2
         reps = max(5, args.steps // 5)
         fp = dp.fft_timed(0, reps)
         ft = dp.fft_timed(1, reps) if fused else fp
         achieved = ft["bytes"] / (ft["ms"] * 1e-3) / 1e9
-        traffic = XPROD_TRAFFIC_MIB_64 if fused else FFT_TRAFFIC_MIB_64
-        roofline = {"bound": "hbm", "kernel": "k_fft_xprod" if fused else "k_fft_axis / k_fft_strided",
-                    "achieved": achieved, "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                    "traffic": (traffic * (1 << 20) if (args.n == 64 and traffic) else None),
-                    "launch_ms": ft["ms"] / ft["launches"], "launches_per_fill": ft["launches"],
-                    "algorithmic_bytes_per_launch": ft["bytes"] / ft["launches"],
-                    "share_of_fill": ft["ms"] / ms_fill,
-                    "traffic_note": ("mean HBM bytes per launch of that kernel in a fill, from rocprofv3 --pmc "
-                                     "FETCH_SIZE/WRITE_SIZE scaled by a 512 MiB copy"),
-                    "note": ("fused x passes of one fill: input pairs and spectrum tables read, output pairs written "
-                             "(algorithmic), HIP events on the launch stream" if fused else
-                             "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"),
-                    "axis_passes": {"ms_per_fill": fp["ms"], "launches_per_fill": fp["launches"],
-                                    "achieved_gb_s": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9,
-                                    "frac": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+        tr = (XPROD_TRAFFIC_MIB if fused else AXIS_TRAFFIC_MIB).get(label)
+        out["roofline"] = {
+            "bound": "hbm", "kernel": "k_fft_xprod_reg" if (fused and dp.info.fft_xreg) else
+            ("k_fft_xprod" if fused else "k_fft_axis / k_fft_strided"),
+            "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+            "traffic": tr * (1 << 20) if tr else None,
+            "launch_ms": ft["ms"] / ft["launches"], "launches_per_fill": ft["launches"],
+            "algorithmic_bytes_per_launch": ft["bytes"] / ft["launches"], "share_of_fill": ft["ms"] / ms_fill,
+            "traffic_note": ("mean HBM bytes per launch of that kernel in a fill, from rocprofv3 --pmc "
+                             "FETCH_SIZE/WRITE_SIZE scaled by a 512 MiB copy"),
+            "note": ("fused x passes of one fill: input pairs and distinct spectrum tables read, output pairs "
+                     "written (algorithmic), HIP events on the launch stream" if fused else
+                     "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"),
+            "axis_passes": {"ms_per_fill": fp["ms"], "launches_per_fill": fp["launches"],
+                            "achieved_gb_s": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9,
+                            "frac": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}}
     else:
-        # direct stencil sums; the FFT form of the mean-field sums is HBM-bound passes, not
-        # direct-form flops, so it is not priced against the FP64 roof (its time is in fill_stage_ms)
-        stage_names = ["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"]
-        per_stage_flops = [0.0, f_fwd, 0.0 if mf_fft else f_mf, 0.0, f_adj]
+        names = ["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"]
+        out["fill_stage_ms"] = dict(zip(names, [float(v) for v in stages]))
+        per_stage_flops = [0.0, f_fwd, 0.0 if dp.info.mf_fft else f_mf, 0.0, f_adj]
         priced = [i for i in range(5) if per_stage_flops[i] > 0]
         dom = max(priced, key=lambda i: stages[i])
-        dom_flops = per_stage_flops[dom] * N
-        achieved = dom_flops / (stages[dom] * 1e-3) / 1e12 if dom_flops > 0 else 0.0
-        roofline = {"bound": "mfma", "kernel": stage_names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                    "traffic": (STAGE_TRAFFIC_MIB_64[stage_names[dom]] * (1 << 20)
-                                if (args.n == 64 and stage_names[dom] in STAGE_TRAFFIC_MIB_64) else None),
-                    "traffic_note": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by a "
-                                     f"512 MiB copy; algorithmic bytes per launch "
-                                     f"{STAGE_ALGO_MIB_64.get(stage_names[dom], float('nan')):.1f} MiB"),
-                    "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
-                            "peak = 78.6 TFLOP/s)"}
-
-    # ---------------------------------------------------------------- Newton time-to-solve
-    newton = {}
-    if not (args.no_newton or args.fill_only):
+        achieved = per_stage_flops[dom] * N / (stages[dom] * 1e-3) / 1e12
+        out["roofline"] = {"bound": "mfma", "kernel": names[dom], "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                           "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                           "note": "direct-form stencil sum, FP64 FMA bound (MI355X FP64 vector peak = FP64 matrix "
+                                   "peak = 78.6 TFLOP/s)"}
+    if newton and not (args.no_newton or args.fill_only):
+        res = {}
+        sols = {}
         for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
             xw = x0.clone()
-            dp.newton(xw, max_iter=1, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)   # allocate scratch
+            dp.newton(xw, max_iter=1, resid_tol=newton_tol, gmres_tol=1e-6, precision=prec)   # scratch, code load
             xs = x0.clone()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            st = dp.newton(xs, max_iter=50, resid_tol=1e-8, gmres_tol=1e-6, precision=prec)
+            st = dp.newton(xs, max_iter=60, resid_tol=newton_tol, gmres_tol=1e-6, precision=prec)
             torch.cuda.synchronize()
             st["wall_s"] = time.perf_counter() - t0
-            newton[name] = st
-            newton[name + "_x"] = xs
-        d = (newton["fp64_x"] - newton["mixed_x"]).abs().max().item()
-        newton["max_profile_diff_mixed_vs_fp64"] = d
-        del newton["fp64_x"], newton["mixed_x"]
+            res[name] = st
+            sols[name] = xs
+        res["mixed_vs_fp64_time_ratio"] = res["fp64"]["wall_s"] / res["mixed"]["wall_s"]
+        res["max_profile_diff_mixed_vs_fp64"] = float((sols["fp64"] - sols["mixed"])[: spec.ncomp].abs().max())
+        out["newton"] = res
+    out["_x0"] = x0
+    return out
 
+
+WORKLOADS = {
+    "solute128": ("3D 128^3 grid at dx = 0.1 sigma, 3-component White Bear FMT + WCA mean field (sigma 1.0/0.9/0.7, "
+                  "eps = 1, cut-off 3 sigma_i, kT = 1), spherical solute of radius 4 sigma with LJ 9-3 surface, "
+                  "eta_b = 0.35 split by seeded Dirichlet weights, periodic box"),
+    "periodic64": ("3D periodic Rosenfeld FMT + WCA mean field, 64^3 grid, dx = 0.1 sigma, cut-off 2.5 sigma, "
+                   "kT = 1, seeded field clip(0.6 + 0.2 sum of 8 random-phase modes of wavelength 2-6 sigma, "
+                   "0.05, 0.9)"),
+}
+
+
+def make_spec(name):
+    from tramonto_amd.problem import periodic_3d, solute_3d
+    return solute_3d() if name == "solute128" else periodic_3d()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="solute128")
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched slit-pore workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D rough-wall adjoint workload")
+    ap.add_argument("--no-periodic", action="store_true", help="skip the 64^3 periodic workload (extra key)")
+    ap.add_argument("--fill-only", action="store_true",
+                    help="only the fill throughput, its stages and the roofline (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 512 MiB device copy three times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_probe:
+        a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    spec = make_spec(args.workload)
+    head = bench_workload(spec, args.workload, args)
+    x0 = head.pop("_x0")
     cpu = None
     if not (args.no_cpu or args.fill_only):
-        cpu = cpu_baseline(spec, x0.cpu().numpy())
+        cpu = cpu_baseline(spec, x0.cpu().numpy(), args.workload)
+    del x0
+    torch.cuda.empty_cache()
 
     extra = {}
+    if args.workload == "solute128" and not (args.no_periodic or args.fill_only):
+        e = bench_workload(make_spec("periodic64"), "periodic64", args)
+        e.pop("_x0")
+        e["workload"] = WORKLOADS["periodic64"]
+        extra["periodic_64"] = e
     if not (args.no_batch or args.fill_only):
         extra["batched_pores"] = bench_batched_pores()
     if not (args.no_adjoint or args.fill_only):
         extra["adjoint_2d"] = bench_adjoint_2d()
-    if args.solute and not args.fill_only:
-        extra["solute_3d"] = bench_solute_3d()
 
     out = {
         "metric": "Newton time-to-solve & fill Gnodes/s, mixed/batched vs plain FP64, res<1e-8",
-        "value": fill_gnodes,
+        "value": head["fill_gnode_s"],
         "unit": "Gnode/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_fill,
+        "ms_per_step": head["fill_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D periodic Rosenfeld FMT + WCA mean field, {args.n}^3 grid, dx=0.1 sigma, "
-                               f"cut-off 2.5 sigma, kT=1, seeded 8-mode density field",
-                   "nodes": N, "unknowns_per_node": spec.nunk,
-                   "stencil_lengths": {"delta": dp.info.sten_len[0][0], "theta": dp.info.sten_len[1][0],
-                                       "uatt": dp.info.sten_len[2][0]}},
-        "wall_s_timed": wall,
-        "fill_stage_ms": dict(zip(stage_names, [float(s) for s in stages])),
-        "jvp": jvp,
-        "newton": newton,
-        "mean_field_form": "fft" if mf_fft else "direct",
-        "fmt_form": ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft],
-        "precond": ["A22 diagonal", "bulk symbol"][dp.info.precond],
-        "flops_per_node_direct": flops_node,
-        "bytes_per_node_residual": bytes_node,
-        "roofline": roofline,
-        "fill_hbm_frac": (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9),
+        "config": {"workload": WORKLOADS[args.workload], "nodes": spec.nnodes, "unknowns_per_node": spec.nunk,
+                   "components": spec.ncomp, "grid": list(spec.nx)},
+        "roofline": head.pop("roofline"),
         "cpu_baseline": cpu,
+        "newton": head.pop("newton", {}),
     }
+    out.update(head)
     out.update(extra)
     print(json.dumps(out, default=float))
 

@@ -70,6 +70,12 @@ def lib() -> ct.CDLL:
         L.tmo_fmt_d2_theta.argtypes = [vp, vp, vp]
         L.tmo_residual.argtypes = [vp, vp, vp]
         L.tmo_residual.restype = d
+        L.tmo_gauss_rule.argtypes = [i, vp, vp]
+        L.tmo_gauss_rule.restype = None
+        L.tmo_set_stencil_rule.argtypes = [i, i, d]
+        L.tmo_set_stencil_rule.restype = None
+        L.tmo_residual_nodes.argtypes = [vp, vp, vp, i, vp]
+        L.tmo_residual_nodes.restype = d
         L.tmo_jacobian_coo.argtypes = [vp, vp, vp, vp, vp, ct.c_long]
         L.tmo_jacobian_coo.restype = ct.c_long
         L.tmo_init_fields.argtypes = [vp, vp]
@@ -153,6 +159,14 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         F = np.zeros_like(x)
         lib().tmo_residual(self.h, _p(x), _p(F))
+        return F
+
+    def residual_nodes(self, x: np.ndarray, nodes: np.ndarray) -> np.ndarray:
+        """F[:, nodes] ([nunk][len(nodes)]) without filling the other nodes' rows."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        F = np.zeros((self.nunk, len(nodes)))
+        lib().tmo_residual_nodes(self.h, _p(x), _p(nodes), len(nodes), _p(F))
         return F
 
     def jacobian_coo(self, x: np.ndarray):

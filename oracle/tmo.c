@@ -297,9 +297,28 @@ static double sten_volume(const tmo_ctx *c, int isten, int i, int j)
     return uattr_volume(c, i, j);
 }
 
+/* the n-point rule on [0, 1] in set_gauss_quad's order (test access) */
+void tmo_gauss_rule(int n, double *gp, double *gw) { set_gauss(n, gp, gw); }
+
+/* Rule overrides for the stencil-pin tests only (tmo_set_stencil_rule): a changed interior Gauss
+ * order, straddling-element sub-point count or 3D delta shell band must move the weights away from
+ * their exact integrals.  Zero keeps the reference's value. */
+static struct {
+    int gauss, npt;
+    double band;
+} g_rule = {0, 0, 0.0};
+
+void tmo_set_stencil_rule(int gauss_order, int npt_boundary, double band_factor)
+{
+    g_rule.gauss = gauss_order;
+    g_rule.npt = npt_boundary;
+    g_rule.band = band_factor;
+}
+
 static int quad_boundary(const tmo_ctx *c, int isten)
 {
     int nd = c->p.ndim;
+    if (g_rule.npt > 0 && isten != TMO_UATT) return g_rule.npt;
     if (isten == TMO_DELTA) return nd == 3 ? 40 : 150;   /* dft_stencil_delta_R.c:50-59 */
     if (isten == TMO_THETA) return nd == 3 ? 20 : 40;    /* dft_stencil_theta_R.c:46-55 */
     return nd == 3 ? 1 : 20;                             /* dft_stencil_theta_uattr.c:99-108 */
@@ -308,6 +327,7 @@ static int quad_boundary(const tmo_ctx *c, int isten)
 static int quad_gauss(int isten, double r)
 {
     if (isten == TMO_UATT) return 1;
+    if (g_rule.gauss > 0) return g_rule.gauss;
     return (r <= 4.000000001) ? 6 : 3;
 }
 
@@ -469,8 +489,9 @@ static void el_on_boundary(const tmo_ctx *c, int isten, double R, int i, int j, 
                     qp[2] = ((double)kg + 0.5) * inv_npt;
                     point[2] = xl[2] + qp[2] * es[2];
                     radius_sq = (point[0] * point[0] + point[1] * point[1] + point[2] * point[2]) * inv_r2;
+                    const double band = g_rule.band > 0.0 ? g_rule.band : 2.0;
                     if (((radius_sq <= 1.00000001) && !delta) ||
-                        (delta && (fabs(radius_sq - 1.0) < 2 * ok_distance))) {
+                        (delta && (fabs(radius_sq - 1.0) < band * ok_distance))) {
                         weight = get_weight(c, isten, i, j, radius_sq, R, ngpu, gpu, gwu);
                         if (fabs(radius_sq - 1.0) <= 1.e-8 && !delta) weight *= 0.5;
                         weight *= (inv_npt * inv_npt * inv_npt) * el_vol;
@@ -1266,6 +1287,35 @@ double tmo_residual(const tmo_ctx *c, const double *x, double *F)
     free(dphi);
     double s = 0.0;
     for (long q = 0; q < (long)c->nunk * N; q++) s += F[q] * F[q];
+    return s;
+}
+
+/* The residual rows of the listed nodes only, F[u][k] for node nodes[k] (compact [nunk][count]):
+ * the same per-node fill after the pointwise derivatives of every node, so a bounded sample of a
+ * large grid costs count/N of the stencil sums.  Returns the sum of their F^2. */
+double tmo_residual_nodes(const tmo_ctx *c, const double *x, const int *nodes, int count, double *F)
+{
+    double *dphi = compute_dphi(c, x);
+    const int N = c->N, nunk = c->nunk;
+    double s = 0.0;
+    /* fill_node writes F[u * N + inode]: give it a full-size scratch and copy the listed columns */
+    double *Fs = (double *)malloc(sizeof(double) * (size_t)nunk * N);
+    if (!Fs) die("out of memory");
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int k = 0; k < count; k++) {
+        const int i = nodes[k];
+        if (i >= 0 && i < N) fill_node(c, x, dphi, i, Fs, NULL);
+    }
+    for (int k = 0; k < count; k++) {
+        const int i = nodes[k];
+        for (int u = 0; u < nunk; u++) {
+            const double f = (i >= 0 && i < N) ? Fs[(size_t)u * N + i] : 0.0;
+            F[(size_t)u * count + k] = f;
+            s += f * f;
+        }
+    }
+    free(Fs);
+    free(dphi);
     return s;
 }
 

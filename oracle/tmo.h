@@ -105,6 +105,10 @@ void tmo_fmt_d2_theta(const tmo_ctx *c, const double *n, double *out);
  * Returns sum of F^2. */
 double tmo_residual(const tmo_ctx *c, const double *x, double *F);
 
+/* residual rows of the listed nodes only: F is compact [nunk][count], F[u][k] the row u of node
+ * nodes[k] (a bounded sample of a large grid).  Returns the sum of their F^2. */
+double tmo_residual_nodes(const tmo_ctx *c, const double *x, const int *nodes, int count, double *F);
+
 /* Jacobian in coordinate form with += semantics already merged; GID = iunk*N + node.
  * First call with rows == NULL to get the count. */
 long tmo_jacobian_coo(const tmo_ctx *c, const double *x, int *rows, int *cols, double *vals, long cap);
@@ -120,6 +124,15 @@ void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, cons
                      double *out);
 
 int tmo_num_threads(void);
+
+/* the n-point Gauss-Legendre rule on [0, 1] the stencils use (n = 1, 3, 6, 12, 20, 40), in
+ * set_gauss_quad's point order (src/dft_quadrature.c:40-159) */
+void tmo_gauss_rule(int n, double *gp, double *gw);
+
+/* test hook: stencils built after this call use interior Gauss order gauss_order, npt_boundary
+ * mid-point sub-points per axis on straddling elements (delta and theta) and a 3D delta shell band
+ * of band_factor * ok_distance; 0 restores the reference's rule (6 or 3; 150/40 or 40/20; 2) */
+void tmo_set_stencil_rule(int gauss_order, int npt_boundary, double band_factor);
 
 #ifdef __cplusplus
 }

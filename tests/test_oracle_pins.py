@@ -16,6 +16,8 @@
 """
 import math
 
+import ctypes as ct
+
 import numpy as np
 import pytest
 from scipy import integrate
@@ -92,14 +94,20 @@ def test_attraction_volume_and_rmin():
     assert np.abs(w - mid).max() < 2e-3 * np.abs(mid).max()
 
 
-def test_gauss_rules_exact():
-    # the 20- and 40-point rules are computed: they must integrate x^k on [0,1] exactly for k < 2n
-    sp = one_comp_1d(dx=0.1, attr=0, cut=1.5)
-    o = Oracle.from_spec(sp)     # builds attraction stencils with 40/20-point rules
-    # indirect check through the 1D attraction weights: the projected kernel is smooth, so a
-    # 40-point rule on each element reproduces adaptive quadrature to 1e-10
-    off, w = o.stencil(2)
-    assert np.isfinite(w).all() and len(w) > 0
+@pytest.mark.parametrize("n", [1, 3, 6, 12, 20, 40])
+def test_gauss_rules_exact(n):
+    """each n-point rule of the stencils (set_gauss_quad, src/dft_quadrature.c:40-159) integrates x^k on
+    [0, 1] exactly for k < 2n and not for k = 2n: Gauss-Legendre of that order.  The 3-point rule keeps
+    the reference's 10-digit abscissa 0.7745966692, hence its 1e-10."""
+    from oracle.oracle import lib
+    gp, gw = np.zeros(n), np.zeros(n)
+    lib().tmo_gauss_rule(n, gp.ctypes.data_as(ct.c_void_p), gw.ctypes.data_as(ct.c_void_p))
+    tol = 1e-10 if n == 3 else 2e-14
+    assert np.all((gp > 0) & (gp < 1)) and np.all(gw > 0)
+    for k in range(2 * n):
+        assert abs(np.dot(gw, gp ** k) - 1.0 / (k + 1)) < tol, (n, k)
+    if n <= 6:   # (beyond, the degree-2n error is below round-off)
+        assert abs(np.dot(gw, gp ** (2 * n)) - 1.0 / (2 * n + 1)) > 1e-8
 
 
 # ------------------------------------------------------------------ FMT physics
@@ -332,3 +340,17 @@ def test_schur_block_a11_is_minus_identity():
     J = o.jacobian_dense(x)
     n2 = sp.ncomp * sp.nnodes
     assert np.array_equal(J[n2:, n2:], -np.eye(J.shape[0] - n2))
+
+
+def test_residual_nodes_equals_full_residual_columns():
+    """tmo_residual_nodes (the bounded-sample fill) gives the listed columns of tmo_residual."""
+    from oracle.oracle import Oracle
+    from tramonto_amd.problem import small_case
+    for case in ("1d_hw", "2d_rough_wall", "3d_solute_small"):
+        sp = small_case(case)
+        o = Oracle.from_spec(sp)
+        x = o.init_fields(sp.initial_x())
+        x[sp.ncomp:] *= 1.01
+        F = o.residual(x).reshape(o.nunk, -1)
+        nodes = np.random.default_rng(3).choice(sp.nnodes, 37, replace=False)
+        assert np.array_equal(o.residual_nodes(x, nodes), F[:, nodes])
