@@ -34,7 +34,7 @@ COPY_PROBE_BYTES = 512 << 20
 # Mean HBM bytes per launch of the fused x pass (k_fft_xprod_reg) in a fill, fetch + write in MiB: rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with --fill-only --copy-probe, scaled by the
 # counters' readings on the 512 MiB copy (x2.0 and x1.0).  Keys: workload.
-XPROD_TRAFFIC_MIB = {"periodic64": 21.600 + 8.000, "solute128": None}
+XPROD_TRAFFIC_MIB = {"periodic64": 21.600 + 8.000, "solute128": 686.4 + 64.0}
 # Mean HBM bytes per launch of the plain FFT axis passes (same runs), MiB
 AXIS_TRAFFIC_MIB = {"periodic64": 8.885 + 8.800, "solute128": None}
 

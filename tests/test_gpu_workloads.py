@@ -185,8 +185,11 @@ def test_config5_residual_matches_oracle_on_sampled_nodes(config5):
     Fg, _ = dp.fill_residual(dev(x))
     Fg = host(Fg)[:, nodes]
     C = spec.ncomp
+    xs = x[:, nodes]
     for a, b in ((0, C), (C, C + spec.nrb), (C + spec.nrb, spec.nunk)):
-        scale = max(np.abs(Fo[a:b]).max(), 1.0)
+        # a rho-bar or mean-field row is x minus a stencil sum of the same size (|u * rho| reaches 14
+        # here): the tolerance is relative to that size
+        scale = max(np.abs(Fo[a:b]).max(), np.abs(xs[a:b]).max(), 1.0)
         assert np.abs(Fg[a:b] - Fo[a:b]).max() < RTOL * scale, (a, b, np.abs(Fg[a:b] - Fo[a:b]).max(), scale)
 
 
