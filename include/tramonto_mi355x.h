@@ -192,6 +192,15 @@ int tmi_compute_observables(void *handle, const double *x_d, const int *nel_hit,
  * right-hand side).  *sumsq (host, may be NULL) receives sum F^2, the value
  * fill_resid_and_matrix_control returns. */
 int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sumsq, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation): tmi_fill_residual with
+ * sum F^2 written to device memory (*sumsq_d, one double) instead of the host, without a host
+ * synchronisation and without any allocation.  Its reduction scratch (the per-workgroup partial
+ * sums of F^2) is a fixed library-owned device array of 131072 doubles (1 MiB) per GPU, held until
+ * the process ends; problems whose fill needs more partial sums are rejected with TMI_EINVAL.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (several streams or host
+ * threads) race.  F and the sum equal tmi_fill_residual's bit for bit. */
+int tmi_fill_residual_noalloc(void *handle, const double *x_d, double *F_d, double *sumsq_d, void *stream);
 /* The residual fill with HIP events recorded on `stream` between its stages;
  * stage_ms[0..4] receive the mean over `reps` fills of: density halo,
  * rho-bar rows, mean-field rows, pointwise FMT + adjoint halo, EL rows. */

@@ -78,7 +78,7 @@ _lib: Optional[ct.CDLL] = None
 # exported symbols declared in include/tramonto_mi355x.h
 DECLARED_SYMBOLS = [
     "tmi_last_error", "tmi_version", "tmi_problem_create", "tmi_problem_destroy", "tmi_problem_info",
-    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
+    "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fill_residual_noalloc", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_set_hard_surface_masks", "tmi_compute_force", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning", "tmi_row_tuning_set",
     "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_compute_observables",
@@ -118,6 +118,7 @@ def lib() -> ct.CDLL:
     L.tmi_host_stencil.argtypes = [ct.POINTER(TmiParams), i, i, i, ct.POINTER(i), vp, vp]
     L.tmi_host_bulk.argtypes = [ct.POINTER(TmiParams), vp]
     L.tmi_fill_residual.argtypes = [vp, vp, vp, ct.POINTER(d), vp]
+    L.tmi_fill_residual_noalloc.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_apply_jacobian.argtypes = [vp, vp, vp, vp, vp]
     L.tmi_fill_residual_timed.argtypes = [vp, vp, vp, i, vp, vp]
     L.tmi_fft_timed.argtypes = [vp, i, i, ct.POINTER(d), ct.POINTER(d), ct.POINTER(i), vp]

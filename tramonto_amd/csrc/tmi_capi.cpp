@@ -473,6 +473,9 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
             P->wsf.rb_part = dev_alloc<float>((size_t)5 * C * N);
         }
         P->d_red = dev_alloc<double>(1024 * 2 + 512 * 1024);
+        if (P->ext_mf)   // the entry points' expansion buffers, so that no fill allocates
+            for (double **b : {&P->d_xi, &P->d_vi, &P->d_fi, &P->d_gi}) *b = dev_alloc<double>((size_t)P->nunk * N);
+        noalloc_scratch();
         autotune_rows1d(*P);
         TMI_HIP_CHECK(hipHostMalloc((void **)&P->h_red, sizeof(double) * 1024, hipHostMallocDefault));
         TMI_HIP_CHECK(hipHostGetDevicePointer((void **)&P->h_red_dev, P->h_red, 0));
@@ -920,6 +923,26 @@ int tmi_fill_residual(void *handle, const double *x_d, double *F_d, double *sums
             *sumsq = residual_sumsq(*P, xi, Fi, s);
         else
             launch_residual(*P, xi, Fi, s);
+        if (P->ext_mf) shrink(*P, Fi, F_d, s);
+        return 0;
+    });
+}
+
+int tmi_fill_residual_noalloc(void *handle, const double *x_d, double *F_d, double *sumsq_d, void *stream)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (!x_d || !F_d || !sumsq_d) throw std::invalid_argument("null device pointer");
+        const long long need = residual_sumsq_partials(*P);
+        if (need > kNoAllocScratchDoubles)
+            throw std::invalid_argument("tmi_fill_residual_noalloc: the fill needs " + std::to_string(need) +
+                                        " partial sums, more than the fixed scratch of " +
+                                        std::to_string(kNoAllocScratchDoubles) + " doubles");
+        double *partials = noalloc_scratch();
+        hipStream_t s = (hipStream_t)stream;
+        const double *xi = expand(*P, x_d, P->d_xi, false, s);   // buffers of the external layout exist from creation
+        double *Fi = P->ext_mf ? P->d_fi : F_d;
+        residual_sumsq_device(*P, xi, Fi, partials, sumsq_d, s);
         if (P->ext_mf) shrink(*P, Fi, F_d, s);
         return 0;
     });

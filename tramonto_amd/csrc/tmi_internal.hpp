@@ -304,6 +304,17 @@ void dev_a22_inverse_diag(Problem &P, const double *x, double *dinv, hipStream_t
 void dev_newton_update(Problem &P, double *x, const double *dx, const tmi_newton_opts &o, double *frac_out,
                        double *unorm_out, hipStream_t s);
 
+// allocation-free fill (tmi_noalloc.hip): the device address of the current GPU's fixed scratch array of
+// kNoAllocScratchDoubles doubles, looked up once per device
+constexpr long long kNoAllocScratchDoubles = 131072;   // 1 MiB
+double *noalloc_scratch();
+// partial sums the fill's Sigma F^2 needs: those of the fused last passes (FFT form), else the blocks
+// of one dot product
+long long residual_sumsq_partials(const Problem &P);
+// F(x) and Sigma F^2 into sumsq_d (device), partial sums in `partials` (at least
+// residual_sumsq_partials(P) doubles); no host synchronisation, no allocation
+void residual_sumsq_device(Problem &P, const double *x, double *F, double *partials, double *sumsq_d, hipStream_t s);
+
 // errors
 void set_error(const std::string &msg);
 #define TMI_HIP_CHECK(expr)                                                                            \

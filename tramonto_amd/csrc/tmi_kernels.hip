@@ -1729,6 +1729,28 @@ void observables_sums(Problem &P, const double *x, const int *nh_d, const int *n
     for (int v = 0; v < kObsVals; ++v) sums[v] = P.h_red[v];
 }
 
+long long residual_sumsq_partials(const Problem &P)
+{
+    long long cap = 0;
+    fft_ss_buffer(P, &cap);
+    return std::max<long long>(cap, 1024);
+}
+
+void residual_sumsq_device(Problem &P, const double *x, double *F, double *partials, double *sumsq_d, hipStream_t s)
+{
+    long long cnt = 0;
+    residual_impl(P, x, F, s, nullptr, P.fft ? partials : nullptr, &cnt);
+    if (cnt > 0) {
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, partials, (int)cnt, 1, sumsq_d, nullptr);
+    } else {   // the blocks and order of dev_dot
+        const long long n = (long long)P.nunk * P.N;
+        const unsigned nb = (unsigned)std::min<long long>(1024, (n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, F, 0LL, 1, F, n, partials);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, partials, (int)nb, 1, sumsq_d, nullptr);
+    }
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
 double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s)
 {
     long long cap = 0, cnt = 0;
