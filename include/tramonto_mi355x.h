@@ -43,9 +43,11 @@ extern "C" {
 #define TMI_PERIODIC 1
 #define TMI_REFLECT 2
 
-/* functionals, Type_func */
-#define TMI_FMT1 0
-#define TMI_FMT3 2
+/* functionals, Type_func (src/dft_globals_const.h:247-250) */
+#define TMI_FMT1 0   /* Rosenfeld */
+#define TMI_FMT2 1   /* Rosenfeld with the (n2 - nv2.nv2/n2)^3 third term */
+#define TMI_FMT3 2   /* White Bear */
+#define TMI_FMT4 3   /* Oleksy and Hansen: White Bear's n3 dependence with FMT2's third term */
 
 /* stencil types, src/dft_globals_const.h:154-156 */
 #define TMI_DELTA 0

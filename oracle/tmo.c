@@ -638,6 +638,7 @@ void tmo_set_hw_masks(tmo_ctx *c, const unsigned char *mask)
 /* FMT physics in n order (n0 n1 n2 n3 nv1[D] nv2[D]).
  * FMT1: dft_physics_FMT1.c:39-75 (Phi, first derivatives), :79-168 (second).
  * FMT3: dft_physics_FMT3.c:39-96, :101-266.
+ * FMT2: dft_physics_FMT2.c:40-250.  FMT4: dft_physics_FMT4.c:38-276.
  * Pinned by: finite differences of Phi (first derivatives) and of the first
  * derivatives (second), bulk pressure = Percus-Yevick (FMT1) and
  * Carnahan-Starling (FMT3), exact hard-sphere limits.                        */
@@ -654,6 +655,16 @@ double tmo_fmt_phi(const tmo_ctx *c, const double *n)
     double p3;
     if (c->p.type_func == TMO_FMT1) {
         p3 = (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) / (24.0 * PI * (1.0 - n[3]) * (1.0 - n[3]));
+    } else if (c->p.type_func == TMO_FMT2) {
+        /* FMT2_energy_density, dft_physics_FMT2.c:40-58; (n2 - nv2.nv2/n2)^3 read as n2^3 where
+           n2 <= 1e-15, as its derivatives do (0/0 in the reference) */
+        const double al = n[2] > 1.e-15 ? n[2] - d22 / n[2] : n[2];
+        p3 = al * al * al / (24.0 * PI * (1.0 - n[3]) * (1.0 - n[3]));
+    } else if (c->p.type_func == TMO_FMT4) {
+        /* FMT4_energy_density, dft_physics_FMT4.c:38-58 (Oleksy and Hansen, Mol. Phys. 104, 2871) */
+        const double f = n[2] > 1.e-15 ? 1.0 - d22 / (n[2] * n[2]) : 1.0;
+        p3 = n[2] * n[2] * n[2] * f * f * f * (n[3] + (1.0 - n[3]) * (1.0 - n[3]) * log(1.0 - n[3])) /
+             (36.0 * PI * n[3] * n[3] * (1.0 - n[3]) * (1.0 - n[3]));
     } else {
         p3 = (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * (n[3] + (1.0 - n[3]) * (1.0 - n[3]) * log(1.0 - n[3])) /
              (36.0 * PI * n[3] * n[3] * (1.0 - n[3]) * (1.0 - n[3]));
@@ -672,6 +683,62 @@ static void dphi_loc(const tmo_ctx *c, const double *n, double d12, double d22, 
         for (int d = 0; d < nd; d++) {
             out[4 + d] = -n[4 + nd + d] * inv[1];
             out[4 + nd + d] = -n[4 + d] * inv[1] - n[2] * n[4 + nd + d] * inv[2] / (4.0 * PI);
+        }
+        return;
+    }
+    if (c->p.type_func == TMO_FMT2) {
+        /* FMT2_1stderiv, dft_physics_FMT2.c:60-100 */
+        out[0] = log(inv[1]);
+        out[1] = n[2] * inv[1];
+        out[2] = n[1] * inv[1];
+        out[3] = n[0] * inv[1] + (n[1] * n[2] - d12) * inv[2];
+        for (int d = 0; d < nd; d++) {
+            out[4 + d] = -n[4 + nd + d] * inv[1];
+            out[4 + nd + d] = -n[4 + d] * inv[1];
+        }
+        double alpha, beta, gamma[3];
+        if (n[2] > 1.e-15) {
+            alpha = n[2] - d22 / n[2];
+            beta = 1.0 + d22 / (n[2] * n[2]);
+            for (int d = 0; d < nd; d++) gamma[d] = n[4 + nd + d] / n[2];
+        } else {
+            alpha = n[2];
+            beta = 1.0;
+            for (int d = 0; d < nd; d++) gamma[d] = 0.0;
+        }
+        const double asq = alpha * alpha, acb = asq * alpha;
+        out[2] += asq * beta * inv[2] / (8.0 * PI);
+        out[3] += acb * inv[3] / (12.0 * PI);
+        for (int d = 0; d < nd; d++) out[4 + nd + d] -= inv[2] * asq * gamma[d] / (4.0 * PI);
+        return;
+    }
+    if (c->p.type_func == TMO_FMT4) {
+        /* FMT4_1stderiv, dft_physics_FMT4.c:60-96; the n2 <= 1e-15 guard of fac is this oracle's (the
+           reference divides by n2^2 there) */
+        out[0] = log(inv[1]);
+        out[1] = n[2] * inv[1];
+        double fac = 0.0, fac2 = 0.0;
+        if (n[3] > 1.e-10) {
+            fac = n[2] > 1.e-15 ? 1 - d22 / (n[2] * n[2]) : 1.0;
+            const double fac1 = n[3] - 2. * (1. - n[3]) * log(1. - n[3]);
+            fac2 = (n[3] + (1. - n[3]) * (1. - n[3]) * log(1. - n[3]));
+            out[2] = n[1] * inv[1] + (n[2] * n[2] * fac * fac * fac + 2 * fac * fac * d22) * inv[2] * fac2 /
+                                         (12.0 * PI * n[3] * n[3]);
+            out[3] = n[0] * inv[1] + (n[1] * n[2] - d12) * inv[2] +
+                     fac1 * n[2] * n[2] * n[2] * fac * fac * fac * inv[2] / (36.0 * PI * n[3] * n[3]) +
+                     fac2 * n[2] * n[2] * n[2] * fac * fac * fac *
+                         (-inv[2] / (n[3] * n[3] * n[3]) + inv[3] / (n[3] * n[3])) / (18.0 * PI);
+        } else {
+            out[2] = 0.0;
+            out[3] = 0.0;
+        }
+        for (int d = 0; d < nd; d++) {
+            const int i1 = 4 + d, i2 = 4 + nd + d;
+            out[i1] = -n[i2] * inv[1];
+            if (n[3] > 1.e-10)
+                out[i2] = -n[i1] * inv[1] - n[2] * n[i2] * fac2 * inv[2] * fac * fac / (6.0 * PI * n[3] * n[3]);
+            else
+                out[i2] = 0.0;
         }
         return;
     }
@@ -749,6 +816,99 @@ void tmo_fmt_d2_delta(const tmo_ctx *c, const double *n, const int *offset, cons
             out[4 + nd + d] = sign[d] * (-n2v[d] * i4pi * a2 - (-i4pir * a - n[2] * i4pi * a2) * vec[d]);
         return;
     }
+    if (c->p.type_func == TMO_FMT2) {
+        /* d2phi_drb2_delta_rb_FMT2, dft_physics_FMT2.c:106-184 */
+        double n1v[3], n2v[3], gam[3], d22 = 0.0, dg = 0.0, alpha, beta, eps;
+        for (int d = 0; d < nd; d++) {
+            n2v[d] = sign[d] * n[4 + nd + d];
+            n1v[d] = sign[d] * n[4 + d];
+            d22 += n2v[d] * n2v[d];
+        }
+        if (n[2] > 1.e-15) {
+            alpha = n[2] - d22 / n[2];
+            beta = 1.0 + d22 / (n[2] * n[2]);
+            for (int d = 0; d < nd; d++) {
+                gam[d] = n2v[d] / n[2];
+                dg += gam[d] * gam[d];
+            }
+            eps = alpha / n[2];
+        } else {
+            alpha = n[2];
+            beta = 1.0;
+            for (int d = 0; d < nd; d++) gam[d] = 0.0;
+            eps = 1.0;
+        }
+        const double asq = alpha * alpha;
+        const double a = 1.0 / (1.0 - n[3]), a2 = a * a, a3 = a2 * a;
+        double S2 = a * i4pir + i4pi * a2 * alpha * (beta * beta - eps * dg);
+        for (int d = 0; d < nd; d++) S2 -= i4pi * a2 * alpha * (eps - 2 * beta) * gam[d] * vec[d];
+        double S3 = (i4pirsq * a + n[2] * i4pir * a2 + n[1] * a2);
+        for (int d = 0; d < nd; d++) S3 += a2 * vec[d] * (n2v[d] * i4pir + n1v[d]);
+        S3 += i4pi * a3 * asq * beta;
+        for (int d = 0; d < nd; d++) S3 += 2.0 * i4pi * a3 * asq * gam[d] * vec[d];
+        out[0] = 0.0;
+        out[1] = a;
+        out[2] = S2;
+        out[3] = S3;
+        for (int d = 0; d < nd; d++) {
+            out[4 + d] = sign[d] * a * vec[d];
+            out[4 + nd + d] = sign[d] * i4pir * a * vec[d];
+        }
+        for (int d = 0; d < nd; d++) {
+            out[4 + nd + d] += sign[d] * i4pi * a2 * alpha * (eps - 2.0 * beta) * gam[d];
+            out[4 + nd + d] -= sign[d] * i4pi * a2 * alpha * (4.0 * dg - eps) * vec[d];
+        }
+        return;
+    }
+    if (c->p.type_func == TMO_FMT4) {
+        /* d2phi_drb2_delta_rb_FMT4, dft_physics_FMT4.c:101-191 (the vector densities enter unsigned
+           except through sign[] on V1 and V2, as there) */
+        double d22 = 0.0, d12 = 0.0;
+        for (int d = 0; d < nd; d++) {
+            d22 += n[4 + nd + d] * n[4 + nd + d];
+            d12 += n[4 + d] * n[4 + nd + d];
+        }
+        double inv[5];
+        inv[0] = (1. - n[3]);
+        inv[1] = 1.0 / (1.0 - n[3]);
+        inv[2] = inv[1] * inv[1];
+        inv[3] = inv[2] * inv[1];
+        inv[4] = inv[3] * inv[1];
+        const double n3sq = n[3] * n[3], n3cb = n3sq * n[3], n2sq = n[2] * n[2], n2cb = n2sq * n[2];
+        if (n[3] > 1.e-10) {
+            const double fac1 = n[3] - 2 * (1. - n[3]) * log(1. - n[3]);
+            const double fac2 = n[3] + (1. - n[3]) * (1. - n[3]) * log(1. - n[3]);
+            const double fac3 = -(inv[2] / n3cb) + (inv[3] / n3sq);
+            const double fac = n[2] > 1.e-15 ? 1 - d22 / (n[2] * n[2]) : 1.0;
+            const double i2 = n[2] > 1.e-15 ? 1.0 / n[2] : 0.0;   /* 1/n2, 1/n2^2, 1/n2^3 of the terms in d22 */
+            double S2 = (inv[1] * i4pir +
+                         fac2 * inv[2] * (n[2] * fac * fac * fac + 3 * d22 * fac * fac * i2 + 4 * d22 * d22 * fac * i2 * i2 * i2) /
+                             (6. * PI * n3sq));
+            for (int d = 0; d < nd; d++)
+                S2 += vec[d] * n[4 + nd + d] * (fac2 * inv[2] / (6. * PI * n3sq)) * (fac * fac + 4 * d22 * fac * i2 * i2);
+            double S3 = (i4pirsq * inv[1] + i4pir * n[2] * inv[2] + n[1] * inv[2] +
+                         (n2sq * fac * fac * fac + 2 * d22 * fac * fac) * ((2. * fac2 * fac3) + (fac1 * inv[2] / n3sq)) / (12. * PI));
+            for (int d = 0; d < nd; d++)
+                S3 += vec[d] * (n[4 + nd + d] * i4pir * inv[2] + n[4 + d] * inv[2] +
+                                n[2] * n[4 + nd + d] * fac * fac * ((2. * fac2 * fac3) + (fac1 * inv[2] / n3sq)) / (6. * PI));
+            out[0] = 0.0;
+            out[1] = inv[1];
+            out[2] = S2;
+            out[3] = S3;
+            for (int d = 0; d < nd; d++) out[4 + d] = sign[d] * (inv[1]) * vec[d];
+            double dnw = 0.0;
+            for (int d = 0; d < nd; d++) dnw += n[4 + nd + d] * vec[d];
+            for (int d = 0; d < nd; d++)
+                out[4 + nd + d] = sign[d] * (-(n[4 + nd + d] * inv[2] * fac2 / (6. * PI * n3sq)) *
+                                                 (fac * fac - 4 * d22 * fac * i2 + 4 * dnw * fac * i2) -
+                                             (-i4pir * inv[1] - n[2] * fac * fac * fac2 * inv[2] / (6. * PI * n3sq)) * vec[d]);
+        } else {
+            for (int q = 0; q < 4 + 2 * nd; q++) out[q] = 0.0;
+        }
+        (void)n2cb;
+        (void)d12;
+        return;
+    }
     /* FMT3 */
     double d22 = 0.0, d12 = 0.0;
     for (int d = 0; d < nd; d++) {
@@ -803,6 +963,68 @@ void tmo_fmt_d2_theta(const tmo_ctx *c, const double *n, double *out)
         out[3] = S3;
         for (int d = 0; d < nd; d++) out[4 + d] = -n[4 + nd + d] * a2;
         for (int d = 0; d < nd; d++) out[4 + nd + d] = -(n[4 + d] * a2 + n[2] * n[4 + nd + d] * a3 / (2.0 * PI));
+        return;
+    }
+    if (c->p.type_func == TMO_FMT2) {
+        /* d2phi_drb2_theta_rb_FMT2, dft_physics_FMT2.c:189-250 */
+        double d22 = 0.0, d12 = 0.0, alpha, beta, gam[3];
+        for (int d = 0; d < nd; d++) {
+            d22 += n[4 + nd + d] * n[4 + nd + d];
+            d12 += n[4 + d] * n[4 + nd + d];
+        }
+        if (n[2] > 1.e-15) {
+            alpha = n[2] - d22 / n[2];
+            beta = 1.0 + d22 / (n[2] * n[2]);
+            for (int d = 0; d < nd; d++) gam[d] = n[4 + nd + d] / n[2];
+        } else {
+            alpha = n[2];
+            beta = 1.0;
+            for (int d = 0; d < nd; d++) gam[d] = 0.0;
+        }
+        const double a = 1.0 / (1.0 - n[3]), a2 = a * a, a3 = a2 * a, a4 = a3 * a;
+        const double asq = alpha * alpha, acb = asq * alpha;
+        out[2] = n[1] * a2 + i4pi * a3 * asq * beta;
+        out[3] = (n[0] * a2 + 2.0 * (n[1] * n[2] - d12) * a3) + i4pi * a4 * acb;
+        out[0] = a;
+        out[1] = n[2] * a2;
+        for (int d = 0; d < nd; d++) {
+            out[4 + d] = -n[4 + nd + d] * a2;
+            out[4 + nd + d] = -n[4 + d] * a2 - 2.0 * i4pi * a3 * asq * gam[d];
+        }
+        return;
+    }
+    if (c->p.type_func == TMO_FMT4) {
+        /* d2phi_drb2_theta_rb_FMT4, dft_physics_FMT4.c:196-276 */
+        double d22 = 0.0, d12 = 0.0;
+        for (int d = 0; d < nd; d++) {
+            d22 += n[4 + nd + d] * n[4 + nd + d];
+            d12 += n[4 + d] * n[4 + nd + d];
+        }
+        double inv[5];
+        inv[0] = (1. - n[3]);
+        inv[1] = 1.0 / (1.0 - n[3]);
+        inv[2] = inv[1] * inv[1];
+        inv[3] = inv[2] * inv[1];
+        inv[4] = inv[3] * inv[1];
+        const double n3sq = n[3] * n[3], n3cb = n3sq * n[3], n3_4 = n3cb * n[3], n2sq = n[2] * n[2], n2cb = n2sq * n[2];
+        if (n[3] > 1.e-10) {
+            const double fac1 = n[3] - 2 * (1. - n[3]) * log(1. - n[3]);
+            const double fac2 = n[3] + (1. - n[3]) * (1. - n[3]) * log(1. - n[3]);
+            const double fac3 = -(inv[2] / n3cb) + (inv[3] / n3sq);
+            const double fac = n[2] > 1.e-15 ? 1 - d22 / (n[2] * n[2]) : 1.0;
+            out[2] = (n[1] * inv[2] + (n2sq * fac * fac * fac + 2 * fac * fac * d22) * (fac1 * inv[2] / n3sq + 2. * fac2 * fac3) / (12 * PI));
+            out[3] = (n[0] * inv[2] + 2 * (n[1] * n[2] - d12) * inv[3] +
+                      ((n2cb * fac * fac * fac) / (36. * PI)) *
+                          (+(((2. * log(1. - n[3]) + 3) * inv[2]) / n3sq) + (4. * fac1 * fac3) +
+                           (fac2 * (6. * inv[2] / n3_4 + 6. * inv[4] / n3sq - 8. * inv[3] / n3cb))));
+            out[0] = inv[1];
+            out[1] = n[2] * inv[2];
+            for (int d = 0; d < nd; d++) out[4 + d] = -n[4 + nd + d] * inv[2];
+            for (int d = 0; d < nd; d++)
+                out[4 + nd + d] = -(n[4 + d] * inv[2] + n[2] * n[4 + nd + d] * fac * fac * ((fac1 * inv[2] / n3sq) + (2. * fac2 * fac3)) / (6. * PI));
+        } else {
+            for (int q = 0; q < 4 + 2 * nd; q++) out[q] = 0.0;
+        }
         return;
     }
     double d22 = 0.0, d12 = 0.0;
@@ -918,7 +1140,7 @@ tmo_ctx *tmo_create(const tmo_params *p, const double *vext, const unsigned char
 static tmo_ctx *create_impl(const tmo_params *p, const double *vext, const unsigned char *zero_tf)
 {
 
-    if (p->type_func != TMO_FMT1 && p->type_func != TMO_FMT3) die("type_func");
+    if (p->type_func < TMO_FMT1 || p->type_func > TMO_FMT4) die("type_func");
     if (p->ncomp < 1 || p->ncomp > NC || p->ndim < 1 || p->ndim > 3) die("dims");
     tmo_ctx *c = (tmo_ctx *)calloc(1, sizeof(tmo_ctx));
     g_pending = c;
@@ -1747,7 +1969,7 @@ void tmo_observables(const tmo_ctx *c, const double *x, const int *nel_hit, cons
         sv_to_n(nd, rb, n);
         /* White Bear's third term is 0/0 at n3 = 0 (the reference would add a NaN): taken as 0 for
            n3 <= 1e-10, the derivatives' cut-off (dft_physics_FMT3.c:68) */
-        if (c->p.type_func != TMO_FMT1 && !(n[3] > 1.e-10)) {
+        if ((c->p.type_func == TMO_FMT3 || c->p.type_func == TMO_FMT4) && !(n[3] > 1.e-10)) {
             double d12 = 0.0;
             for (int d = 0; d < nd; d++) d12 += n[4 + d] * n[4 + nd + d];
             hs += (-n[0] * log(1.0 - n[3]) + (n[1] * n[2] - d12) / (1.0 - n[3])) * nel_hit[node];

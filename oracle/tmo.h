@@ -28,6 +28,7 @@ extern "C" {
 #define TMO_FMT1 0
 #define TMO_FMT2 1
 #define TMO_FMT3 2
+#define TMO_FMT4 3
 
 /* stencil types, dft_globals_const.h:154-156 */
 #define TMO_DELTA 0
@@ -39,7 +40,7 @@ typedef struct {
     int nx[3];
     double dx[3];
     int bc[3][2];
-    int type_func;      /* TMO_FMT1 / TMO_FMT3 */
+    int type_func;      /* TMO_FMT1 .. TMO_FMT4 */
     int type_attr;      /* -1: none; 0: strict MF, u = u(r_min) for r < r_min */
     int nwall;          /* > 0: external field added to the Euler-Lagrange rows */
     int lhard_surf;     /* hard surfaces: boundary-node weights from tmo_set_hw_masks */

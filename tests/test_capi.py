@@ -96,7 +96,11 @@ def test_rejects_unsupported_problems():
         p = _lib.fill_params(_lib.TmiParams(), d)
         assert _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None) == 0
     d["lhard_surf"] = 0
-    d["type_func"] = 1   # FMT2
+    for func in (1, 3):   # FMT2, FMT4
+        d["type_func"] = func
+        p = _lib.fill_params(_lib.TmiParams(), d)
+        assert _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None) == 0
+    d["type_func"] = 4   # no such functional
     p = _lib.fill_params(_lib.TmiParams(), d)
     rc = _lib.lib().tmi_host_stencil(ct.byref(p), 0, 0, 0, ct.byref(n), None, None)
     assert rc == -2

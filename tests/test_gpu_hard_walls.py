@@ -56,12 +56,33 @@ def test_residual_and_products_match_oracle(name):
     J = o.jacobian_dense(x)
     rng = np.random.default_rng(7)
     v = rng.uniform(-1.0, 1.0, x.shape)
+    w = rng.uniform(-1.0, 1.0, x.shape)
+    if spec.type_func in (1, 3):
+        # FMT2/FMT4: the reference's delta-stencil Hessian (src/dft_physics_FMT4.c:101-191) differs from
+        # the exact derivative in its nv2 columns; the device product is exact.  Directions without nv2
+        # components match the assembled J entry for entry, and the full product matches central
+        # differences of F.
+        C, D = spec.ncomp, spec.ndim
+        nv2 = [C + 4 + d for d in range(D)]
+        v[nv2] = 0.0
+        h = 1e-6
+        u = rng.uniform(-1.0, 1.0, x.shape) * np.abs(x) * 0.1
+        Fp, _ = dp.fill_residual(dev(x + h * u))
+        Fm, _ = dp.fill_residual(dev(x - h * u))
+        fd = ((Fp - Fm) / (2 * h)).cpu().numpy().ravel()
+        Ju = dp.apply_jacobian(dev(x), dev(u)).cpu().numpy().ravel()
+        assert np.abs(Ju - fd).max() <= 1e-6 * np.abs(fd).max()
     Jv = dp.apply_jacobian(dev(x), dev(v)).cpu().numpy().ravel()
     r = J @ v.ravel()
     assert np.abs(Jv - r).max() <= 1e-12 * np.abs(r).max()
-    w = rng.uniform(-1.0, 1.0, x.shape)
-    JTw = dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy().ravel()
-    r = J.T @ w.ravel()
+    JTw = dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy()
+    r = (J.T @ w.ravel()).reshape(x.shape)
+    if spec.type_func in (1, 3):
+        keep = [q for q in range(spec.nunk) if q not in nv2]
+        JTw, r = JTw[keep], r[keep]
+        Jw = dp.apply_jacobian(dev(x), dev(u)).cpu().numpy()
+        a, b = float((w * Jw).sum()), float((dp.apply_jacobian_transpose(dev(x), dev(w)).cpu().numpy() * u).sum())
+        assert abs(a - b) < 1e-12 * np.abs(w * Jw).sum()
     assert np.abs(JTw - r).max() <= 1e-12 * np.abs(r).max()
 
 
@@ -85,10 +106,12 @@ def test_device_newton_reproduces_hard_wall_example(name):
     h1, h2, scale = integration_weights(spec, g)
     obs = dp.observables(x, h1, h2, scale)
     assert abs(obs["omega_s"] - ref["energy"]) < printed_tol(ref["energy_text"]), (obs, ref)
-    for c in range(spec.ncomp):
-        assert abs(obs["ads"][c] - ref["adsorption"][c]) < 2e-6, (obs, ref)
+    for c in range(spec.ncomp):   # to the printed digits (HS8's README prints 5 decimals), at least 2e-6
+        assert abs(obs["ads"][c] - ref["adsorption"][c]) < max(2e-6, printed_tol(repr(ref["adsorption"][c]))), (obs, ref)
     W = force_weights(spec, g)
     force = dp.force(x, W)
+    # the reference harness matches each README key against every value of the output line
+    # (Examples/check_results/main.cpp, matchParameters), so any wall's ptilde may carry "force"
     assert min(abs(f - ref["force"]) for f in force) < printed_tol(ref["force_text"]), (force, ref)
     fo = [(W[k] * xg[: spec.ncomp]).sum() for k in range(W.shape[0])]
     assert np.abs(np.array(fo) - force).max() < 1e-12 * max(1.0, np.abs(fo).max())

@@ -124,7 +124,7 @@ def rand_n(rng, nd, scale=1.0):
     return np.concatenate([[n0, n1, n2, n3], v1, v2])
 
 
-@pytest.mark.parametrize("func", [FMT1, FMT3])
+@pytest.mark.parametrize("func", [FMT1, 1, FMT3, 3])
 @pytest.mark.parametrize("nd", [1, 2, 3])
 def test_fmt_first_derivatives_fd(func, nd):
     sp = one_comp_1d(func=func)
@@ -188,6 +188,62 @@ def test_fmt_second_derivatives_fd(func):
             ref = (c @ Hm) * s_full
             d = o.d2_delta(n, off, sg)
             assert np.allclose(d, ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+
+
+def _hessian_fd(o, n):
+    H = np.zeros((len(n), len(n)))
+    for b in range(len(n)):
+        h = 1e-6 * max(1.0, abs(n[b]))
+        e = np.zeros_like(n)
+        e[b] = h
+        H[:, b] = (o.dphi(n + e) - o.dphi(n - e)) / (2 * h)
+    return H
+
+
+@pytest.mark.parametrize("func", [1, 3])
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_fmt2_fmt4_second_derivatives(func, nd):
+    """FMT2 and FMT4 (src/dft_physics_FMT2.c:106-250, FMT4.c:101-276, restated by the oracle): the theta
+    Hessian row and every delta-Hessian column except nv2 equal finite differences of the first
+    derivatives.  The nv2 columns of the delta Hessian do not (FMT2 in 2D and 3D, FMT4 in every
+    dimension): the reference's assembled Jacobian is inexact there, which is why the device, whose
+    product is the exact derivative, is compared with it on the other columns only."""
+    sp = one_comp_1d(func=func)
+    sp.ndim, sp.nx, sp.dx, sp.bc = nd, [12] * nd, [0.1] * nd, [[PERIODIC, PERIODIC]] * nd
+    sp.rho0 = np.full((1, 12 ** nd), 0.6)
+    o = Oracle.from_spec(sp)
+    rng = np.random.default_rng(4)
+    R = 0.5
+    worst_nv2 = 0.0
+    for _ in range(4):
+        n = rand_n(rng, nd)
+        H = _hessian_fd(o, n)
+        t = o.d2_theta(n)
+        assert np.allclose(t, H[3], rtol=1e-6, atol=1e-6 * np.abs(H[3]).max())
+        off = rng.integers(-5, 6, nd).astype(np.int32)
+        v = off * 0.1 / R
+        c = np.concatenate([[1 / (4 * PI * R * R), 1 / (4 * PI * R), 1.0, 0.0], -v / (4 * PI * R), -v])
+        d = o.d2_delta(n, off, np.ones(nd))
+        ref = c @ H
+        scale = np.abs(ref).max()
+        assert np.allclose(d[: 4 + nd], ref[: 4 + nd], rtol=1e-6, atol=1e-6 * scale)
+        worst_nv2 = max(worst_nv2, float(np.abs(d[4 + nd:] - ref[4 + nd:]).max() / scale))
+    if func == 1 and nd == 1:
+        assert worst_nv2 < 1e-6
+    else:
+        assert worst_nv2 > 1e-3
+
+
+@pytest.mark.parametrize("rho", [0.1, 0.5, 0.8])
+def test_bulk_fmt2_and_fmt4_reduce_to_rosenfeld_and_white_bear(rho):
+    """at nv2 = 0, FMT2 is FMT1 (Percus-Yevick) and FMT4 is White Bear (Carnahan-Starling)"""
+    for func, other in ((1, FMT1), (3, FMT3)):
+        b = Oracle.from_spec(one_comp_1d(rho_b=rho, func=func)).bulk()
+        bo = Oracle.from_spec(one_comp_1d(rho_b=rho, func=other)).bulk()
+        assert np.allclose(b, bo, rtol=1e-14, atol=1e-14)
+    eta = PI / 6 * rho
+    mu_cs = eta * (8 - 9 * eta + 3 * eta * eta) / (1 - eta) ** 3
+    assert abs(Oracle.from_spec(one_comp_1d(rho_b=rho, func=3)).bulk()[20] - mu_cs) < 1e-11 * max(1, abs(mu_cs))
 
 
 def py_mu_p(etas_by_comp, d):

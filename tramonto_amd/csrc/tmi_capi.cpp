@@ -61,8 +61,8 @@ void validate(const tmi_params &p)
 {
     if (p.ndim < 1 || p.ndim > 3) throw std::invalid_argument("ndim must be 1, 2 or 3");
     if (p.ncomp < 1 || p.ncomp > TMI_NCOMP_MAX) throw std::invalid_argument("ncomp must be 1..4");
-    if (p.type_func != TMI_FMT1 && p.type_func != TMI_FMT3)
-        throw std::domain_error("type_func: only Rosenfeld (0) and White Bear (2) are implemented");
+    if (p.type_func < TMI_FMT1 || p.type_func > TMI_FMT4)
+        throw std::domain_error("type_func: 0 (FMT1), 1 (FMT2), 2 (FMT3) or 3 (FMT4)");
     if (p.type_attr != -1 && p.type_attr != 0) throw std::domain_error("type_attr: only -1 and 0 are implemented");
 
     for (int d = 0; d < p.ndim; ++d) {

@@ -1,17 +1,21 @@
 // Pointwise fundamental-measure free-energy derivatives, shared by host and
 // device code.  Densities are in "n order": n0 n1 n2 n3 nv1[D] nv2[D].
 //
-// Both functionals are written as
-//   Phi = -n0 ln(1-n3) + (n1 n2 - nv1.nv2)/(1-n3) + (n2^3 - 3 n2 nv2.nv2) f(n3) / (36 pi)
-// with f = 1.5/(1-n3)^2 for Rosenfeld (FMT1, src/dft_physics_FMT1.c:39-57) and
-// f = [n3 + (1-n3)^2 ln(1-n3)] / (n3^2 (1-n3)^2) for White Bear
-// (FMT3, src/dft_physics_FMT3.c:39-59).  The first derivatives reproduce
-// FMT1_1stderiv / FMT3_1stderiv (src/dft_physics_FMT1.c:59-75,
-// src/dft_physics_FMT3.c:61-96), including White Bear's n3 <= 1e-10 cut-off.
-// The Jacobian-vector product uses the exact Hessian of Phi; contracted with a
-// stencil kernel vector it equals the reference's d2phi_drb2_{delta,theta}_rb
-// values (src/dft_physics_FMT1.c:79-168, src/dft_physics_FMT3.c:101-266),
-// which are zero for White Bear when n3 <= 1e-10.
+// The four functionals are written as
+//   Phi = -n0 ln(1-n3) + (n1 n2 - nv1.nv2)/(1-n3) + q(n2, nv2) f(n3) / (36 pi)
+// with
+//   FMT1 (Rosenfeld, src/dft_physics_FMT1.c:39-57):   q = n2^3 - 3 n2 nv2.nv2,   f = 1.5/(1-n3)^2
+//   FMT2 (src/dft_physics_FMT2.c:40-58):              q = (n2 - nv2.nv2/n2)^3,   f = 1.5/(1-n3)^2
+//   FMT3 (White Bear, src/dft_physics_FMT3.c:39-59):  q = n2^3 - 3 n2 nv2.nv2,   f = [n3 + (1-n3)^2 ln(1-n3)] / (n3^2 (1-n3)^2)
+//   FMT4 (Oleksy-Hansen, src/dft_physics_FMT4.c:38-58): q = (n2 - nv2.nv2/n2)^3, f as White Bear.
+// The first derivatives reproduce FMTk_1stderiv (src/dft_physics_FMT1.c:59-75, FMT2.c:60-100,
+// FMT3.c:61-96, FMT4.c:60-96), including White Bear's and FMT4's n3 <= 1e-10 cut-off and FMT2's
+// n2 <= 1e-15 convention (q, its derivatives taken at nv2 = 0), which FMT4 shares here (the reference
+// divides by n2^2 there).  The Jacobian-vector product uses the exact Hessian of Phi; contracted with a
+// stencil kernel vector it equals the reference's d2phi_drb2_{delta,theta}_rb values
+// (src/dft_physics_FMT1.c:79-168, FMT3.c:101-266, and the theta parts of FMT2/FMT4), which are zero
+// below the cut-off.  The reference's FMT2 (2D/3D) and FMT4 delta Hessians differ from the exact
+// derivative in their nv2 columns; the product here is the exact one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +31,11 @@ struct FmtScalars {
     bool cut;    // White Bear below the n3 cut-off
 };
 
+// Rosenfeld's f(n3) for FMT1 and FMT2 (Type_func 0, 1), White Bear's for FMT3 and FMT4 (2, 3)
+__host__ __device__ __attribute__((always_inline)) inline bool fmt_rosenfeld_f(int func) { return func <= 1; }
+// q = (n2 - nv2.nv2/n2)^3 for FMT2 and FMT4, n2^3 - 3 n2 nv2.nv2 for FMT1 and FMT3
+__host__ __device__ __attribute__((always_inline)) inline bool fmt_cubic_q(int func) { return func == 1 || func == 3; }
+
 __host__ __device__ __attribute__((always_inline)) inline FmtScalars fmt_scalars(int func, double n3)
 {
     FmtScalars s;
@@ -35,7 +44,7 @@ __host__ __device__ __attribute__((always_inline)) inline FmtScalars fmt_scalars
     s.lnA = log(s.a);
     s.cut = false;
     const double a = s.a, a2 = a * a, a3 = a2 * a, a4 = a3 * a;
-    if (func == 0) {
+    if (fmt_rosenfeld_f(func)) {
         s.f0 = 1.5 * a2;
         s.f1 = 3.0 * a3;
         s.f2 = 9.0 * a4;
@@ -58,8 +67,43 @@ __host__ __device__ __attribute__((always_inline)) inline FmtScalars fmt_scalars
     return s;
 }
 
-// Phi(n) (FMT1_energy_density / FMT3_energy_density, src/dft_physics_FMT1.c:39-57,
-// src/dft_physics_FMT3.c:39-59); White Bear's third term, 0/0 at n3 = 0, is 0 below the cut-off
+// q(n2, nv2) and its derivatives, with D = nv2.nv2:
+//   dq/dn2 = q2, dq/dnv2_d = qv nv2_d, d2q/dn2^2 = q22, d2q/dn2 dnv2_d = q2v nv2_d,
+//   d2q/dnv2_d dnv2_e = qvvo nv2_d nv2_e + qvvd delta_de
+struct FmtQ {
+    double q, q2, qv, q22, q2v, qvvo, qvvd;
+};
+
+__host__ __device__ __attribute__((always_inline)) inline FmtQ fmt_q(int func, double n2, double D)
+{
+    FmtQ r;
+    if (!fmt_cubic_q(func)) {   // n2^3 - 3 n2 D
+        r.q = n2 * n2 * n2 - 3.0 * n2 * D;
+        r.q2 = 3.0 * n2 * n2 - 3.0 * D;
+        r.qv = -6.0 * n2;
+        r.q22 = 6.0 * n2;
+        r.q2v = -6.0;
+        r.qvvo = 0.0;
+        r.qvvd = -6.0 * n2;
+        return r;
+    }
+    // alpha^3, alpha = n2 - D/n2, beta = d alpha/dn2 = 1 + D/n2^2; for n2 <= 1e-15 taken at nv2 = 0
+    const bool small = !(n2 > 1.e-15);
+    const double in2 = small ? 0.0 : 1.0 / n2;
+    const double Dn = small ? 0.0 : D;
+    const double al = n2 - Dn * in2, be = 1.0 + Dn * in2 * in2, al2 = al * al;
+    r.q = al2 * al;
+    r.q2 = 3.0 * al2 * be;
+    r.qv = -6.0 * al2 * in2;
+    r.q22 = 6.0 * al * be * be - 6.0 * al2 * Dn * in2 * in2 * in2;
+    r.q2v = (-12.0 * al * be + 6.0 * al2 * in2) * in2;
+    r.qvvo = 24.0 * al * in2 * in2;
+    r.qvvd = small ? -6.0 * n2 : -6.0 * al2 * in2;
+    return r;
+}
+
+// Phi(n) (FMTk_energy_density, src/dft_physics_FMT1.c:39-57, FMT2.c:40-58, FMT3.c:39-59, FMT4.c:38-58);
+// the White Bear f's third term, 0/0 at n3 = 0, is 0 below the cut-off
 template <int ND>
 __host__ __device__ __attribute__((always_inline)) inline double fmt_phi(int func, const double *n)
 {
@@ -71,7 +115,8 @@ __host__ __device__ __attribute__((always_inline)) inline double fmt_phi(int fun
         d12 += n[4 + d] * n[4 + ND + d];
         d22 += n[4 + ND + d] * n[4 + ND + d];
     }
-    return n[0] * s.lnA + (n[1] * n[2] - d12) * s.a + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f0 * (inv_pi / 36.0);
+    const FmtQ q = fmt_q(func, n[2], d22);
+    return n[0] * s.lnA + (n[1] * n[2] - d12) * s.a + q.q * s.f0 * (inv_pi / 36.0);
 }
 
 // dPhi/dn in n order
@@ -87,9 +132,11 @@ __host__ __device__ __attribute__((always_inline)) inline void fmt_dphi(int func
         d22 += n[4 + ND + d] * n[4 + ND + d];
     }
     const double a = s.a, a2 = a * a;
+    const FmtQ q = fmt_q(func, n[2], d22);
+    const double c0 = s.f0 * (inv_pi / 36.0), c1 = s.f1 * (inv_pi / 36.0);
     // every element is written once, selects instead of branches: the array stays in registers
-    const double o2 = n[1] * a + (n[2] * n[2] - d22) * s.f0 * (inv_pi / 12.0);
-    const double o3 = n[0] * a + (n[1] * n[2] - d12) * a2 + (n[2] * n[2] * n[2] - 3.0 * n[2] * d22) * s.f1 * (inv_pi / 36.0);
+    const double o2 = n[1] * a + q.q2 * c0;
+    const double o3 = n[0] * a + (n[1] * n[2] - d12) * a2 + q.q * c1;
     out[0] = s.lnA;
     out[1] = n[2] * a;
     out[2] = s.cut ? 0.0 : o2;
@@ -97,7 +144,7 @@ __host__ __device__ __attribute__((always_inline)) inline void fmt_dphi(int func
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         out[4 + d] = -n[4 + ND + d] * a;
-        out[4 + ND + d] = s.cut ? 0.0 : (-n[4 + d] * a - n[2] * n[4 + ND + d] * s.f0 * (inv_pi / 6.0));
+        out[4 + ND + d] = s.cut ? 0.0 : (-n[4 + d] * a + q.qv * n[4 + ND + d] * c0);
     }
 }
 
@@ -108,29 +155,33 @@ __host__ __device__ __attribute__((always_inline)) inline void fmt_hvp(int func,
     constexpr double inv_pi = 0.318309886183790671537767526745028724;
     const FmtScalars s = fmt_scalars(func, n[3]);
     const bool z = s.cut;   // every element is written once, through selects (no early return)
-    double d12 = 0.0, d22 = 0.0;
+    double d12 = 0.0, d22 = 0.0, vdv = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         d12 += n[4 + d] * n[4 + ND + d];
         d22 += n[4 + ND + d] * n[4 + ND + d];
+        vdv += n[4 + ND + d] * dn[4 + ND + d];
     }
+    const FmtQ q = fmt_q(func, n[2], d22);
+    const double c0 = s.f0 * (inv_pi / 36.0), c1 = s.f1 * (inv_pi / 36.0), c2 = s.f2 * (inv_pi / 36.0);
     const double a = s.a, a2 = a * a, a3 = a2 * a;
     const double n0 = n[0], n1 = n[1], n2 = n[2];
-    const double h22 = n2 * s.f0 * (inv_pi / 6.0);
-    const double h23 = n1 * a2 + (n2 * n2 - d22) * s.f1 * (inv_pi / 12.0);
-    const double h33 = n0 * a2 + 2.0 * (n1 * n2 - d12) * a3 + (n2 * n2 * n2 - 3.0 * n2 * d22) * s.f2 * (inv_pi / 36.0);
-    const double c2v = s.f0 * (inv_pi / 6.0);   // -H(2,v2d) / nv2d and -H(v2d,v2d)/n2
-    const double c3v = s.f1 * (inv_pi / 6.0);
+    const double h22 = q.q22 * c0;
+    const double h23 = n1 * a2 + q.q2 * c1;
+    const double h33 = n0 * a2 + 2.0 * (n1 * n2 - d12) * a3 + q.q * c2;
+    const double c2v = q.q2v * c0;   // H(2, v2d) / nv2d
+    const double c3v = q.qv * c1;    // the nv2 part of H(3, v2d) / nv2d
     double o2 = a * dn[1] + h22 * dn[2] + h23 * dn[3];
     double o3 = a * dn[0] + n2 * a2 * dn[1] + h23 * dn[2] + h33 * dn[3];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         const double v1 = n[4 + d], v2 = n[4 + ND + d];
         const double dv1 = dn[4 + d], dv2 = dn[4 + ND + d];
-        o2 += -v2 * c2v * dv2;
-        o3 += -v2 * a2 * dv1 + (-v1 * a2 - n2 * v2 * c3v) * dv2;
+        o2 += v2 * c2v * dv2;
+        o3 += -v2 * a2 * dv1 + (-v1 * a2 + v2 * c3v) * dv2;
         const double ov1 = -v2 * a2 * dn[3] - a * dv2;
-        const double ov2 = -v2 * c2v * dn[2] + (-v1 * a2 - n2 * v2 * c3v) * dn[3] - a * dv1 - n2 * c2v * dv2;
+        const double ov2 = v2 * c2v * dn[2] + (-v1 * a2 + v2 * c3v) * dn[3] - a * dv1 +
+                           (q.qvvd * dv2 + q.qvvo * v2 * vdv) * c0;
         out[4 + d] = z ? 0.0 : ov1;
         out[4 + ND + d] = z ? 0.0 : ov2;
     }

@@ -520,10 +520,10 @@ void compute_bulk_constants(const tmi_params &p, BulkConstants &bc)
     const double a = 1.0 / (1.0 - n3), a2 = a * a, a3 = a2 * a;
     bc.dphi_b[0] = std::log(a);
     bc.dphi_b[1] = n2 * a;
-    if (p.type_func == TMI_FMT1) {
+    if (p.type_func == TMI_FMT1 || p.type_func == TMI_FMT2) {   // FMT2 equals FMT1 at nv2 = 0
         bc.dphi_b[2] = n1 * a + (n2 * n2) * a2 / (8.0 * kPi);
         bc.dphi_b[3] = n0 * a + (n1 * n2) * a2 + (n2 * n2 * n2) * a3 / (12.0 * kPi);
-    } else if (n3 > 1.e-10) {
+    } else if (n3 > 1.e-10) {   // FMT3, and FMT4, which equals it at nv2 = 0
         const double L = std::log(1. - n3), g = n3 + (1. - n3) * (1. - n3) * L;
         bc.dphi_b[2] = n1 * a + (n2 * n2) * a2 * g / (12.0 * kPi * n3 * n3);
         bc.dphi_b[3] = n0 * a + (n1 * n2) * a2 +

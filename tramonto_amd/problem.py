@@ -460,6 +460,16 @@ def small_case(kind: str, seed: int = 7) -> ProblemSpec:
         return rough_wall_2d(seed=seed, nx=40, ny=32, dx=0.1, cut=1.5, wall_cut=2.0)
     if kind == "3d_solute_small":
         return solute_3d(seed=seed, n=20, dx=0.25, sig=(1.0, 0.8), radius=1.2, cut_fac=1.2, wall_cut=2.0)
+    if kind == "3d_periodic_fmt2":
+        return periodic_3d(seed=seed, n=12, dx=0.25, cut=1.2, func=1)
+    if kind == "2d_wall_fmt4":
+        sp = small_case("2d_wall_reflect", seed)
+        sp.type_func = 3
+        return sp
+    if kind == "3d_mix_fmt4":   # White Bear mixture case with FMT4 (reflect, wall and periodic edges)
+        sp = small_case("3d_mix_bulk", seed)
+        sp.type_func = 3
+        return sp
     if kind == "3d_solute3_small":   # the 128^3 solute's three species, small box
         return solute_3d(seed=seed, n=20, dx=0.25, sig=(1.0, 0.9, 0.7), radius=1.2, cut_fac=1.2, wall_cut=2.0)
     # periodic boxes whose axis lengths select the register FFT kernels (x 64 / 128, strided y, z)
