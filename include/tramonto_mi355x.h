@@ -103,6 +103,11 @@ typedef struct tmi_info {
 #define TMI_PRECOND_DIAG 0   /* the A22 diagonal, as the reference's HardSphere LinProbMgr */
 #define TMI_PRECOND_BULK 1   /* A22 diagonal times the inverse bulk Schur symbol (periodic FFT grids) */
 
+/* convergence tests of tmi_newton_solve */
+#define TMI_CONV_RESID 0    /* ||F||_2 < resid_tol, tested before each step (the benchmark's criterion) */
+#define TMI_CONV_UPDATE 1   /* the reference's: the weighted update norm ||frac dx / (nl_rel_tol |x| + nl_abs_tol)||_2
+                               of the step just taken is <= 1 (update_solution, src/dft_newton.c:266-380) */
+
 typedef struct tmi_newton_opts {
     int max_iter;
     double nl_rel_tol, nl_abs_tol, update_scaling;   /* update_solution damping, src/dft_newton.c:266 */
@@ -111,6 +116,7 @@ typedef struct tmi_newton_opts {
     double gmres_tol;        /* relative tolerance of each Schur solve */
     int precision;           /* TMI_PREC_* */
     int verbose;
+    int convergence;         /* TMI_CONV_* (zero-initialised options keep TMI_CONV_RESID) */
 } tmi_newton_opts;
 
 typedef struct tmi_newton_stats {

@@ -417,3 +417,31 @@ def test_hard_surface_weights_in_2d_with_attraction(case, mf_fft, monkeypatch):
     Fa, _ = dpa.fill_residual(dev(np.ascontiguousarray(x[:ne])))
     ref = o.residual(xi)[:ne]
     assert np.abs(Fa.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_reference_convergence_test_gives_readme_niter(name):
+    """With the reference's convergence test (weighted update norm <= 1, update_solution,
+    src/dft_newton.c:266-380) the device Newton stops where the oracle's does (within 1: the
+    GMRES tolerances differ) and within 1 of the README's niter (the reference harness allows 4,
+    Examples/CMakeLists.txt:17), on a profile within the update tolerance of the converged one."""
+    from tramonto_amd.device import CONV_UPDATE, DeviceProblem
+    g = EXAMPLES[name]
+    spec = spec_from_tramonto(g, name)
+    dp = DeviceProblem(spec)
+    kw = dict(max_iter=g["max_nl_iter"], nl_rel_tol=g["nl_rel_tol"], nl_abs_tol=g["nl_abs_tol"],
+              update_scaling=g["nl_update_scaling"])
+    x = dp.initial_state()
+    st = dp.newton(x, convergence=CONV_UPDATE, gmres_tol=1e-9, gmres_restart=100, gmres_max_iter=2000, **kw)
+    assert st["converged"], st
+    o = Oracle.from_spec(spec)
+    _, sto = o.newton(o.init_fields(spec.initial_x()), resid_tol=0.0, **kw)
+    assert sto["converged"]
+    assert abs(st["iters"] - sto["iters"]) <= 1, (st, sto)
+    assert abs(st["iters"] - g["key_outputs"]["niter"]) <= 1, (st["iters"], g["key_outputs"]["niter"])
+    xf = dp.initial_state()
+    assert dp.newton(xf, resid_tol=1e-10, gmres_tol=1e-9, gmres_restart=100, gmres_max_iter=2000, **kw)["converged"]
+    C = spec.ncomp
+    a, b = x.cpu().numpy()[:C], xf.cpu().numpy()[:C]
+    assert np.all(np.abs(a - b) <= g["nl_rel_tol"] * np.abs(b) + g["nl_abs_tol"])

@@ -49,6 +49,7 @@ class TmiNewtonOpts(ct.Structure):
         ("max_iter", ct.c_int), ("nl_rel_tol", ct.c_double), ("nl_abs_tol", ct.c_double),
         ("update_scaling", ct.c_double), ("resid_tol", ct.c_double), ("gmres_restart", ct.c_int),
         ("gmres_max_iter", ct.c_int), ("gmres_tol", ct.c_double), ("precision", ct.c_int), ("verbose", ct.c_int),
+        ("convergence", ct.c_int),
     ]
 
 

@@ -537,6 +537,7 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->wsf.corr_f);
     (void)hipFree(P->d_red);
     (void)hipFree(P->d_scratch);
+    (void)hipFree(P->d_newton);
     (void)hipFree(P->d_xi);
     (void)hipFree(P->d_vi);
     (void)hipFree(P->d_fi);
@@ -730,48 +731,41 @@ int schur_solve_export(Problem &P, const double *x, const double *F, double *dx,
 
 int newton_solve(Problem &P, double *x, const tmi_newton_opts &o, tmi_newton_stats &st, hipStream_t s)
 {
+    if (o.convergence != TMI_CONV_RESID && o.convergence != TMI_CONV_UPDATE)
+        throw std::invalid_argument("convergence must be TMI_CONV_RESID or TMI_CONV_UPDATE");
     const size_t N = P.N, ntot = (size_t)P.nunk * N;
-    std::vector<double *> bufs;
-    double *F = nullptr, *dx = nullptr;
-    TMI_HIP_CHECK(hipMalloc((void **)&F, sizeof(double) * ntot));
-    try {
-        TMI_HIP_CHECK(hipMalloc((void **)&dx, sizeof(double) * ntot));
-    } catch (...) {
-        (void)hipFree(F);
-        throw;
-    }
+    // F and dx: allocated at the first solve and kept with the problem (no allocation in later solves)
+    if (!P.d_newton) P.d_newton = dev_alloc<double>(2 * ntot);
+    double *F = P.d_newton, *dx = P.d_newton + ntot;
     auto t0 = std::chrono::steady_clock::now();
     st = tmi_newton_stats{};
-    try {
-        int iter = 0;
-        while (true) {
-            const double rn = std::sqrt(residual_sumsq(P, x, F, s));
-            st.resid_norm = rn;
-            if (o.verbose) std::fprintf(stderr, "[tmi newton] iter %d ||F|| = %.6e\n", iter, rn);
-            if (!(rn == rn)) throw std::runtime_error("residual is NaN");
-            if (rn < o.resid_tol) {
-                st.converged = 1;
-                break;
-            }
-            if (iter >= o.max_iter) break;
-            int its = 0;
-            schur_solve(P, x, F, dx, o.gmres_tol, o.gmres_restart, o.gmres_max_iter, o.precision, &its, s);
-            st.gmres_iters_total += its;
-            double frac = 1.0, unorm = 0.0;
-            dev_newton_update(P, x, dx, o, &frac, &unorm, s);
-            if (o.verbose) std::fprintf(stderr, "[tmi newton]   gmres %d its, frac %.3g, weighted update %.3e\n", its, frac, unorm);
-            ++iter;
+    int iter = 0;
+    while (true) {
+        const double rn = std::sqrt(residual_sumsq(P, x, F, s));
+        st.resid_norm = rn;
+        if (o.verbose) std::fprintf(stderr, "[tmi newton] iter %d ||F|| = %.6e\n", iter, rn);
+        if (!(rn == rn)) throw std::runtime_error("residual is NaN");
+        if (o.convergence == TMI_CONV_RESID && rn < o.resid_tol) {
+            st.converged = 1;
+            break;
         }
-        st.iters = iter;
-    } catch (...) {
-        (void)hipFree(F);
-        (void)hipFree(dx);
-        throw;
+        if (iter >= o.max_iter) break;
+        int its = 0;
+        schur_solve(P, x, F, dx, o.gmres_tol, o.gmres_restart, o.gmres_max_iter, o.precision, &its, s);
+        st.gmres_iters_total += its;
+        double frac = 1.0, unorm = 0.0;
+        dev_newton_update(P, x, dx, o, &frac, &unorm, s);
+        if (o.verbose) std::fprintf(stderr, "[tmi newton]   gmres %d its, frac %.3g, weighted update %.3e\n", its, frac, unorm);
+        ++iter;
+        if (o.convergence == TMI_CONV_UPDATE && unorm <= 1.0) {   // src/dft_newton.c:378
+            st.converged = 1;
+            st.resid_norm = std::sqrt(residual_sumsq(P, x, F, s));
+            break;
+        }
     }
+    st.iters = iter;
     TMI_HIP_CHECK(hipStreamSynchronize(s));
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    (void)hipFree(F);
-    (void)hipFree(dx);
     return 0;
 }
 
@@ -1210,8 +1204,10 @@ int tmi_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opts, tmi
     return guarded([&] {
         Problem *P = as_problem(handle);
         if (!opts || !stats) throw std::invalid_argument("null argument");
-        if (P->npores > 0)
+        if (P->npores > 0) {
+            if (opts->convergence != TMI_CONV_RESID) throw std::domain_error("batches of pores converge on ||F_p||_2");
             return batch_newton_solve(*P, x_d, *opts, *stats, nullptr, nullptr, nullptr, nullptr, (hipStream_t)stream);
+        }
         if (!P->ext_mf) return newton_solve(*P, x_d, *opts, *stats, (hipStream_t)stream);
         hipStream_t s = (hipStream_t)stream;
         double *xi = const_cast<double *>(expand(*P, x_d, P->d_xi, false, s));
@@ -1286,6 +1282,7 @@ int tmi_batch_newton_solve(void *handle, double *x_d, const tmi_newton_opts *opt
         Problem *P = as_problem(handle);
         if (!opts || !stats || !x_d) throw std::invalid_argument("null argument");
         if (P->npores < 1) throw std::invalid_argument("not a batched problem");
+        if (opts->convergence != TMI_CONV_RESID) throw std::domain_error("batches of pores converge on ||F_p||_2");
         return batch_newton_solve(*P, x_d, *opts, *stats, pore_iters, pore_converged, pore_gmres, pore_resid,
                                   (hipStream_t)stream);
     });

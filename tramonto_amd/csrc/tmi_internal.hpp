@@ -167,6 +167,7 @@ struct Problem {
     BatchWork *batch_work = nullptr;  // batched Newton buffers and graphs (tmi_batch.hip)
     FftPlan *fft = nullptr;           // FFT forms of the stencil sums (tmi_fft.hip), 2D/3D
     // solver scratch
+    double *d_newton = nullptr;   // [2][nunk][N]: F and dx of tmi_newton_solve, from its first call on
     double *d_scratch = nullptr;
     size_t scratch_bytes = 0;
     double *d_red = nullptr;     // reduction buffer

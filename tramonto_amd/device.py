@@ -23,6 +23,7 @@ from . import _lib
 from .problem import ProblemSpec
 
 PREC_FP64, PREC_MIXED = 0, 1
+CONV_RESID, CONV_UPDATE = 0, 1
 PRECOND_DIAG, PRECOND_BULK = 0, 1
 
 
@@ -211,9 +212,11 @@ class DeviceProblem:
 
     def newton(self, x: torch.Tensor, max_iter=50, resid_tol=1e-8, nl_rel_tol=1e-3, nl_abs_tol=1e-8,
                update_scaling=0.1, gmres_restart=60, gmres_max_iter=600, gmres_tol=1e-6,
-               precision=PREC_FP64, verbose=False) -> Dict:
+               precision=PREC_FP64, verbose=False, convergence=CONV_RESID) -> Dict:
+        """newton_solver (src/dft_newton.c:131-196) on the device; convergence CONV_RESID stops when
+        ||F||_2 < resid_tol, CONV_UPDATE on the reference's weighted update norm <= 1."""
         o = _lib.TmiNewtonOpts(max_iter, nl_rel_tol, nl_abs_tol, update_scaling, resid_tol, gmres_restart,
-                               gmres_max_iter, gmres_tol, precision, 1 if verbose else 0)
+                               gmres_max_iter, gmres_tol, precision, 1 if verbose else 0, convergence)
         st = _lib.TmiNewtonStats()
         _lib.check(_lib.lib().tmi_newton_solve(self.handle, _ptr(x), ct.byref(o), ct.byref(st), _stream()),
                    "tmi_newton_solve")
