@@ -139,9 +139,16 @@ struct FftPlan {
     // kernel-family switches, read from the environment when the plan is built: regpass 0 keeps the
     // LDS Stockham pass on strided axes (TMI_FFT_REGPASS=0), 1 uses the one-wave register pass where it
     // has enough waves, 2 wherever it has radices (TMI_FFT_REGPASS=2); xreg false keeps the LDS form of
-    // the fused x pass (TMI_FFT_XREG=0)
+    // the fused x pass (TMI_FFT_XREG=0); xcdmap 0 keeps the plain workgroup order of the register
+    // passes instead of the XCD-contiguous one (TMI_FFT_XCDMAP=0)
     int regpass = 1;
     bool xreg = true;
+    int xcdmap = 1;
+    // adjoint fields of the Euler-Lagrange product on a wrapping grid: false the per-component fields
+    // A_c, B_c,d and S3 (C (1 + D) + 1), true the derivatives of Phi themselves in the rho-bar order
+    // (n3 n2 n1 n0 nv2 nv1: 4 + 2 D), whichever makes fewer pairs; the product then forms
+    // A_c = D2 + D1/(4 pi R_c) + D0/(4 pi R_c^2) and B_c,d = Dv2_d + Dv1_d/(4 pi R_c) per wave number
+    bool dform = false;
 };
 
 namespace {
@@ -321,6 +328,7 @@ struct PassIO {
     double *ss = nullptr;
     // kPassGather: optional factor of the gathered values; kPassPc: the rows kept where elzero is set
     const double *mul = nullptr, *pv = nullptr;
+    int xcd = 1;   // register passes: XCD-contiguous workgroup order (xcd_item)
 };
 
 // sum over the workgroup (kFftBlock threads), valid in thread 0
@@ -354,12 +362,14 @@ template <typename T, int MODE>
 __device__ inline Cx<T> strided_load(const Cx<T> *__restrict__ data, const StridedGeo &g, const PassIO &io, int field,
                                      int i, int j)
 {
-    if constexpr (MODE == kPassGather) {
+    if constexpr (MODE == kPassGather) {   // every load unconditional: none waits for another
         const size_t a = (size_t)(2 * field) * io.N + strided_node(g, io, i, j), b = a + io.N;
         const bool has_b = 2 * field + 1 < io.C;
         const double fa = io.mul ? io.mul[a] : 1.0, fb = (io.mul && has_b) ? io.mul[b] : 1.0;
-        const T re = io.zero[a] ? (T)0 : (T)(io.rho[a] * fa);
-        const T im = (has_b && !io.zero[b]) ? (T)(io.rho[b] * fb) : (T)0;
+        const double ra = io.rho[a], rb = has_b ? io.rho[b] : 0.0;
+        const unsigned char za = io.zero[a], zb = has_b ? io.zero[b] : 1;
+        const T re = za ? (T)0 : (T)(ra * fa);
+        const T im = zb ? (T)0 : (T)(rb * fb);
         return Cx<T>{re, im};
     } else {
         return data[g.base0 + i + (long long)j * g.step];
@@ -429,6 +439,93 @@ __device__ inline void strided_store(Cx<T> *__restrict__ data, const StridedGeo 
         }
     } else {
         data[g.base0 + i + (long long)j * g.step] = v;
+    }
+}
+
+// The R2 values u[k2] (elements k1 + R1 k2 of line i) of one phase of a register pass, stored as
+// strided_store does; for the row-writing ends the node-side inputs of all R2 values of an output
+// (or component) are loaded before any of them is stored, so R2 loads are in flight at once (the
+// compiler cannot move a load past a store to a pointer it may alias).
+template <typename T, int MODE, int R1, int R2>
+__device__ inline void strided_store_phase(Cx<T> *__restrict__ data, const StridedGeo &g, const PassIO &io, int field,
+                                           int i, int k1, const Cx<T> (&u)[R2], double &ssq)
+{
+    if constexpr (MODE == kPassRows) {
+        int nd[R2];
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) nd[k2] = strided_node(g, io, i, k1 + R1 * k2);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {   // compile-time h: u[k2].x / .y stay in registers
+            const int o = 2 * field + h;
+            if (o >= io.nout) break;
+            for (int r = 0; r < io.nrow[o]; ++r) {
+                const int row = io.row[o][r];
+                const bool rb = row < io.nrb;
+                const size_t base = rb ? (size_t)row * io.N : (size_t)(row - io.nrb) * io.N;
+                const double *in = rb ? io.in_rb : io.in_mf, *sub = rb ? io.sub_rb : io.sub_mf;
+                double *out = rb ? io.out_rb : io.out_mf;
+                double pin[R2], psub[R2];
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    pin[k2] = in ? in[base + nd[k2]] : 0.0;
+                    psub[k2] = sub ? sub[base + nd[k2]] : 0.0;
+                }
+                const double sc = io.scale[o][r];
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {   // the operations and their order of strided_store
+                    double val = (double)(h ? u[k2].y : u[k2].x) * sc;
+                    if (in) val -= pin[k2];
+                    if (sub) val -= psub[k2];
+                    out[base + nd[k2]] = val;
+                    ssq += val * val;
+                }
+            }
+        }
+    } else if constexpr (MODE == kPassEl) {
+        int nd[R2];
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) nd[k2] = strided_node(g, io, i, k1 + R1 * k2);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = 2 * field + h;
+            if (c >= io.C) break;
+            const size_t base = (size_t)c * io.N;
+            unsigned char zr[R2];
+            double xv[R2], inr[R2], vx[R2], mf[R2], ad[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const size_t row = base + nd[k2];
+                zr[k2] = io.elzero[row];
+                xv[k2] = io.x_rho[row];
+                inr[k2] = io.el_in_rho ? io.el_in_rho[row] : 0.0;
+                vx[k2] = (!io.linear && io.nwall > 0) ? io.vext[row] : 0.0;
+                mf[k2] = io.el_in_mf ? io.el_in_mf[row] : 0.0;
+                ad[k2] = io.add ? io.add[row] : 0.0;
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {   // the operations and their order of strided_store
+                double r;
+                if (zr[k2]) {
+                    r = inr[k2];
+                } else {
+                    r = (double)(h ? u[k2].y : u[k2].x);
+                    if (!io.linear) {
+                        r += log(xv[k2]) + io.mu[c];
+                        if (io.nwall > 0) r += vx[k2];
+                        if (io.el_in_mf) r += mf[k2];
+                    } else {
+                        if (io.el_in_rho) r += inr[k2] / xv[k2];
+                        if (io.el_in_mf) r += mf[k2];
+                    }
+                }
+                if (io.add) r += ad[k2];
+                io.out_el[base + nd[k2]] = r;
+                ssq += r * r;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) strided_store<T, MODE>(data, g, io, field, i, k1 + R1 * k2, u[k2], ssq);
     }
 }
 
@@ -553,6 +650,16 @@ __device__ __forceinline__ void dft_reg(Cx<T> *a)
     }
 }
 
+// Workgroup b of nb -> its work item.  Workgroups are dispatched to the 8 XCDs round robin (b mod
+// 8), each XCD with its own L2; here the workgroups of one XCD take one contiguous range of items,
+// so neighbouring items (adjacent x blocks whose real rows share 128-byte lines, the lines that
+// share a spectrum-table row) meet in one L2.
+__device__ inline int xcd_item(int b, int nb)
+{
+    const int x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // Strided axis pass of length M = R1 R2 (R2 <= R1 <= 16, both powers of two) with one wave per
 // workgroup: lane (i, n2) takes line i of L = 64 / R2 consecutive x and loads elements
 // R2 n1 + n2 straight into registers; an R1-point DFT, the twiddles W_M^(n2 k1), a transpose
@@ -568,11 +675,13 @@ __global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, lo
     constexpr int M = R1 * R2, L = 64 / R2, LD = R2 * R2 + 1;
     __shared__ Cx<T> tw[M];
     __shared__ Cx<T> buf[L * LD];
-    const int field = blockIdx.y;
+    // XCD-contiguous order within each field (every XCD gets an equal share of every field)
+    const int field = blockIdx.y, bx = io.xcd ? xcd_item(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int item = field * gridDim.x + bx;
     data += (long long)field * fstride;
     for (int j = threadIdx.x; j < M; j += 64) tw[j] = twtab[j];
     const int xg = (M0 + L - 1) / L;
-    const int other = blockIdx.x / xg, x0 = (blockIdx.x % xg) * L;
+    const int other = bx / xg, x0 = (bx % xg) * L;
     const StridedGeo sg{axis == 1 ? x0 + S2 * other : x0 + S1 * other, axis == 1 ? S1 : S2, x0, other, axis};
     const int i = threadIdx.x % L, n2 = threadIdx.x / L;
     const bool live = x0 + i < M0;
@@ -599,15 +708,60 @@ __global__ __launch_bounds__(64) void k_fft_strided(Cx<T> *__restrict__ data, lo
 #pragma unroll
         for (int m = 0; m < R2; ++m) u[m] = buf[i * LD + n2 * R2 + m];
         dft_reg<R2, SIGN>(u);
-        if (live) {
-#pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) strided_store<T, MODE>(data, sg, io, field, i, k1 + R1 * k2, u[k2], ssq);
-        }
+        if (live) strided_store_phase<T, MODE, R1, R2>(data, sg, io, field, i, k1, u, ssq);
     }
     if constexpr (MODE == kPassRows || MODE == kPassEl) {
         if (io.ss) {
             for (int o = 32; o > 0; o >>= 1) ssq += __shfl_down(ssq, o, 64);
-            if (threadIdx.x == 0) io.ss[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = ssq;
+            if (threadIdx.x == 0) io.ss[item] = ssq;
+        }
+    }
+}
+
+// The x axis (contiguous lines) as a register pass of length M = R1 R2 with one wave per
+// workgroup: lane (i, n2) takes line i of L = 64 / R2 consecutive x-lines, loads elements
+// R2 n1 + n2 (R2 lanes read R2 consecutive elements), an R1-point DFT, the twiddles W_M^(n2 k1),
+// and, in R1/R2 phases of R2 values of k1 through an LDS buffer of L R2^2 values (as
+// k_fft_strided), the R2-point DFTs; stores k1 + R1 k2.  Every load of the workgroup precedes
+// its first barrier, so the pass runs in place.
+template <typename T, int SIGN, int R1, int R2>
+__global__ __launch_bounds__(64) void k_fft_xline(Cx<T> *__restrict__ data, long long fstride, int M1, int M2,
+                                                  long long S1, long long S2, const Cx<T> *__restrict__ twtab)
+{
+    constexpr int M = R1 * R2, L = 64 / R2, LD = R2 * R2 + 1;
+    __shared__ Cx<T> tw[M];
+    __shared__ Cx<T> buf[L * LD];
+    data += (long long)blockIdx.y * fstride;
+    for (int j = threadIdx.x; j < M; j += 64) tw[j] = twtab[j];
+    const int i = threadIdx.x / R2, n2 = threadIdx.x - i * R2;
+    const int l = blockIdx.x * L + i;
+    const bool live = l < M1 * M2;
+    const int z = live ? l / M1 : 0, y = live ? l - z * M1 : 0;
+    const long long row = (long long)y * S1 + (long long)z * S2;
+    Cx<T> v[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? data[row + R2 * n1 + n2] : Cx<T>{(T)0, (T)0};
+    __syncthreads();   // tw
+    dft_reg<R1, SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < R1 / R2; ++q) {
+        if (q > 0) __syncthreads();   // the previous phase's reads are done
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {
+            const int k1 = R2 * q + t;
+            Cx<T> w = tw[n2 * k1];
+            if constexpr (SIGN > 0) w.y = -w.y;
+            buf[i * LD + t * R2 + n2] = cmul(v[k1], w);
+        }
+        __syncthreads();
+        const int k1 = n2 + R2 * q;
+        Cx<T> u[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) u[m] = buf[i * LD + n2 * R2 + m];
+        dft_reg<R2, SIGN>(u);
+        if (live) {
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) data[row + k1 + R1 * k2] = u[k2];
         }
     }
 }
@@ -751,6 +905,97 @@ __global__ void k_fwd_mul(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z, cons
     }
 }
 
+// The products in groups of 8 wave numbers {(+-kx, +-ky, +-kz)} (folded tables only): thread
+// (kx, ky, kz), 0 <= k_a <= M_a/2, takes the whole group, which holds the negative of each of its
+// points (point s, bit a set where axis a is negated; -point(s) = point(s ^ 7)) and whose table
+// values sit in one folded row (ky, kz) at the columns kx and -kx.  Every input point, table
+// element and output point is then read or written once (points repeat where k_a = 0 or M_a/2:
+// the same values are computed and stored twice by one thread).
+struct Group8 {
+    long long q[8];   // storage index of point s
+    long long trow;   // folded table row
+    int tx[2];        // table columns of +kx and -kx
+    int neg[8];       // bit d: an odd table changes sign at point s (axis d folded from above M_d/2)
+};
+__device__ inline bool group8(const FftGeo &f, long long t, Group8 &g)
+{
+    const int hx = f.M[0] / 2, hy = f.M[1] / 2, hz = f.M[2] / 2;
+    const long long nxr = hx + 1, nyr = hy + 1;
+    if (t >= nxr * nyr * (hz + 1)) return false;
+    const int kx = (int)(t % nxr);
+    const long long r = t / nxr;
+    const int ky = (int)(r % nyr), kz = (int)(r / nyr);
+    const int xs[2] = {kx, kx ? f.M[0] - kx : 0}, ys[2] = {ky, ky ? f.M[1] - ky : 0}, zs[2] = {kz, kz ? f.M[2] - kz : 0};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int y = ys[(s >> 1) & 1], z = zs[s >> 2];
+        g.q[s] = xs[s & 1] + (long long)y * f.S1 + (long long)z * f.S2;
+        g.neg[s] = (tab_neg(f.tg, 1, y) ? 2 : 0) | (tab_neg(f.tg, 2, z) ? 4 : 0);
+    }
+    g.trow = (long long)ky * f.tg.p1 + (long long)kz * f.tg.p2;
+    g.tx[0] = xs[0];
+    g.tx[1] = xs[1];
+    return true;
+}
+
+template <typename T, int ND>
+__global__ __launch_bounds__(kBlock) void k_fwd_mul8(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z,
+                                                     const T *__restrict__ tk, const T *__restrict__ uk,
+                                                     Cx<T> *__restrict__ O)
+{
+    Group8 g;
+    if (!group8(f, (long long)blockIdx.x * kBlock + threadIdx.x, g)) return;
+    constexpr int NP = (kNC + 1) / 2;
+    Cx<T> z[NP][8];
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) z[j][s] = (2 * j < d.C) ? Z[(size_t)j * f.ns + g.q[s]] : Cx<T>{(T)0, (T)0};
+    auto X = [&](int c, int s) { return unpack(z[c >> 1][s], z[c >> 1][s ^ 7], c & 1); };
+    Cx<T> acc[8];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o) {
+        if (o < d.nout) {
+            const int kind = d.kind[o];
+            Cx<T> y[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) y[s] = Cx<T>{(T)0, (T)0};
+#pragma unroll
+            for (int c = 0; c < kNC; ++c) {
+                if (c < d.C) {
+                    const T *tb = kind < kKindMf ? tk + ((size_t)c * (2 + ND) + kind) * f.tg.tn + g.trow
+                                                 : uk + (size_t)(kind - kKindMf + d.C * c) * f.tg.tn + g.trow;
+                    const T cf = kind < kKindMf ? (T)d.coef[o][c] : (T)1;
+                    const T w2[2] = {cf * tb[g.tx[0]], cf * tb[g.tx[1]]};
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        T w = w2[s & 1];
+                        const Cx<T> xv = X(c, s);
+                        if (kind >= 2 && kind < kKindMf) {   // imaginary spectrum: (i w) X
+                            if ((g.neg[s] >> (kind - 2)) & 1) w = -w;
+                            y[s] = cadd(y[s], Cx<T>{-w * xv.y, w * xv.x});
+                        } else {
+                            y[s] = cadd(y[s], cscale(xv, w));
+                        }
+                    }
+                }
+            }
+            if ((o & 1) == 0) {
+#pragma unroll
+                for (int s = 0; s < 8; ++s) acc[s] = y[s];
+                if (o + 1 == d.nout) {
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) O[(size_t)(o >> 1) * f.ns + g.q[s]] = acc[s];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    O[(size_t)(o >> 1) * f.ns + g.q[s]] = Cx<T>{acc[s].x - y[s].y, acc[s].y + y[s].x};
+            }
+        }
+    }
+}
+
 // rho-bar and mean-field rows at the nodes: row = Y - in - sub
 template <typename T>
 __global__ void k_fwd_extract(FftGeo f, Geo g, RowMap rm, const Cx<T> *__restrict__ O, const double *__restrict__ in_rb,
@@ -783,9 +1028,15 @@ __global__ void k_fwd_extract(FftGeo f, Geo g, RowMap rm, const Cx<T> *__restric
 
 // Euler-Lagrange sums of the adjoint fields (pairs Z, fields A_c, B_c,d in
 // the order c (1 + ND) + {0, 1 + d}, then S3):
-// E_c = K_delta,c A_c - sum_d (i K'_c,d) B_c,d + K_theta,c S3, packed in pairs
-template <typename T, int ND>
-__global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *__restrict__ tk, Cx<T> *__restrict__ O)
+// E_c = K_delta,c A_c - sum_d (i K'_c,d) B_c,d + K_theta,c S3, packed in pairs.
+// DF: the fields are the derivatives of Phi in the rho-bar order (D3 D2 D1 D0 Dv2_d Dv1_d) and
+// A_c = D2 + D1 i4pir_c + D0 i4pirsq_c, B_c,d = Dv2_d + Dv1_d i4pir_c, S3 = D3 are formed here.
+struct ElConsts {
+    double i4pir[kNC], i4pirsq[kNC];
+};
+template <typename T, int ND, bool DF>
+__global__ void k_el_mul(FftGeo f, int C, ElConsts ec, const Cx<T> *__restrict__ Z, const T *__restrict__ tk,
+                         Cx<T> *__restrict__ O)
 {
     const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (m >= f.n) return;
@@ -794,18 +1045,34 @@ __global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *
         const Cx<T> *z = Z + (size_t)(j >> 1) * f.ns;
         return unpack(z[ix.q], z[ix.qn], j & 1);
     };
-    const Cx<T> S3 = field(C * (1 + ND));
+    Cx<T> S3, D[4 + 2 * ND];
+    if constexpr (DF) {
+#pragma unroll
+        for (int j = 0; j < 4 + 2 * ND; ++j) D[j] = field(j);
+        S3 = D[0];
+    } else {
+        S3 = field(C * (1 + ND));
+    }
     Cx<T> acc{(T)0, (T)0};
 #pragma unroll
     for (int c = 0; c < kNC; ++c) {
         if (c < C) {
             const T *t = tk + (size_t)c * (2 + ND) * f.tg.tn + ix.t;
-            Cx<T> e = cadd(cscale(S3, t[0]), cscale(field(c * (1 + ND)), t[f.tg.tn]));
+            Cx<T> A;
+            if constexpr (DF)
+                A = cadd(D[1], cadd(cscale(D[2], (T)ec.i4pir[c]), cscale(D[3], (T)ec.i4pirsq[c])));
+            else
+                A = field(c * (1 + ND));
+            Cx<T> e = cadd(cscale(S3, t[0]), cscale(A, t[f.tg.tn]));
 #pragma unroll
             for (int dd = 0; dd < ND; ++dd) {
                 T kv = t[(size_t)(2 + dd) * f.tg.tn];
                 if ((ix.neg >> dd) & 1) kv = -kv;
-                const Cx<T> B = field(c * (1 + ND) + 1 + dd);
+                Cx<T> B;
+                if constexpr (DF)
+                    B = cadd(D[4 + dd], cscale(D[4 + ND + dd], (T)ec.i4pir[c]));
+                else
+                    B = field(c * (1 + ND) + 1 + dd);
                 e = cadd(e, Cx<T>{kv * B.y, -kv * B.x});   // -(i kv) B
             }
             if ((c & 1) == 0) {
@@ -813,6 +1080,80 @@ __global__ void k_el_mul(FftGeo f, int C, const Cx<T> *__restrict__ Z, const T *
                 if (c + 1 == C) O[(size_t)(c >> 1) * f.ns + ix.q] = acc;
             } else {
                 O[(size_t)(c >> 1) * f.ns + ix.q] = Cx<T>{acc.x - e.y, acc.y + e.x};
+            }
+        }
+    }
+}
+
+// k_el_mul over groups of 8 wave numbers (Group8; folded tables only): the table values of the
+// group's row are loaded once, then the four point pairs (s, s ^ 7) are formed in turn.
+template <typename T, int ND, bool DF>
+__global__ __launch_bounds__(kBlock) void k_el_mul8(FftGeo f, int C, ElConsts ec, const Cx<T> *__restrict__ Z,
+                                                    const T *__restrict__ tk, Cx<T> *__restrict__ O)
+{
+    Group8 g;
+    if (!group8(f, (long long)blockIdx.x * kBlock + threadIdx.x, g)) return;
+    constexpr int NT = 2 + ND;
+    T tv[kNC][NT][2];
+#pragma unroll
+    for (int c = 0; c < kNC; ++c)
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const T *tb = tk + ((size_t)c * NT + k) * f.tg.tn + g.trow;
+            tv[c][k][0] = c < C ? tb[g.tx[0]] : (T)0;
+            tv[c][k][1] = c < C ? tb[g.tx[1]] : (T)0;
+        }
+    constexpr int NF = DF ? 4 + 2 * ND : kNC * (1 + ND) + 1;
+    const int nf = DF ? 4 + 2 * ND : C * (1 + ND) + 1;
+#pragma unroll
+    for (int s0 = 0; s0 < 4; ++s0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {   // point s and its negative
+            const int s = h ? (s0 ^ 7) : s0;
+            Cx<T> fld[NF];
+#pragma unroll
+            for (int j = 0; j < NF; j += 2) {
+                if (j < nf) {
+                    const Cx<T> *zp = Z + (size_t)(j >> 1) * f.ns;
+                    const Cx<T> a = zp[g.q[s]], b = zp[g.q[s ^ 7]];
+                    fld[j] = unpack(a, b, 0);
+                    if (j + 1 < NF) fld[j + 1] = unpack(a, b, 1);
+                }
+            }
+            const T sg[2] = {(T)1, (T)-1};
+            Cx<T> S3 = fld[0];
+            if constexpr (!DF) {   // field C (1 + ND), selected with compile-time indices
+#pragma unroll
+                for (int j = 0; j < NF; ++j)
+                    if (j == C * (1 + ND)) S3 = fld[j];
+            }
+            Cx<T> acc{(T)0, (T)0};
+#pragma unroll
+            for (int c = 0; c < kNC; ++c) {
+                if (c < C) {
+                    Cx<T> A;
+                    if constexpr (DF)
+                        A = cadd(fld[1], cadd(cscale(fld[2], (T)ec.i4pir[c]), cscale(fld[3], (T)ec.i4pirsq[c])));
+                    else
+                        A = fld[c * (1 + ND)];
+                    Cx<T> e = cadd(cscale(S3, tv[c][0][s & 1]), cscale(A, tv[c][1][s & 1]));
+#pragma unroll
+                    for (int dd = 0; dd < ND; ++dd) {
+                        const T kv = tv[c][2 + dd][s & 1] * sg[(g.neg[s] >> dd) & 1];
+                        Cx<T> B;
+                        if constexpr (DF)
+                            B = cadd(fld[4 + dd], cscale(fld[4 + ND + dd], (T)ec.i4pir[c]));
+                        else
+                            B = fld[c * (1 + ND) + 1 + dd];
+                        e = cadd(e, Cx<T>{kv * B.y, -kv * B.x});   // -(i kv) B
+                    }
+                    if ((c & 1) == 0) {
+                        acc = e;
+                        if (c + 1 == C) O[(size_t)(c >> 1) * f.ns + g.q[s]] = acc;
+                    } else {
+                        O[(size_t)(c >> 1) * f.ns + g.q[s]] = Cx<T>{acc.x - e.y, acc.y + e.x};
+                    }
+                }
             }
         }
     }
@@ -906,46 +1247,70 @@ __global__ __launch_bounds__(kFftBlock) void k_fft_xprod(const Cx<T> *__restrict
     }
 }
 
+// Primary x-lines of the fused register x pass, numbered l = 0, 1, ...: every line (y, z) of the
+// planes 0 < z < M2 - z (z-major, y fastest), then the self-mirror planes (z = 0, and z = M2/2
+// when M2 is even) with 0 <= y <= M1/2.  The mirror lines (-y, -z) of the primary lines are the
+// other lines, each once, except the self-mirror lines (y, z) = (-y, -z).  False past the last.
+__device__ inline bool xprod_line(int l, int M1, int M2, int &y, int &z)
+{
+    const int zr = (M2 - 1) / 2;
+    if (l < zr * M1) {
+        z = 1 + l / M1;
+        y = l - (z - 1) * M1;
+        return true;
+    }
+    l -= zr * M1;
+    const int ms = M1 / 2 + 1, nself = (M2 > 1 && M2 % 2 == 0) ? 2 : 1;
+    const int sp = l / ms;
+    if (sp >= nself) return false;
+    z = sp ? M2 / 2 : 0;
+    y = l - sp * ms;
+    return true;
+}
+inline long long xprod_primary_lines(int M1, int M2)
+{
+    const int zr = (M2 - 1) / 2, nself = (M2 > 1 && M2 % 2 == 0) ? 2 : 1;
+    return (long long)zr * M1 + (long long)nself * (M1 / 2 + 1);
+}
+
 // Fused x pass with the x transforms in registers, for x-lines of length M = R1 R2 (R1 = 8 or 16,
 // R2 = 8 or 4) and NOP output pairs whose NOP x R1 spectrum values fit in registers.  The
-// workgroup has 2 L R2 threads: lanes (i, n2) of the first half take line i of L consecutive
-// x-lines (y, z), the second half their mirror lines (-y, -z).  Per input pair every lane loads
-// elements R2 n1 + n2 of its line, runs an R1-point DFT, the twiddles W_M^(n2 k1), a transpose
-// through LDS and R1/R2 R2-point DFTs, leaving spectrum values k = n2 + R2 q + R1 k2; those go
-// back to LDS in natural order, and the first half reads the mirror spectrum at -k, splits the
-// pair and accumulates the terms into output pairs held in registers.  The outputs return
-// through LDS two pairs at a time (one per half), through the same two-stage transform with
-// SIGN +1, and are stored for the y and z passes.
+// workgroup has 2 L R2 threads: lanes (i, n2) of the first half take primary line i of L
+// consecutive primary x-lines (y, z) (xprod_line), the second half their mirror lines (-y, -z),
+// so every line is loaded once.  Per input pair every lane loads elements R2 n1 + n2 of its
+// line, runs an R1-point DFT, the twiddles W_M^(n2 k1), a transpose through LDS and R1/R2
+// R2-point DFTs, leaving spectrum values k = n2 + R2 q + R1 k2; those go back to LDS in natural
+// order, every lane reads the partner line's spectrum at -k (the spectrum of its line's
+// negative), splits the pair and accumulates the terms of its own line into output pairs held
+// in registers (the table row of a line and of its mirror is the same folded row).  The outputs
+// return through LDS one pair at a time, through the same two-stage transform with SIGN +1,
+// and both halves store their lines (a self-mirror line once) for the y and z passes.
 template <typename T, int R1, int R2, int NOP>
 __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T> *__restrict__ Zin,
                                                                     Cx<T> *__restrict__ Oout, long long ns, int M1,
                                                                     int M2, long long S1, long long S2, int L,
                                                                     const Cx<T> *__restrict__ twtab,
                                                                     const T *__restrict__ tk, const T *__restrict__ uk,
-                                                                    TabGeo tg, TermList tl)
+                                                                    TabGeo tg, TermList tl, int xcd)
 {
     constexpr int M = R1 * R2, LD = M + 1;
     extern __shared__ char smem_raw[];
     Cx<T> *tw = reinterpret_cast<Cx<T> *>(smem_raw);   // [M]
-    Cx<T> *buf = tw + M;                               // [2 L][LD]: lines, then mirror lines
+    Cx<T> *buf = tw + M;                               // [2 L][LD]: primary lines, then mirror lines
     const int tid = threadIdx.x, half = tid / (L * R2), r0 = tid - half * L * R2;
     const int i = r0 / R2, n2 = r0 - i * R2;
     const int nth = 2 * L * R2;
     for (int j = tid; j < M; j += nth) tw[j] = twtab[j];
-    const int l = blockIdx.x * L + i;
-    const bool live = l < M1 * M2;
-    int y = 0, z = 0;
-    if (live) {
-        z = l / M1;
-        y = l - z * M1;
-    }
-    const int ym = y ? M1 - y : 0, zm = z ? M2 - z : 0;
+    int yp = 0, zp = 0;
+    const bool live = xprod_line((xcd ? xcd_item(blockIdx.x, gridDim.x) : (int)blockIdx.x) * L + i, M1, M2, yp, zp);
+    const int ym = yp ? M1 - yp : 0, zm = zp ? M2 - zp : 0;
+    const bool self = ym == yp && zm == zp;
+    const int y = half ? ym : yp, z = half ? zm : zp;   // this lane's line
     const long long row = (long long)y * S1 + (long long)z * S2;
-    const long long src_row = half ? (long long)ym * S1 + (long long)zm * S2 : row;
     const long long trow = tab_fold(tg, 1, y) * tg.p1 + tab_fold(tg, 2, z) * tg.p2;   // the line's table row
     const bool neg_y = tab_neg(tg, 1, y), neg_z = tab_neg(tg, 2, z);
-    Cx<T> *mine = buf + (size_t)(half * L + i) * LD;   // this lane's line in LDS
-    const Cx<T> *mirror = buf + (size_t)(L + i) * LD;  // the mirror line of line i
+    Cx<T> *mine = buf + (size_t)(half * L + i) * LD;                 // this lane's line in LDS
+    const Cx<T> *partner = buf + (size_t)((1 - half) * L + i) * LD;  // the line of the negative spectrum
     // spectrum position of register r (= q R2 + k2)
     auto kpos = [&](int r) { return n2 + R2 * (r / R2) + R1 * (r % R2); };
     Cx<T> acc[NOP][R1];
@@ -977,18 +1342,18 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
         const Cx<T> *src = Zin + (size_t)jp * ns;
         Cx<T> v[R1], X[R1];
 #pragma unroll
-        for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? src[src_row + R2 * n1 + n2] : Cx<T>{(T)0, (T)0};
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? src[row + R2 * n1 + n2] : Cx<T>{(T)0, (T)0};
         forward(v, X);
         // spectra in natural order: every half writes its own line
 #pragma unroll
         for (int r = 0; r < R1; ++r) mine[kpos(r)] = X[r];
         __syncthreads();
-        if (half == 0 && live) {
-            Cx<T> Yr[R1];   // mirror spectrum at -k
+        if (live) {
+            Cx<T> Yr[R1];   // the spectrum at -k
 #pragma unroll
             for (int r = 0; r < R1; ++r) {
                 const int k = kpos(r);
-                Yr[r] = mirror[k ? M - k : 0];
+                Yr[r] = partner[k ? M - k : 0];
             }
             for (int h = 0; h < 2; ++h) {
                 const int f = 2 * jp + h;
@@ -1024,21 +1389,15 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
         }
         __syncthreads();
     }
-    // output pairs back along x, two at a time: the first half writes both, each half transforms one
+    // output pairs back along x, one at a time, every lane its own line
     const int nop = (tl.nout + 1) / 2;
+    const bool store = live && (half == 0 || !self);
 #pragma unroll
-    for (int p0 = 0; p0 < NOP; p0 += 2) {
-        if (p0 >= nop) break;
-        if (half == 0) {
+    for (int p = 0; p < NOP; ++p) {
+        if (p >= nop) break;
 #pragma unroll
-            for (int r = 0; r < R1; ++r) buf[(size_t)i * LD + kpos(r)] = acc[p0][r];
-            if (p0 + 1 < NOP) {
-#pragma unroll
-                for (int r = 0; r < R1; ++r) buf[(size_t)(L + i) * LD + kpos(r)] = acc[p0 + 1 < NOP ? p0 + 1 : p0][r];
-            }
-        }
+        for (int r = 0; r < R1; ++r) mine[kpos(r)] = acc[p][r];
         __syncthreads();
-        const int p = p0 + half;
         Cx<T> v[R1];
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) v[n1] = mine[R2 * n1 + n2];
@@ -1051,7 +1410,7 @@ __global__ __launch_bounds__(2 * (64 / R2) * R2) void k_fft_xprod_reg(const Cx<T
             mine[k1 * R2 + n2] = cmul(v[k1], w);
         }
         __syncthreads();
-        if (p < nop && live) {
+        if (store) {
 #pragma unroll
             for (int q = 0; q < R1 / R2; ++q) {
                 const int k1 = n2 + R2 * q;
@@ -1276,12 +1635,17 @@ inline int reg_radices(int M, int &R1, int &R2)
 // The one-wave register pass where it has radices and at least kRegPassWaves waves (one per
 // workgroup); fewer would leave SIMDs idle, and the LDS pass runs 4 waves per workgroup.
 constexpr long long kRegPassWaves = 1024;
+inline long long reg_pass_groups(const FftPlan &F, int axis, int R2)
+{
+    const int L = 64 / R2;
+    if (axis == 0) return (F.n / F.M[0] + L - 1) / L;
+    return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]);
+}
 inline bool use_reg_pass(const FftPlan &F, int axis, int nfield, int &R1, int &R2)
 {
-    if (axis == 0 || F.regpass == 0 || !reg_radices(F.M[axis], R1, R2)) return false;
+    if (F.regpass == 0 || !reg_radices(F.M[axis], R1, R2)) return false;
     if (F.regpass == 2) return true;
-    const int L = 64 / R2;
-    return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]) * nfield >= kRegPassWaves;
+    return reg_pass_groups(F, axis, R2) * nfield >= kRegPassWaves;
 }
 
 // workgroups of a pass over nfield fields
@@ -1289,10 +1653,7 @@ template <typename T>
 long long pass_blocks(const FftPlan &F, int axis, int nfield)
 {
     int R1 = 0, R2 = 0;
-    if (use_reg_pass(F, axis, nfield, R1, R2)) {
-        const int L = 64 / R2;
-        return (long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1]) * nfield;
-    }
+    if (use_reg_pass(F, axis, nfield, R1, R2)) return reg_pass_groups(F, axis, R2) * nfield;
     return pass_shape<T>(F, axis, nfield).groups * nfield;
 }
 
@@ -1300,11 +1661,12 @@ template <typename T, int SIGN, int MODE>
 void launch_strided_reg(const FftPlan &F, Cx<T> *data, int nfield, int axis, int R1, int R2, hipStream_t s,
                         const PassIO &io)
 {
-    const int L = 64 / R2;
-    const dim3 grid((unsigned)((long long)((F.M[0] + L - 1) / L) * (axis == 1 ? F.M[2] : F.M[1])), (unsigned)nfield);
+    const dim3 grid((unsigned)reg_pass_groups(F, axis, R2), (unsigned)nfield);
+    PassIO iox = io;
+    iox.xcd = F.xcdmap;
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, data, F.ns, F.M[0], F.M[1], F.M[2], F.S1, F.S2, axis,
-                           twiddles<T>(F, axis), io);
+                           twiddles<T>(F, axis), iox);
     };
     if (R1 == 8 && R2 == 4)
         go(k_fft_strided<T, SIGN, MODE, 8, 4>);
@@ -1325,6 +1687,28 @@ long long fft_pass(const FftPlan &F, Cx<T> *data, int nfield, int sign, int axis
     if (M == 1 || nfield <= 0) return 0;
     if (mode != kPassPlain && (axis == 0 || !F.wraps)) throw std::logic_error("fused pass ends need a strided axis of a wrapping grid");
     int R1 = 0, R2 = 0;
+    if (use_reg_pass(F, axis, nfield, R1, R2) && axis == 0) {
+        const dim3 grid((unsigned)reg_pass_groups(F, 0, R2), (unsigned)nfield);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(64), 0, s, data, F.ns, F.M[1], F.M[2], F.S1, F.S2, twiddles<T>(F, 0));
+        };
+        auto by_sign = [&](auto r1, auto r2) {
+            constexpr int A = decltype(r1)::value, B = decltype(r2)::value;
+            if (sign < 0)
+                go(k_fft_xline<T, -1, A, B>);
+            else
+                go(k_fft_xline<T, 1, A, B>);
+        };
+        if (R1 == 8 && R2 == 4)
+            by_sign(std::integral_constant<int, 8>{}, std::integral_constant<int, 4>{});
+        else if (R1 == 8 && R2 == 8)
+            by_sign(std::integral_constant<int, 8>{}, std::integral_constant<int, 8>{});
+        else if (R1 == 16 && R2 == 8)
+            by_sign(std::integral_constant<int, 16>{}, std::integral_constant<int, 8>{});
+        else
+            by_sign(std::integral_constant<int, 16>{}, std::integral_constant<int, 16>{});
+        return pass_blocks<T>(F, axis, nfield);
+    }
     if (use_reg_pass(F, axis, nfield, R1, R2)) {
         if (sign < 0) {
             if (mode == kPassGather)
@@ -1412,12 +1796,12 @@ bool xprod_reg(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, c
     if (!xreg_fits(F, tl, R1, R2)) return false;
     const int nop = (tl.nout + 1) / 2;
     const int L = 64 / R2;
-    const long long lines = F.n / F.M[0];
+    const long long lines = xprod_primary_lines(F.M[1], F.M[2]);
     const dim3 grid((unsigned)((lines + L - 1) / L)), block(2 * L * R2);
     const size_t lds = sizeof(Cx<T>) * ((size_t)F.M[0] + 2 * (size_t)L * (F.M[0] + 1));
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, block, lds, s, Z, O, F.ns, F.M[1], F.M[2], F.S1, F.S2, L, twiddles<T>(F, 0), tk,
-                           uk, tab_geo(F), tl);
+                           uk, tab_geo(F), tl, F.xcdmap);
     };
     auto by_nop = [&](auto r1, auto r2) {
         constexpr int A = decltype(r1)::value, B = decltype(r2)::value;
@@ -1457,6 +1841,12 @@ void fft_xprod(const FftPlan &F, const Cx<T> *Z, Cx<T> *O, const TermList &tl, i
     const size_t lds = sizeof(Cx<T>) * ((size_t)M + (4 + (size_t)nop) * lpb * (M + 1));
     hipLaunchKernelGGL(k_fft_xprod<T>, dim3((unsigned)((lines + lpb - 1) / lpb)), dim3(kFftBlock), lds, s, Z, O, F.ns,
                        F.M[0], F.M[1], F.M[2], F.S1, F.S2, rc, lpb, twiddles<T>(F, 0), tk, uk, tab_geo(F), tl);
+}
+
+// threads of the group-of-8 product kernels: one per (kx, ky, kz) with 0 <= k_a <= M_a / 2
+long long group8_count(const FftPlan &F)
+{
+    return (long long)(F.M[0] / 2 + 1) * (F.M[1] / 2 + 1) * (F.M[2] / 2 + 1);
 }
 
 FftGeo fft_geo(const FftPlan &F, int which)
@@ -1536,6 +1926,7 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         F->C = C;
         if (const char *e = std::getenv("TMI_FFT_REGPASS")) F->regpass = std::string(e) == "0" ? 0 : (std::string(e) == "2" ? 2 : 1);
         if (const char *e = std::getenv("TMI_FFT_XREG")) F->xreg = std::string(e) != "0";
+        if (const char *e = std::getenv("TMI_FFT_XCDMAP")) F->xcdmap = std::string(e) != "0";
         for (int d = 0; d < nd; ++d) {
             const int n = P.p.nx[d], h = P.pad_u.h[d], hf = P.pad_f.h[d];
             const bool per = P.p.bc[d][0] == TMI_PERIODIC;
@@ -1585,7 +1976,9 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             TMI_HIP_CHECK(hipMemcpy(F->twf[d], tf.data(), M * sizeof(Cx<float>), hipMemcpyHostToDevice));
         }
         // output descriptors
-        const int ND = nd, nadj = C * (1 + ND) + 1;
+        const int ND = nd;
+        F->dform = F->wraps && fmt && (4 + 2 * ND + 1) / 2 < (C * (1 + ND) + 2) / 2;
+        const int nadj = F->dform ? 4 + 2 * ND : C * (1 + ND) + 1;
         const auto &b = P.bulk;
         FwdDesc &f0 = F->fwd[0], &f1 = F->fwd[1];
         RowMap &r0 = F->rows[0], &r1 = F->rows[1];
@@ -1658,7 +2051,26 @@ void fft_setup(Problem &P, bool mf, bool fmt)
                 close_input(tl);
             }
         }
-        {
+        if (F->dform) {   // inputs D3, D2, D1, D0, Dv2_d, Dv1_d (the derivatives of Phi)
+            TermList &tl = F->tel;
+            tl.nout = C;
+            for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND), 0, 1.0);   // D3: theta
+            close_input(tl);
+            for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 1, 0, 1.0);   // D2: delta
+            close_input(tl);
+            for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 1, 0, b.inv_4pir[c]);   // D1
+            close_input(tl);
+            for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 1, 0, b.inv_4pirsq[c]);   // D0
+            close_input(tl);
+            for (int d = 0; d < ND; ++d) {   // Dv2_d: -(i K'_d)
+                for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 2 + d, 1, -1.0);
+                close_input(tl);
+            }
+            for (int d = 0; d < ND; ++d) {   // Dv1_d
+                for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 2 + d, 1, -b.inv_4pir[c]);
+                close_input(tl);
+            }
+        } else {
             TermList &tl = F->tel;
             tl.nout = C;
             for (int c = 0; c < C; ++c) {
@@ -1843,7 +2255,7 @@ void fft_free(Problem &P)
 
 FftPairs fft_pairs(const Problem &P)
 {
-    return FftPairs{P.fft->zbuf, P.fft->ns, P.fft->S1, P.fft->S2};
+    return FftPairs{P.fft->zbuf, P.fft->ns, P.fft->S1, P.fft->S2, P.fft->dform ? 1 : 0};
 }
 bool fft_node_input(const Problem &P)
 {
@@ -1944,12 +2356,21 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
             fft_xprod<T>(F, Z, O, F.tfwd[w], F.xlpb[0], fmt_spectra<T>(F), attr_spectra<T>(F), s);
         } else {
             fft_pass<T>(F, Z, nip, -1, 0, s);
-            if (P.p.ndim == 2)
+            if (F.fold) {
+                const dim3 g8((unsigned)grid_for(group8_count(F)));
+                if (P.p.ndim == 2)
+                    hipLaunchKernelGGL((k_fwd_mul8<T, 2>), g8, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                                       attr_spectra<T>(F), O);
+                else
+                    hipLaunchKernelGGL((k_fwd_mul8<T, 3>), g8, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                                       attr_spectra<T>(F), O);
+            } else if (P.p.ndim == 2) {
                 hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z,
                                    fmt_spectra<T>(F), attr_spectra<T>(F), O);
-            else
+            } else {
                 hipLaunchKernelGGL((k_fwd_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z,
                                    fmt_spectra<T>(F), attr_spectra<T>(F), O);
+            }
             fft_pass<T>(F, O, nop, +1, 0, s);
         }
         if (ends) io.ss = ss;
@@ -1979,7 +2400,7 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
 {
     if (nss) *nss = 0;
     const FftPlan &F = *P.fft;
-    const int C = P.p.ncomp, nd = P.p.ndim, nadj = C * (1 + nd) + 1;
+    const int C = P.p.ncomp, nd = P.p.ndim, nadj = F.dform ? 4 + 2 * nd : C * (1 + nd) + 1;
     Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
     Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gf = make_geo(P, P.pad_f);
@@ -2018,12 +2439,25 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
             fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
         } else {
             fft_pass<T>(F, Z, nip, -1, 0, s);
-            if (nd == 2)
-                hipLaunchKernelGGL((k_el_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z,
-                                   fmt_spectra<T>(F), O);
-            else
-                hipLaunchKernelGGL((k_el_mul<T, 3>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z,
-                                   fmt_spectra<T>(F), O);
+            ElConsts ec{};
+            for (int c = 0; c < C; ++c) {
+                ec.i4pir[c] = P.bulk.inv_4pir[c];
+                ec.i4pirsq[c] = P.bulk.inv_4pirsq[c];
+            }
+            const long long nthr = F.fold ? group8_count(F) : F.n;
+            auto go = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(grid_for(nthr)), dim3(kBlock), 0, s, fg, C, ec, Z, fmt_spectra<T>(F), O);
+            };
+            if (F.fold) {
+                if (nd == 2)
+                    F.dform ? go(k_el_mul8<T, 2, true>) : go(k_el_mul8<T, 2, false>);
+                else
+                    F.dform ? go(k_el_mul8<T, 3, true>) : go(k_el_mul8<T, 3, false>);
+            } else if (nd == 2) {
+                F.dform ? go(k_el_mul<T, 2, true>) : go(k_el_mul<T, 2, false>);
+            } else {
+                F.dform ? go(k_el_mul<T, 3, true>) : go(k_el_mul<T, 3, false>);
+            }
             fft_pass<T>(F, O, nop, +1, 0, s);
         }
         if (ends) io.ss = ss;
@@ -2060,7 +2494,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
     if (!fft_fmt(P)) throw std::logic_error("the FMT sums are not in FFT form");
     const FftPlan &F = *P.fft;
     if (which == 1 && !(F.xlpb[0] || F.xlpb[1])) throw std::logic_error("the fused x pass is not in use");
-    const int C = P.p.ncomp, nadj = C * (1 + P.p.ndim) + 1;
+    const int C = P.p.ncomp, nadj = F.dform ? 4 + 2 * P.p.ndim : C * (1 + P.p.ndim) + 1;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
     std::function<void()> fill;

@@ -247,6 +247,7 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
 struct FftPairs {
     void *buf;
     long long ns, S1, S2;
+    int dform;   // the pairs carry the derivatives of Phi (rho-bar order) instead of A_c, B_c,d, S3
 };
 FftPairs fft_pairs(const Problem &P);
 bool fft_node_input(const Problem &P);   // every density product reads the node arrays (no density halo)

@@ -119,6 +119,18 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
         else
             dst[(size_t)f * fstride + p] = v;
     };
+    if (pairs.buf && pairs.dform) {   // the derivatives themselves, rho-bar order: D3 D2 D1 D0 Dv2_d Dv1_d
+        put(0, (T)out[3]);
+        put(1, (T)out[2]);
+        put(2, (T)out[1]);
+        put(3, (T)out[0]);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            put(4 + d, (T)out[4 + ND + d]);
+            put(4 + ND + d, (T)out[4 + d]);
+        }
+        return;   // 4 + 2 ND fields: whole pairs
+    }
 #pragma unroll
     for (int c = 0; c < kNC; ++c) {   // compile-time component index: the constants stay in registers
         if (c < C) {
@@ -1066,6 +1078,29 @@ __global__ void k_reduce_rows(const double *__restrict__ partial, int nb, int k,
     }
 }
 
+// sum of nb partial sums by one workgroup of 1024 threads, four independent loads in flight per
+// thread (a single wide block: the sum is one short dependent chain instead of nb / 256 loads);
+// the order is fixed, so the result is the same on every call
+constexpr int kSumBlock = 1024;
+__global__ __launch_bounds__(kSumBlock) void k_sum_partials(const double *__restrict__ partial, int nb,
+                                                            double *__restrict__ out, double *__restrict__ mirror)
+{
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * kSumBlock < nb; b += 4 * kSumBlock) {
+        a0 += partial[b];
+        a1 += partial[b + kSumBlock];
+        a2 += partial[b + 2 * kSumBlock];
+        a3 += partial[b + 3 * kSumBlock];
+    }
+    for (; b < nb; b += kSumBlock) a0 += partial[b];
+    const double s = block_sum<kSumBlock>((a0 + a1) + (a2 + a3));
+    if (threadIdx.x == 0) {
+        if (out) out[0] = s;
+        if (mirror) mirror[0] = s;
+    }
+}
+
 __global__ void k_axpy(double a, const double *__restrict__ x, double *__restrict__ y, long long n)
 {
     long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -1741,7 +1776,7 @@ void residual_sumsq_device(Problem &P, const double *x, double *F, double *parti
     long long cnt = 0;
     residual_impl(P, x, F, s, nullptr, P.fft ? partials : nullptr, &cnt);
     if (cnt > 0) {
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, partials, (int)cnt, 1, sumsq_d, nullptr);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumBlock), 0, s, partials, (int)cnt, sumsq_d, nullptr);
     } else {   // the blocks and order of dev_dot
         const long long n = (long long)P.nunk * P.N;
         const unsigned nb = (unsigned)std::min<long long>(1024, (n + kBlock - 1) / kBlock);
@@ -1758,7 +1793,7 @@ double residual_sumsq(Problem &P, const double *x, double *F, hipStream_t s)
     residual_impl(P, x, F, s, nullptr, ss, &cnt);
     if (cnt > cap) throw std::logic_error("partial sums past their buffer");
     if (cnt > 0) {
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, ss, (int)cnt, 1, nullptr, P.h_red_dev);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kSumBlock), 0, s, ss, (int)cnt, nullptr, P.h_red_dev);
         TMI_HIP_CHECK(hipStreamSynchronize(s));
         return P.h_red[0];
     }
