@@ -31,12 +31,15 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 COPY_PROBE_BYTES = 512 << 20
-# Mean HBM bytes per launch of the fused x pass (k_fft_xprod_reg) in a fill, fetch + write in MiB: rocprofv3
-# --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with --fill-only --copy-probe, scaled by the
-# counters' readings on the 512 MiB copy (x2.0 and x1.0).  Keys: workload.
-XPROD_TRAFFIC_MIB = {"periodic64": 21.600 + 8.000, "solute128": 686.4 + 64.0}
-# Mean HBM bytes per launch of the plain FFT axis passes (same runs), MiB
-AXIS_TRAFFIC_MIB = {"periodic64": 8.885 + 8.800, "solute128": None}
+# HBM bytes per launch, fetch + write in MiB, of the density product's rows pass in a fill (the dominant
+# kernel on wrapping grids): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with
+# --fill-only --copy-probe, that launch's counters scaled by their readings on the 512 MiB copy (x2.0 and
+# x1.0).  Keys: workload.
+ROWS_TRAFFIC_MIB = {"periodic64": 34.1 + 22.0, "solute128": 442.0 + 222.9}
+# The same for the fused x pass (mean per launch) and the plain FFT axis passes, where they are the
+# dominant kernel
+XPROD_TRAFFIC_MIB = {"periodic64": (7.6 + 12.0 + 14.9 + 4.0) / 2, "solute128": None}
+AXIS_TRAFFIC_MIB = {"periodic64": None, "solute128": None}
 
 
 def stencil_flops_per_node(dp, spec):
@@ -252,32 +255,52 @@ def bench_workload(spec, label, args, newton_tol=1e-8, newton=True):
                               "theta": [dp.info.sten_len[1][c] for c in range(C)],
                               "uatt": [dp.info.sten_len[2][q] for q in range(C * C)]}
     if dp.info.fmt_fft:
-        # every stencil sum is an FFT convolution.  Dominant kernel: the fused x pass (forward x transform,
-        # spectral products, inverse x transform; the longest launch of a fill) when it is in use, else the
-        # axis passes; both timed with HIP events around their launches in a fill's order (tmi_fft_timed)
+        # every stencil sum is an FFT convolution.  Dominant kernel: the longer launch of the density
+        # product's last pass on a wrapping grid (inverse z transform of the output pairs fused with the
+        # rho-bar and mean-field rows) and the fused x pass when it is in use, else the axis passes; each
+        # timed with HIP events around its launches in a fill's order (tmi_fft_timed)
         out["fill_stage_ms"] = dict(zip(["pad_rho", "rb_mf_rows_fft", "mf_rows_direct", "fmt_point", "el_rows_fft"],
                                         [float(v) for v in stages]))
         fused = dp.info.fmt_fft == 2
         reps = max(5, args.steps // 5)
         fp = dp.fft_timed(0, reps)
-        ft = dp.fft_timed(1, reps) if fused else fp
+        try:
+            fr = dp.fft_timed(2, reps)
+        except RuntimeError:
+            fr = None
+        axis = {"ms_per_fill": fp["ms"], "launches_per_fill": fp["launches"],
+                "achieved_gb_s": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9,
+                "frac": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                "kernels": "k_fft_strided<double, +-1, 0, R1, R2> (y, z) and k_fft_xline (x), plain mode"}
+        fx = dp.fft_timed(1, reps) if fused else None
+        if fr is not None and fx is not None and fx["ms"] / fx["launches"] > fr["ms"] / fr["launches"]:
+            fr = None   # the fused x pass is the longer launch
+        if fr is not None:
+            ft, tr = fr, ROWS_TRAFFIC_MIB.get(label)
+            kernel = "k_fft_strided<double, 1, 2, 16, 8> (density rows pass)" if label == "solute128" else \
+                "k_fft_strided<double, 1, 2, R1, R2> (density rows pass)"
+            note = ("last inverse z pass of the density product fused with the rho-bar and mean-field rows: output "
+                    "pairs read once, each row's unknowns read and residual written once (algorithmic), HIP events "
+                    "on the launch stream")
+        elif fused:
+            ft = fx
+            tr = XPROD_TRAFFIC_MIB.get(label)
+            kernel = "k_fft_xprod_reg" if dp.info.fft_xreg else "k_fft_xprod"
+            note = ("fused x passes of one fill: input pairs and distinct spectrum tables read, output pairs "
+                    "written (algorithmic), HIP events on the launch stream")
+        else:
+            ft, tr, kernel = fp, AXIS_TRAFFIC_MIB.get(label), "k_fft_strided / k_fft_xline"
+            note = "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"
         achieved = ft["bytes"] / (ft["ms"] * 1e-3) / 1e9
-        tr = (XPROD_TRAFFIC_MIB if fused else AXIS_TRAFFIC_MIB).get(label)
         out["roofline"] = {
-            "bound": "hbm", "kernel": "k_fft_xprod_reg" if (fused and dp.info.fft_xreg) else
-            ("k_fft_xprod" if fused else "k_fft_axis / k_fft_strided"),
+            "bound": "hbm", "kernel": kernel,
             "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
             "traffic": tr * (1 << 20) if tr else None,
             "launch_ms": ft["ms"] / ft["launches"], "launches_per_fill": ft["launches"],
             "algorithmic_bytes_per_launch": ft["bytes"] / ft["launches"], "share_of_fill": ft["ms"] / ms_fill,
-            "traffic_note": ("mean HBM bytes per launch of that kernel in a fill, from rocprofv3 --pmc "
+            "traffic_note": ("HBM bytes per launch of that kernel in a fill, from rocprofv3 --pmc "
                              "FETCH_SIZE/WRITE_SIZE scaled by a 512 MiB copy"),
-            "note": ("fused x passes of one fill: input pairs and distinct spectrum tables read, output pairs "
-                     "written (algorithmic), HIP events on the launch stream" if fused else
-                     "FFT axis passes of one fill (HIP events on the launch stream), HBM bound"),
-            "axis_passes": {"ms_per_fill": fp["ms"], "launches_per_fill": fp["launches"],
-                            "achieved_gb_s": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9,
-                            "frac": fp["bytes"] / (fp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+            "note": note, "axis_passes": axis}
     else:
         names = ["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"]
         out["fill_stage_ms"] = dict(zip(names, [float(v) for v in stages]))

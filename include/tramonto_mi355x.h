@@ -217,11 +217,14 @@ int tmi_fill_residual_timed(void *handle, const double *x_d, double *F_d, int re
  * FFT form (tmi_info.fmt_fft), times one kernel family of a residual fill on
  * the problem's work buffers, in the fill's order and batch sizes, with HIP
  * events on `stream`: which = 0 the plain FFT axis passes, which = 1 the
- * fused x passes (tmi_info.fmt_fft == 2).  *ms receives their mean time per
- * fill over `reps` fills, *bytes their algorithmic traffic per fill (axis
- * pass: every point of every pair field read and written once, 16 B each way;
- * fused x pass: input pairs and distinct spectrum tables read once, output
- * pairs written once) and *launches their launches per fill. */
+ * fused x passes (tmi_info.fmt_fft == 2), which = 2 the density product's last
+ * pass writing the rho-bar and mean-field rows (wrapping grids).  *ms receives
+ * their mean time per fill over `reps` fills, *bytes their algorithmic traffic
+ * per fill (axis pass: every point of every pair field read and written once,
+ * 16 B each way; fused x pass: input pairs and distinct spectrum tables read
+ * once, output pairs written once; rows pass: the output pairs read once, each
+ * row's unknowns read and its residual written, 8 B each) and *launches their
+ * launches per fill. */
 int tmi_fft_timed(void *handle, int which, int reps, double *ms, double *bytes, int *launches, void *stream);
 /* Jv = J(x) v, the assembled-Jacobian product, src/dft_Schur_Epetra_Operator.cpp:149 (ApplyGlobal) */
 int tmi_apply_jacobian(void *handle, const double *x_d, const double *v_d, double *Jv_d, void *stream);

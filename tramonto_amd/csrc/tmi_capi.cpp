@@ -971,7 +971,7 @@ int tmi_fft_timed(void *handle, int which, int reps, double *ms, double *bytes, 
 {
     return guarded([&] {
         Problem *P = as_problem(handle);
-        if (!ms || !bytes || !launches || reps < 1 || which < 0 || which > 1) throw std::invalid_argument("bad argument");
+        if (!ms || !bytes || !launches || reps < 1 || which < 0 || which > 2) throw std::invalid_argument("bad argument");
         fft_time_kernels(*P, which, reps, ms, bytes, launches, (hipStream_t)stream);
         return 0;
     });

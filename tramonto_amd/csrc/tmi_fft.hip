@@ -2293,6 +2293,38 @@ void fft_set_fmt(Problem &P, bool on)
     if (P.fft) P.fft->fmt = on && P.fft->fmt_ok;
 }
 
+// The fused ends of the density product on a wrapping grid (product w): the first pass gathers the
+// density pairs from the node arrays, the last one writes the rho-bar and mean-field rows
+PassIO rows_io(const Problem &P, int w, const double *rho_nodes, const double *in_rb, const double *sub_rb,
+               double *out_rb, const double *in_mf, const double *sub_mf, double *out_mf)
+{
+    const FftPlan &F = *P.fft;
+    const FwdDesc &d = F.fwd[w];
+    PassIO io;
+    io.C = P.p.ncomp;
+    io.nx0 = P.p.nx[0];
+    io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
+    io.N = P.N;
+    io.rho = rho_nodes;
+    io.zero = P.d_zero;
+    io.nout = d.nout;
+    io.nrb = F.rows[w].nrb;
+    for (int o = 0; o < d.nout; ++o) {
+        io.nrow[o] = F.onrow[w][o];
+        for (int k = 0; k < 3; ++k) {
+            io.row[o][k] = F.orow[w][o][k];
+            io.scale[o][k] = F.oscale[w][o][k];
+        }
+    }
+    io.in_rb = in_rb;
+    io.sub_rb = sub_rb;
+    io.out_rb = out_rb;
+    io.in_mf = in_mf;
+    io.sub_mf = sub_mf;
+    io.out_mf = out_mf;
+    return io;
+}
+
 // Density products from the halo-padded densities in workspace<T>(P).rho_pad:
 // with_rb: the rho-bar rows (out_rb = sums - in_rb - sub_rb), then the
 // mean-field rows when they are in FFT form; else the mean-field rows only.
@@ -2316,29 +2348,12 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
     {   // z, y, x passes; the product (fused with the x passes or not); x, y, z passes back
         // on a wrapping grid the first pass reads the node arrays and the last one writes the rows
         const bool ends = F.wraps && rho_nodes;
-        PassIO io;
+        PassIO io = ends ? rows_io(P, w, rho_nodes, in_rb, sub_rb, out_rb, in_mf, sub_mf, out_mf) : PassIO();
         io.C = C;
         io.nx0 = P.p.nx[0];
         io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
         io.N = P.N;
         if (ends) {
-            io.rho = rho_nodes;
-            io.zero = P.d_zero;
-            io.nout = d.nout;
-            io.nrb = rm.nrb;
-            for (int o = 0; o < d.nout; ++o) {
-                io.nrow[o] = F.onrow[w][o];
-                for (int k = 0; k < 3; ++k) {
-                    io.row[o][k] = F.orow[w][o][k];
-                    io.scale[o][k] = F.oscale[w][o][k];
-                }
-            }
-            io.in_rb = in_rb;
-            io.sub_rb = sub_rb;
-            io.out_rb = out_rb;
-            io.in_mf = in_mf;
-            io.sub_mf = sub_mf;
-            io.out_mf = out_mf;
         } else if (rho_nodes) {
             hipLaunchKernelGGL(k_gather_nodes<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, gu, rho_nodes, P.d_zero, Z);
         } else {
@@ -2481,7 +2496,8 @@ template void fft_el<float>(Problem &, bool, const double *, const double *, con
 // Times one kernel family of a residual fill in FFT form on the plan's work
 // buffers, in the fill's order and batch sizes, with HIP events on s.
 // which = 0: the plain axis passes (without fused ends); which = 1: the fused
-// x passes (density and Euler-Lagrange products).  *ms is their mean time per
+// x passes (density and Euler-Lagrange products); which = 2: the density
+// product's rows pass (fused end, on scratch rows).  *ms is their mean time per
 // fill over reps fills, *bytes their algorithmic traffic per fill and
 // *launches their launches per fill.  Algorithmic bytes: an axis pass reads
 // and writes each grid point of each pair field once (16 B each way); a fused
@@ -2500,7 +2516,25 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
     std::function<void()> fill;
     int count = 0;
     double b = 0.0;
-    if (which == 1) {
+    double *tmp = nullptr;
+    if (which == 2) {   // the density product's last pass writing the rows (fused end), as in a residual fill
+        if (!(F.wraps && fft_node_input(P))) throw std::logic_error("the density product has no fused rows pass");
+        const RowMap &rm = F.rows[1];
+        const int nrows = rm.nrb + rm.nmf, nop = (F.fwd[1].nout + 1) / 2;
+        int last = -1;
+        for (int a : {1, 2})
+            if (F.M[a] > 1) last = a;
+        TMI_HIP_CHECK(hipMalloc((void **)&tmp, sizeof(double) * 2 * (size_t)nrows * P.N));
+        TMI_HIP_CHECK(hipMemsetAsync(tmp, 0, sizeof(double) * 2 * (size_t)nrows * P.N, s));
+        const size_t N = P.N;
+        PassIO io = rows_io(P, 1, nullptr, tmp, nullptr, tmp + (size_t)nrows * N, rm.nmf ? tmp + rm.nrb * N : nullptr,
+                            nullptr, tmp + (size_t)nrows * N + rm.nrb * N);
+        long long cap = 0;
+        io.ss = fft_ss_buffer(P, &cap);
+        count = 1;
+        b = 16.0 * (double)F.n * nop + 16.0 * (double)nrows * N;
+        fill = [&, io, last, nop] { fft_pass<double>(F, O, nop, +1, last, s, kPassRows, io); };
+    } else if (which == 1) {
         const TermList *tls[2] = {&F.tfwd[1], &F.tel};
         for (int w = 0; w < 2; ++w) {
             if (!F.xlpb[w]) continue;
@@ -2566,10 +2600,13 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(tmp);
         throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    TMI_HIP_CHECK(hipStreamSynchronize(s));
+    (void)hipFree(tmp);
     *ms = (double)t / reps;
     *bytes = b;
     *launches = count;

@@ -178,8 +178,8 @@ class DeviceProblem:
 
     def fft_timed(self, which: int = 0, reps: int = 10) -> Dict:
         """One kernel family of a residual fill in FFT form (HIP events on the launch stream):
-        which = 0 the plain axis passes, 1 the fused x passes; mean ms per fill, algorithmic
-        bytes per fill, launches per fill."""
+        which = 0 the plain axis passes, 1 the fused x passes, 2 the density product's rows pass
+        (wrapping grids); mean ms per fill, algorithmic bytes per fill, launches per fill."""
         ms, nbytes, launches = ct.c_double(), ct.c_double(), ct.c_int()
         _lib.check(_lib.lib().tmi_fft_timed(self.handle, which, reps, ct.byref(ms), ct.byref(nbytes),
                                             ct.byref(launches), _stream()), "tmi_fft_timed")
