@@ -286,8 +286,10 @@ void free_geometry_maps(PadGeom &pg)
 
 void destroy_problem(Problem *P);
 
+// symbol_only: a bulk fluid built only for its Schur symbol (FFT forms taken without timing the
+// direct ones)
 Problem *create_problem(const tmi_params &p_in, const double *vext, const unsigned char *zero_tf,
-                        const BatchLayout *batch = nullptr)
+                        const BatchLayout *batch = nullptr, bool symbol_only = false)
 {
     tmi_params p = p_in;
     std::vector<int> pore_off;
@@ -490,10 +492,15 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
             const bool mf = p.type_attr >= 0 && (fe ? std::string(fe) == "1" : lsum >= kFftMinStencil);
             const bool fmt = !(fm && std::string(fm) == "0");
             fft_setup(*P, mf, fmt);
-            choose_fmt_form(*P);
-            // an unused plan is kept on wrapping grids for the bulk preconditioner
-            if (P->fft && !fft_mf(*P) && !fft_fmt(*P) && !fft_wraps(*P)) fft_free(*P);
+            if (!symbol_only) {
+                choose_fmt_form(*P);
+            } else if (fft_fmt_available(*P)) {
+                fft_set_fmt(*P, true);
+                fft_set_fused(*P, 0);
+            }
             precond_setup(*P);
+            // an unused plan is kept only for the bulk preconditioner
+            if (P->fft && !fft_mf(*P) && !fft_fmt(*P) && !precond_available(*P)) fft_free(*P);
         }
     } catch (...) {
         destroy_problem(P);
@@ -721,6 +728,11 @@ int schur_solve(Problem &P, const double *x, const double *F, double *dx, double
 }
 
 }  // namespace
+
+// a periodic bulk fluid of the given parameters (no walls, no zero nodes), for the bulk Schur
+// symbol of a walled grid's preconditioner (tmi_fft.hip); free with destroy_bulk_problem
+Problem *create_bulk_problem(const tmi_params &p) { return create_problem(p, nullptr, nullptr, nullptr, true); }
+void destroy_bulk_problem(Problem *P) { destroy_problem(P); }
 
 // exported for the LinProbMgr layer
 int schur_solve_export(Problem &P, const double *x, const double *F, double *dx, double rtol, int restart, int maxit,

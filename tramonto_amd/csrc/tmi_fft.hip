@@ -1465,8 +1465,8 @@ __global__ void k_el_extract(FftGeo f, Geo g, CompConsts cc, int linear, int nwa
     out_el[row] = r;
 }
 
-// Preconditioner input u_c = rho_c v_c on the free rows (0 on the zero rows),
-// as pairs on a wrapping FFT grid (grid point = node)
+// Preconditioner input u_c = rho_c v_c on the free rows (0 on the zero rows), as pairs at the
+// nodes' FFT-grid positions (node + off; the rest of a padded grid is zeroed beforehand)
 __global__ void k_pc_gather(FftGeo f, Geo g, const double *__restrict__ x_rho, const double *__restrict__ v,
                             const unsigned char *__restrict__ elzero, Cx<double> *__restrict__ Z)
 {
@@ -1475,7 +1475,7 @@ __global__ void k_pc_gather(FftGeo f, Geo g, const double *__restrict__ x_rho, c
     const int x = i % g.nx[0];
     const int r = i / g.nx[0];
     const int y = r % g.nx[1], z = r / g.nx[1];
-    const long long q = x + y * f.S1 + z * f.S2;
+    const long long q = (x + f.off[0]) + (y + f.off[1]) * f.S1 + (long long)(z + f.off[2]) * f.S2;
     for (int j = 0; 2 * j < g.C; ++j) {
         const size_t a = (size_t)(2 * j) * g.N + i, b = a + g.N;
         const double re = elzero[a] ? 0.0 : x_rho[a] * v[a];
@@ -1533,8 +1533,15 @@ __global__ void k_pc_extract(FftGeo f, Geo g, const Cx<double> *__restrict__ O, 
     const int x = i % g.nx[0];
     const int r = i / g.nx[0];
     const int y = r % g.nx[1], z = r / g.nx[1];
-    const Cx<double> o = O[(size_t)(c >> 1) * f.ns + x + y * f.S1 + z * f.S2];
+    const Cx<double> o =
+        O[(size_t)(c >> 1) * f.ns + (x + f.off[0]) + (y + f.off[1]) * f.S1 + (long long)(z + f.off[2]) * f.S2];
     zout[row] = (c & 1) ? o.y : o.x;
+}
+
+__global__ void k_zero_cx(Cx<double> *__restrict__ Z, long long n)
+{
+    const long long m = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (m < n) Z[m] = Cx<double>{0.0, 0.0};
 }
 
 __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long long n)
@@ -2635,7 +2642,41 @@ void precond_setup(Problem &P)
     (void)hipFree(F.pqr);
     F.pq = nullptr;
     F.pqr = nullptr;
-    if (!F.wraps || P.npores > 0) return;
+    if (P.npores > 0) return;
+    if (!F.wraps) {
+        // A walled or reflecting grid: its FFT grid pads every such axis (M = a 2-3-5 length of
+        // n + 2 h).  The bulk Schur operator's symbol on that grid comes from a periodic bulk fluid
+        // of M nodes per axis (same functional, components, potentials and rho_b; no walls), whose
+        // FFT grid is the same grid with the same storage layout.  Applied to the zero-padded rho v,
+        // it is a preconditioner only (not the inverse at any state): GMRES is right-preconditioned,
+        // so the solution is unchanged.
+        tmi_params q = P.p;
+        for (int d = 0; d < P.p.ndim; ++d) {
+            q.nx[d] = F.M[d];
+            q.bc[d][0] = q.bc[d][1] = TMI_PERIODIC;
+        }
+        q.nwall = 0;
+        q.lhard_surf = 0;
+        q.att_in_a22 = 0;
+        Problem *B = create_bulk_problem(q);
+        const FftPlan *FB = B->fft;
+        const bool same = FB && FB->pc_ok && FB->pq && FB->ns == F.ns && FB->S1 == F.S1 && FB->S2 == F.S2 &&
+                          FB->M[0] == F.M[0] && FB->M[1] == F.M[1] && FB->M[2] == F.M[2];
+        if (same) {
+            const size_t bytes = (size_t)P.p.ncomp * P.p.ncomp * (size_t)F.ns * sizeof(Cx<double>);
+            try {
+                TMI_HIP_CHECK(hipMalloc((void **)&F.pq, bytes));
+                TMI_HIP_CHECK(hipMemcpy(F.pq, FB->pq, bytes, hipMemcpyDeviceToDevice));
+            } catch (...) {
+                destroy_bulk_problem(B);
+                throw;
+            }
+            F.pc_ok = true;
+            F.pc_on = F.pc_want != 0;
+        }
+        destroy_bulk_problem(B);
+        return;
+    }
     const int C = P.p.ncomp;
     const size_t N = P.N, ns = (size_t)F.ns, nq = (size_t)P.nrb + P.nmf;
     double *xb = nullptr, *e = nullptr, *t1 = nullptr, *y = nullptr;
@@ -2800,7 +2841,7 @@ bool precond_active(const Problem &P) { return P.fft && P.fft->pc_on; }
 void precond_enable(Problem &P, bool on)
 {
     if (on && !precond_available(P))
-        throw std::domain_error("the bulk preconditioner needs a periodic FFT grid and a stable bulk");
+        throw std::domain_error("the bulk preconditioner needs an FFT grid and a stable bulk");
     if (P.fft) {
         P.fft->pc_want = on ? 1 : 0;
         P.fft->pc_on = on;
@@ -2840,6 +2881,10 @@ void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, 
     }
     const Geo g = make_geo(P, P.pad_u);
     const FftGeo fg = fft_geo(F, 0);
+    if (!F.wraps) {   // zero padding around the nodes (a fill kernel: no memset nodes in captured graphs)
+        const long long nz = (long long)((C + 1) / 2) * F.ns;
+        hipLaunchKernelGGL(k_zero_cx, dim3(grid_for(nz)), dim3(kBlock), 0, s, Z, nz);
+    }
     hipLaunchKernelGGL(k_pc_gather, dim3(grid_for(P.N)), dim3(kBlock), 0, s, fg, g, x_rho, v, P.d_elzero, Z);
     fft3d<double>(F, Z, (C + 1) / 2, -1, s);
     hipLaunchKernelGGL(k_pc_mul, dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, C, Z, F.pq, O);

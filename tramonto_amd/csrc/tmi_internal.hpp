@@ -256,8 +256,10 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
             double *out_el, hipStream_t s, double *ss = nullptr, long long *nss = nullptr);
 double *fft_ss_buffer(const Problem &P, long long *cap);   // partial sums of F^2 of the fused last passes
 void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes, int *launches, hipStream_t s);
-// bulk-circulant Schur preconditioner on wrapping FFT grids (tmi_fft.hip)
+// bulk-circulant Schur preconditioner on FFT grids (tmi_fft.hip)
 void precond_setup(Problem &P);
+Problem *create_bulk_problem(const tmi_params &p);   // tmi_capi.cpp
+void destroy_bulk_problem(Problem *P);
 bool precond_available(const Problem &P);
 bool precond_active(const Problem &P);
 void precond_enable(Problem &P, bool on);
