@@ -93,6 +93,7 @@ typedef struct tmi_info {
     int fft_regpass; /* strided FFT axes of length 32-256: 0 LDS Stockham passes (TMI_FFT_REGPASS=0), 1 the
                         one-wave register pass where a pass has at least 1024 waves, 2 wherever the length
                         allows (TMI_FFT_REGPASS=2); read at creation */
+    int jacobian;    /* TMI_JAC_EXACT or TMI_JAC_REFERENCE (tmi_problem_set_jacobian) */
 } tmi_info;
 
 /* precision of the inner Krylov products in tmi_newton_solve */
@@ -162,11 +163,27 @@ int tmi_problem_set_vext(void *handle, const double *vext);
  * TMI_PRECOND_BULK, the default where available, multiplies the diagonal by
  * the inverse symbol of the Schur operator at the uniform bulk state,
  * M^-1 v = F^-1[S_b(k)^-1 diag(1/rho_b) F(diag(rho) v)], exact for a uniform
- * fluid.  It exists when every axis is periodic with an FFT-friendly length
- * (2^a 3^b 5^c), the FFT forms are in use and the bulk is stable (S_b(k)
- * positive definite); TMI_PRECOND=diag in the environment at creation
+ * fluid on a periodic FFT grid (every length 2^a 3^b 5^c).  On 2D/3D grids
+ * with walls or reflecting boundaries the FFT grid pads those axes, and the
+ * symbol of a periodic bulk fluid on that padded grid serves as an
+ * approximation.  It exists where an FFT grid exists and the bulk is stable
+ * (S_b(k) positive definite); TMI_PRECOND=diag in the environment at creation
  * selects the diagonal.  Fails with TMI_PRECOND_BULK where unavailable. */
 int tmi_problem_set_preconditioner(void *handle, int kind);
+
+/* Jacobian of the products (Jacobian-vector products, the Schur operator,
+ * getmatrixvalue): TMI_JAC_EXACT (default) the exact derivative of the
+ * residual; TMI_JAC_REFERENCE the reference's assembled entries
+ * (load_nonlocal_hs_rosen_rb with d2phi_drb2_delta_rb_FMTk,
+ * src/dft_fill_FMT.c:41-189, src/dft_physics_FMT1.c:79-126, FMT2.c:106-184,
+ * FMT3.c:101-160, FMT4.c:101-191), which differ from the exact derivative in
+ * the nv2 columns of FMT2 (2D/3D) and FMT4 and, for White Bear and FMT4,
+ * across REFLECT boundaries.  The residual and the transpose product
+ * (tmi_apply_jacobian_transpose, exact) are unaffected.  TMI_JAC_REFERENCE=1
+ * in the environment at creation selects the reference entries. */
+#define TMI_JAC_EXACT 0
+#define TMI_JAC_REFERENCE 1
+int tmi_problem_set_jacobian(void *handle, int mode);
 
 /* Integrated observables of a profile, in the units of the reference's output. */
 typedef struct tmi_observables {

@@ -445,3 +445,27 @@ def test_reference_convergence_test_gives_readme_niter(name):
     C = spec.ncomp
     a, b = x.cpu().numpy()[:C], xf.cpu().numpy()[:C]
     assert np.all(np.abs(a - b) <= g["nl_rel_tol"] * np.abs(b) + g["nl_abs_tol"])
+
+
+@pytest.mark.gpu
+def test_reference_jacobian_newton_follows_the_oracle_on_fmt4():
+    """HS8_FMT4_1D: the reference's FMT4 delta Hessian is inexact in its nv2 columns.  With the
+    reference-Jacobian products and the reference's convergence test the device Newton takes the
+    oracle's (assembled-Jacobian) steps: the same iteration count, the README's niter within 1."""
+    from tramonto_amd.device import CONV_UPDATE, JAC_REFERENCE, DeviceProblem
+    name = "HS8_FMT4_1D"
+    g = EXAMPLES[name]
+    spec = spec_from_tramonto(g, name)
+    dp = DeviceProblem(spec)
+    dp.set_jacobian(JAC_REFERENCE)
+    kw = dict(max_iter=g["max_nl_iter"], nl_rel_tol=g["nl_rel_tol"], nl_abs_tol=g["nl_abs_tol"],
+              update_scaling=g["nl_update_scaling"])
+    x = dp.initial_state()
+    st = dp.newton(x, convergence=CONV_UPDATE, gmres_tol=1e-11, gmres_restart=200, gmres_max_iter=4000, **kw)
+    o = Oracle.from_spec(spec)
+    xo, sto = o.newton(o.init_fields(spec.initial_x()), resid_tol=0.0, gmres_tol=1e-11, **kw)
+    assert st["converged"] and sto["converged"], (st, sto)
+    assert st["iters"] == sto["iters"], (st, sto)
+    assert abs(st["iters"] - g["key_outputs"]["niter"]) <= 1
+    C = spec.ncomp
+    assert np.abs(x.cpu().numpy()[:C] - xo[:C]).max() < 1e-8

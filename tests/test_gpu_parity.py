@@ -111,6 +111,46 @@ def test_jvp_matches_assembled_jacobian(case, device_problem):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("form", ["chosen", "fft"])
+@pytest.mark.parametrize("case", CASES)
+def test_reference_jacobian_mode_matches_assembled_jacobian(case, form, device_problem, monkeypatch):
+    """TMI_JAC_REFERENCE: the products use the reference's delta-stencil Hessian rows, so the JVP
+    equals the assembled Jacobian on every case, the EXACT_ONLY ones included (FMT2/FMT4 nv2
+    columns, White Bear across REFLECT), in the form the set-up chose and with the FMT sums forced
+    into FFT form (adjoint fields as transform pairs on wrapping grids, per component even where
+    the residual uses the derivative form)."""
+    from tramonto_amd.device import JAC_EXACT, JAC_REFERENCE, DeviceProblem
+    spec, o, dp = device_problem(case)
+    if form == "fft":
+        if spec.ndim == 1:
+            pytest.skip("1D sums run direct")
+        monkeypatch.setenv("TMI_FMT_FFT", "1")
+        dp = DeviceProblem(spec)
+        assert dp.info.fmt_fft >= 1
+    x = generic_state(spec, o, 13)
+    r, c, v = o.jacobian_coo(x)
+    n = spec.nunk * spec.nnodes
+    J = sps.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+    rng = np.random.default_rng(9)
+    try:
+        dp.set_jacobian(JAC_REFERENCE)
+        assert dp.info.jacobian == JAC_REFERENCE
+        for trial in range(2):
+            vec = rng.uniform(-1.0, 1.0, n)
+            Jv_o = (J @ vec).reshape(spec.nunk, spec.nnodes)
+            Jv_g = dp.apply_jacobian(dev(x), dev(vec.reshape(spec.nunk, spec.nnodes))).cpu().numpy()
+            err = np.abs(Jv_g - Jv_o).max() / np.abs(Jv_o).max()
+            assert err < RTOL, (trial, err)
+    finally:
+        dp.set_jacobian(JAC_EXACT)
+    if case in EXACT_ONLY:   # the exact product differs from the assembled one there
+        vec = rng.uniform(-1.0, 1.0, n)
+        Jv_o = (J @ vec).reshape(spec.nunk, spec.nnodes)
+        Jv_g = dp.apply_jacobian(dev(x), dev(vec.reshape(spec.nunk, spec.nnodes))).cpu().numpy()
+        assert np.abs(Jv_g - Jv_o).max() / np.abs(Jv_o).max() > 1e-9
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", [c for c in CASES if c not in EXACT_ONLY])
 def test_vjp_matches_assembled_transpose(case, device_problem):
     spec, o, dp = device_problem(case)

@@ -33,7 +33,7 @@ class TmiInfo(ct.Structure):
         ("nunk", ct.c_int), ("nnodes", ct.c_int), ("iu_rho", ct.c_int), ("iu_rb", ct.c_int), ("n_rb", ct.c_int),
         ("iu_mf", ct.c_int), ("n_mf", ct.c_int), ("sten_len", (ct.c_int * (NC * NC)) * 3), ("mf_fft", ct.c_int),
         ("fmt_fft", ct.c_int), ("precond", ct.c_int), ("fft_fold", ct.c_int), ("fft_xreg", ct.c_int),
-        ("fft_regpass", ct.c_int),
+        ("fft_regpass", ct.c_int), ("jacobian", ct.c_int),
     ]
 
 
@@ -82,7 +82,7 @@ DECLARED_SYMBOLS = [
     "tmi_stencil_get", "tmi_bulk_get", "tmi_host_stencil", "tmi_host_bulk", "tmi_apply_jacobian_fp32", "tmi_fill_residual_timed", "tmi_fill_residual_noalloc", "tmi_fft_timed", "tmi_fill_residual", "tmi_apply_jacobian", "tmi_apply_jacobian_transpose",
     "tmi_init_fields", "tmi_set_hard_surface_masks", "tmi_compute_force", "tmi_newton_solve", "tmi_schur_solve", "tmi_set_active_problem",
     "tmi_batch_create", "tmi_batch_info", "tmi_batch_newton_solve", "tmi_row_tuning", "tmi_row_tuning_set",
-    "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_compute_observables",
+    "tmi_problem_set_rho_b", "tmi_problem_set_vext", "tmi_problem_set_preconditioner", "tmi_problem_set_jacobian", "tmi_compute_observables",
     "fill_resid_and_matrix_control", "tmi_linprobmgr_last_solve",
     "dft_parameterlist_create", "dft_parameterlist_destruct", "dft_parameterlist_set_int", "dft_parameterlist_set_double",
     "dft_parameterlist_set_int_array", "dft_parameterlist_set_double_array", "dft_parameterlist_set_all_aztec",
@@ -136,6 +136,7 @@ def lib() -> ct.CDLL:
     L.tmi_problem_set_rho_b.argtypes = [vp, vp]
     L.tmi_problem_set_vext.argtypes = [vp, vp]
     L.tmi_problem_set_preconditioner.argtypes = [vp, i]
+    L.tmi_problem_set_jacobian.argtypes = [vp, i]
     L.tmi_compute_observables.argtypes = [vp, vp, vp, vp, d, ct.POINTER(TmiObservables), vp]
     L.tmi_batch_create.argtypes = [ct.POINTER(TmiParams), i, vp, vp, vp, vp, ct.POINTER(vp)]
     L.tmi_batch_info.argtypes = [vp, ct.POINTER(i), vp]

@@ -330,6 +330,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
     if (p.lhard_surf && P->npores > 0) throw std::domain_error("lhard_surf in a batch of pores is not supported");
     // Nlists_HW == 2 with hard surfaces (src/dft_mesh.c:330-345): one component and walls
     P->one_size = p.lhard_surf && p.nwall > 0 && p.ncomp == 1;
+    if (const char *e = std::getenv("TMI_JAC_REFERENCE")) P->jac_ref = std::string(e) == "1";
     if (P->one_size) TMI_HIP_CHECK(hipMalloc((void **)&P->d_w0, sizeof(double) * (size_t)P->nunk * P->N));
     P->nunk_ext = P->ext_mf ? C + P->nrb : P->nunk;
     if (P->ext_mf && batch) throw std::domain_error("att_in_a22 is not supported for batches of pores");
@@ -829,6 +830,7 @@ int tmi_problem_info(void *handle, tmi_info *info)
         info->fft_fold = fft_folded(*P) ? 1 : 0;
         info->fft_xreg = fft_xreg_mask(*P);
         info->fft_regpass = fft_regpass_mode(*P);
+        info->jacobian = P->jac_ref ? TMI_JAC_REFERENCE : TMI_JAC_EXACT;
         return 0;
     });
 }
@@ -1089,8 +1091,18 @@ int tmi_problem_set_preconditioner(void *handle, int kind)
         Problem *P = as_problem(handle);
         if (kind != TMI_PRECOND_DIAG && kind != TMI_PRECOND_BULK) throw std::invalid_argument("preconditioner kind");
         if (kind == TMI_PRECOND_BULK && !precond_available(*P))
-            throw std::domain_error("the bulk preconditioner needs a periodic FFT grid and a stable bulk");
+            throw std::domain_error("the bulk preconditioner needs an FFT grid and a stable bulk");
         precond_enable(*P, kind == TMI_PRECOND_BULK);
+        return 0;
+    });
+}
+
+int tmi_problem_set_jacobian(void *handle, int mode)
+{
+    return guarded([&] {
+        Problem *P = as_problem(handle);
+        if (mode != TMI_JAC_EXACT && mode != TMI_JAC_REFERENCE) throw std::invalid_argument("Jacobian mode");
+        P->jac_ref = mode == TMI_JAC_REFERENCE;
         return 0;
     });
 }

@@ -117,7 +117,9 @@ struct FftPlan {
     void *obuf = nullptr;       // products, then the output pairs
     FwdDesc fwd[2];             // [0] mean-field rows only, [1] rho-bar rows then mean-field rows
     RowMap rows[2];
-    TermList tfwd[2], tel;      // the same products as terms of the fused x pass
+    TermList tfwd[2], tel;      // the same products as terms of the fused x pass (tel: adjoint fields
+                                // A_c, B_c,d, S3; tel_d: the derivative form, when dform)
+    TermList tel_d;
     // rows fed by each output of the density product (fused last pass): [w][output][k]
     int onrow[2][kMaxOut] = {}, orow[2][kMaxOut][3] = {};
     double oscale[2][kMaxOut][3] = {};
@@ -1985,7 +1987,6 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         // output descriptors
         const int ND = nd;
         F->dform = F->wraps && fmt && (4 + 2 * ND + 1) / 2 < (C * (1 + ND) + 2) / 2;
-        const int nadj = F->dform ? 4 + 2 * ND : C * (1 + ND) + 1;
         const auto &b = P.bulk;
         FwdDesc &f0 = F->fwd[0], &f1 = F->fwd[1];
         RowMap &r0 = F->rows[0], &r1 = F->rows[1];
@@ -2059,7 +2060,7 @@ void fft_setup(Problem &P, bool mf, bool fmt)
             }
         }
         if (F->dform) {   // inputs D3, D2, D1, D0, Dv2_d, Dv1_d (the derivatives of Phi)
-            TermList &tl = F->tel;
+            TermList &tl = F->tel_d;
             tl.nout = C;
             for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND), 0, 1.0);   // D3: theta
             close_input(tl);
@@ -2077,7 +2078,8 @@ void fft_setup(Problem &P, bool mf, bool fmt)
                 for (int c = 0; c < C; ++c) add_term(tl, c, c * (2 + ND) + 2 + d, 1, -b.inv_4pir[c]);
                 close_input(tl);
             }
-        } else {
+        }
+        {   // the adjoint fields A_c, B_c,d, S3 (without dform, and for reference-Jacobian products)
             TermList &tl = F->tel;
             tl.nout = C;
             for (int c = 0; c < C; ++c) {
@@ -2102,7 +2104,7 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         }
         F->xfit[0] = F->xlpb[0] = xprod_lpb(*F, std::max((F->fwd[0].nout + 1) / 2, (F->fwd[1].nout + 1) / 2));
         F->xfit[1] = F->xlpb[1] = xprod_lpb(*F, (C + 1) / 2);
-        const int zpairs = std::max((C + 1) / 2, fmt ? (nadj + 1) / 2 : 0);
+        const int zpairs = std::max((C + 1) / 2, fmt ? (C * (1 + ND) + 2) / 2 : 0);   // either adjoint form
         const int opairs = std::max((f0.nout + 1) / 2, fmt ? std::max((f1.nout + 1) / 2, (C + 1) / 2) : 0);
         TMI_HIP_CHECK(hipMalloc(&F->zbuf, (size_t)zpairs * ns * sizeof(Cx<double>)));
         TMI_HIP_CHECK(hipMalloc(&F->obuf, (size_t)opairs * ns * sizeof(Cx<double>)));
@@ -2260,9 +2262,13 @@ void fft_free(Problem &P)
     P.fft = nullptr;
 }
 
-FftPairs fft_pairs(const Problem &P)
+// the Euler-Lagrange product of a residual (linear = false) or a Jacobian-vector product runs on the
+// derivatives of Phi (dform) unless the reference-Jacobian mode asks for per-component fields
+bool el_dform(const Problem &P, bool linear) { return P.fft && P.fft->dform && !(linear && P.jac_ref); }
+
+FftPairs fft_pairs(const Problem &P, bool linear)
 {
-    return FftPairs{P.fft->zbuf, P.fft->ns, P.fft->S1, P.fft->S2, P.fft->dform ? 1 : 0};
+    return FftPairs{P.fft->zbuf, P.fft->ns, P.fft->S1, P.fft->S2, el_dform(P, linear) ? 1 : 0};
 }
 bool fft_node_input(const Problem &P)
 {
@@ -2422,7 +2428,8 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
 {
     if (nss) *nss = 0;
     const FftPlan &F = *P.fft;
-    const int C = P.p.ncomp, nd = P.p.ndim, nadj = F.dform ? 4 + 2 * nd : C * (1 + nd) + 1;
+    const bool df = el_dform(P, linear);
+    const int C = P.p.ncomp, nd = P.p.ndim, nadj = df ? 4 + 2 * nd : C * (1 + nd) + 1;
     Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
     Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
     const Geo gf = make_geo(P, P.pad_f);
@@ -2458,7 +2465,7 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
         fft_pass<T>(F, Z, nip, -1, 2, s);
         fft_pass<T>(F, Z, nip, -1, 1, s);
         if (F.xlpb[1]) {
-            fft_xprod<T>(F, Z, O, F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
+            fft_xprod<T>(F, Z, O, df ? F.tel_d : F.tel, F.xlpb[1], fmt_spectra<T>(F), attr_spectra<T>(F), s);
         } else {
             fft_pass<T>(F, Z, nip, -1, 0, s);
             ElConsts ec{};
@@ -2472,13 +2479,13 @@ void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, 
             };
             if (F.fold) {
                 if (nd == 2)
-                    F.dform ? go(k_el_mul8<T, 2, true>) : go(k_el_mul8<T, 2, false>);
+                    df ? go(k_el_mul8<T, 2, true>) : go(k_el_mul8<T, 2, false>);
                 else
-                    F.dform ? go(k_el_mul8<T, 3, true>) : go(k_el_mul8<T, 3, false>);
+                    df ? go(k_el_mul8<T, 3, true>) : go(k_el_mul8<T, 3, false>);
             } else if (nd == 2) {
-                F.dform ? go(k_el_mul<T, 2, true>) : go(k_el_mul<T, 2, false>);
+                df ? go(k_el_mul<T, 2, true>) : go(k_el_mul<T, 2, false>);
             } else {
-                F.dform ? go(k_el_mul<T, 3, true>) : go(k_el_mul<T, 3, false>);
+                df ? go(k_el_mul<T, 3, true>) : go(k_el_mul<T, 3, false>);
             }
             fft_pass<T>(F, O, nop, +1, 0, s);
         }
@@ -2542,7 +2549,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
         b = 16.0 * (double)F.n * nop + 16.0 * (double)nrows * N;
         fill = [&, io, last, nop] { fft_pass<double>(F, O, nop, +1, last, s, kPassRows, io); };
     } else if (which == 1) {
-        const TermList *tls[2] = {&F.tfwd[1], &F.tel};
+        const TermList *tls[2] = {&F.tfwd[1], F.dform ? &F.tel_d : &F.tel};
         for (int w = 0; w < 2; ++w) {
             if (!F.xlpb[w]) continue;
             const TermList *tl = tls[w];
@@ -2555,7 +2562,7 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
         }
         fill = [&] {
             if (F.xlpb[0]) fft_xprod<double>(F, Z, O, F.tfwd[1], F.xlpb[0], F.td, F.ud, s);
-            if (F.xlpb[1]) fft_xprod<double>(F, Z, O, F.tel, F.xlpb[1], F.td, F.ud, s);
+            if (F.xlpb[1]) fft_xprod<double>(F, Z, O, F.dform ? F.tel_d : F.tel, F.xlpb[1], F.td, F.ud, s);
         };
     } else {
         // which passes run with fused ends (those are other kernels): the density product's first and last

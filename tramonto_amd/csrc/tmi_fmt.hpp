@@ -191,4 +191,143 @@ __host__ __device__ __attribute__((always_inline)) inline void fmt_hvp(int func,
     out[3] = z ? 0.0 : o3;
 }
 
+// The reference's delta-stencil Hessian row d2phi_drb2_delta_rb_FMTk (src/dft_physics_FMT1.c:79-126,
+// FMT2.c:106-184, FMT3.c:101-160, FMT4.c:101-191) for one stencil point: out (n order) is the
+// derivative of that point's Euler-Lagrange term with respect to the rho-bar unknowns at the node it
+// reads, for the kernel vector vec = offset dx / R and the mirror signs sign[d] (-1 across a
+// REFLECT boundary).  It is affine in vec.  It differs from the exact Hessian in the nv2 columns of
+// FMT2 (2D/3D) and FMT4, and for FMT3/FMT4 across REFLECT boundaries, where the reference leaves
+// the vector densities unmirrored; the reference-Jacobian mode of the product uses it to reproduce
+// the reference's assembled entries.
+template <int ND>
+__host__ __device__ inline void fmt_ref_delta_row(int func, const double *n, const double *vec, const double *sign,
+                                                  double i4pir, double i4pirsq, double *out)
+{
+    constexpr double kPi = 3.14159265358979323846, i4pi = 1.0 / (4.0 * kPi);
+    const double a = 1.0 / (1.0 - n[3]), a2 = a * a, a3 = a2 * a;
+    if (func == 0 || func == 1) {   // Rosenfeld and FMT2: mirrored vector densities
+        double n1v[ND > 0 ? ND : 1], n2v[ND > 0 ? ND : 1];
+        double d22 = 0.0;
+        for (int d = 0; d < ND; ++d) {
+            n2v[d] = sign[d] * n[4 + ND + d];
+            n1v[d] = sign[d] * n[4 + d];
+            d22 += n2v[d] * n2v[d];
+        }
+        if (func == 0) {
+            double S2 = a * i4pir + n[2] * i4pi * a2, S3 = i4pirsq * a + n[2] * i4pir * a2 + n[1] * a2 + n[2] * n[2] * i4pi * a3;
+            for (int d = 0; d < ND; ++d) {
+                S2 += vec[d] * n2v[d] * i4pi * a2;
+                S3 += a2 * (-n2v[d] * n2v[d] * i4pi * a + vec[d] * (n2v[d] * i4pir + n1v[d] + n[2] * n2v[d] * a / (2.0 * kPi)));
+            }
+            out[0] = 0.0;
+            out[1] = a;
+            out[2] = S2;
+            out[3] = S3;
+            for (int d = 0; d < ND; ++d) {
+                out[4 + d] = sign[d] * a * vec[d];
+                out[4 + ND + d] = sign[d] * (-n2v[d] * i4pi * a2 + (i4pir * a + n[2] * i4pi * a2) * vec[d]);
+            }
+            return;
+        }
+        double al, be, ep, gam[ND > 0 ? ND : 1], dg = 0.0;
+        if (n[2] > 1e-15) {
+            al = n[2] - d22 / n[2];
+            be = 1.0 + d22 / (n[2] * n[2]);
+            for (int d = 0; d < ND; ++d) {
+                gam[d] = n2v[d] / n[2];
+                dg += gam[d] * gam[d];
+            }
+            ep = al / n[2];
+        } else {
+            al = n[2];
+            be = 1.0;
+            for (int d = 0; d < ND; ++d) gam[d] = 0.0;
+            ep = 1.0;
+        }
+        double S2 = a * i4pir + i4pi * a2 * al * (be * be - ep * dg);
+        double S3 = i4pirsq * a + n[2] * i4pir * a2 + n[1] * a2 + i4pi * a3 * al * al * be;
+        for (int d = 0; d < ND; ++d) {
+            S2 -= i4pi * a2 * al * (ep - 2.0 * be) * gam[d] * vec[d];
+            S3 += a2 * vec[d] * (n2v[d] * i4pir + n1v[d]) + 2.0 * i4pi * a3 * al * al * gam[d] * vec[d];
+        }
+        out[0] = 0.0;
+        out[1] = a;
+        out[2] = S2;
+        out[3] = S3;
+        for (int d = 0; d < ND; ++d) {
+            out[4 + d] = sign[d] * a * vec[d];
+            out[4 + ND + d] = sign[d] * (i4pir * a * vec[d] + i4pi * a2 * al * (ep - 2.0 * be) * gam[d] -
+                                         i4pi * a2 * al * (4.0 * dg - ep) * vec[d]);
+        }
+        return;
+    }
+    // White Bear and FMT4: the vector densities enter unmirrored, the signs only on the V columns
+    for (int q = 0; q < 4 + 2 * ND; ++q) out[q] = 0.0;
+    if (!(n[3] > 1e-10)) return;
+    double d22 = 0.0, dnw = 0.0;
+    for (int d = 0; d < ND; ++d) {
+        d22 += n[4 + ND + d] * n[4 + ND + d];
+        dnw += n[4 + ND + d] * vec[d];
+    }
+    const double n3sq = n[3] * n[3], n3cb = n3sq * n[3], l = log(1.0 - n[3]);
+    const double fac1 = n[3] - 2.0 * (1.0 - n[3]) * l, fac2 = n[3] + (1.0 - n[3]) * (1.0 - n[3]) * l;
+    const double fac3 = -(a2 / n3cb) + (a3 / n3sq), g = 2.0 * fac2 * fac3 + fac1 * a2 / n3sq;
+    const double h = fac2 * a2 / (6.0 * kPi * n3sq);
+    if (func == 2) {
+        double S2 = a * i4pir + n[2] * h, S3 = i4pirsq * a + i4pir * n[2] * a2 + n[1] * a2 + (n[2] * n[2] - d22) * g / (12.0 * kPi);
+        for (int d = 0; d < ND; ++d) {
+            S2 += vec[d] * n[4 + ND + d] * h;
+            S3 += vec[d] * (n[4 + ND + d] * i4pir * a2 + n[4 + d] * a2 + n[2] * n[4 + ND + d] * g / (6.0 * kPi));
+        }
+        out[1] = a;
+        out[2] = S2;
+        out[3] = S3;
+        for (int d = 0; d < ND; ++d) {
+            out[4 + d] = sign[d] * a * vec[d];
+            out[4 + ND + d] = sign[d] * (-n[4 + ND + d] * h + (i4pir * a + n[2] * h) * vec[d]);
+        }
+        return;
+    }
+    const double fac = n[2] > 1e-15 ? 1.0 - d22 / (n[2] * n[2]) : 1.0, i2 = n[2] > 1e-15 ? 1.0 / n[2] : 0.0;
+    double S2 = a * i4pir + h * (n[2] * fac * fac * fac + 3.0 * d22 * fac * fac * i2 + 4.0 * d22 * d22 * fac * i2 * i2 * i2);
+    double S3 = i4pirsq * a + i4pir * n[2] * a2 + n[1] * a2 + (n[2] * n[2] * fac * fac * fac + 2.0 * d22 * fac * fac) * g / (12.0 * kPi);
+    for (int d = 0; d < ND; ++d) {
+        S2 += vec[d] * n[4 + ND + d] * h * (fac * fac + 4.0 * d22 * fac * i2 * i2);
+        S3 += vec[d] * (n[4 + ND + d] * i4pir * a2 + n[4 + d] * a2 + n[2] * n[4 + ND + d] * fac * fac * g / (6.0 * kPi));
+    }
+    out[1] = a;
+    out[2] = S2;
+    out[3] = S3;
+    for (int d = 0; d < ND; ++d) {
+        out[4 + d] = sign[d] * a * vec[d];
+        out[4 + ND + d] = sign[d] * (-n[4 + ND + d] * h * (fac * fac - 4.0 * d22 * fac * i2 + 4.0 * dnw * fac * i2) +
+                                     (i4pir * a + n[2] * fac * fac * h) * vec[d]);
+    }
+}
+
+// Adjoint fields of the Jacobian-vector product in the reference-Jacobian mode for component c:
+// A = t0 . dn and B_d = -(t_d . dn), where the reference's delta row (fmt_ref_delta_row) is
+// t0 + sum_d vec_d t_d; the Euler-Lagrange sum then reproduces the reference's assembled entries
+// (sum_k w_k row(vec_k) . dn at the node that stencil point k reads)
+template <int ND>
+__host__ __device__ inline void fmt_ref_adjoint(int func, const double *n, const double *dn, const double *sign,
+                                                double i4pir, double i4pirsq, double &A, double *B)
+{
+    constexpr int NR = 4 + 2 * ND;
+    double vec[ND > 0 ? ND : 1], row[NR];
+    for (int d = 0; d < ND; ++d) vec[d] = 0.0;
+    fmt_ref_delta_row<ND>(func, n, vec, sign, i4pir, i4pirsq, row);
+    A = 0.0;
+    for (int q = 0; q < NR; ++q) A += row[q] * dn[q];
+    for (int d = 0; d < ND; ++d) {
+        vec[d] = 1.0;
+        double r1[NR];
+        fmt_ref_delta_row<ND>(func, n, vec, sign, i4pir, i4pirsq, r1);
+        vec[d] = 0.0;
+        double t = 0.0;
+        for (int q = 0; q < NR; ++q) t += (r1[q] - row[q]) * dn[q];
+        B[d] = -t;
+    }
+}
+
 }  // namespace tmi

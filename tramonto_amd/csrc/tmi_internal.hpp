@@ -183,6 +183,9 @@ struct Problem {
     // one particle size with hard surfaces (one component, walls): n1, n0, nv1 rows are multiples
     // of n2, nv2 as in the reference; d_w0 is the transpose product's copy of w
     bool one_size = false;
+    // Jacobian-vector products with the reference's delta-stencil Hessian rows (TMI_JAC_REFERENCE:
+    // its assembled entries, including the inexact ones) instead of the exact Hessian
+    bool jac_ref = false;
     double *d_w0 = nullptr;
     int nunk_ext = 0;
     double *d_xi = nullptr, *d_vi = nullptr, *d_fi = nullptr, *d_gi = nullptr;   // [nunk][N] each
@@ -249,7 +252,7 @@ struct FftPairs {
     long long ns, S1, S2;
     int dform;   // the pairs carry the derivatives of Phi (rho-bar order) instead of A_c, B_c,d, S3
 };
-FftPairs fft_pairs(const Problem &P);
+FftPairs fft_pairs(const Problem &P, bool linear);
 bool fft_node_input(const Problem &P);   // every density product reads the node arrays (no density halo)
 template <typename T>
 void fft_el(Problem &P, bool linear, const double *x_rho, const double *in_rho, const double *in_mf, const double *add,

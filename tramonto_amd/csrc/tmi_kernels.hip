@@ -64,7 +64,7 @@ __global__ void k_pad_rho(Geo g, const double *__restrict__ in, const unsigned c
 // pairs of the transform buffer of a wrapping FFT grid
 template <typename T, int ND>
 __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, const double *__restrict__ dnrb, int linear,
-                            CompConsts cc, T *__restrict__ adj, FftPairs pairs)
+                            CompConsts cc, T *__restrict__ adj, FftPairs pairs, int jref)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.N) return;
@@ -119,6 +119,38 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
         else
             dst[(size_t)f * fstride + p] = v;
     };
+    if (linear && jref) {   // the reference's delta rows for A_c and B_c,d; S3 from the theta row (exact)
+        double sg[ND > 0 ? ND : 1];
+        for (int d = 0; d < ND; ++d) sg[d] = 1.0;
+        double dn[NR];
+        {
+            double dr[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) dr[q] = dnrb[(size_t)q * g.N + i];
+            dn[0] = dr[3];
+            dn[1] = dr[2];
+            dn[2] = dr[1];
+            dn[3] = dr[0];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                dn[4 + d] = dr[4 + ND + d];
+                dn[4 + ND + d] = dr[4 + d];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            if (c < C) {
+                double A, B[ND > 0 ? ND : 1];
+                fmt_ref_adjoint<ND>(func, n, dn, sg, cc.i4pir[c], cc.i4pirsq[c], A, B);
+                put(c * (1 + ND), (T)A);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) put(c * (1 + ND) + 1 + d, (T)B[d]);
+            }
+        }
+        put(C * (1 + ND), (T)out[3]);
+        if (pairs.buf && ((C * (1 + ND) + 1) & 1)) put(C * (1 + ND) + 1, (T)0);
+        return;
+    }
     if (pairs.buf && pairs.dform) {   // the derivatives themselves, rho-bar order: D3 D2 D1 D0 Dv2_d Dv1_d
         put(0, (T)out[3]);
         put(1, (T)out[2]);
@@ -144,7 +176,8 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
 }
 
 template <typename T, int ND>
-__global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj)
+__global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj, int func, const double *__restrict__ xrb,
+                          const double *__restrict__ dnrb, int jref)
 {
     long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= g.np) return;
@@ -154,6 +187,43 @@ __global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj)
     if (code == 0) {
         const long long ps = pad_pos(g, src);
         if (ps == pos) return;   // interior position
+        if (linear && jref && refl) {
+            // reference-Jacobian mode across a REFLECT boundary: the reference evaluates its delta row
+            // at the node with the mirror signs (src/dft_fill_FMT.c:41-189 with sign[] of
+            // offset_to_node_box), which is not the mirror image of the node's own value for every
+            // functional; S3 (theta row, no signs) is copied
+            constexpr int NR = 4 + 2 * ND;
+            double n[NR], dn[NR], sg[ND > 0 ? ND : 1];
+            {
+                double r[NR], dr[NR];
+#pragma unroll
+                for (int q = 0; q < NR; ++q) {
+                    r[q] = xrb[(size_t)q * g.N + src];
+                    dr[q] = dnrb[(size_t)q * g.N + src];
+                }
+                n[0] = r[3], n[1] = r[2], n[2] = r[1], n[3] = r[0];
+                dn[0] = dr[3], dn[1] = dr[2], dn[2] = dr[1], dn[3] = dr[0];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    n[4 + d] = r[4 + ND + d];
+                    n[4 + ND + d] = r[4 + d];
+                    dn[4 + d] = dr[4 + ND + d];
+                    dn[4 + ND + d] = dr[4 + d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) sg[d] = ((refl >> d) & 1) ? -1.0 : 1.0;
+            for (int c = 0; c < C; ++c) {
+                const size_t b = (size_t)(c * (1 + ND));
+                double A, B[ND > 0 ? ND : 1];
+                fmt_ref_adjoint<ND>(func, n, dn, sg, cc.i4pir[c], cc.i4pirsq[c], A, B);
+                adj[b * g.np + pos] = (T)A;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) adj[(b + 1 + d) * g.np + pos] = (T)B[d];
+            }
+            adj[(size_t)(C * (1 + ND)) * g.np + pos] = adj[(size_t)(C * (1 + ND)) * g.np + ps];
+            return;
+        }
         for (int c = 0; c < C; ++c) {
             const size_t b = (size_t)(c * (1 + ND));
             adj[b * g.np + pos] = adj[b * g.np + ps];
@@ -1416,27 +1486,28 @@ void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear,
     T *adj = workspace<T>(P).adj_pad;
     const int lin = linear ? 1 : 0;
     const int func = P.p.type_func;
+    const int jref = (linear && P.jac_ref) ? 1 : 0;
     if (fft_fmt(P) && fft_wraps(P)) {   // straight into the transform buffer as pairs: no halo
-        const FftPairs fp = fft_pairs(P);
+        const FftPairs fp = fft_pairs(P, linear);
         if (P.p.ndim == 2)
-            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp);
+            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp, jref);
         else
-            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp);
+            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp, jref);
         return;
     }
-    const FftPairs nop{nullptr, 0, 0, 0};
+    const FftPairs nop{nullptr, 0, 0, 0, 0};
     switch (P.p.ndim) {
     case 1:
-        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
-        hipLaunchKernelGGL((k_pad_adj<T, 1>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
+        hipLaunchKernelGGL((k_pad_adj<T, 1>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
         break;
     case 2:
-        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
-        hipLaunchKernelGGL((k_pad_adj<T, 2>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
+        hipLaunchKernelGGL((k_pad_adj<T, 2>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
         break;
     default:
-        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop);
-        hipLaunchKernelGGL((k_pad_adj<T, 3>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj);
+        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
+        hipLaunchKernelGGL((k_pad_adj<T, 3>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
         break;
     }
 }

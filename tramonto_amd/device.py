@@ -25,6 +25,7 @@ from .problem import ProblemSpec
 PREC_FP64, PREC_MIXED = 0, 1
 CONV_RESID, CONV_UPDATE = 0, 1
 PRECOND_DIAG, PRECOND_BULK = 0, 1
+JAC_EXACT, JAC_REFERENCE = 0, 1
 
 
 def _stream() -> ct.c_void_p:
@@ -123,6 +124,12 @@ class DeviceProblem:
         (diagonal times the inverse bulk Schur symbol, periodic FFT grids)."""
         _lib.check(_lib.lib().tmi_problem_set_preconditioner(self.handle, int(kind)), "tmi_problem_set_preconditioner")
         self.info.precond = int(kind)
+
+    def set_jacobian(self, mode: int) -> None:
+        """Jacobian of the products: JAC_EXACT (the exact derivative) or JAC_REFERENCE (the
+        reference's assembled entries, inexact for FMT2/FMT4 and White Bear across REFLECT)."""
+        _lib.check(_lib.lib().tmi_problem_set_jacobian(self.handle, int(mode)), "tmi_problem_set_jacobian")
+        self.info.jacobian = int(mode)
 
     def observables(self, x: torch.Tensor, nel_hit: np.ndarray, nel_hit2: Optional[np.ndarray] = None,
                     scale: float = 1.0) -> Dict:
