@@ -285,13 +285,17 @@ int tmi_row_tuning(void *handle, int *shape, double *ms);
 int tmi_row_tuning_set(void *handle, int prec, int kind, int per_lane, int lanes);
 
 /* --------------------------------------------------------------------------
- * Batched 1D problems.  No counterpart in the reference, which solves one
- * problem per run (src/dft_main.c): npores independent 1D problems ("pores",
- * IN_WALL at both ends of each) that share the fluid of *fluid (ndim = 1;
- * fluid->nx[0], fluid->bc and fluid->rho_b are ignored) and differ in node
- * count nodes[p], bulk densities rho_b[p][ncomp], V_ext and Zero_density_TF
- * (vext, zero_tf: [N][ncomp] node-major over the packed nodes, pore 0 first,
- * N = sum of nodes).  Each pore is the problem the reference would set up
+ * Batched 1D or 2D problems.  No counterpart in the reference, which solves
+ * one problem per run (src/dft_main.c): npores independent problems
+ * ("pores") that share the fluid of *fluid and differ in node count nodes[p],
+ * bulk densities rho_b[p][ncomp], V_ext and Zero_density_TF (vext, zero_tf:
+ * [N][ncomp] node-major over the packed nodes, pore 0 first, N = sum of
+ * nodes).  fluid->ndim = 1: 1D pores with IN_WALL at both ends
+ * (fluid->nx[0], fluid->bc and fluid->rho_b are ignored).  fluid->ndim = 2:
+ * 2D pieces of fluid->nx[0] columns and nodes[p] / nx[0] rows (x fastest),
+ * stacked along y, each with IN_WALL at both y ends; fluid->bc[0] (both
+ * ends PERIODIC, or both IN_WALL) is their common x boundary; their sums run
+ * in the direct form.  Each pore is the problem the reference would set up
  * with those inputs (one fill_resid_and_matrix_control system per pore).
  * The handle works with tmi_problem_info (nnodes = N), tmi_fill_residual,
  * tmi_apply_jacobian[_fp32], tmi_init_fields, tmi_schur_solve and

@@ -163,3 +163,61 @@ def test_every_row_kernel_shape_matches_oracle(small, per_lane, lanes):
         assert np.abs(J32 - Jv).max() < 1e-5 * np.abs(Jv).max()
     finally:
         db.set_row_tuning(8, 256)
+
+
+# ---------------------------------------------------------------- 2D pieces
+@pytest.fixture(scope="module")
+def small2d():
+    """four rough slit pores of 64 columns (periodic x, walls at both y ends) and uneven heights"""
+    from tramonto_amd.batch import DeviceBatch, rough_slit_pore_batch_2d
+    b = rough_slit_pore_batch_2d(npores=4, seed=5, nx=64, h_range=(3.0, 5.0), dx=0.1)
+    oracles = [Oracle.from_spec(p) for p in b.pores]
+    return b, oracles, DeviceBatch(b)
+
+
+@pytest.mark.gpu
+def test_batch_2d_residual_and_jvp_per_piece(small2d):
+    """2D pieces stacked along y between wall rows: each piece's residual and Jacobian product equal
+    the oracle's for that piece on its own (1e-12), at a perturbed state."""
+    b, oracles, db = small2d
+    assert db.info.fmt_fft == 0 and db.info.mf_fft == 0   # batches of 2D pieces keep the direct sums
+    x = generic_state(b, oracles, 31)
+    Fg, _ = db.fill_residual(dev(x))
+    for Fp, xp, o in zip(b.split(Fg.cpu().numpy()), b.split(x), oracles):
+        Fo = o.residual(np.ascontiguousarray(xp))
+        assert np.abs(Fp - Fo).max() < RTOL * np.abs(Fo).max()
+    rng = np.random.default_rng(6)
+    v = rng.uniform(-1.0, 1.0, x.shape)
+    Jv = db.apply_jacobian(dev(x), dev(v)).cpu().numpy()
+    for p, o, xp, vp, Jp in zip(b.pores, oracles, b.split(x), b.split(v), b.split(Jv)):
+        r, c, val = o.jacobian_coo(np.ascontiguousarray(xp))
+        n = p.nunk * p.nnodes
+        J = sps.coo_matrix((val, (r, c)), shape=(n, n)).tocsr()
+        ref = (J @ np.ascontiguousarray(vp).reshape(-1)).reshape(p.nunk, p.nnodes)
+        assert np.abs(Jp - ref).max() < RTOL * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", ["1", "0"])
+def test_batch_2d_newton_matches_single_solves(graph, small2d, monkeypatch):
+    """The batched per-piece Newton-Krylov on 2D pieces: every piece converges, to the profile of
+    the piece solved as one device problem (1e-8; that path's Newton profiles are pinned to the
+    oracle's by the parity suite; the first piece is also checked against the oracle here)."""
+    from tramonto_amd.device import DeviceProblem
+    b, oracles, db = small2d
+    monkeypatch.setenv("TMI_BATCH_GRAPH", graph)
+    x = db.initial_state()
+    st = db.newton(x, resid_tol=1e-10, gmres_tol=1e-9, gmres_restart=100, gmres_max_iter=2000, max_iter=60)
+    print("2D batch:", st["pore_iters"], st["pore_gmres"], st["pore_resid"], flush=True)
+    assert st["converged"], st
+    assert st["pore_converged"].all() and (st["pore_resid"] < 1e-10).all()
+    for k, (xp, p, o) in enumerate(zip(b.split(x.cpu().numpy()), b.pores, oracles)):
+        if k == 0:
+            xo, sto = o.newton(o.init_fields(p.initial_x()), resid_tol=1e-10)
+            assert sto["converged"], sto
+            assert np.abs(xp[: p.ncomp] - xo[: p.ncomp]).max() < 1e-8
+        dp = DeviceProblem(p)
+        xs = dp.initial_state()
+        assert dp.newton(xs, resid_tol=1e-11, gmres_tol=1e-10, gmres_restart=100, gmres_max_iter=2000,
+                         max_iter=60)["converged"]
+        assert np.abs(xp[: p.ncomp] - xs.cpu().numpy()[: p.ncomp]).max() < 1e-8

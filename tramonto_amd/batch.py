@@ -1,11 +1,13 @@
-"""Batches of independent 1D problems on one device ("pores").
+"""Batches of independent 1D or 2D problems on one device ("pores").
 
-A batch packs P 1D problems that share the fluid (components, functional,
+A batch packs P problems that share the fluid (components, functional,
 potentials, grid spacing) into one device problem: unknown rows hold pore
 0's nodes, then pore 1's, and so on.  Each pore keeps its own node count,
 bulk densities, V_ext and zero-density flags, and the stencil kernels see
-every pore between its own walls.  ``DeviceBatch.newton`` solves all pores
-at once with a per-pore Newton-Krylov (tmi_batch_newton_solve).
+every pore between its own walls (1D: walls at both ends; 2D: pieces of a
+common x width, walled at both y ends, stacked along y).
+``DeviceBatch.newton`` solves all pores at once with a per-pore
+Newton-Krylov (tmi_batch_newton_solve).
 
 ``slit_pore_batch`` builds the uneven slit-pore workload: binary Rosenfeld +
 WCA fluid in P pores of width H ~ U(4, 30) sigma between LJ 9-3 walls with
@@ -22,7 +24,7 @@ import torch
 
 from . import _lib
 from .device import PREC_FP64, DeviceProblem, _ptr, _stream
-from .problem import IN_WALL, ProblemSpec, SplitMix64, slit_pore_1d
+from .problem import IN_WALL, PERIODIC, ProblemSpec, SplitMix64, rough_slit_pore_2d, slit_pore_1d
 
 
 @dataclasses.dataclass
@@ -33,15 +35,20 @@ class BatchSpec:
         if not self.pores:
             raise ValueError("empty batch")
         f = self.pores[0]
+        if f.ndim not in (1, 2):
+            raise ValueError("batches hold 1D or 2D problems")
         for p in self.pores:
-            if p.ndim != 1 or p.ncomp != f.ncomp or p.dx != f.dx or p.type_func != f.type_func \
+            if p.ndim != f.ndim or p.ncomp != f.ncomp or p.dx != f.dx or p.type_func != f.type_func \
                     or p.type_attr != f.type_attr or list(p.hs_diam) != list(f.hs_diam):
                 raise ValueError("pores of a batch must share the fluid and the grid spacing")
             for a, b in ((p.sigma_ff, f.sigma_ff), (p.eps_ff, f.eps_ff), (p.cut_ff, f.cut_ff)):
                 if (a is None) != (b is None) or (a is not None and not np.array_equal(a, b)):
                     raise ValueError("pores of a batch must share the pair potentials")
-            if p.bc[0] != [IN_WALL, IN_WALL]:
-                raise ValueError("batched pores have walls (IN_WALL) at both ends")
+            if p.bc[p.ndim - 1] != [IN_WALL, IN_WALL]:
+                raise ValueError("batched pores have walls (IN_WALL) at both ends of their last axis")
+            if p.ndim == 2 and (p.nx[0] != f.nx[0] or p.bc[0] != f.bc[0] or p.bc[0][0] != p.bc[0][1]
+                                or p.bc[0][0] not in (PERIODIC, IN_WALL)):
+                raise ValueError("2D pieces share nx and a periodic or walled x boundary")
 
     @property
     def npores(self) -> int:
@@ -95,6 +102,24 @@ def slit_pore_batch(npores: int = 256, seed: int = 9, h_range=(4.0, 30.0), dx: f
     rho = rng.uniform(npores * len(sig), *rho_range).reshape(npores, len(sig))
     pores = [slit_pore_1d(H=float(np.round(H[i] / dx) * dx), dx=dx, sig=sig, rho_b=tuple(rho[i]), eps_w=float(eps[i]),
                           cut_fac=cut_fac, seed=seed * 1000 + i, eps_ff=eps_ff) for i in range(npores)]
+    return BatchSpec(pores)
+
+
+def rough_slit_pore_batch_2d(npores: int = 16, seed: int = 11, nx: int = 64, h_range=(4.0, 10.0), dx: float = 0.1,
+                             eps_range=(0.5, 1.5), rho_range=(0.3, 0.5), cut: float = 2.5,
+                             eps_ff: float = 0.5) -> BatchSpec:
+    """Uneven 2D state points: P rough slit pores (rough_slit_pore_2d) of a common width nx dx along
+    x (periodic), heights H ~ U(h_range) sigma, wall strengths eps_w ~ U(eps_range) and bulk
+    densities rho_b ~ U(rho_range), all seeded.  The fluid-fluid eps_ff/kT = 0.5 keeps the fluid
+    above its mean-field critical temperature: at eps_ff/kT = 1 these narrow pores condense and a
+    Newton start from the gas-like Boltzmann profile diverges (the oracle's as well)."""
+    rng = SplitMix64(seed)
+    H = rng.uniform(npores, *h_range)
+    eps = rng.uniform(npores, *eps_range)
+    rho = rng.uniform(npores, *rho_range)
+    pores = [rough_slit_pore_2d(seed=seed * 1000 + i, nx=nx, ny=int(np.round(H[i] / dx)) + 1, dx=dx,
+                                eps_w=float(eps[i]), rho_b=float(rho[i]), cut=cut, eps_ff=eps_ff)
+             for i in range(npores)]
     return BatchSpec(pores)
 
 

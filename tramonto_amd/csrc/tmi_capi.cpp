@@ -277,6 +277,48 @@ void batch_geometry(Problem &P, PadGeom &pg, int h)
     upload(pg.d_pad_node, pad_node.data(), pad_node.size());
 }
 
+// The padded copy of a batch of 2D pieces stacked along y (piece q: rows pore_off[q] / nx ..): every
+// piece is preceded by hy wall rows and the last one followed by hy more; along x the halo of hx
+// columns holds the periodic image (periodic x) or wall (walled x).  Stencil offsets keep the
+// padded grid's row pitch nx + 2 hx.
+void batch_geometry_2d(Problem &P, PadGeom &pg, int hx, int hy)
+{
+    const int nx = P.p.nx[0];
+    const bool per = P.p.bc[0][0] == TMI_PERIODIC;
+    if (per && hx > nx) throw std::domain_error("stencil reach exceeds the periodic domain");
+    const long long P0 = nx + 2LL * hx, rows = (long long)P.p.nx[1] + (long long)(P.npores + 1) * hy;
+    const long long n = P0 * rows;
+    if (n > INT32_MAX) throw std::domain_error("batch too large for 32-bit padded positions");
+    pg.h[0] = hx;
+    pg.h[1] = hy;
+    pg.h[2] = 0;
+    pg.P[0] = (int)P0;
+    pg.P[1] = (int)rows;
+    pg.P[2] = 1;
+    pg.n = n;
+    std::vector<int> node_pad(P.N), pad_node((size_t)n, -1);
+    for (int q = 0; q < P.npores; ++q) {
+        const int r0 = P.pore_off[q] / nx, r1 = P.pore_off[q + 1] / nx;
+        for (int r = r0; r < r1; ++r) {
+            const long long prow = r + (long long)(q + 1) * hy;
+            for (int xp = 0; xp < P0; ++xp) {
+                int x = xp - hx;
+                if (x < 0 || x >= nx) {
+                    if (!per) continue;
+                    x = ((x % nx) + nx) % nx;
+                }
+                const int node = x + r * nx;
+                pad_node[(size_t)(xp + prow * P0)] = node;
+                if (xp - hx == x) node_pad[node] = (int)(xp + prow * P0);
+            }
+        }
+    }
+    pg.d_node_pad = dev_alloc<int>(node_pad.size());
+    pg.d_pad_node = dev_alloc<int>(pad_node.size());
+    upload(pg.d_node_pad, node_pad.data(), node_pad.size());
+    upload(pg.d_pad_node, pad_node.data(), pad_node.size());
+}
+
 void free_geometry_maps(PadGeom &pg)
 {
     (void)hipFree(pg.d_node_pad);
@@ -294,16 +336,27 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
     tmi_params p = p_in;
     std::vector<int> pore_off;
     if (batch) {
-        if (p.ndim != 1) throw std::invalid_argument("batched problems are 1D");
+        if (p.ndim != 1 && p.ndim != 2) throw std::invalid_argument("batched problems are 1D or 2D");
         if (batch->npores < 1 || !batch->nodes || !batch->rho_b) throw std::invalid_argument("batch layout");
+        if (p.ndim == 2) {
+            if (p.nx[0] < 1) throw std::invalid_argument("2D pieces need a common nx[0]");
+            if (p.bc[0][0] != p.bc[0][1] || (p.bc[0][0] != TMI_PERIODIC && p.bc[0][0] != TMI_IN_WALL))
+                throw std::domain_error("2D pieces of a batch are periodic or walled along x");
+        }
         pore_off.assign(1, 0);
         for (int q = 0; q < batch->npores; ++q) {
             if (batch->nodes[q] < 1) throw std::invalid_argument("pore node count must be positive");
+            if (p.ndim == 2 && batch->nodes[q] % p.nx[0]) throw std::invalid_argument("2D piece node count not a multiple of nx");
             if ((long long)pore_off.back() + batch->nodes[q] > INT32_MAX) throw std::domain_error("batch too large");
             pore_off.push_back(pore_off.back() + batch->nodes[q]);
         }
-        p.nx[0] = pore_off.back();
-        p.bc[0][0] = p.bc[0][1] = TMI_IN_WALL;
+        if (p.ndim == 1) {
+            p.nx[0] = pore_off.back();
+            p.bc[0][0] = p.bc[0][1] = TMI_IN_WALL;
+        } else {   // pieces stacked along y, each between walls in y
+            p.nx[1] = pore_off.back() / p.nx[0];
+            p.bc[1][0] = p.bc[1][1] = TMI_IN_WALL;
+        }
         for (int c = 0; c < p.ncomp; ++c) p.rho_b[c] = batch->rho_b[c];
         for (int q = 0; q < batch->npores; ++q)
             for (int c = 0; c < p.ncomp; ++c)
@@ -376,8 +429,13 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         }
     try {
         if (batch) {
-            batch_geometry(*P, P->pad_u, hu[0]);
-            batch_geometry(*P, P->pad_f, hf[0]);
+            if (nd == 1) {
+                batch_geometry(*P, P->pad_u, hu[0]);
+                batch_geometry(*P, P->pad_f, hf[0]);
+            } else {
+                batch_geometry_2d(*P, P->pad_u, hu[0], hu[1]);
+                batch_geometry_2d(*P, P->pad_f, hf[0], hf[1]);
+            }
             std::vector<int> pore_of(N);
             for (int q = 0; q < P->npores; ++q)
                 for (int i = P->pore_off[q]; i < P->pore_off[q + 1]; ++i) pore_of[i] = q;
@@ -485,7 +543,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
         // FFT forms of the stencil sums on 2D/3D grids: the mean-field sums when the attraction
         // stencils are wide (TMI_MF_FFT=1 forces it, 0 keeps the direct sums), the FMT sums when
         // that form fills faster (TMI_FMT_FFT=1 / 0 forces the FFT / direct form)
-        if (nd >= 2) {
+        if (nd >= 2 && !batch) {   // batches of 2D pieces keep the direct sums (their padded copy has gap rows)
             const char *fe = std::getenv("TMI_MF_FFT"), *fm = std::getenv("TMI_FMT_FFT");
             long long lsum = 0;
             if (p.type_attr >= 0)

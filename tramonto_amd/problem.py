@@ -307,6 +307,40 @@ def rough_wall_2d(seed: int = 4, nx: int = 256, ny: int = 256, dx: float = 0.05,
                        cut_ff=np.full((1, 1), cut), rho_b=[rho_b], vext=vext, zero_tf=zero, rho0=rho[None, :])
 
 
+def rough_slit_pore_2d(seed: int = 3, nx: int = 64, ny: int = 61, dx: float = 0.1, eps_w: float = 1.0,
+                       amplitude: float = 0.3, corr_len: float = 1.5, rho_b: float = 0.5, cut: float = 2.5,
+                       wall_pos: float = 0.3, wall_cut: float = 3.0, z_excl: float = 0.62,
+                       vext_max: float = 10.0, eps_ff: float = 1.0) -> ProblemSpec:
+    """2D Rosenfeld + WCA fluid (sigma = 1, eps = eps_ff, cut-off `cut`, kT = 1) in a slit between two
+    rough LJ 9-3 walls, periodic along x.  The lower surface sits at y_l(x) = wall_pos + h_l(x),
+    the upper at y_u(x) = (ny - 1) dx - wall_pos - h_u(x), with h_l, h_u seeded Gaussian-correlated
+    periodic profiles (gaussian_rough_profile); V_ext = lj93_wall(y - y_l) + lj93_wall(y_u - y)
+    (eps_w, sigma_w = 1, cut wall_cut).  Nodes closer than z_excl to either surface are excluded
+    (Zero_density_TF, V_ext = VEXT_MAX).  Boundaries: periodic in x, IN_WALL at both y ends."""
+    xv = np.arange(nx) * dx
+    yv = np.arange(ny) * dx
+    hl = gaussian_rough_profile(nx, dx, amplitude, corr_len, seed)
+    hu = gaussian_rough_profile(nx, dx, amplitude, corr_len, seed + 7)
+    zl = yv[:, None] - (wall_pos + hl[None, :])                      # [ny, nx]
+    zu = ((ny - 1) * dx - wall_pos - hu[None, :]) - yv[:, None]
+    excl = (zl < z_excl) | (zu < z_excl)
+    v = lj93_wall(np.where(excl, 1.0, zl), eps_w, 1.0, wall_cut) + lj93_wall(np.where(excl, 1.0, zu), eps_w, 1.0,
+                                                                             wall_cut)
+    if np.any(v[~excl] >= vext_max):
+        raise ValueError("V_ext reaches VEXT_MAX outside the excluded region; raise z_excl")
+    v = np.where(excl, vext_max, v)
+    vext = v.reshape(-1)[:, None]                                   # node = i + j*nx: x fastest
+    zero = excl.reshape(-1).astype(np.uint8)[:, None]
+    rng = SplitMix64(seed + 1)
+    # exp(-max(V, 0)): the attractive wells are not amplified into an over-packed start
+    rho = rho_b * np.exp(-np.clip(vext[:, 0], 0.0, 30.0)) * (1.0 + 0.02 * rng.uniform(nx * ny, -1.0, 1.0))
+    rho[zero[:, 0] == 1] = 0.0
+    return ProblemSpec(name="rough_slit_pore_2d", ndim=2, ncomp=1, nx=[nx, ny], dx=[dx, dx],
+                       bc=[[PERIODIC, PERIODIC], [IN_WALL, IN_WALL]], type_func=FMT1, type_attr=0, nwall=1,
+                       vext_max=vext_max, hs_diam=[1.0], sigma_ff=np.ones((1, 1)), eps_ff=np.full((1, 1), eps_ff),
+                       cut_ff=np.full((1, 1), cut), rho_b=[rho_b], vext=vext, zero_tf=zero, rho0=rho[None, :])
+
+
 def dirichlet_split(total: float, k: int, seed: int) -> np.ndarray:
     """total * (w_1..w_k), w ~ Dirichlet(1,...,1) from SplitMix64 (normalised exponentials)."""
     e = -np.log(SplitMix64(seed).uniform(k, 1e-300, 1.0))
