@@ -914,10 +914,12 @@ __global__ void k_fwd_mul(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z, cons
 // element and output point is then read or written once (points repeat where k_a = 0 or M_a/2:
 // the same values are computed and stored twice by one thread).
 struct Group8 {
-    long long q[8];   // storage index of point s
+    int q[8];         // storage index of point s (a field holds fewer than 2^31 elements when tables fold)
     long long trow;   // folded table row
     int tx[2];        // table columns of +kx and -kx
-    int neg[8];       // bit d: an odd table changes sign at point s (axis d folded from above M_d/2)
+    int nyz;          // bit 1: the y of the points with s bit 1 set folds from above M_y/2; bit 2: the same for z
+    // bit d of the result: an odd table in axis d changes sign at point s (d = 1, 2; x is never folded)
+    __device__ int neg(int s) const { return nyz & (((s >> 1) & 1) << 1 | ((s >> 2) & 1) << 2); }
 };
 __device__ inline bool group8(const FftGeo &f, long long t, Group8 &g)
 {
@@ -931,68 +933,124 @@ __device__ inline bool group8(const FftGeo &f, long long t, Group8 &g)
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const int y = ys[(s >> 1) & 1], z = zs[s >> 2];
-        g.q[s] = xs[s & 1] + (long long)y * f.S1 + (long long)z * f.S2;
-        g.neg[s] = (tab_neg(f.tg, 1, y) ? 2 : 0) | (tab_neg(f.tg, 2, z) ? 4 : 0);
+        g.q[s] = (int)(xs[s & 1] + (long long)y * f.S1 + (long long)z * f.S2);
     }
+    // ky, kz <= M/2 never fold; their mirrors fold when they exceed M/2
+    g.nyz = (tab_neg(f.tg, 1, ys[1]) ? 2 : 0) | (tab_neg(f.tg, 2, zs[1]) ? 4 : 0);
     g.trow = (long long)ky * f.tg.p1 + (long long)kz * f.tg.p2;
     g.tx[0] = xs[0];
     g.tx[1] = xs[1];
     return true;
 }
 
+// A group of 8 split over two adjacent lanes: lane half h takes the 4 points s_j = 2h ^ S[j],
+// S = {0, 7, 6, 1} (bits x, y, z of s: the axes negated).  Entries j and j ^ 1 are each other's
+// negatives, and the x bit of s_j is j & 1, so entry j reads table column j & 1.  The two lanes read the same table elements in one
+// instruction, so the tables still come from memory once.
+struct Group4 {
+    int q[4];         // storage index of point s_j
+    long long trow;   // folded table row
+    int tx[2];        // table columns of +kx and -kx
+    int sneg[4];      // bit d: an odd table in axis d changes sign at point s_j
+};
+__device__ inline bool group4(const FftGeo &f, long long t, Group4 &g)
+{
+    const int hx = f.M[0] / 2, hy = f.M[1] / 2, hz = f.M[2] / 2;
+    const long long nxr = hx + 1, nyr = hy + 1, grp = t >> 1;
+    if (grp >= nxr * nyr * (hz + 1)) return false;
+    const int h = (int)(t & 1);
+    const int kx = (int)(grp % nxr);
+    const long long r = grp / nxr;
+    const int ky = (int)(r % nyr), kz = (int)(r / nyr);
+    const int x0 = kx, x1 = kx ? f.M[0] - kx : 0, y1 = ky ? f.M[1] - ky : 0, z1 = kz ? f.M[2] - kz : 0;
+    // the y of entry j is mirrored when bit 1 of s_j = 2h ^ S[j] is set
+    const int ya = h ? y1 : ky, yb = h ? ky : y1;   // entries with S[j] bit 1 clear / set
+    const bool ny = tab_neg(f.tg, 1, y1), nz = tab_neg(f.tg, 2, z1);
+    const int na = (h && ny) ? 2 : 0, nb = (!h && ny) ? 2 : 0, nzb = nz ? 4 : 0;
+    // S[j] as (x, y, z) bits: j = 0: (0,0,0), 1: (1,1,1), 2: (0,1,1), 3: (1,0,0); h flips y
+    g.q[0] = (int)(x0 + (long long)ya * f.S1 + (long long)kz * f.S2);
+    g.q[1] = (int)(x1 + (long long)yb * f.S1 + (long long)z1 * f.S2);
+    g.q[2] = (int)(x0 + (long long)yb * f.S1 + (long long)z1 * f.S2);
+    g.q[3] = (int)(x1 + (long long)ya * f.S1 + (long long)kz * f.S2);
+    g.sneg[0] = na;
+    g.sneg[1] = nb | nzb;
+    g.sneg[2] = nb | nzb;
+    g.sneg[3] = na;
+    g.trow = (long long)ky * f.tg.p1 + (long long)kz * f.tg.p2;
+    g.tx[0] = x0;
+    g.tx[1] = x1;
+    return true;
+}
+__host__ __device__ constexpr long long group4_threads(long long ngroups) { return 2 * ngroups; }
+
+// The products over Group4 (folded tables only), one output pair (outputs o = 2 p, 2 p + 1) at a
+// time: its table values for every component are loaded once, then each of the lane's 4 points
+// is formed and stored.
 template <typename T, int ND>
-__global__ __launch_bounds__(kBlock) void k_fwd_mul8(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z,
+__global__ __launch_bounds__(kBlock) void k_fwd_mul4(FftGeo f, FwdDesc d, const Cx<T> *__restrict__ Z,
                                                      const T *__restrict__ tk, const T *__restrict__ uk,
                                                      Cx<T> *__restrict__ O)
 {
-    Group8 g;
-    if (!group8(f, (long long)blockIdx.x * kBlock + threadIdx.x, g)) return;
+    Group4 g;
+    if (!group4(f, (long long)blockIdx.x * kBlock + threadIdx.x, g)) return;
     constexpr int NP = (kNC + 1) / 2;
-    Cx<T> z[NP][8];
+    Cx<T> z[NP][4];
 #pragma unroll
     for (int j = 0; j < NP; ++j)
 #pragma unroll
-        for (int s = 0; s < 8; ++s) z[j][s] = (2 * j < d.C) ? Z[(size_t)j * f.ns + g.q[s]] : Cx<T>{(T)0, (T)0};
-    auto X = [&](int c, int s) { return unpack(z[c >> 1][s], z[c >> 1][s ^ 7], c & 1); };
-    Cx<T> acc[8];
+        for (int e = 0; e < 4; ++e) z[j][e] = (2 * j < d.C) ? Z[(size_t)j * f.ns + g.q[e]] : Cx<T>{(T)0, (T)0};
+    auto X = [&](int c, int e) { return unpack(z[c >> 1][e], z[c >> 1][e ^ 1], c & 1); };
+    // table values of output o for component c at +kx and -kx (0 past the components)
+    auto load_w = [&](int o, T (&w)[kNC][2]) {
+        const int kind = d.kind[o];
 #pragma unroll
-    for (int o = 0; o < kMaxOut; ++o) {
-        if (o < d.nout) {
-            const int kind = d.kind[o];
-            Cx<T> y[8];
+        for (int c = 0; c < kNC; ++c) {
+            w[c][0] = w[c][1] = (T)0;
+            if (c < d.C) {
+                const T *tb = kind < kKindMf ? tk + ((size_t)c * (2 + ND) + kind) * f.tg.tn + g.trow
+                                             : uk + (size_t)(kind - kKindMf + d.C * c) * f.tg.tn + g.trow;
+                const T cf = kind < kKindMf ? (T)d.coef[o][c] : (T)1;
+                w[c][0] = cf * tb[g.tx[0]];
+                w[c][1] = cf * tb[g.tx[1]];
+            }
+        }
+    };
+    // output o at entry e: the sum over components in ascending order
+    auto term = [&](int kind, const T (&w)[kNC][2], int e) {
+        Cx<T> y{(T)0, (T)0};
 #pragma unroll
-            for (int s = 0; s < 8; ++s) y[s] = Cx<T>{(T)0, (T)0};
-#pragma unroll
-            for (int c = 0; c < kNC; ++c) {
-                if (c < d.C) {
-                    const T *tb = kind < kKindMf ? tk + ((size_t)c * (2 + ND) + kind) * f.tg.tn + g.trow
-                                                 : uk + (size_t)(kind - kKindMf + d.C * c) * f.tg.tn + g.trow;
-                    const T cf = kind < kKindMf ? (T)d.coef[o][c] : (T)1;
-                    const T w2[2] = {cf * tb[g.tx[0]], cf * tb[g.tx[1]]};
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        T w = w2[s & 1];
-                        const Cx<T> xv = X(c, s);
-                        if (kind >= 2 && kind < kKindMf) {   // imaginary spectrum: (i w) X
-                            if ((g.neg[s] >> (kind - 2)) & 1) w = -w;
-                            y[s] = cadd(y[s], Cx<T>{-w * xv.y, w * xv.x});
-                        } else {
-                            y[s] = cadd(y[s], cscale(xv, w));
-                        }
-                    }
+        for (int c = 0; c < kNC; ++c) {
+            if (c < d.C) {
+                T wv = w[c][e & 1];
+                const Cx<T> xv = X(c, e);
+                if (kind >= 2 && kind < kKindMf) {   // imaginary spectrum: (i w) X
+                    if ((g.sneg[e] >> (kind - 2)) & 1) wv = -wv;
+                    y = cadd(y, Cx<T>{-wv * xv.y, wv * xv.x});
+                } else {
+                    y = cadd(y, cscale(xv, wv));
                 }
             }
-            if ((o & 1) == 0) {
+        }
+        return y;
+    };
+#pragma unroll 1
+    for (int p = 0; 2 * p < d.nout; ++p) {
+        const int oa = 2 * p, ob = 2 * p + 1;
+        const bool has_b = ob < d.nout;
+        T wa[kNC][2], wb[kNC][2];
+        load_w(oa, wa);
+        if (has_b)
+            load_w(ob, wb);
+        const int ka = d.kind[oa], kb = has_b ? d.kind[ob] : 0;
+        Cx<T> *out = O + (size_t)p * f.ns;
 #pragma unroll
-                for (int s = 0; s < 8; ++s) acc[s] = y[s];
-                if (o + 1 == d.nout) {
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) O[(size_t)(o >> 1) * f.ns + g.q[s]] = acc[s];
-                }
+        for (int e = 0; e < 4; ++e) {
+            const Cx<T> ya = term(ka, wa, e);
+            if (has_b) {
+                const Cx<T> yb = term(kb, wb, e);
+                out[g.q[e]] = Cx<T>{ya.x - yb.y, ya.y + yb.x};
             } else {
-#pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    O[(size_t)(o >> 1) * f.ns + g.q[s]] = Cx<T>{acc[s].x - y[s].y, acc[s].y + y[s].x};
+                out[g.q[e]] = ya;
             }
         }
     }
@@ -1141,7 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_el_mul8(FftGeo f, int C, ElConsts ec
                     Cx<T> e = cadd(cscale(S3, tv[c][0][s & 1]), cscale(A, tv[c][1][s & 1]));
 #pragma unroll
                     for (int dd = 0; dd < ND; ++dd) {
-                        const T kv = tv[c][2 + dd][s & 1] * sg[(g.neg[s] >> dd) & 1];
+                        const T kv = tv[c][2 + dd][s & 1] * sg[(g.neg(s) >> dd) & 1];
                         Cx<T> B;
                         if constexpr (DF)
                             B = cadd(fld[4 + dd], cscale(fld[4 + ND + dd], (T)ec.i4pir[c]));
@@ -2151,7 +2209,7 @@ void fft_setup(Problem &P, bool mf, bool fmt)
         if (fmt)
             for (int c = 0; c < C; ++c) fold = fold && reflective(P.hs[TMI_THETA][c]) && reflective(P.hs[TMI_DELTA][c]);
         const char *fold_env = std::getenv("TMI_FFT_FOLD");   // TMI_FFT_FOLD=0 keeps whole-grid tables
-        F->fold = fold && !(fold_env && std::string(fold_env) == "0");
+        F->fold = fold && F->ns <= INT32_MAX && !(fold_env && std::string(fold_env) == "0");
         if (F->fold) {
             const long long H0 = F->M[0], H1 = F->M[1] / 2 + 1, H2 = F->M[2] / 2 + 1;
             F->tp1 = H0;
@@ -2385,12 +2443,12 @@ void fft_rb_mf(Problem &P, const double *rho_nodes, bool with_rb, const double *
         } else {
             fft_pass<T>(F, Z, nip, -1, 0, s);
             if (F.fold) {
-                const dim3 g8((unsigned)grid_for(group8_count(F)));
+                const dim3 g4((unsigned)grid_for(group4_threads(group8_count(F))));
                 if (P.p.ndim == 2)
-                    hipLaunchKernelGGL((k_fwd_mul8<T, 2>), g8, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                    hipLaunchKernelGGL((k_fwd_mul4<T, 2>), g4, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
                                        attr_spectra<T>(F), O);
                 else
-                    hipLaunchKernelGGL((k_fwd_mul8<T, 3>), g8, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
+                    hipLaunchKernelGGL((k_fwd_mul4<T, 3>), g4, dim3(kBlock), 0, s, fg, d, Z, fmt_spectra<T>(F),
                                        attr_spectra<T>(F), O);
             } else if (P.p.ndim == 2) {
                 hipLaunchKernelGGL((k_fwd_mul<T, 2>), dim3(grid_for(F.n)), dim3(kBlock), 0, s, fg, d, Z,

@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "tmi_device.hpp"
 #include "tmi_fmt.hpp"
@@ -38,6 +39,12 @@ namespace {
 using namespace dev;
 
 // ---------------------------------------------------------------- padding
+// two values stored as one vector (a complex element of a pair transform buffer)
+template <typename T>
+struct alignas(2 * sizeof(T)) TPair {
+    T a, b;
+};
+
 // rho_pad[c][pos] from in[c*N + node]; bulk value rho_b (residual) or 0 (linear)
 template <typename T>
 __global__ void k_pad_rho(Geo g, const double *__restrict__ in, const unsigned char *__restrict__ zero, CompConsts cc,
@@ -61,11 +68,16 @@ __global__ void k_pad_rho(Geo g, const double *__restrict__ in, const unsigned c
 
 // pointwise FMT: residual (dphi) or linear (H dn); writes the adjoint fields
 // at the interior positions of the FMT-halo geometry, or (pairs.buf set) as
-// pairs of the transform buffer of a wrapping FFT grid
-template <typename T, int ND>
-__global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, const double *__restrict__ dnrb, int linear,
-                            CompConsts cc, T *__restrict__ adj, FftPairs pairs, int jref)
+// pairs of the transform buffer of a wrapping FFT grid.  MODE (kFmtResidual,
+// kFmtLinear, kFmtLinearRef) is a template parameter so that each mode is
+// compiled with only its own registers.
+constexpr int kFmtResidual = 0, kFmtLinear = 1, kFmtLinearRef = 2;
+template <typename T, int ND, int MODE>
+__global__ __launch_bounds__(kBlock) void k_fmt_point(Geo g, int func, const double *__restrict__ xrb,
+                                                      const double *__restrict__ dnrb, CompConsts cc,
+                                                      T *__restrict__ adj, FftPairs pairs)
 {
+    constexpr bool linear = MODE != kFmtResidual, jref = MODE == kFmtLinearRef;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.N) return;
     constexpr int NR = 4 + 2 * ND;
@@ -82,9 +94,9 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
         n[4 + d] = rb[4 + ND + d];
         n[4 + ND + d] = rb[4 + d];
     }
-    if (!linear) {
+    if constexpr (!linear) {
         fmt_dphi<ND>(func, n, out);
-    } else {
+    } else {   // both linear modes: the exact product (the reference mode keeps its S3 = out[3])
         double dr[NR], dn[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q) dr[q] = dnrb[(size_t)q * g.N + i];
@@ -99,27 +111,16 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
         }
         fmt_hvp<ND>(func, n, dn, out);
     }
+    // the adjoint fields in v[0 .. nf): A_c, B_c,d (component c at c (1 + ND)), S3 at C (1 + ND), or in
+    // the derivative form D3 D2 D1 D0 Dv2_d Dv1_d; every index below is a compile-time one
     const int C = g.C;
-    long long p, fstride;
-    T *dst;
-    if (pairs.buf) {   // field f at 2 ((f / 2) ns + q) + f % 2
-        const int x = i % g.nx[0];
-        const int r = i / g.nx[0];
-        p = 2 * ((long long)x + (r % g.nx[1]) * pairs.S1 + (r / g.nx[1]) * pairs.S2);
-        fstride = pairs.ns;
-        dst = reinterpret_cast<T *>(pairs.buf);
-    } else {
-        p = pad_pos(g, i);
-        fstride = g.np;
-        dst = adj;
-    }
-    auto put = [&](int f, T v) {
-        if (pairs.buf)
-            dst[(size_t)(f >> 1) * 2 * fstride + p + (f & 1)] = v;
-        else
-            dst[(size_t)f * fstride + p] = v;
-    };
-    if (linear && jref) {   // the reference's delta rows for A_c and B_c,d; S3 from the theta row (exact)
+    constexpr int NV = 2 * ((kNC * (1 + ND) + 2) / 2);
+    T v[NV];
+#pragma unroll
+    for (int f = 0; f < NV; ++f) v[f] = (T)0;
+    int nf;
+    bool dform_out = false;
+    if constexpr (jref) {   // the reference's delta rows for A_c and B_c,d; S3 from the theta row (exact)
         double sg[ND > 0 ? ND : 1];
         for (int d = 0; d < ND; ++d) sg[d] = 1.0;
         double dn[NR];
@@ -142,43 +143,61 @@ __global__ void k_fmt_point(Geo g, int func, const double *__restrict__ xrb, con
             if (c < C) {
                 double A, B[ND > 0 ? ND : 1];
                 fmt_ref_adjoint<ND>(func, n, dn, sg, cc.i4pir[c], cc.i4pirsq[c], A, B);
-                put(c * (1 + ND), (T)A);
+                v[c * (1 + ND)] = (T)A;
 #pragma unroll
-                for (int d = 0; d < ND; ++d) put(c * (1 + ND) + 1 + d, (T)B[d]);
+                for (int d = 0; d < ND; ++d) v[c * (1 + ND) + 1 + d] = (T)B[d];
             }
         }
-        put(C * (1 + ND), (T)out[3]);
-        if (pairs.buf && ((C * (1 + ND) + 1) & 1)) put(C * (1 + ND) + 1, (T)0);
-        return;
-    }
-    if (pairs.buf && pairs.dform) {   // the derivatives themselves, rho-bar order: D3 D2 D1 D0 Dv2_d Dv1_d
-        put(0, (T)out[3]);
-        put(1, (T)out[2]);
-        put(2, (T)out[1]);
-        put(3, (T)out[0]);
+        nf = C * (1 + ND) + 1;
+    } else if (pairs.buf && pairs.dform) {   // the derivatives themselves, rho-bar order
+        v[0] = (T)out[3];
+        v[1] = (T)out[2];
+        v[2] = (T)out[1];
+        v[3] = (T)out[0];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            put(4 + d, (T)out[4 + ND + d]);
-            put(4 + ND + d, (T)out[4 + d]);
+            v[4 + d] = (T)out[4 + ND + d];
+            v[4 + ND + d] = (T)out[4 + d];
         }
-        return;   // 4 + 2 ND fields: whole pairs
-    }
+        nf = 4 + 2 * ND;
+        dform_out = true;
+    } else {
 #pragma unroll
-    for (int c = 0; c < kNC; ++c) {   // compile-time component index: the constants stay in registers
-        if (c < C) {
-            put(c * (1 + ND), (T)(out[0] * cc.i4pirsq[c] + out[1] * cc.i4pir[c] + out[2]));
+        for (int c = 0; c < kNC; ++c) {   // compile-time component index: the constants stay in registers
+            if (c < C) {
+                v[c * (1 + ND)] = (T)(out[0] * cc.i4pirsq[c] + out[1] * cc.i4pir[c] + out[2]);
 #pragma unroll
-            for (int d = 0; d < ND; ++d) put(c * (1 + ND) + 1 + d, (T)(out[4 + d] * cc.i4pir[c] + out[4 + ND + d]));
+                for (int d = 0; d < ND; ++d) v[c * (1 + ND) + 1 + d] = (T)(out[4 + d] * cc.i4pir[c] + out[4 + ND + d]);
+            }
         }
+        nf = C * (1 + ND) + 1;
     }
-    put(C * (1 + ND), (T)out[3]);
-    if (pairs.buf && ((C * (1 + ND) + 1) & 1)) put(C * (1 + ND) + 1, (T)0);   // odd field count: empty partner
+    if (!dform_out) {   // S3 after the components (v[nf] stays 0: an odd count's empty partner)
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+            if (f == C * (1 + ND)) v[f] = (T)out[3];
+    }
+    if (pairs.buf) {   // pair j = fields (2 j, 2 j + 1) at element j ns + q of the transform buffer: one 16-byte store
+        const int x = i % g.nx[0];
+        const int r = i / g.nx[0];
+        const long long q = (long long)x + (r % g.nx[1]) * pairs.S1 + (r / g.nx[1]) * pairs.S2;
+        TPair<T> *dst = reinterpret_cast<TPair<T> *>(pairs.buf);
+#pragma unroll
+        for (int j = 0; j < NV / 2; ++j)
+            if (2 * j < nf) dst[(size_t)j * pairs.ns + q] = TPair<T>{v[2 * j], v[2 * j + 1]};
+    } else {
+        const long long p = pad_pos(g, i);
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+            if (f < nf) adj[(size_t)f * g.np + p] = v[f];
+    }
 }
 
-template <typename T, int ND>
-__global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj, int func, const double *__restrict__ xrb,
-                          const double *__restrict__ dnrb, int jref)
+template <typename T, int ND, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pad_adj(Geo g, CompConsts cc, T *__restrict__ adj, int func,
+                                                    const double *__restrict__ xrb, const double *__restrict__ dnrb)
 {
+    constexpr bool linear = MODE != kFmtResidual;
     long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= g.np) return;
     int src = 0, refl = 0;
@@ -187,7 +206,7 @@ __global__ void k_pad_adj(Geo g, int linear, CompConsts cc, T *__restrict__ adj,
     if (code == 0) {
         const long long ps = pad_pos(g, src);
         if (ps == pos) return;   // interior position
-        if (linear && jref && refl) {
+        if (MODE == kFmtLinearRef && refl) {
             // reference-Jacobian mode across a REFLECT boundary: the reference evaluates its delta row
             // at the node with the mirror signs (src/dft_fill_FMT.c:41-189 with sign[] of
             // offset_to_node_box), which is not the mirror image of the node's own value for every
@@ -1484,32 +1503,32 @@ void fmt_point(Problem &P, const double *x_rb, const double *dn_rb, bool linear,
     Geo gf = make_geo(P, P.pad_f);
     const CompConsts cc = make_consts(P);
     T *adj = workspace<T>(P).adj_pad;
-    const int lin = linear ? 1 : 0;
     const int func = P.p.type_func;
-    const int jref = (linear && P.jac_ref) ? 1 : 0;
-    if (fft_fmt(P) && fft_wraps(P)) {   // straight into the transform buffer as pairs: no halo
-        const FftPairs fp = fft_pairs(P, linear);
-        if (P.p.ndim == 2)
-            hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp, jref);
+    const int mode = !linear ? kFmtResidual : (P.jac_ref ? kFmtLinearRef : kFmtLinear);
+    const bool pairs = fft_fmt(P) && fft_wraps(P);   // straight into the transform buffer as pairs: no halo
+    const FftPairs fp = pairs ? fft_pairs(P, linear) : FftPairs{nullptr, 0, 0, 0, 0};
+    auto go = [&](auto nd, auto md) {
+        constexpr int ND = decltype(nd)::value, MODE = decltype(md)::value;
+        hipLaunchKernelGGL((k_fmt_point<T, ND, MODE>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb,
+                           dn_rb, cc, adj, fp);
+        if (!pairs)
+            hipLaunchKernelGGL((k_pad_adj<T, ND, MODE>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, cc, adj, func,
+                               x_rb, dn_rb);
+    };
+    auto by_mode = [&](auto nd) {
+        if (mode == kFmtResidual)
+            go(nd, std::integral_constant<int, kFmtResidual>{});
+        else if (mode == kFmtLinear)
+            go(nd, std::integral_constant<int, kFmtLinear>{});
         else
-            hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, fp, jref);
-        return;
-    }
-    const FftPairs nop{nullptr, 0, 0, 0, 0};
-    switch (P.p.ndim) {
-    case 1:
-        hipLaunchKernelGGL((k_fmt_point<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
-        hipLaunchKernelGGL((k_pad_adj<T, 1>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
-        break;
-    case 2:
-        hipLaunchKernelGGL((k_fmt_point<T, 2>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
-        hipLaunchKernelGGL((k_pad_adj<T, 2>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
-        break;
-    default:
-        hipLaunchKernelGGL((k_fmt_point<T, 3>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gf, func, x_rb, dn_rb, lin, cc, adj, nop, jref);
-        hipLaunchKernelGGL((k_pad_adj<T, 3>), dim3(grid_for(gf.np)), dim3(kBlock), 0, s, gf, lin, cc, adj, func, x_rb, dn_rb, jref);
-        break;
-    }
+            go(nd, std::integral_constant<int, kFmtLinearRef>{});
+    };
+    if (P.p.ndim == 1)
+        by_mode(std::integral_constant<int, 1>{});
+    else if (P.p.ndim == 2)
+        by_mode(std::integral_constant<int, 2>{});
+    else
+        by_mode(std::integral_constant<int, 3>{});
 }
 
 template <typename T>
