@@ -100,21 +100,29 @@ def bench_batched_pores(npores=256):
     db = DeviceBatch(b)
     x0 = db.initial_state()
     F = db.zeros()
+    ssd = torch.zeros(1, dtype=torch.float64, device="cuda")
     for _ in range(3):
+        db.fill_residual_device_norm(x0, F, ssd)
         db.fill_residual(x0, F)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):   # sum F^2 on the device, as the headline fill
+        db.fill_residual_device_norm(x0, F, ssd)
+    e1.record()
+    torch.cuda.synchronize()
+    fill_ms = e0.elapsed_time(e1) / 20
     e0.record()
     for _ in range(20):
         db.fill_residual(x0, F)
     e1.record()
     torch.cuda.synchronize()
-    fill_ms = e0.elapsed_time(e1) / 20
+    fill_ms_host = e0.elapsed_time(e1) / 20
     out = {"workload": f"{npores} uneven 1D slit pores, H ~ U(4,30) sigma at dx = 0.02 sigma, binary "
                        "(sigma 1.0/0.8) Rosenfeld FMT + WCA (cut-off 2.5 sigma_i, eps_ff/kT = 0.7), LJ 9-3 walls "
                        "eps_w ~ U(0.5,2), rho_b,i ~ U(0.05,0.35)",
            "pores": npores, "nodes": int(b.nnodes), "pore_nodes_min_max": [int(b.nodes.min()), int(b.nodes.max())],
-           "fill_ms": fill_ms, "fill_gnode_s": b.nnodes / (fill_ms * 1e-3) / 1e9,
+           "fill_ms": fill_ms, "fill_gnode_s": b.nnodes / (fill_ms * 1e-3) / 1e9, "fill_ms_host_norm": fill_ms_host,
            "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
                                      [float(v) for v in db.fill_residual_timed(x0, 10)]))}
     sols = {}
@@ -211,20 +219,32 @@ def bench_workload(spec, label, args, newton_tol=1e-8, newton=True):
     torch.cuda.synchronize()
     N = spec.nnodes
     F = dp.zeros()
+    ssd = torch.zeros(1, dtype=torch.float64, device="cuda")
+    # the timed fill: residual rows and sum F^2 on the device (tmi_fill_residual_noalloc, no host read
+    # per fill); the same fill returning sum F^2 to the host each time is timed beside it
     for _ in range(args.warmup):
+        dp.fill_residual_device_norm(x0, F, ssd)
         dp.fill_residual(x0, F)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
     for _ in range(args.steps):
-        dp.fill_residual(x0, F)
+        dp.fill_residual_device_norm(x0, F, ssd)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_fill = e0.elapsed_time(e1) / args.steps
+    e0.record()
+    for _ in range(args.steps):
+        _, ss_host = dp.fill_residual(x0, F)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fill_host = e0.elapsed_time(e1) / args.steps
+    if abs(float(ssd.item()) - ss_host) > 1e-12 * max(ss_host, 1e-300):
+        raise RuntimeError(f"device and host sums of squares differ: {float(ssd.item())} vs {ss_host}")
     out = {"nodes": N, "unknowns_per_node": spec.nunk, "fill_ms": ms_fill, "fill_gnode_s": N / (ms_fill * 1e-3) / 1e9,
-           "wall_s_timed": wall}
+           "fill_ms_host_norm": ms_fill_host, "wall_s_timed": wall}
     stages = dp.fill_residual_timed(x0, reps=max(3, args.steps // 5))
     jvp = {}
     v = torch.rand_like(x0)

@@ -174,6 +174,14 @@ class DeviceProblem:
                    "tmi_fill_residual")
         return F, s.value
 
+    def fill_residual_device_norm(self, x: torch.Tensor, F: torch.Tensor, sumsq: torch.Tensor) -> None:
+        """The residual fill with sum F^2 written to the device scalar `sumsq` (float64, one element)
+        instead of returned to the host: no host synchronisation (tmi_fill_residual_noalloc)."""
+        if sumsq.dtype != torch.float64 or sumsq.numel() < 1 or not sumsq.is_cuda:
+            raise ValueError("sumsq must be a float64 device tensor with at least one element")
+        _lib.check(_lib.lib().tmi_fill_residual_noalloc(self.handle, _ptr(x), _ptr(F), _ptr(sumsq), _stream()),
+                   "tmi_fill_residual_noalloc")
+
     def fill_residual_timed(self, x: torch.Tensor, reps: int = 10) -> np.ndarray:
         """Mean per-stage milliseconds of `reps` fills (HIP events on the launch stream)."""
         F = self.zeros()
