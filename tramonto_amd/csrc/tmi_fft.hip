@@ -301,6 +301,17 @@ __device__ inline Cx<T> *fft_lines(Cx<T> *a, Cx<T> *b, int ld, int nl, int M, Ra
     return a;
 }
 
+// non-temporal 16-byte store of a complex value (written once, read by a later kernel)
+template <typename T>
+__device__ __forceinline__ void nt_store(Cx<T> *p, Cx<T> v)
+{
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    V2 t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<V2 *>(p));
+}
+
 // Fused ends of a product on a wrapping grid (grid point = node), for the
 // strided passes: kPassGather reads the density pairs of the first forward
 // pass straight from the node arrays (0 where Zero_density_TF); kPassRows
@@ -468,9 +479,9 @@ __device__ inline void strided_store_phase(Cx<T> *__restrict__ data, const Strid
                 double *out = rb ? io.out_rb : io.out_mf;
                 double pin[R2], psub[R2];
 #pragma unroll
-                for (int k2 = 0; k2 < R2; ++k2) {
-                    pin[k2] = in ? in[base + nd[k2]] : 0.0;
-                    psub[k2] = sub ? sub[base + nd[k2]] : 0.0;
+                for (int k2 = 0; k2 < R2; ++k2) {   // read once, written rows never re-read here: non-temporal
+                    pin[k2] = in ? __builtin_nontemporal_load(in + base + nd[k2]) : 0.0;
+                    psub[k2] = sub ? __builtin_nontemporal_load(sub + base + nd[k2]) : 0.0;
                 }
                 const double sc = io.scale[o][r];
 #pragma unroll
@@ -478,7 +489,7 @@ __device__ inline void strided_store_phase(Cx<T> *__restrict__ data, const Strid
                     double val = (double)(h ? u[k2].y : u[k2].x) * sc;
                     if (in) val -= pin[k2];
                     if (sub) val -= psub[k2];
-                    out[base + nd[k2]] = val;
+                    __builtin_nontemporal_store(val, out + base + nd[k2]);
                     ssq += val * val;
                 }
             }
@@ -1209,9 +1220,9 @@ __global__ __launch_bounds__(kBlock) void k_el_mul8(FftGeo f, int C, ElConsts ec
                     }
                     if ((c & 1) == 0) {
                         acc = e;
-                        if (c + 1 == C) O[(size_t)(c >> 1) * f.ns + g.q[s]] = acc;
+                        if (c + 1 == C) nt_store(O + (size_t)(c >> 1) * f.ns + g.q[s], acc);
                     } else {
-                        O[(size_t)(c >> 1) * f.ns + g.q[s]] = Cx<T>{acc.x - e.y, acc.y + e.x};
+                        nt_store(O + (size_t)(c >> 1) * f.ns + g.q[s], Cx<T>{acc.x - e.y, acc.y + e.x});
                     }
                 }
             }
