@@ -1138,16 +1138,31 @@ __device__ inline double block_sum(double v)
     return v;
 }
 
-// partial[j*gridDim.x + blk] = sum over the block's range of V_j . w
-__global__ void k_multidot(const double *__restrict__ V, long long ld, int k, const double *__restrict__ w, long long n,
-                           double *__restrict__ partial)
+// partial[j*gridDim.x + blk] = sum over the block's range of V_j . w, four vectors per sweep over the
+// range (w is re-read once per four vectors, from cache): V is read once, with few registers.  Each
+// V_j . w is summed in the same order as by a sweep per vector.
+constexpr int kDotGroup = 4;
+__global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ V, long long ld, int k,
+                                                     const double *__restrict__ w, long long n,
+                                                     double *__restrict__ partial)
 {
-    for (int j = 0; j < k; ++j) {
-        double s = 0.0;
-        for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-            s += V[(size_t)j * ld + i] * w[i];
-        s = block_sum<kBlock>(s);
-        if (threadIdx.x == 0) partial[(size_t)j * gridDim.x + blockIdx.x] = s;
+    for (int j0 = 0; j0 < k; j0 += kDotGroup) {
+        double acc[kDotGroup];
+#pragma unroll
+        for (int g = 0; g < kDotGroup; ++g) acc[g] = 0.0;
+        for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+            const double wi = w[i];
+#pragma unroll
+            for (int g = 0; g < kDotGroup; ++g)
+                if (j0 + g < k) acc[g] += V[(size_t)(j0 + g) * ld + i] * wi;
+        }
+#pragma unroll
+        for (int g = 0; g < kDotGroup; ++g) {
+            if (j0 + g < k) {
+                const double t = block_sum<kBlock>(acc[g]);
+                if (threadIdx.x == 0) partial[(size_t)(j0 + g) * gridDim.x + blockIdx.x] = t;
+            }
+        }
     }
 }
 
@@ -1273,49 +1288,8 @@ __global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, co
     w[i] -= s;
 }
 
-// Second Gram-Schmidt pass fused with the first update (k <= KM columns): w -= V h, then
-// partial[j][blk] = sum over the block's elements of V_j . w (updated w), j < k.  Every V_j
-// element is read once for both: the update and the new projections.
-template <int KM>
-__global__ __launch_bounds__(kBlock) void k_gemv_sub_dots(const double *__restrict__ V, long long ld, int k,
-                                                          const double *__restrict__ h, double *__restrict__ w,
-                                                          long long n, double *__restrict__ partial)
-{
-    double acc[KM];
-#pragma unroll
-    for (int j = 0; j < KM; ++j) acc[j] = 0.0;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-        double v[KM];
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < KM; ++j) {
-            v[j] = j < k ? V[(size_t)j * ld + i] : 0.0;
-            if (j < k) s += v[j] * h[j];
-        }
-        const double wi = w[i] - s;
-        w[i] = wi;
-#pragma unroll
-        for (int j = 0; j < KM; ++j) acc[j] += v[j] * wi;
-    }
-    __shared__ double sh[kBlock / 64][KM];   // per-wave sums
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < KM; ++j) {
-        if (j < k) {
-            double a = acc[j];
-            for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-            if (lane == 0) sh[wid][j] = a;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < k) {
-        double a = 0.0;
-        for (int q = 0; q < kBlock / 64; ++q) a += sh[q][threadIdx.x];
-        partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
-    }
-}
-
-// w -= V h (k columns), partial[blk] = sum over the block's elements of w^2 (updated w)
+// w -= V h (k columns); NORM: partial[blk] = sum over the block's elements of w^2 (updated w)
+template <bool NORM>
 __global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const double *__restrict__ V, long long ld, int k,
                                                           const double *__restrict__ h, double *__restrict__ w,
                                                           long long n, double *__restrict__ partial)
@@ -1328,8 +1302,10 @@ __global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const double *__restri
         w[i] = wi;
         q += wi * wi;
     }
-    q = block_sum<kBlock>(q);
-    if (threadIdx.x == 0) partial[blockIdx.x] = q;
+    if constexpr (NORM) {
+        q = block_sum<kBlock>(q);
+        if (threadIdx.x == 0) partial[blockIdx.x] = q;
+    }
 }
 
 __global__ void k_gemv_add(const double *__restrict__ V, long long ld, int k, const double *__restrict__ y,
@@ -1971,29 +1947,29 @@ void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d, mirror);
 }
 
+// Workgroups of the Gram-Schmidt passes: 2048 (the streams of V then run near the copy rate), fewer
+// when k rows of partial sums would not fit below the coefficient slot of P.d_red (512 Ki doubles)
+static unsigned gs_blocks(long long n, int k)
+{
+    const long long cap = (512LL * 1024) / std::max(k, 1);
+    return (unsigned)std::max<long long>(1, std::min<long long>({2048, cap, (n + kBlock - 1) / kBlock}));
+}
+
+// classical Gram-Schmidt with one reorthogonalisation pass: h1 = V^T w, w -= V h1, h2 = V^T w,
+// w -= V h2 with |w|^2, next = w / |w|; the coefficients and |w|^2 land in coef_d and in the
+// pinned host mirror (read after the caller synchronises)
 void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
               hipStream_t s)
 {
-    const unsigned nb = red_blocks(n);
+    const unsigned nb = gs_blocks(n, k);
     double *h1 = coef_d, *h2 = coef_d + k, *nrm = coef_d + 2 * k;
     double *m1 = P.h_red_dev, *m2 = m1 + k, *mn = m1 + 2 * k;   // host copies (P.h_red)
-    dev_multi_dot_async(P, V, ld, k, w, n, h1, s, m1);
-    auto fused = [&](auto km) {
-        constexpr int KM = decltype(km)::value;
-        hipLaunchKernelGGL(k_gemv_sub_dots<KM>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, P.d_red);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2, m2);
-    };
-    if (k <= 8)
-        fused(std::integral_constant<int, 8>{});
-    else if (k <= 16)
-        fused(std::integral_constant<int, 16>{});
-    else if (k <= 32)
-        fused(std::integral_constant<int, 32>{});
-    else {
-        dev_gemv_sub(V, ld, k, h1, w, n, s);
-        dev_multi_dot_async(P, V, ld, k, w, n, h2, s, m2);
-    }
-    hipLaunchKernelGGL(k_gemv_sub_norm, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h1, m1);
+    hipLaunchKernelGGL(k_gemv_sub_norm<false>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, nullptr);
+    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2, m2);
+    hipLaunchKernelGGL(k_gemv_sub_norm<true>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb, 1, nrm, mn);
     dev_scale_by_norm(nrm, w, next, n, s);
 }
