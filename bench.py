@@ -35,10 +35,10 @@ COPY_PROBE_BYTES = 512 << 20
 # kernel on wrapping grids): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script with
 # --fill-only --copy-probe, that launch's counters scaled by their readings on the 512 MiB copy (x2.0 and
 # x1.0).  Keys: workload.
-ROWS_TRAFFIC_MIB = {"periodic64": 34.1 + 22.0, "solute128": 442.0 + 222.9}
+ROWS_TRAFFIC_MIB = {"periodic64": 35.7 + 24.6, "solute128": 534.6 + 234.3}
 # The same for the fused x pass (mean per launch) and the plain FFT axis passes, where they are the
 # dominant kernel
-XPROD_TRAFFIC_MIB = {"periodic64": (7.6 + 12.0 + 14.9 + 4.0) / 2, "solute128": None}
+XPROD_TRAFFIC_MIB = {"periodic64": (7.5 + 12.0 + 14.9 + 4.0) / 2, "solute128": None}
 AXIS_TRAFFIC_MIB = {"periodic64": None, "solute128": None}
 
 
