@@ -7,13 +7,15 @@ Dirichlet weights (tramonto_amd.problem.solute_3d): 2,097,152 nodes, 16 unknowns
 largest single-GPU configuration of the benchmark set.  Inputs are resident in HBM before the
 timed region starts.
 
-A step is one FP64 residual fill over the whole grid; `value` is the fill throughput in Gnode/s
-over K timed steps.  The same JSON line carries the Jacobian-vector product throughput (FP64 and
-FP32), the Newton time-to-solve to ||F||_2 < 1e-8 for the plain FP64 path and the mixed path
+A step is one FP64 residual fill over the whole grid (rows and sum F^2 computed on the device);
+`value` is the fill throughput in Gnode/s over K timed steps.  The same JSON line carries the fill
+time when each fill also returns sum F^2 to the host, the Jacobian-vector product throughput (FP64
+and FP32), the Newton time-to-solve to ||F||_2 < 1e-8 for the plain FP64 path and the mixed path
 (FP32 inner Krylov products, FP64 residual/update), the per-stage kernel times of one fill (HIP
-events), the roofline of the dominant kernel (the fused x pass of the FFT form), and the CPU
-baseline (the oracle's OpenMP fill on the host cores, on a bounded sample of nodes).  Extra keys:
-the 64^3 periodic Rosenfeld + WCA workload, the batched slit pores and the 2D adjoint workload.
+events), the roofline of the dominant kernel (the longest launch of the fill: the density rows pass
+at 128^3), and the CPU baseline (the oracle's OpenMP fill on the host cores, on a bounded sample of
+nodes).  Extra keys: the 64^3 periodic Rosenfeld + WCA workload, the batched 1D slit pores, the
+batched 2D rough slit pores and the 2D adjoint workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-newton] [--no-cpu]
                        [--no-batch] [--no-adjoint] [--no-periodic] [--fill-only] [--copy-probe]
@@ -88,15 +90,36 @@ def bench_batched_pores(npores=256):
     pores at once by the batched per-pore Newton-Krylov (graph replay and direct
     launches, FP64 and mixed), (b) pore by pore with the plain single-problem
     FP64 path.  Problem set-up is outside the timed regions."""
+    from tramonto_amd.batch import slit_pore_batch
+
+    return bench_batch(slit_pore_batch(npores=npores),
+                       f"{npores} uneven 1D slit pores, H ~ U(4,30) sigma at dx = 0.02 sigma, binary "
+                       "(sigma 1.0/0.8) Rosenfeld FMT + WCA (cut-off 2.5 sigma_i, eps_ff/kT = 0.7), LJ 9-3 walls "
+                       "eps_w ~ U(0.5,2), rho_b,i ~ U(0.05,0.35)")
+
+
+def bench_batched_2d(npores=16):
+    """Uneven 2D state points: 16 rough slit pores of 64 columns (periodic x) stacked along y, solved
+    as bench_batched_pores solves the 1D pores."""
+    from tramonto_amd.batch import rough_slit_pore_batch_2d
+
+    return bench_batch(rough_slit_pore_batch_2d(npores=npores, seed=11),
+                       f"{npores} uneven 2D rough slit pores, 64 columns (periodic x) at dx = 0.1 sigma, heights "
+                       "H ~ U(4,10) sigma between rough LJ 9-3 walls (eps_w ~ U(0.5,1.5)), Rosenfeld FMT + WCA "
+                       "(cut-off 2.5 sigma, eps_ff/kT = 0.5), rho_b ~ U(0.3,0.5)")
+
+
+def bench_batch(b, workload):
+    """A batch (BatchSpec) solved all at once and pore by pore; see bench_batched_pores."""
     import os as _os
 
     import numpy as np
     import torch
 
-    from tramonto_amd.batch import DeviceBatch, slit_pore_batch
+    from tramonto_amd.batch import DeviceBatch
     from tramonto_amd.device import PREC_FP64, PREC_MIXED, DeviceProblem
 
-    b = slit_pore_batch(npores=npores)
+    npores = b.npores
     db = DeviceBatch(b)
     x0 = db.initial_state()
     F = db.zeros()
@@ -118,9 +141,7 @@ def bench_batched_pores(npores=256):
     e1.record()
     torch.cuda.synchronize()
     fill_ms_host = e0.elapsed_time(e1) / 20
-    out = {"workload": f"{npores} uneven 1D slit pores, H ~ U(4,30) sigma at dx = 0.02 sigma, binary "
-                       "(sigma 1.0/0.8) Rosenfeld FMT + WCA (cut-off 2.5 sigma_i, eps_ff/kT = 0.7), LJ 9-3 walls "
-                       "eps_w ~ U(0.5,2), rho_b,i ~ U(0.05,0.35)",
+    out = {"workload": workload,
            "pores": npores, "nodes": int(b.nnodes), "pore_nodes_min_max": [int(b.nodes.min()), int(b.nodes.max())],
            "fill_ms": fill_ms, "fill_gnode_s": b.nnodes / (fill_ms * 1e-3) / 1e9, "fill_ms_host_norm": fill_ms_host,
            "fill_stage_ms": dict(zip(["pad_rho", "rb_rows", "mf_rows", "fmt_point+pad_adj", "el_rows"],
@@ -376,7 +397,7 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="solute128")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched slit-pore workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workloads (1D slit pores, 2D pieces)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D rough-wall adjoint workload")
     ap.add_argument("--no-periodic", action="store_true", help="skip the 64^3 periodic workload (extra key)")
     ap.add_argument("--fill-only", action="store_true",
@@ -416,6 +437,7 @@ def main():
         extra["periodic_64"] = e
     if not (args.no_batch or args.fill_only):
         extra["batched_pores"] = bench_batched_pores()
+        extra["batched_2d"] = bench_batched_2d()
     if not (args.no_adjoint or args.fill_only):
         extra["adjoint_2d"] = bench_adjoint_2d()
 
