@@ -11,6 +11,7 @@
 #include <string>
 #include <cstdlib>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "tmi_fmt.hpp"
@@ -657,17 +658,21 @@ double *scratch(Problem &P, size_t doubles)
 
 // right-preconditioned restarted GMRES on the Schur complement
 // S = A22 - A21 A11^{-1} A12 (dft_HardSphereLinProbMgr.cpp:345-428);
-// classical Gram-Schmidt with one reorthogonalisation pass.
-int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, double rtol, int m, int maxit, int prec,
-                double *work, hipStream_t s)
+// classical Gram-Schmidt with one reorthogonalisation pass.  TV is the Krylov basis type: FP32 on
+// the mixed path (half the Gram-Schmidt traffic; each basis vector is widened to FP64 before its
+// products), FP64 otherwise; the residual, w, the Arnoldi coefficients and x2 are FP64 throughout.
+template <typename TV>
+int gmres_schur_t(Problem &P, const double *x, const double *bS, double *x2, double rtol, int m, int maxit, int prec,
+                  double *work, hipStream_t s)
 {
     const long long n = (long long)P.p.ncomp * P.N;
     m = std::max(1, std::min(m, 500));
-    double *V = work;                                  // (m+1) n
-    double *z = V + (size_t)(m + 1) * n;               // n
+    TV *V = reinterpret_cast<TV *>(work);                                      // (m+1) n
+    double *z = work + ((size_t)(m + 1) * n * sizeof(TV) + sizeof(double) - 1) / sizeof(double);   // n
     double *w = z + n;                                 // n
     double *dinv = w + n;                              // n
-    double *t1 = dinv + n;                             // (nrb + nmf) N
+    double *vk = dinv + n;                             // n: V_j widened to FP64 (FP32 basis only)
+    double *t1 = vk + (std::is_same<TV, float>::value ? n : 0);   // (nrb + nmf) N
     // right preconditioner: the A22 diagonal (dinv), or its bulk-circulant correction on wrapping FFT grids
     const bool bulk_pc = precond_active(P);
     const double *x_rho = x + (size_t)P.iu_rho * P.N;
@@ -678,10 +683,17 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     double *dh = P.d_red + 1024 * 1024 / 2;
     // w = S M^-1 V_j, the products of an Arnoldi step
     auto products = [&](int j) {
+        const double *vj;
+        if constexpr (std::is_same<TV, float>::value) {
+            dev_widen(V + (size_t)j * n, vk, n, s);
+            vj = vk;
+        } else {
+            vj = V + (size_t)j * n;
+        }
         if (bulk_pc)
-            precond_apply(P, x_rho, V + (size_t)j * n, z, s);
+            precond_apply(P, x_rho, vj, z, s);
         else
-            dev_mul(dinv, V + (size_t)j * n, z, n, s);
+            dev_mul(dinv, vj, z, n, s);
         if (prec == TMI_PREC_MIXED)
             launch_schur_apply<float>(P, x, z, t1, w, s);
         else
@@ -692,21 +704,21 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     int its = 0;
     bool first = true;
     while (its < maxit) {
-        // r = bS - S x2
-        if (first) {
-            TMI_HIP_CHECK(hipMemcpyAsync(V, bS, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        } else {
+        // r = bS - S x2 (in z), V_0 = r / |r|
+        const double *r = bS;
+        if (!first) {
             if (prec == TMI_PREC_MIXED)
                 launch_schur_apply<float>(P, x, x2, t1, w, s);
             else
                 launch_schur_apply<double>(P, x, x2, t1, w, s);
-            dev_scale_copy(1.0, bS, V, n, s);
-            dev_axpy(-1.0, w, V, n, s);
+            dev_scale_copy(1.0, bS, z, n, s);
+            dev_axpy(-1.0, w, z, n, s);
+            r = z;
         }
+        const double beta = first ? bn0 : std::sqrt(dev_dot(P, r, r, n, s));
         first = false;
-        const double beta = std::sqrt(dev_dot(P, V, V, n, s));
         if (beta / bn <= rtol || beta == 0.0) break;
-        dev_scale_copy(1.0 / beta, V, V, n, s);
+        dev_cast_scaled<TV>(1.0 / beta, r, V, n, s);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int k = 0;
@@ -715,7 +727,7 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
             products(k);
             // classical Gram-Schmidt and one reorthogonalisation pass with the coefficients kept
             // on the device; one host synchronisation per step, for the Hessenberg column
-            dev_cgs2(P, V, n, k + 1, w, n, V + (size_t)(k + 1) * n, dh, s);
+            dev_cgs2<TV>(P, V, n, k + 1, w, n, V + (size_t)(k + 1) * n, dh, s);
             TMI_HIP_CHECK(hipStreamSynchronize(s));
             for (int j = 0; j <= k; ++j) h[j] = P.h_red[j] + P.h_red[k + 1 + j];
             const double hn = std::sqrt(P.h_red[2 * (k + 1)]);
@@ -748,7 +760,7 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
         }
         TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
         dev_fill(0.0, z, (long long)n, s);
-        dev_gemv_add(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
+        dev_gemv_add<TV>(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
         if (bulk_pc)
             precond_apply(P, x_rho, z, w, s);
         else
@@ -760,10 +772,19 @@ int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, doubl
     return its;
 }
 
+int gmres_schur(Problem &P, const double *x, const double *bS, double *x2, double rtol, int m, int maxit, int prec,
+                double *work, hipStream_t s)
+{
+    if (prec == TMI_PREC_MIXED) return gmres_schur_t<float>(P, x, bS, x2, rtol, m, maxit, prec, work, s);
+    return gmres_schur_t<double>(P, x, bS, x2, rtol, m, maxit, prec, work, s);
+}
+
 size_t gmres_work_doubles(const Problem &P, int m)
 {
-    const size_t n = (size_t)P.p.ncomp * P.N;
-    return (size_t)(m + 1) * n + 3 * n + (size_t)(P.nrb + P.nmf) * P.N;
+    const size_t n = (size_t)P.p.ncomp * P.N, t1 = (size_t)(P.nrb + P.nmf) * P.N;
+    const size_t fp64 = (size_t)(m + 1) * n + 3 * n + t1;           // FP64 basis, z, w, dinv, t1
+    const size_t fp32 = ((size_t)(m + 1) * n + 1) / 2 + 4 * n + t1;   // FP32 basis, z, w, dinv, the widened V_j, t1
+    return std::max(fp64, fp32);
 }
 
 int schur_solve(Problem &P, const double *x, const double *F, double *dx, double rtol, int restart, int maxit, int prec,

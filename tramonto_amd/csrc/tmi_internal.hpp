@@ -284,9 +284,14 @@ void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long lo
 // classical Gram-Schmidt of w against V_0 .. V_k-1 with one reorthogonalisation pass, all on the
 // device: coef_d[0, k) and coef_d[k, 2k) the two passes' projections, coef_d[2k] the squared
 // norm of the result (copied to P.h_red[0, 2k] as well), next = the result normalised (0 if
-// its norm is 0); w holds the result
-void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
+// its norm is 0); w holds the result.  TV (double or float): the type of the basis V and next.
+template <typename TV>
+void dev_cgs2(Problem &P, const TV *V, long long ld, int k, double *w, long long n, TV *next, double *coef_d,
               hipStream_t s);
+// y = (TV)(a x); b = (double) a
+template <typename TV>
+void dev_cast_scaled(double a, const double *x, TV *y, long long n, hipStream_t s);
+void dev_widen(const float *a, double *b, long long n, hipStream_t s);
 void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const double *w, long long n, double *out,
                    hipStream_t s);
 void dev_axpy(double a, const double *x, double *y, long long n, hipStream_t s);
@@ -306,7 +311,8 @@ void dev_fill(double v, double *y, long long n, hipStream_t s);
 void dev_scale_copy(double a, const double *x, double *y, long long n, hipStream_t s);
 void dev_mul(const double *a, const double *b, double *y, long long n, hipStream_t s);
 void dev_gemv_sub(const double *V, long long ld, int k, const double *h, double *w, long long n, hipStream_t s);
-void dev_gemv_add(const double *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s);
+template <typename TV>
+void dev_gemv_add(const TV *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s);
 void dev_a22_inverse_diag(Problem &P, const double *x, double *dinv, hipStream_t s);
 void dev_newton_update(Problem &P, double *x, const double *dx, const tmi_newton_opts &o, double *frac_out,
                        double *unorm_out, hipStream_t s);

@@ -1141,8 +1141,10 @@ __device__ inline double block_sum(double v)
 // partial[j*gridDim.x + blk] = sum over the block's range of V_j . w, four vectors per sweep over the
 // range (w is re-read once per four vectors, from cache): V is read once, with few registers.  Each
 // V_j . w is summed in the same order as by a sweep per vector.
+// V may be an FP32 basis (the mixed path's Krylov vectors); products and sums are FP64.
 constexpr int kDotGroup = 4;
-__global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ V, long long ld, int k,
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_multidot(const TV *__restrict__ V, long long ld, int k,
                                                      const double *__restrict__ w, long long n,
                                                      double *__restrict__ partial)
 {
@@ -1154,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) void k_multidot(const double *__restrict__ 
             const double wi = w[i];
 #pragma unroll
             for (int g = 0; g < kDotGroup; ++g)
-                if (j0 + g < k) acc[g] += V[(size_t)(j0 + g) * ld + i] * wi;
+                if (j0 + g < k) acc[g] += (double)V[(size_t)(j0 + g) * ld + i] * wi;
         }
 #pragma unroll
         for (int g = 0; g < kDotGroup; ++g) {
@@ -1268,14 +1270,30 @@ __global__ void k_mul(const double *__restrict__ a, const double *__restrict__ b
 }
 
 // w -= V h (k columns)
-// y = x / sqrt(*nrm2) (0 when *nrm2 is 0): the next Arnoldi vector without a host round trip
-__global__ void k_scale_by_norm(const double *__restrict__ nrm2, const double *__restrict__ x, double *__restrict__ y,
+// y = x / sqrt(*nrm2) (0 when *nrm2 is 0): the next Arnoldi vector without a host round trip, stored
+// in the basis type
+template <typename TV>
+__global__ void k_scale_by_norm(const double *__restrict__ nrm2, const double *__restrict__ x, TV *__restrict__ y,
                                 long long n)
 {
     long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double q = *nrm2;
-    y[i] = q > 0.0 ? x[i] * (1.0 / sqrt(q)) : 0.0;
+    y[i] = (TV)(q > 0.0 ? x[i] * (1.0 / sqrt(q)) : 0.0);
+}
+
+// y = (TV)(a x), and its widening b = (double) a
+template <typename TV>
+__global__ void k_cast_scaled(double a, const double *__restrict__ x, TV *__restrict__ y, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] = (TV)(a * x[i]);
+}
+template <typename TV>
+__global__ void k_widen(const TV *__restrict__ a, double *__restrict__ b, long long n)
+{
+    long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
 }
 
 __global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, const double *__restrict__ h,
@@ -1289,15 +1307,15 @@ __global__ void k_gemv_sub(const double *__restrict__ V, long long ld, int k, co
 }
 
 // w -= V h (k columns); NORM: partial[blk] = sum over the block's elements of w^2 (updated w)
-template <bool NORM>
-__global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const double *__restrict__ V, long long ld, int k,
+template <typename TV, bool NORM>
+__global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const TV *__restrict__ V, long long ld, int k,
                                                           const double *__restrict__ h, double *__restrict__ w,
                                                           long long n, double *__restrict__ partial)
 {
     double q = 0.0;
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
         double s = 0.0;
-        for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * h[j];
+        for (int j = 0; j < k; ++j) s += (double)V[(size_t)j * ld + i] * h[j];
         const double wi = w[i] - s;
         w[i] = wi;
         q += wi * wi;
@@ -1308,13 +1326,14 @@ __global__ __launch_bounds__(kBlock) void k_gemv_sub_norm(const double *__restri
     }
 }
 
-__global__ void k_gemv_add(const double *__restrict__ V, long long ld, int k, const double *__restrict__ y,
+template <typename TV>
+__global__ void k_gemv_add(const TV *__restrict__ V, long long ld, int k, const double *__restrict__ y,
                            double *__restrict__ x, long long n)
 {
     long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < k; ++j) s += V[(size_t)j * ld + i] * y[j];
+    for (int j = 0; j < k; ++j) s += (double)V[(size_t)j * ld + i] * y[j];
     x[i] += s;
 }
 
@@ -1846,7 +1865,7 @@ void residual_sumsq_device(Problem &P, const double *x, double *F, double *parti
     } else {   // the blocks and order of dev_dot
         const long long n = (long long)P.nunk * P.N;
         const unsigned nb = (unsigned)std::min<long long>(1024, (n + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, F, 0LL, 1, F, n, partials);
+        hipLaunchKernelGGL(k_multidot<double>, dim3(nb), dim3(kBlock), 0, s, F, 0LL, 1, F, n, partials);
         hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, partials, (int)nb, 1, sumsq_d, nullptr);
     }
     TMI_HIP_CHECK(hipGetLastError());
@@ -1933,7 +1952,7 @@ void dev_multi_dot(Problem &P, const double *V, long long ld, int k, const doubl
 {
     unsigned nb = red_blocks(n);
     double *partial = P.d_red;            // [k][nb]
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, partial);
+    hipLaunchKernelGGL(k_multidot<double>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, partial);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, partial, (int)nb, k, nullptr, P.h_red_dev);
     TMI_HIP_CHECK(hipStreamSynchronize(s));
     for (int j = 0; j < k; ++j) out[j] = P.h_red[j];
@@ -1943,7 +1962,7 @@ void dev_multi_dot_async(Problem &P, const double *V, long long ld, int k, const
                          hipStream_t s, double *mirror)
 {
     unsigned nb = red_blocks(n);
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL(k_multidot<double>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, out_d, mirror);
 }
 
@@ -1957,26 +1976,44 @@ static unsigned gs_blocks(long long n, int k)
 
 // classical Gram-Schmidt with one reorthogonalisation pass: h1 = V^T w, w -= V h1, h2 = V^T w,
 // w -= V h2 with |w|^2, next = w / |w|; the coefficients and |w|^2 land in coef_d and in the
-// pinned host mirror (read after the caller synchronises)
-void dev_cgs2(Problem &P, const double *V, long long ld, int k, double *w, long long n, double *next, double *coef_d,
+// pinned host mirror (read after the caller synchronises).  TV: the basis type (FP32 on the mixed
+// path); w and every sum stay FP64.
+template <typename TV>
+void dev_cgs2(Problem &P, const TV *V, long long ld, int k, double *w, long long n, TV *next, double *coef_d,
               hipStream_t s)
 {
     const unsigned nb = gs_blocks(n, k);
     double *h1 = coef_d, *h2 = coef_d + k, *nrm = coef_d + 2 * k;
     double *m1 = P.h_red_dev, *m2 = m1 + k, *mn = m1 + 2 * k;   // host copies (P.h_red)
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL(k_multidot<TV>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h1, m1);
-    hipLaunchKernelGGL(k_gemv_sub_norm<false>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, nullptr);
-    hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
+    hipLaunchKernelGGL((k_gemv_sub_norm<TV, false>), dim3(nb), dim3(kBlock), 0, s, V, ld, k, h1, w, n, nullptr);
+    hipLaunchKernelGGL(k_multidot<TV>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(k), dim3(kBlock), 0, s, P.d_red, (int)nb, k, h2, m2);
-    hipLaunchKernelGGL(k_gemv_sub_norm<true>, dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
+    hipLaunchKernelGGL((k_gemv_sub_norm<TV, true>), dim3(nb), dim3(kBlock), 0, s, V, ld, k, h2, w, n, P.d_red);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, s, P.d_red, (int)nb, 1, nrm, mn);
-    dev_scale_by_norm(nrm, w, next, n, s);
+    hipLaunchKernelGGL(k_scale_by_norm<TV>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrm, w, next, n);
 }
+template void dev_cgs2<double>(Problem &, const double *, long long, int, double *, long long, double *, double *,
+                               hipStream_t);
+template void dev_cgs2<float>(Problem &, const float *, long long, int, double *, long long, float *, double *,
+                              hipStream_t);
 
 void dev_scale_by_norm(const double *nrm2_d, const double *x, double *y, long long n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scale_by_norm, dim3(grid_for(n)), dim3(kBlock), 0, s, nrm2_d, x, y, n);
+    hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, nrm2_d, x, y, n);
+}
+
+template <typename TV>
+void dev_cast_scaled(double a, const double *x, TV *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cast_scaled<TV>, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, y, n);
+}
+template void dev_cast_scaled<double>(double, const double *, double *, long long, hipStream_t);
+template void dev_cast_scaled<float>(double, const double *, float *, long long, hipStream_t);
+void dev_widen(const float *a, double *b, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_widen<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, n);
 }
 
 double dev_dot(Problem &P, const double *a, const double *b, long long n, hipStream_t s)
@@ -2027,10 +2064,13 @@ void dev_gemv_sub(const double *V, long long ld, int k, const double *h, double 
 {
     hipLaunchKernelGGL(k_gemv_sub, dim3(grid_for(n)), dim3(kBlock), 0, s, V, ld, k, h, w, n);
 }
-void dev_gemv_add(const double *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s)
+template <typename TV>
+void dev_gemv_add(const TV *V, long long ld, int k, const double *y, double *x, long long n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gemv_add, dim3(grid_for(n)), dim3(kBlock), 0, s, V, ld, k, y, x, n);
+    hipLaunchKernelGGL(k_gemv_add<TV>, dim3(grid_for(n)), dim3(kBlock), 0, s, V, ld, k, y, x, n);
 }
+template void dev_gemv_add<double>(const double *, long long, int, const double *, double *, long long, hipStream_t);
+template void dev_gemv_add<float>(const float *, long long, int, const double *, double *, long long, hipStream_t);
 void dev_a22_inverse_diag(Problem &P, const double *x, double *dinv, hipStream_t s)
 {
     long long n = (long long)P.p.ncomp * P.N;
