@@ -660,7 +660,8 @@ double *scratch(Problem &P, size_t doubles)
 // S = A22 - A21 A11^{-1} A12 (dft_HardSphereLinProbMgr.cpp:345-428);
 // classical Gram-Schmidt with one reorthogonalisation pass.  TV is the Krylov basis type: FP32 on
 // the mixed path (half the Gram-Schmidt traffic; each basis vector is widened to FP64 before its
-// products), FP64 otherwise; the residual, w, the Arnoldi coefficients and x2 are FP64 throughout.
+// products, and the fused bulk preconditioner transforms in FP32), FP64 otherwise; the residual, w,
+// the Arnoldi coefficients and x2 are FP64 throughout.
 template <typename TV>
 int gmres_schur_t(Problem &P, const double *x, const double *bS, double *x2, double rtol, int m, int maxit, int prec,
                   double *work, hipStream_t s)
@@ -691,7 +692,7 @@ int gmres_schur_t(Problem &P, const double *x, const double *bS, double *x2, dou
             vj = V + (size_t)j * n;
         }
         if (bulk_pc)
-            precond_apply(P, x_rho, vj, z, s);
+            precond_apply(P, x_rho, vj, z, s, std::is_same<TV, float>::value);
         else
             dev_mul(dinv, vj, z, n, s);
         if (prec == TMI_PREC_MIXED)
@@ -761,8 +762,8 @@ int gmres_schur_t(Problem &P, const double *x, const double *bS, double *x2, dou
         TMI_HIP_CHECK(hipMemcpyAsync(P.d_red + 1024 * 1024 / 2, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
         dev_fill(0.0, z, (long long)n, s);
         dev_gemv_add<TV>(V, n, k, P.d_red + 1024 * 1024 / 2, z, n, s);
-        if (bulk_pc)
-            precond_apply(P, x_rho, z, w, s);
+        if (bulk_pc)   // the same preconditioner as in the products
+            precond_apply(P, x_rho, z, w, s, std::is_same<TV, float>::value);
         else
             dev_mul(dinv, z, w, n, s);
         dev_axpy(1.0, w, x2, n, s);

@@ -132,6 +132,7 @@ struct FftPlan {
     // Q real and even in k_y, k_z (the bulk operator is real and reflection-symmetric): its real part
     // in the spectrum-table layout, applied by the fused x pass between fused-end passes
     double *pqr = nullptr;      // [C*C][tn]
+    float *pqr_f = nullptr;     // the same in FP32, for the mixed path's GMRES
     TermList tpc;               // output i += Q_ij X_j
     int pc_lpb = 0;             // lines per workgroup of its LDS form
     bool pc_fused = false;
@@ -1973,6 +1974,7 @@ void free_plan(FftPlan *F)
     (void)hipFree(F->tf);
     (void)hipFree(F->pq);
     (void)hipFree(F->pqr);
+    (void)hipFree(F->pqr_f);
     (void)hipFree(F->ss);
     for (int d = 0; d < 3; ++d) {
         (void)hipFree(F->twd[d]);
@@ -2716,8 +2718,10 @@ void precond_setup(Problem &P)
     F.pc_ok = F.pc_on = F.pc_fused = false;
     (void)hipFree(F.pq);
     (void)hipFree(F.pqr);
+    (void)hipFree(F.pqr_f);
     F.pq = nullptr;
     F.pqr = nullptr;
+    F.pqr_f = nullptr;
     if (P.npores > 0) return;
     if (!F.wraps) {
         // A walled or reflecting grid: its FFT grid pads every such axis (M = a 2-3-5 length of
@@ -2900,6 +2904,10 @@ void precond_setup(Problem &P)
     if (bad <= 1e-9 * qmax && F.pc_lpb > 0 && !(pf && std::string(pf) == "0")) {
         TMI_HIP_CHECK(hipMalloc((void **)&F.pqr, qr.size() * sizeof(double)));
         TMI_HIP_CHECK(hipMemcpy(F.pqr, qr.data(), qr.size() * sizeof(double), hipMemcpyHostToDevice));
+        TMI_HIP_CHECK(hipMalloc((void **)&F.pqr_f, qr.size() * sizeof(float)));
+        hipLaunchKernelGGL(k_to_float, dim3(grid_for((long long)qr.size())), dim3(kBlock), 0, 0, F.pqr, F.pqr_f,
+                           (long long)qr.size());
+        TMI_HIP_CHECK(hipStreamSynchronize(0));
         TermList &tl = F.tpc;
         tl = TermList();
         tl.nout = C;
@@ -2924,35 +2932,53 @@ void precond_enable(Problem &P, bool on)
     }
 }
 
-// z = M^-1 v (bulk-circulant preconditioner), x_rho the densities of the Newton iterate
-void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s)
+// The fused apply in precision T: z, y passes (the first gathers rho v); x pass fused with Q; y, z
+// passes (the last writes z, FP64)
+template <typename T>
+static void precond_apply_fused(Problem &P, const double *x_rho, const double *v, double *z, const T *qr,
+                                hipStream_t s)
+{
+    const FftPlan &F = *P.fft;
+    const int C = P.p.ncomp;
+    Cx<T> *Z = reinterpret_cast<Cx<T> *>(F.zbuf);
+    Cx<T> *O = reinterpret_cast<Cx<T> *>(F.obuf);
+    const int np = (C + 1) / 2;
+    PassIO io;
+    io.C = C;
+    io.nx0 = P.p.nx[0];
+    io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
+    io.N = P.N;
+    io.rho = x_rho;
+    io.mul = v;
+    io.zero = P.d_elzero;
+    io.pv = v;
+    io.elzero = P.d_elzero;
+    io.out_el = z;
+    int first = -1, last = -1;
+    for (int a : {2, 1})
+        if (F.M[a] > 1 && first < 0) first = a;
+    for (int a : {1, 2})
+        if (F.M[a] > 1) last = a;
+    for (int a : {2, 1}) fft_pass<T>(F, Z, np, -1, a, s, a == first ? kPassGather : kPassPlain, io);
+    fft_xprod<T>(F, Z, O, F.tpc, F.pc_lpb, qr, nullptr, s);
+    for (int a : {1, 2}) fft_pass<T>(F, O, np, +1, a, s, a == last ? kPassPc : kPassPlain, io);
+    TMI_HIP_CHECK(hipGetLastError());
+}
+
+// z = M^-1 v (bulk-circulant preconditioner), x_rho the densities of the Newton iterate; fp32: the
+// fused apply's transforms in FP32 (the mixed path)
+void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s, bool fp32)
 {
     const FftPlan &F = *P.fft;
     const int C = P.p.ncomp;
     Cx<double> *Z = reinterpret_cast<Cx<double> *>(F.zbuf);
     Cx<double> *O = reinterpret_cast<Cx<double> *>(F.obuf);
-    if (F.pc_fused) {   // z, y passes (the first gathers rho v); x pass fused with Q; y, z passes (the last writes z)
-        const int np = (C + 1) / 2;
-        PassIO io;
-        io.C = C;
-        io.nx0 = P.p.nx[0];
-        io.nx1 = P.p.ndim > 1 ? P.p.nx[1] : 1;
-        io.N = P.N;
-        io.rho = x_rho;
-        io.mul = v;
-        io.zero = P.d_elzero;
-        io.pv = v;
-        io.elzero = P.d_elzero;
-        io.out_el = z;
-        int first = -1, last = -1;
-        for (int a : {2, 1})
-            if (F.M[a] > 1 && first < 0) first = a;
-        for (int a : {1, 2})
-            if (F.M[a] > 1) last = a;
-        for (int a : {2, 1}) fft_pass<double>(F, Z, np, -1, a, s, a == first ? kPassGather : kPassPlain, io);
-        fft_xprod<double>(F, Z, O, F.tpc, F.pc_lpb, F.pqr, nullptr, s);
-        for (int a : {1, 2}) fft_pass<double>(F, O, np, +1, a, s, a == last ? kPassPc : kPassPlain, io);
-        TMI_HIP_CHECK(hipGetLastError());
+    if (F.pc_fused && fp32 && F.pqr_f) {
+        precond_apply_fused<float>(P, x_rho, v, z, F.pqr_f, s);
+        return;
+    }
+    if (F.pc_fused) {
+        precond_apply_fused<double>(P, x_rho, v, z, F.pqr, s);
         return;
     }
     const Geo g = make_geo(P, P.pad_u);

@@ -266,7 +266,7 @@ void destroy_bulk_problem(Problem *P);
 bool precond_available(const Problem &P);
 bool precond_active(const Problem &P);
 void precond_enable(Problem &P, bool on);
-void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s);
+void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s, bool fp32 = false);
 // times the fill with the FMT sums in both forms and keeps the faster (tmi_kernels.hip)
 void choose_fmt_form(Problem &P);
 
