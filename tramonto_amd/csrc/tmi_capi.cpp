@@ -499,6 +499,31 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                 }
             P->use_columns = true;
         }
+        // dense 2D convolution weights of the attraction sums (TMI_STENCIL_KERNEL=gather keeps the
+        // per-point gather kernel)
+        if (nd == 2 && p.type_attr >= 0 && !(kern && std::string(kern) == "gather")) {
+            const int hx = P->pad_u.h[0], hy = P->pad_u.h[1], wx = 2 * hx + 1, wy = 2 * hy + 1;
+            std::vector<double> wd((size_t)C * C * wx * wy, 0.0);
+            bool fits = true;
+            for (int q = 0; q < C * C; ++q) {
+                const HostStencil &hs = P->hs[TMI_UATT][q];
+                for (int k = 0; k < hs.L; ++k) {
+                    const int dx = hs.off[3 * (size_t)k], dy = hs.off[3 * (size_t)k + 1];
+                    if (dx < -hx || dx > hx || dy < -hy || dy > hy || hs.off[3 * (size_t)k + 2] != 0) {
+                        fits = false;
+                        break;
+                    }
+                    wd[(size_t)q * wx * wy + (size_t)(dy + hy) * wx + (dx + hx)] += hs.w[k];
+                }
+            }
+            if (fits) {
+                std::vector<float> wf(wd.begin(), wd.end());
+                P->d_mfw2 = dev_alloc<double>(wd.size());
+                P->f_mfw2 = dev_alloc<float>(wf.size());
+                upload(P->d_mfw2, wd.data(), wd.size());
+                upload(P->f_mfw2, wf.data(), wf.size());
+            }
+        }
         // dense row kernels on 1D grids (TMI_STENCIL_KERNEL=gather keeps the gather kernels)
         if (nd == 1 && !(kern && std::string(kern) == "gather")) {
             for (int c = 0; c < C; ++c)
@@ -603,6 +628,8 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->wsf.corr_u);
     (void)hipFree(P->wsf.corr_f);
     (void)hipFree(P->d_red);
+    (void)hipFree(P->d_mfw2);
+    (void)hipFree(P->f_mfw2);
     (void)hipFree(P->d_scratch);
     (void)hipFree(P->d_newton);
     (void)hipFree(P->d_xi);

@@ -186,6 +186,8 @@ struct Problem {
     // Jacobian-vector products with the reference's delta-stencil Hessian rows (TMI_JAC_REFERENCE:
     // its assembled entries, including the inexact ones) instead of the exact Hessian
     bool jac_ref = false;
+    double *d_mfw2 = nullptr;   // 2D direct form: dense (2 hy + 1) x (2 hx + 1) attraction weights per component pair
+    float *f_mfw2 = nullptr;    // the same in FP32
     double *d_w0 = nullptr;
     int nunk_ext = 0;
     double *d_xi = nullptr, *d_vi = nullptr, *d_fi = nullptr, *d_gi = nullptr;   // [nunk][N] each

@@ -360,6 +360,87 @@ __global__ void k_mf_rows(Geo g, UArgs ua, const T *__restrict__ rho_pad, const 
     out_mf[(size_t)c * g.N + i] = r;
 }
 
+// 2D mean-field rows as a dense convolution of the padded densities.  A workgroup of 4 waves takes 64
+// consecutive interior x of TY consecutive padded rows; wave w sums the stencil columns dx = -hx + w,
+// -hx + w + 4, ... and the four partial sums meet in LDS.  Each density a wave loads feeds up to TY
+// outputs (register blocking along y), and the dense (2 hy + 1) x (2 hx + 1) weights of a component
+// pair (zeros off the stencil) are wave-uniform scalars.  Padded rows holding no node (the gap rows of
+// a batch) produce no output.  Sums run in another order than k_mf_rows's.
+constexpr int kMf2dWaves = 4;
+template <typename T, int TY>
+__global__ __launch_bounds__(64 * kMf2dWaves) void k_mf_rows_2d(Geo g, const T *__restrict__ W,
+                                                                const T *__restrict__ rho_pad,
+                                                                const double *__restrict__ in_mf,
+                                                                const double *__restrict__ sub,
+                                                                double *__restrict__ out_mf)
+{
+    __shared__ T part[kMf2dWaves - 1][TY][64];
+    const int c = blockIdx.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
+    const int wx = 2 * hx + 1, wy = 2 * hy + 1;
+    const int xi = blockIdx.x * 64 + lane;   // interior column
+    const bool col_ok = xi < g.nx[0];
+    const int px = hx + (col_ok ? xi : 0);
+    const int py0 = hy + blockIdx.y * TY;    // first padded row of the tile
+    // input row py0 + yy feeds output j through weight row dy = yy - j, |dy| <= hy; rows at or past P1
+    // feed only outputs past the last padded node row
+    const int yend = min(hy + TY - 1, P1 - 1 - py0);
+    T acc[TY];
+#pragma unroll
+    for (int j = 0; j < TY; ++j) acc[j] = (T)0;
+    for (int cj = 0; cj < g.C; ++cj) {
+        const T *Wq = W + (size_t)(c + g.C * cj) * wx * wy;
+        const T *src = rho_pad + (size_t)cj * g.np;
+        for (int dx = -hx + wave; dx <= hx; dx += kMf2dWaves) {
+            const T *col = src + px + dx + (long long)py0 * P0;
+            const T *wcol = Wq + (dx + hx) + (size_t)hy * wx;   // weight row dy at wcol[dy * wx]
+            int yy = -hy;
+            for (; yy < -hy + TY - 1 && yy <= yend; ++yy) {   // ramp: only outputs j <= yy + hy
+                const T v = col[(long long)yy * P0];
+#pragma unroll
+                for (int j = 0; j < TY; ++j)
+                    if (yy - j >= -hy && yy - j <= hy) acc[j] += wcol[(yy - j) * wx] * v;
+            }
+            const int ysteady = min(hy, yend);   // from yy >= -hy + TY - 1 on, every output's weight row exists
+#pragma unroll 4
+            for (; yy <= ysteady; ++yy) {   // every output
+                const T v = col[(long long)yy * P0];
+#pragma unroll
+                for (int j = 0; j < TY; ++j) acc[j] += wcol[(yy - j) * wx] * v;
+            }
+            for (; yy <= yend; ++yy) {   // tail: only outputs j >= yy - hy
+                const T v = col[(long long)yy * P0];
+#pragma unroll
+                for (int j = 0; j < TY; ++j)
+                    if (yy - j >= -hy && yy - j <= hy) acc[j] += wcol[(yy - j) * wx] * v;
+            }
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int j = 0; j < TY; ++j) part[wave - 1][j][lane] = acc[j];
+    }
+    __syncthreads();
+    if (wave > 0 || !col_ok) return;
+#pragma unroll
+    for (int j = 0; j < TY; ++j) {
+        const int py = py0 + j;
+        if (py >= P1 - hy) break;
+        T a = acc[j];
+#pragma unroll
+        for (int w = 0; w < kMf2dWaves - 1; ++w) a += part[w][j][lane];
+        const long long pos = (long long)py * P0 + px;
+        const int node = g.pad_node ? g.pad_node[pos] : xi + (py - hy) * g.nx[0];
+        if (node < 0) continue;
+        double r = (double)a;
+        if (in_mf) r -= in_mf[(size_t)c * g.N + node];
+        if (sub) r -= sub[(size_t)c * g.N + node];
+        out_mf[(size_t)c * g.N + node] = r;
+    }
+}
+constexpr int kMf2dRows = 4;
+
 // ---------------------------------------------------------------- 3D column kernels
 // A column pass streams one input field along z for one stencil column and
 // adds, for each of the lane's B consecutive z outputs, the column's weights
@@ -1458,6 +1539,14 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
         for (int q = 0; q < P.p.ncomp * P.p.ncomp; ++q) ca.u[q] = col_ptrs(dev_cols<T>(P, TMI_UATT, q), kGeomU);
         dim3 grid(col_tiles(P));
         hipLaunchKernelGGL((k_mf_rows_cols<T, kColB, kColS>), grid, dim3(64, kColS), 0, s, gu, ca,
+                           workspace<T>(P).rho_pad, in_mf, sub, out_mf);
+        return;
+    }
+    if (P.p.ndim == 2 && P.d_mfw2) {   // dense 2D convolution
+        const T *W = std::is_same<T, float>::value ? (const T *)P.f_mfw2 : (const T *)P.d_mfw2;
+        const dim3 grid((unsigned)((P.p.nx[0] + 63) / 64),
+                        (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + kMf2dRows - 1) / kMf2dRows), (unsigned)P.p.ncomp);
+        hipLaunchKernelGGL((k_mf_rows_2d<T, kMf2dRows>), grid, dim3(64 * kMf2dWaves), 0, s, gu, W,
                            workspace<T>(P).rho_pad, in_mf, sub, out_mf);
         return;
     }
