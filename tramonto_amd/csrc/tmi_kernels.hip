@@ -403,11 +403,19 @@ __global__ __launch_bounds__(64 * kMf2dWaves) void k_mf_rows_2d(Geo g, const T *
                     if (yy - j >= -hy && yy - j <= hy) acc[j] += wcol[(yy - j) * wx] * v;
             }
             const int ysteady = min(hy, yend);   // from yy >= -hy + TY - 1 on, every output's weight row exists
-#pragma unroll 4
-            for (; yy <= ysteady; ++yy) {   // every output
-                const T v = col[(long long)yy * P0];
+            if (yy <= ysteady) {   // every output; wr[j] = weight row yy - j, rotated: one new weight per row
+                T wr[TY];
 #pragma unroll
-                for (int j = 0; j < TY; ++j) acc[j] += wcol[(yy - j) * wx] * v;
+                for (int j = 1; j < TY; ++j) wr[j] = wcol[(yy - j) * wx];
+#pragma unroll TY
+                for (; yy <= ysteady; ++yy) {
+                    wr[0] = wcol[yy * wx];
+                    const T v = col[(long long)yy * P0];
+#pragma unroll
+                    for (int j = 0; j < TY; ++j) acc[j] += wr[j] * v;
+#pragma unroll
+                    for (int j = TY - 1; j > 0; --j) wr[j] = wr[j - 1];
+                }
             }
             for (; yy <= yend; ++yy) {   // tail: only outputs j >= yy - hy
                 const T v = col[(long long)yy * P0];
