@@ -360,13 +360,13 @@ __global__ void k_mf_rows(Geo g, UArgs ua, const T *__restrict__ rho_pad, const 
     out_mf[(size_t)c * g.N + i] = r;
 }
 
-// 2D mean-field rows as a dense convolution of the padded densities.  A workgroup of 4 waves takes 64
-// consecutive interior x of TY consecutive padded rows; wave w sums the stencil columns dx = -hx + w,
-// -hx + w + 4, ... and the four partial sums meet in LDS.  Each density a wave loads feeds up to TY
+// 2D mean-field rows as a dense convolution of the padded densities.  A workgroup of kMf2dWaves waves
+// takes 64 consecutive interior x of TY consecutive padded rows; wave w sums the stencil columns
+// dx = -hx + w, -hx + w + kMf2dWaves, ... and the partial sums meet in LDS.  Each density a wave loads feeds up to TY
 // outputs (register blocking along y), and the dense (2 hy + 1) x (2 hx + 1) weights of a component
 // pair (zeros off the stencil) are wave-uniform scalars.  Padded rows holding no node (the gap rows of
 // a batch) produce no output.  Sums run in another order than k_mf_rows's.
-constexpr int kMf2dWaves = 4;
+constexpr int kMf2dWaves = 8;
 template <typename T, int TY>
 __global__ __launch_bounds__(64 * kMf2dWaves) void k_mf_rows_2d(Geo g, const T *__restrict__ W,
                                                                 const T *__restrict__ rho_pad,
