@@ -59,25 +59,44 @@ __device__ inline double seg_block_sum(double v, double *sh)
     return t;   // valid in thread 0
 }
 
-// out[p*ldo + j] = sum over pore p of V_j . w   (rows x nodes of the pore)
+// out[p*ldo + j] = sum over pore p of V_j . w   (rows x nodes of the pore): one workgroup per pore,
+// kSegGroup vectors per sweep over the pore's elements (w re-read per group, from cache) and one
+// reduction round per group
+constexpr int kSegGroup = 8;
 __global__ __launch_bounds__(kSegBlock) void k_seg_dots(const double *__restrict__ V, long long ldv, int k,
                                                         const double *__restrict__ w, int rows, int N,
                                                         const int *__restrict__ off, double *__restrict__ out, int ldo)
 {
-    __shared__ double sh[kSegBlock / 64];
+    __shared__ double sh[kSegBlock / 64][kSegGroup];
     const int p = blockIdx.x;
     const int i0 = off[p], n = off[p + 1] - i0;
-    const int tot = rows * n;
-    for (int j = 0; j < k; ++j) {
-        const double *Vj = V + (size_t)j * ldv;
-        double s = 0.0;
-        for (int e = threadIdx.x; e < tot; e += kSegBlock) {
-            const int r = e / n, i = e - r * n;
-            const size_t q = (size_t)r * N + i0 + i;
-            s += Vj[q] * w[q];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j0 = 0; j0 < k; j0 += kSegGroup) {
+        double acc[kSegGroup];
+#pragma unroll
+        for (int g = 0; g < kSegGroup; ++g) acc[g] = 0.0;
+        for (int r = 0; r < rows; ++r) {
+            const size_t base = (size_t)r * N + i0;
+            for (int i = threadIdx.x; i < n; i += kSegBlock) {
+                const double wv = w[base + i];
+#pragma unroll
+                for (int g = 0; g < kSegGroup; ++g)
+                    if (j0 + g < k) acc[g] += V[(size_t)(j0 + g) * ldv + base + i] * wv;
+            }
         }
-        s = seg_block_sum(s, sh);
-        if (threadIdx.x == 0) out[(size_t)p * ldo + j] = s;
+#pragma unroll
+        for (int g = 0; g < kSegGroup; ++g) {
+            double a = acc[g];
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+            if (lane == 0) sh[wid][g] = a;
+        }
+        __syncthreads();
+        if (threadIdx.x < kSegGroup && j0 + (int)threadIdx.x < k) {
+            double t = 0.0;
+            for (int q = 0; q < kSegBlock / 64; ++q) t += sh[q][threadIdx.x];
+            out[(size_t)p * ldo + j0 + threadIdx.x] = t;
+        }
+        __syncthreads();   // sh is reused by the next group
     }
 }
 
