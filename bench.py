@@ -42,6 +42,9 @@ ROWS_TRAFFIC_MIB = {"periodic64": 35.7 + 24.6, "solute128": 534.6 + 234.3}
 # dominant kernel
 XPROD_TRAFFIC_MIB = {"periodic64": (7.5 + 12.0 + 14.9 + 4.0) / 2, "solute128": None}
 AXIS_TRAFFIC_MIB = {"periodic64": None, "solute128": None}
+# HBM bytes of one whole fill in FFT form (every launch of the fill, FETCH + WRITE in MiB, from the
+# same counter runs): the fill kernels' measured traffic, against the fill's live time
+FILL_TRAFFIC_MIB = {"periodic64": 245.6, "solute128": 4595.7}
 
 
 def stencil_flops_per_node(dp, spec):
@@ -288,6 +291,13 @@ def bench_workload(spec, label, args, newton_tol=1e-8, newton=True):
     out["flops_per_node_direct"] = flops_node
     out["bytes_per_node_residual"] = bytes_node
     out["fill_hbm_frac"] = (N * bytes_node / (ms_fill * 1e-3)) / (PEAK_HBM_GBS * 1e9)
+    ft = FILL_TRAFFIC_MIB.get(label) if dp.info.fmt_fft and dp.info.mf_fft else None
+    if ft:
+        gbs = ft * (1 << 20) / (ms_fill * 1e-3) / 1e9
+        out["fill_traffic"] = {"mib_per_fill": ft, "achieved_gb_s": gbs, "frac": gbs / PEAK_HBM_GBS,
+                               "note": "HBM bytes of all the fill's launches (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                       "launch, scaled by a 512 MiB copy) over the live fill time; fill_hbm_frac "
+                                       "counts only the compulsory bytes (x, F, V_ext, zero flags)"}
     out["mean_field_form"] = "fft" if dp.info.mf_fft else "direct"
     out["fmt_form"] = ["direct", "fft", "fft, x pass fused with the products"][dp.info.fmt_fft]
     out["fft_xreg"] = dp.info.fft_xreg
