@@ -1883,7 +1883,11 @@ int tmo_newton(const tmo_ctx *c, double *x, const tmo_newton_opts *o, tmo_newton
             for (int i = 0; i < N; i++) {
                 double *xp = &X(iu, i);
                 double d = dx[(size_t)iu * N + i];
-                if (*xp + frac_min * d < 1.e-99 && iu < c->iu_rb)
+                /* x + frac_min d < 1e-99 (dft_newton.c:344) read in exact arithmetic: a density
+                   whose own fraction x / (-d) is at or below frac_min reaches x + frac_min d <= 0,
+                   where the rounded sum is a residue of either sign */
+                int limited = iu < c->iu_rb && *xp + d < 0.0 && fmin(1.0, *xp / (-d)) <= frac_min;
+                if ((*xp + frac_min * d < 1.e-99 || limited) && iu < c->iu_rb)
                     *xp = 0.1 * *xp;
                 else if (iu == c->iu_rb && *xp + frac_min * d > 1.0)
                     *xp += 0.5 * (1.0 - *xp);

@@ -318,7 +318,10 @@ __global__ void k_seg_apply_update(double *__restrict__ x, const double *__restr
     const double f = frac[pore_of[i]];
     if (f == 0.0) return;
     const double xv = x[e], d = dx[e];
-    if (u < C && xv + f * d < 1.e-99)
+    // the limiting density goes to 0.1 x as in exact arithmetic, not to the rounding residue of
+    // x + f dx (k_apply_update)
+    const bool limited = u < C && xv + d < 0.0 && fmin(1.0, xv / (-d)) <= f;
+    if (u < C && (xv + f * d < 1.e-99 || limited))
         x[e] = 0.1 * xv;
     else if (u == iu_n3 && xv + f * d > 1.0)
         x[e] = xv + 0.5 * (1.0 - xv);

@@ -1477,7 +1477,12 @@ __global__ void k_apply_update(double *__restrict__ x, const double *__restrict_
     const double xv = x[i], d = dx[i];
     const bool dens = i < n_rho;
     const bool is_n3 = (i / N) == iu_n3;
-    if (xv + frac * d < 1.e-99 && dens)
+    // a density whose own fraction x / (-dx) is at or below frac (the node that set frac, or one
+    // past it) reaches x + frac dx <= 0 in exact arithmetic; in floating point the limiting node's
+    // x + frac dx is a residue of either sign (~1e-16 x), and a positive one would leave a density
+    // of ~1e-20 whose 1/rho term swamps the next Jacobian, so it takes the 0.1 x branch by this test
+    const bool limited = dens && xv + d < 0.0 && fmin(1.0, xv / (-d)) <= frac;
+    if ((xv + frac * d < 1.e-99 || limited) && dens)
         x[i] = 0.1 * xv;
     else if (is_n3 && xv + frac * d > 1.0)
         x[i] = xv + 0.5 * (1.0 - xv);
