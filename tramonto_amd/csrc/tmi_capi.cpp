@@ -1166,7 +1166,7 @@ int tmi_problem_set_rho_b(void *handle, const double *rho_b)
         for (int c = 0; c < P->p.ncomp; ++c) P->p.rho_b[c] = rho_b[c];
         compute_bulk_constants(P->p, P->bulk);
         if (P->p.type_attr >= 0) compute_attraction_constants(P->p, P->bulk, P->hs);
-        precond_setup(*P);   // the bulk Schur symbol depends on rho_b
+        precond_invalidate(*P);   // the bulk Schur symbol depends on rho_b: rebuilt when GMRES next needs it
         return 0;
     });
 }

@@ -139,6 +139,7 @@ struct FftPlan {
     bool pc_ok = false;         // Q built
     int pc_want = -1;           // wanted (TMI_PRECOND=diag: no; -1 until the first set-up reads it)
     bool pc_on = false;         // GMRES uses it: built and wanted
+    bool pc_stale = false;      // rho_b changed since Q was built: rebuilt when next asked for
     // kernel-family switches, read from the environment when the plan is built: regpass 0 keeps the
     // LDS Stockham pass on strided axes (TMI_FFT_REGPASS=0), 1 uses the one-wave register pass where it
     // has enough waves, 2 wherever it has radices (TMI_FFT_REGPASS=2); xreg false keeps the LDS form of
@@ -2920,8 +2921,29 @@ void precond_setup(Problem &P)
     }
 }
 
-bool precond_available(const Problem &P) { return P.fft && P.fft->pc_ok; }
-bool precond_active(const Problem &P) { return P.fft && P.fft->pc_on; }
+// a new rho_b marks Q stale; the queries below rebuild it first, so parameter sweeps that only fill
+// (finite differences in the bulk density) never build the bulk problem of a walled grid
+void precond_invalidate(Problem &P)
+{
+    if (P.fft) P.fft->pc_stale = true;
+}
+static void precond_refresh(Problem &P)
+{
+    if (!P.fft || !P.fft->pc_stale) return;
+    TMI_HIP_CHECK(hipDeviceSynchronize());   // the set-up runs on the null stream and reuses the transform buffers
+    precond_setup(P);
+    P.fft->pc_stale = false;
+}
+bool precond_available(Problem &P)
+{
+    precond_refresh(P);
+    return P.fft && P.fft->pc_ok;
+}
+bool precond_active(Problem &P)
+{
+    precond_refresh(P);
+    return P.fft && P.fft->pc_on;
+}
 void precond_enable(Problem &P, bool on)
 {
     if (on && !precond_available(P))

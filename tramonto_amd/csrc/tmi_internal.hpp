@@ -265,8 +265,9 @@ void fft_time_kernels(Problem &P, int which, int reps, double *ms, double *bytes
 void precond_setup(Problem &P);
 Problem *create_bulk_problem(const tmi_params &p);   // tmi_capi.cpp
 void destroy_bulk_problem(Problem *P);
-bool precond_available(const Problem &P);
-bool precond_active(const Problem &P);
+bool precond_available(Problem &P);   // both rebuild a stale Q first (precond_invalidate)
+bool precond_active(Problem &P);
+void precond_invalidate(Problem &P);
 void precond_enable(Problem &P, bool on);
 void precond_apply(Problem &P, const double *x_rho, const double *v, double *z, hipStream_t s, bool fp32 = false);
 // times the fill with the FMT sums in both forms and keeps the faster (tmi_kernels.hip)
