@@ -151,7 +151,9 @@ int tmi_host_bulk(const tmi_params *p, double *out60);
  * its globals between continuation steps, src/dft_continuation.c; these serve
  * continuation in a bulk or wall parameter and the adjoint harness).
  * set_rho_b: new bulk densities rho_b[ncomp]; the chemical potentials and the
- * bulk boundary values follow, stencils and tables are kept.  set_vext: new
+ * bulk boundary values follow, stencils and tables are kept; the bulk-symbol
+ * preconditioner is rebuilt for the new rho_b by the next Schur solve (or
+ * tmi_problem_info / set_preconditioner call), not here.  set_vext: new
  * V_ext [nnodes][ncomp] (host, node-major as at creation); Zero_density_TF is
  * kept and rows with V_ext = VEXT_MAX become fixed rows as at creation.  Both
  * wait for the device to be idle first. */
