@@ -1,7 +1,9 @@
-"""The dense 2D convolution of the direct attraction rows (k_mf_rows_2d) against the per-point gather
-kernel it replaces (TMI_STENCIL_KERNEL=gather): residual and Jacobian products in FP64 (1e-13 of the
-largest row) and in the FP32 sums of the mixed path (1e-5), on walled/reflecting and periodic 2D
-grids, two components, and a batch of 2D pieces (gap rows between the pieces)."""
+"""The dense 2D convolution of the direct attraction rows against the per-point gather kernel it
+replaces (TMI_STENCIL_KERNEL=gather): residual and Jacobian products in FP64 (1e-13 of the largest
+row; the FP64 sums run on the matrix cores, k_mf_rows_2d_mfma, or with TMI_MF2D_MFMA=0 in the FMA
+kernel k_mf_rows_2d) and in the FP32 sums of the mixed path (k_mf_rows_2d, 1e-5), on
+walled/reflecting and periodic 2D grids, two components, and a batch of 2D pieces (gap rows between
+the pieces)."""
 import dataclasses
 
 import numpy as np
@@ -28,11 +30,12 @@ def binary_2d():
         rho0=np.stack([0.55 * s.rho0[0], 0.35 * s.rho0[0]]).reshape(2, N))
 
 
-def make_pair(spec_or_batch, monkeypatch):
+def make_pair(spec_or_batch, monkeypatch, mfma="1"):
     from tramonto_amd.batch import BatchSpec, DeviceBatch
     from tramonto_amd.device import DeviceProblem
     monkeypatch.setenv("TMI_MF_FFT", "0")
     monkeypatch.setenv("TMI_FMT_FFT", "0")
+    monkeypatch.setenv("TMI_MF2D_MFMA", mfma)
     mk = DeviceBatch if isinstance(spec_or_batch, BatchSpec) else DeviceProblem
     monkeypatch.delenv("TMI_STENCIL_KERNEL", raising=False)
     dense = mk(spec_or_batch)
@@ -63,16 +66,18 @@ def check(dense, gather, x, seed):
         assert np.abs(Jd - Jg).max() < tol * np.abs(Jg).max(), fp32
 
 
+@pytest.mark.parametrize("mfma", ["1", "0"])
 @pytest.mark.parametrize("case", ["2d_wall_reflect", "2d_periodic", "binary"])
-def test_dense_2d_attraction_matches_gather(case, monkeypatch):
+def test_dense_2d_attraction_matches_gather(case, mfma, monkeypatch):
     spec = binary_2d() if case == "binary" else small_case(case)
-    dense, gather = make_pair(spec, monkeypatch)
+    dense, gather = make_pair(spec, monkeypatch, mfma)
     check(dense, gather, state(spec, 3), 4)
 
 
-def test_dense_2d_attraction_in_a_batch_of_pieces(monkeypatch):
+@pytest.mark.parametrize("mfma", ["1", "0"])
+def test_dense_2d_attraction_in_a_batch_of_pieces(mfma, monkeypatch):
     from tramonto_amd.batch import rough_slit_pore_batch_2d
     b = rough_slit_pore_batch_2d(npores=3, seed=5, nx=64, h_range=(3.0, 5.0), dx=0.1)
-    dense, gather = make_pair(b, monkeypatch)
+    dense, gather = make_pair(b, monkeypatch, mfma)
     x = np.concatenate([state(p, 7 + k) for k, p in enumerate(b.pores)], axis=1)
     check(dense, gather, x, 8)

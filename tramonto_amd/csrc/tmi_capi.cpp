@@ -522,6 +522,8 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                 P->f_mfw2 = dev_alloc<float>(wf.size());
                 upload(P->d_mfw2, wd.data(), wd.size());
                 upload(P->f_mfw2, wf.data(), wf.size());
+                const char *mm = std::getenv("TMI_MF2D_MFMA");
+                P->mf2d_mfma = !(mm && std::string(mm) == "0");
             }
         }
         // dense row kernels on 1D grids (TMI_STENCIL_KERNEL=gather keeps the gather kernels)

@@ -187,6 +187,7 @@ struct Problem {
     // its assembled entries, including the inexact ones) instead of the exact Hessian
     bool jac_ref = false;
     double *d_mfw2 = nullptr;   // 2D direct form: dense (2 hy + 1) x (2 hx + 1) attraction weights per component pair
+    bool mf2d_mfma = true;      // FP64 dense 2D sums on the matrix cores (TMI_MF2D_MFMA=0: the FMA kernel)
     float *f_mfw2 = nullptr;    // the same in FP32
     double *d_w0 = nullptr;
     int nunk_ext = 0;
