@@ -526,6 +526,40 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                 P->mf2d_mfma = !(mm && std::string(mm) == "0");
             }
         }
+        // dense 2D weights of the rho-bar sums of one component for the matrix-core kernel: theta,
+        // delta and the delta vector weights w (o_d dx_d / R) along x and y, as the gather kernel's
+        // w and wv (TMI_STENCIL_KERNEL=gather keeps the gather kernel)
+        if (nd == 2 && C == 1 && !P->use_columns && !(kern && std::string(kern) == "gather")) {
+            const HostStencil &th = P->hs[TMI_THETA][0], &de = P->hs[TMI_DELTA][0];
+            int fx = 0, fy = 0;
+            bool flat = true;
+            for (const HostStencil *hs : {&th, &de})
+                for (int k = 0; k < hs->L; ++k) {
+                    fx = std::max(fx, std::abs(hs->off[3 * (size_t)k]));
+                    fy = std::max(fy, std::abs(hs->off[3 * (size_t)k + 1]));
+                    flat = flat && hs->off[3 * (size_t)k + 2] == 0;
+                }
+            if (flat && fx <= P->pad_u.h[0] && fy <= P->pad_u.h[1]) {
+                const int wx = 2 * fx + 1, wy = 2 * fy + 1;
+                std::vector<double> wr((size_t)4 * wx * wy, 0.0);
+                auto at = [&](int t, const int *o) -> double & {
+                    return wr[((size_t)t * wy + (o[1] + fy)) * wx + (o[0] + fx)];
+                };
+                for (int k = 0; k < th.L; ++k) at(0, &th.off[3 * (size_t)k]) += th.w[k];
+                for (int k = 0; k < de.L; ++k) {
+                    const int *o = &de.off[3 * (size_t)k];
+                    at(1, o) += de.w[k];
+                    for (int d = 0; d < 2; ++d)
+                        at(2 + d, o) += de.w[k] * ((double)o[d] * p.dx[d] * P->bulk.inv_rad[0]);
+                }
+                P->d_rbw2 = dev_alloc<double>(wr.size());
+                upload(P->d_rbw2, wr.data(), wr.size());
+                P->rbw_h[0] = fx;
+                P->rbw_h[1] = fy;
+                const char *rm = std::getenv("TMI_RB2D_MFMA");
+                P->rb2d_mfma = !(rm && std::string(rm) == "0");
+            }
+        }
         // dense row kernels on 1D grids (TMI_STENCIL_KERNEL=gather keeps the gather kernels)
         if (nd == 1 && !(kern && std::string(kern) == "gather")) {
             for (int c = 0; c < C; ++c)
@@ -632,6 +666,7 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->d_red);
     (void)hipFree(P->d_mfw2);
     (void)hipFree(P->f_mfw2);
+    (void)hipFree(P->d_rbw2);
     (void)hipFree(P->d_scratch);
     (void)hipFree(P->d_newton);
     (void)hipFree(P->d_xi);

@@ -514,6 +514,84 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, cons
     }
 }
 
+// The rho-bar rows of one component on a 2D grid in FP64 on the matrix cores, as k_mf_rows_2d_mfma
+// with the FMT stencils' own reach (fx, fy) inside the density halo: four Toeplitz products per dx
+// and 4-row chunk share each density load (theta, delta, delta x, delta y weights), and the rows are
+// those of k_rb_rows for C = 1: n3, n2, n1 = n2 / (4 pi R), n0 = n2 / (4 pi R^2), nv2, nv1 = nv2 / (4 pi R).
+constexpr int kRbMfmaWaves = 8;
+__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, int fx, int fy, const double *__restrict__ W,
+                                                                       const double *__restrict__ rho_pad, double i4pir,
+                                                                       double i4pirsq, const double *__restrict__ in_rb,
+                                                                       const double *__restrict__ sub,
+                                                                       double *__restrict__ out_rb)
+{
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ double part[kRbMfmaWaves - 1][4][4][64];   // [wave - 1][sum][register][lane]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
+    const int wx = 2 * fx + 1, wy = 2 * fy + 1;
+    const size_t wsz = (size_t)wx * wy;
+    const int lo = lane & 15, kq = lane >> 4;
+    const int x0 = blockIdx.x * 16;          // first interior column of the tile
+    const int py0 = hy + blockIdx.y * 16;    // first padded row of the tile's outputs
+    const bool col_ok = x0 + lo < g.nx[0];
+    const int nk = 16 + 2 * fy;              // window rows: padded rows py0 - fy + k (>= 0 as fy <= hy)
+    d4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int dx = -fx + wave; dx <= fx; dx += kRbMfmaWaves) {
+        const long long colx = hx + x0 + lo + dx;
+        const double *wcol = W + (dx + fx);
+#pragma unroll 2
+        for (int k0 = 0; k0 < nk; k0 += 4) {
+            const int k = k0 + kq;
+            const int prow = py0 - fy + k;
+            const double b = (col_ok && k < nk && prow < P1) ? rho_pad[(long long)prow * P0 + colx] : 0.0;
+            const int wr = k - lo;
+            const bool win = wr >= 0 && wr < wy;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const double a = win ? wcol[t * wsz + (size_t)wr * wx] : 0.0;
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[wave - 1][t][r][lane] = acc[t][r];
+    }
+    __syncthreads();
+    if (wave > 0 || !col_ok) return;
+    const int xi = x0 + lo;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int py = py0 + kq + 4 * r;
+        if (py >= P1 - hy) continue;
+        double sum[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            double a = acc[t][r];
+#pragma unroll
+            for (int w = 0; w < kRbMfmaWaves - 1; ++w) a += part[w][t][r][lane];
+            sum[t] = a;
+        }
+        const long long pos = (long long)py * P0 + hx + xi;
+        const int node = g.pad_node ? g.pad_node[pos] : xi + (py - hy) * g.nx[0];
+        if (node < 0) continue;
+        const double vals[8] = {sum[0], sum[1], i4pir * sum[1], i4pirsq * sum[1],
+                                sum[2], sum[3], i4pir * sum[2], i4pir * sum[3]};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double v = vals[q];
+            if (in_rb) v -= in_rb[(size_t)q * g.N + node];
+            if (sub) v -= sub[(size_t)q * g.N + node];
+            out_rb[(size_t)q * g.N + node] = v;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- 3D column kernels
 // A column pass streams one input field along z for one stencil column and
 // adds, for each of the lane's B consecutive z outputs, the column's weights
@@ -1591,6 +1669,14 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
         if (part)
             hipLaunchKernelGGL(k_rb_combine<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, part, cc, in_rb, sub, out_rb);
         return;
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma) {   // one component: 16 x 16 tiles on the matrix cores
+            const dim3 grid((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + 15) / 16));
+            hipLaunchKernelGGL(k_rb_rows_2d_mfma, grid, dim3(64 * kRbMfmaWaves), 0, s, gu, P.rbw_h[0], P.rbw_h[1],
+                               P.d_rbw2, rp, cc.i4pir[0], cc.i4pirsq[0], in_rb, sub, out_rb);
+            return;
+        }
     }
     switch (P.p.ndim) {
     case 1: hipLaunchKernelGGL((k_rb_rows<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
