@@ -449,21 +449,38 @@ __global__ __launch_bounds__(64 * kMf2dWaves) void k_mf_rows_2d(Geo g, const T *
 }
 constexpr int kMf2dRows = 4;
 
-// The same sums in FP64 on the matrix cores.  A wave owns a tile of 16 padded rows x 16 columns; for
-// each dx the tile gains the Toeplitz product A_dx B_dx over the 16 + 2 hy input rows k of its
-// window (padded row py0 - hy + k): A_dx[i][k] = W[k - i][dx] (weight row dy + hy = k - i, 0 outside
-// [0, 2 hy]), B_dx[k][j] = rho[py0 - hy + k][column j + dx], 4 rows k per v_mfma_f64_16x16x4f64.
-// Operand lanes: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k; D[i][j] at lane j + 16 (i % 4),
-// register i / 4.  The kMfmaWaves waves of a workgroup split the dx range and add their tiles in LDS.
+// The same sums on the matrix cores (FP64 v_mfma_f64_16x16x4f64; FP32 v_mfma_f32_16x16x4f32 for the
+// mixed path's sums).  A wave owns a tile of 16 padded rows x 16 columns; for each dx the tile gains
+// the Toeplitz product A_dx B_dx over the 16 + 2 hy input rows k of its window (padded row
+// py0 - hy + k): A_dx[i][k] = W[k - i][dx] (weight row dy + hy = k - i, 0 outside [0, 2 hy]),
+// B_dx[k][j] = rho[py0 - hy + k][column j + dx], 4 rows k per instruction.  Operand lanes (both
+// types): A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k; D[i][j] at lane j + 16 (i % 4), register
+// i / 4 in FP64 and at lane j + 16 (i / 4), register i % 4 in FP32.  The kMfmaWaves waves of a
+// workgroup split the dx range and add their tiles in LDS.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 step(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int kq, int r) { return kq + 4 * r; }   // tile row of register r
+};
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 step(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int kq, int r) { return 4 * kq + r; }
+};
 constexpr int kMfmaWaves = 16;
-__global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, const double *__restrict__ W,
-                                                                    const double *__restrict__ rho_pad,
+template <typename T>
+__global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, const T *__restrict__ W,
+                                                                    const T *__restrict__ rho_pad,
                                                                     const double *__restrict__ in_mf,
                                                                     const double *__restrict__ sub,
                                                                     double *__restrict__ out_mf)
 {
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    __shared__ double part[kMfmaWaves - 1][4][64];
+    typedef typename Mfma16<T>::v4 d4;
+    __shared__ T part[kMfmaWaves - 1][4][64];
     const int c = blockIdx.z;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
@@ -473,21 +490,21 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, cons
     const int py0 = hy + blockIdx.y * 16;       // first padded row of the tile's outputs
     const bool col_ok = x0 + lo < g.nx[0];
     const int nk = 16 + 2 * hy;
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    d4 acc = {(T)0, (T)0, (T)0, (T)0};
     for (int cj = 0; cj < g.C; ++cj) {
-        const double *Wq = W + (size_t)(c + g.C * cj) * wx * wy;
-        const double *src = rho_pad + (size_t)cj * g.np;
+        const T *Wq = W + (size_t)(c + g.C * cj) * wx * wy;
+        const T *src = rho_pad + (size_t)cj * g.np;
         for (int dx = -hx + wave; dx <= hx; dx += kMfmaWaves) {
             const long long colx = hx + x0 + lo + dx;   // padded column read by this lane's B entries
-            const double *wcol = Wq + (dx + hx);        // weight row r at wcol[r * wx]
+            const T *wcol = Wq + (dx + hx);             // weight row r at wcol[r * wx]
 #pragma unroll 4
             for (int k0 = 0; k0 < nk; k0 += 4) {
                 const int k = k0 + kq;
                 const int prow = py0 - hy + k;   // >= 0; rows at or past P1 feed no written output
-                const double b = (col_ok && k < nk && prow < P1) ? src[(long long)prow * P0 + colx] : 0.0;
+                const T b = (col_ok && k < nk && prow < P1) ? src[(long long)prow * P0 + colx] : (T)0;
                 const int wr = k - lo;
-                const double a = (wr >= 0 && wr < wy) ? wcol[(size_t)wr * wx] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                const T a = (wr >= 0 && wr < wy) ? wcol[(size_t)wr * wx] : (T)0;
+                acc = Mfma16<T>::step(a, b, acc);
             }
         }
     }
@@ -500,17 +517,18 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, cons
     const int xi = x0 + lo;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int py = py0 + kq + 4 * r;
+        const int py = py0 + Mfma16<T>::row(kq, r);
         if (py >= P1 - hy) continue;
-        double a = acc[r];
+        T a = acc[r];
 #pragma unroll
         for (int w = 0; w < kMfmaWaves - 1; ++w) a += part[w][r][lane];
         const long long pos = (long long)py * P0 + hx + xi;
         const int node = g.pad_node ? g.pad_node[pos] : xi + (py - hy) * g.nx[0];
         if (node < 0) continue;
-        if (in_mf) a -= in_mf[(size_t)c * g.N + node];
-        if (sub) a -= sub[(size_t)c * g.N + node];
-        out_mf[(size_t)c * g.N + node] = a;
+        double v = (double)a;
+        if (in_mf) v -= in_mf[(size_t)c * g.N + node];
+        if (sub) v -= sub[(size_t)c * g.N + node];
+        out_mf[(size_t)c * g.N + node] = v;
     }
 }
 
@@ -1707,16 +1725,14 @@ void mf_rows(Problem &P, const double *in_mf, const double *sub, double *out_mf,
         return;
     }
     if (P.p.ndim == 2 && P.d_mfw2) {   // dense 2D convolution
-        if constexpr (std::is_same<T, double>::value) {
-            if (P.mf2d_mfma) {   // 16 x 16 output tiles on the FP64 matrix cores
-                const dim3 grid((unsigned)((P.p.nx[0] + 15) / 16),
-                                (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + 15) / 16), (unsigned)P.p.ncomp);
-                hipLaunchKernelGGL(k_mf_rows_2d_mfma, grid, dim3(64 * kMfmaWaves), 0, s, gu, P.d_mfw2,
-                                   workspace<double>(P).rho_pad, in_mf, sub, out_mf);
-                return;
-            }
-        }
         const T *W = std::is_same<T, float>::value ? (const T *)P.f_mfw2 : (const T *)P.d_mfw2;
+        if (P.mf2d_mfma) {   // 16 x 16 output tiles on the matrix cores
+            const dim3 grid((unsigned)((P.p.nx[0] + 15) / 16),
+                            (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + 15) / 16), (unsigned)P.p.ncomp);
+            hipLaunchKernelGGL(k_mf_rows_2d_mfma<T>, grid, dim3(64 * kMfmaWaves), 0, s, gu, W, workspace<T>(P).rho_pad,
+                               in_mf, sub, out_mf);
+            return;
+        }
         const dim3 grid((unsigned)((P.p.nx[0] + 63) / 64),
                         (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + kMf2dRows - 1) / kMf2dRows), (unsigned)P.p.ncomp);
         hipLaunchKernelGGL((k_mf_rows_2d<T, kMf2dRows>), grid, dim3(64 * kMf2dWaves), 0, s, gu, W,
