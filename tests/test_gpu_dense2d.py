@@ -1,8 +1,8 @@
 """The dense 2D convolution of the direct attraction rows against the per-point gather kernel it
 replaces (TMI_STENCIL_KERNEL=gather): residual and Jacobian products in FP64 (1e-13 of the largest
-row; the FP64 sums run on the matrix cores, k_mf_rows_2d_mfma and for one component the rho-bar
-rows k_rb_rows_2d_mfma, or with TMI_MF2D_MFMA=0 TMI_RB2D_MFMA=0 in the FMA kernel k_mf_rows_2d and the
-gather kernel k_rb_rows) and in the FP32 sums of the mixed path (k_mf_rows_2d, 1e-5), on
+row; the FP64 sums run on the matrix cores, k_mf_rows_2d_mfma and for one component the rho-bar and
+Euler-Lagrange rows k_rb_rows_2d_mfma, k_el_rows_2d_mfma, or with TMI_MF2D_MFMA=0 TMI_RB2D_MFMA=0 in
+the FMA kernel k_mf_rows_2d and the gather kernels) and in the FP32 sums of the mixed path (1e-5), on
 walled/reflecting and periodic 2D grids, two components, and a batch of 2D pieces (gap rows between
 the pieces)."""
 import dataclasses

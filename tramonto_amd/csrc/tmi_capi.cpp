@@ -556,6 +556,7 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                 upload(P->d_rbw2, wr.data(), wr.size());
                 P->rbw_h[0] = fx;
                 P->rbw_h[1] = fy;
+                P->rbw_el = fx <= P->pad_f.h[0] && fy <= P->pad_f.h[1];
                 const char *rm = std::getenv("TMI_RB2D_MFMA");
                 P->rb2d_mfma = !(rm && std::string(rm) == "0");
             }

@@ -610,6 +610,90 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, in
     }
 }
 
+// The Euler-Lagrange rows of one component on a 2D grid in FP64 on the matrix cores, as k_el_rows for
+// C = 1: the adjoint fields (A, B_x, B_y, S3, halo-padded in the adjoint geometry g) correlated with the
+// delta, -delta x, -delta y and theta weights of k_rb_rows_2d_mfma's table (reach fx, fy inside the
+// adjoint halo), four products per dx and 4-row chunk into one sum, then the local terms of k_el_rows.
+__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, int fx, int fy, const double *__restrict__ W,
+                                                                       const double *__restrict__ adj, CompConsts cc,
+                                                                       int linear, int nwall,
+                                                                       const double *__restrict__ x_rho,
+                                                                       const double *__restrict__ in_rho,
+                                                                       const double *__restrict__ in_mf,
+                                                                       const double *__restrict__ vext,
+                                                                       const unsigned char *__restrict__ elzero,
+                                                                       const double *__restrict__ add,
+                                                                       double *__restrict__ out_el)
+{
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ double part[kRbMfmaWaves - 1][4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
+    const int wx = 2 * fx + 1, wy = 2 * fy + 1;
+    const size_t wsz = (size_t)wx * wy;
+    const int lo = lane & 15, kq = lane >> 4;
+    const int x0 = blockIdx.x * 16, py0 = hy + blockIdx.y * 16;
+    const bool col_ok = x0 + lo < g.nx[0];
+    const int nk = 16 + 2 * fy;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int dx = -fx + wave; dx <= fx; dx += kRbMfmaWaves) {
+        const long long colx = hx + x0 + lo + dx;
+        const double *wcol = W + (dx + fx);
+#pragma unroll 2
+        for (int k0 = 0; k0 < nk; k0 += 4) {
+            const int k = k0 + kq;
+            const int prow = py0 - fy + k;
+            const bool inb = col_ok && k < nk && prow < P1;
+            const long long at = (long long)prow * P0 + colx;
+            const int wr = k - lo;
+            const bool win = wr >= 0 && wr < wy;
+            const size_t wo = (size_t)wr * wx;
+            const double bA = inb ? adj[at] : 0.0, bX = inb ? adj[(size_t)g.np + at] : 0.0;
+            const double bY = inb ? adj[2 * (size_t)g.np + at] : 0.0, bS = inb ? adj[3 * (size_t)g.np + at] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? wcol[wsz + wo] : 0.0, bA, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? -wcol[2 * wsz + wo] : 0.0, bX, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? -wcol[3 * wsz + wo] : 0.0, bY, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? wcol[wo] : 0.0, bS, acc, 0, 0, 0);
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[wave - 1][r][lane] = acc[r];
+    }
+    __syncthreads();
+    if (wave > 0 || !col_ok) return;
+    const int xi = x0 + lo;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int py = py0 + kq + 4 * r;
+        if (py >= P1 - hy) continue;
+        double sum = acc[r];
+#pragma unroll
+        for (int w = 0; w < kRbMfmaWaves - 1; ++w) sum += part[w][r][lane];
+        const long long pos = (long long)py * P0 + hx + xi;
+        const int node = g.pad_node ? g.pad_node[pos] : xi + (py - hy) * g.nx[0];
+        if (node < 0) continue;
+        const size_t row = (size_t)node;   // component 0
+        double v;
+        if (elzero[row]) {
+            v = in_rho ? in_rho[row] : 0.0;
+        } else {
+            v = sum;
+            const double xv = x_rho[row];
+            if (!linear) {
+                v += log(xv) + (g.mu ? g.mu[row] : cc.mu[0]);
+                if (nwall > 0) v += vext[row];
+                if (in_mf) v += in_mf[row];
+            } else {
+                if (in_rho) v += in_rho[row] / xv;
+                if (in_mf) v += in_mf[row];
+            }
+        }
+        if (add) v += add[row];
+        out_el[row] = v;
+    }
+}
+
 // ---------------------------------------------------------------- 3D column kernels
 // A column pass streams one input field along z for one stencil column and
 // adds, for each of the lane's B consecutive z outputs, the column's weights
@@ -1836,6 +1920,15 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
         hipLaunchKernelGGL((k_el_rows_cols<T, kColB, kColS>), cgrid, dim3(64, kColS), 0, s, gf, fa, adj, cc, lin,
                            P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
         return;
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma && P.rbw_el) {   // one component: 16 x 16 tiles on the matrix cores
+            const dim3 g2((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_f.P[1] - 2 * P.pad_f.h[1] + 15) / 16));
+            hipLaunchKernelGGL(k_el_rows_2d_mfma, g2, dim3(64 * kRbMfmaWaves), 0, s, gf, P.rbw_h[0], P.rbw_h[1], P.d_rbw2,
+                               (const double *)adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero,
+                               add, out_el);
+            return;
+        }
     }
     dim3 grid(grid_for(P.N), P.p.ncomp);
     switch (P.p.ndim) {
