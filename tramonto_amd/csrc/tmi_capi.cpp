@@ -552,8 +552,11 @@ Problem *create_problem(const tmi_params &p_in, const double *vext, const unsign
                     for (int d = 0; d < 2; ++d)
                         at(2 + d, o) += de.w[k] * ((double)o[d] * p.dx[d] * P->bulk.inv_rad[0]);
                 }
+                std::vector<float> wrf(wr.begin(), wr.end());
                 P->d_rbw2 = dev_alloc<double>(wr.size());
+                P->f_rbw2 = dev_alloc<float>(wrf.size());
                 upload(P->d_rbw2, wr.data(), wr.size());
+                upload(P->f_rbw2, wrf.data(), wrf.size());
                 P->rbw_h[0] = fx;
                 P->rbw_h[1] = fy;
                 P->rbw_el = fx <= P->pad_f.h[0] && fy <= P->pad_f.h[1];
@@ -668,6 +671,7 @@ void destroy_problem(Problem *P)
     (void)hipFree(P->d_mfw2);
     (void)hipFree(P->f_mfw2);
     (void)hipFree(P->d_rbw2);
+    (void)hipFree(P->f_rbw2);
     (void)hipFree(P->d_scratch);
     (void)hipFree(P->d_newton);
     (void)hipFree(P->d_xi);

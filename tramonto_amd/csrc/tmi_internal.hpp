@@ -189,6 +189,7 @@ struct Problem {
     double *d_mfw2 = nullptr;   // 2D direct form: dense (2 hy + 1) x (2 hx + 1) attraction weights per component pair
     bool mf2d_mfma = true;      // FP64 dense 2D sums on the matrix cores (TMI_MF2D_MFMA=0: the FMA kernel)
     double *d_rbw2 = nullptr;   // 2D, one component: dense theta, delta, delta x and delta y weights [4][2 fy + 1][2 fx + 1]
+    float *f_rbw2 = nullptr;    // the same in FP32 (mixed path)
     int rbw_h[2] = {0, 0};      // their reach fx, fy (nodes)
     bool rbw_el = false;        // the reach fits the adjoint halo too (Euler-Lagrange rows on the matrix cores)
     bool rb2d_mfma = true;      // FP64 rho-bar rows on the matrix cores (TMI_RB2D_MFMA=0: the gather kernel)

@@ -537,14 +537,15 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mf_rows_2d_mfma(Geo g, cons
 // and 4-row chunk share each density load (theta, delta, delta x, delta y weights), and the rows are
 // those of k_rb_rows for C = 1: n3, n2, n1 = n2 / (4 pi R), n0 = n2 / (4 pi R^2), nv2, nv1 = nv2 / (4 pi R).
 constexpr int kRbMfmaWaves = 8;
-__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, int fx, int fy, const double *__restrict__ W,
-                                                                       const double *__restrict__ rho_pad, double i4pir,
+template <typename T>
+__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, int fx, int fy, const T *__restrict__ W,
+                                                                       const T *__restrict__ rho_pad, double i4pir,
                                                                        double i4pirsq, const double *__restrict__ in_rb,
                                                                        const double *__restrict__ sub,
                                                                        double *__restrict__ out_rb)
 {
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    __shared__ double part[kRbMfmaWaves - 1][4][4][64];   // [wave - 1][sum][register][lane]
+    typedef typename Mfma16<T>::v4 d4;
+    __shared__ T part[kRbMfmaWaves - 1][4][4][64];   // [wave - 1][sum][register][lane]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
     const int wx = 2 * fx + 1, wy = 2 * fy + 1;
@@ -556,21 +557,21 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, in
     const int nk = 16 + 2 * fy;              // window rows: padded rows py0 - fy + k (>= 0 as fy <= hy)
     d4 acc[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < 4; ++t) acc[t] = (d4){(T)0, (T)0, (T)0, (T)0};
     for (int dx = -fx + wave; dx <= fx; dx += kRbMfmaWaves) {
         const long long colx = hx + x0 + lo + dx;
-        const double *wcol = W + (dx + fx);
+        const T *wcol = W + (dx + fx);
 #pragma unroll 2
         for (int k0 = 0; k0 < nk; k0 += 4) {
             const int k = k0 + kq;
             const int prow = py0 - fy + k;
-            const double b = (col_ok && k < nk && prow < P1) ? rho_pad[(long long)prow * P0 + colx] : 0.0;
+            const T b = (col_ok && k < nk && prow < P1) ? rho_pad[(long long)prow * P0 + colx] : (T)0;
             const int wr = k - lo;
             const bool win = wr >= 0 && wr < wy;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const double a = win ? wcol[t * wsz + (size_t)wr * wx] : 0.0;
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                const T a = win ? wcol[t * wsz + (size_t)wr * wx] : (T)0;
+                acc[t] = Mfma16<T>::step(a, b, acc[t]);
             }
         }
     }
@@ -585,12 +586,12 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, in
     const int xi = x0 + lo;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int py = py0 + kq + 4 * r;
+        const int py = py0 + Mfma16<T>::row(kq, r);
         if (py >= P1 - hy) continue;
-        double sum[4];
+        T sum[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            double a = acc[t][r];
+            T a = acc[t][r];
 #pragma unroll
             for (int w = 0; w < kRbMfmaWaves - 1; ++w) a += part[w][t][r][lane];
             sum[t] = a;
@@ -598,8 +599,9 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, in
         const long long pos = (long long)py * P0 + hx + xi;
         const int node = g.pad_node ? g.pad_node[pos] : xi + (py - hy) * g.nx[0];
         if (node < 0) continue;
-        const double vals[8] = {sum[0], sum[1], i4pir * sum[1], i4pirsq * sum[1],
-                                sum[2], sum[3], i4pir * sum[2], i4pir * sum[3]};
+        const T f1 = (T)i4pir, f0 = (T)i4pirsq;   // the products in T, as k_rb_rows forms them
+        const double vals[8] = {(double)sum[0], (double)sum[1], (double)(f1 * sum[1]), (double)(f0 * sum[1]),
+                                (double)sum[2], (double)sum[3], (double)(f1 * sum[2]), (double)(f1 * sum[3])};
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             double v = vals[q];
@@ -614,8 +616,9 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_rb_rows_2d_mfma(Geo g, in
 // C = 1: the adjoint fields (A, B_x, B_y, S3, halo-padded in the adjoint geometry g) correlated with the
 // delta, -delta x, -delta y and theta weights of k_rb_rows_2d_mfma's table (reach fx, fy inside the
 // adjoint halo), four products per dx and 4-row chunk into one sum, then the local terms of k_el_rows.
-__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, int fx, int fy, const double *__restrict__ W,
-                                                                       const double *__restrict__ adj, CompConsts cc,
+template <typename T>
+__global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, int fx, int fy, const T *__restrict__ W,
+                                                                       const T *__restrict__ adj, CompConsts cc,
                                                                        int linear, int nwall,
                                                                        const double *__restrict__ x_rho,
                                                                        const double *__restrict__ in_rho,
@@ -625,8 +628,8 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, in
                                                                        const double *__restrict__ add,
                                                                        double *__restrict__ out_el)
 {
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    __shared__ double part[kRbMfmaWaves - 1][4][64];
+    typedef typename Mfma16<T>::v4 d4;
+    __shared__ T part[kRbMfmaWaves - 1][4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int hx = g.h[0], hy = g.h[1], P0 = g.P[0], P1 = g.P[1];
     const int wx = 2 * fx + 1, wy = 2 * fy + 1;
@@ -635,10 +638,10 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, in
     const int x0 = blockIdx.x * 16, py0 = hy + blockIdx.y * 16;
     const bool col_ok = x0 + lo < g.nx[0];
     const int nk = 16 + 2 * fy;
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    d4 acc = {(T)0, (T)0, (T)0, (T)0};
     for (int dx = -fx + wave; dx <= fx; dx += kRbMfmaWaves) {
         const long long colx = hx + x0 + lo + dx;
-        const double *wcol = W + (dx + fx);
+        const T *wcol = W + (dx + fx);
 #pragma unroll 2
         for (int k0 = 0; k0 < nk; k0 += 4) {
             const int k = k0 + kq;
@@ -648,12 +651,12 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, in
             const int wr = k - lo;
             const bool win = wr >= 0 && wr < wy;
             const size_t wo = (size_t)wr * wx;
-            const double bA = inb ? adj[at] : 0.0, bX = inb ? adj[(size_t)g.np + at] : 0.0;
-            const double bY = inb ? adj[2 * (size_t)g.np + at] : 0.0, bS = inb ? adj[3 * (size_t)g.np + at] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? wcol[wsz + wo] : 0.0, bA, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? -wcol[2 * wsz + wo] : 0.0, bX, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? -wcol[3 * wsz + wo] : 0.0, bY, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(win ? wcol[wo] : 0.0, bS, acc, 0, 0, 0);
+            const T bA = inb ? adj[at] : (T)0, bX = inb ? adj[(size_t)g.np + at] : (T)0;
+            const T bY = inb ? adj[2 * (size_t)g.np + at] : (T)0, bS = inb ? adj[3 * (size_t)g.np + at] : (T)0;
+            acc = Mfma16<T>::step(win ? wcol[wsz + wo] : (T)0, bA, acc);
+            acc = Mfma16<T>::step(win ? -wcol[2 * wsz + wo] : (T)0, bX, acc);
+            acc = Mfma16<T>::step(win ? -wcol[3 * wsz + wo] : (T)0, bY, acc);
+            acc = Mfma16<T>::step(win ? wcol[wo] : (T)0, bS, acc);
         }
     }
     if (wave > 0) {
@@ -665,9 +668,9 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, in
     const int xi = x0 + lo;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int py = py0 + kq + 4 * r;
+        const int py = py0 + Mfma16<T>::row(kq, r);
         if (py >= P1 - hy) continue;
-        double sum = acc[r];
+        T sum = acc[r];
 #pragma unroll
         for (int w = 0; w < kRbMfmaWaves - 1; ++w) sum += part[w][r][lane];
         const long long pos = (long long)py * P0 + hx + xi;
@@ -678,7 +681,7 @@ __global__ __launch_bounds__(64 * kRbMfmaWaves) void k_el_rows_2d_mfma(Geo g, in
         if (elzero[row]) {
             v = in_rho ? in_rho[row] : 0.0;
         } else {
-            v = sum;
+            v = (double)sum;
             const double xv = x_rho[row];
             if (!linear) {
                 v += log(xv) + (g.mu ? g.mu[row] : cc.mu[0]);
@@ -1772,13 +1775,12 @@ void rb_rows(Problem &P, const double *in_rb, const double *sub, double *out_rb,
             hipLaunchKernelGGL(k_rb_combine<T>, dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, part, cc, in_rb, sub, out_rb);
         return;
     }
-    if constexpr (std::is_same<T, double>::value) {
-        if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma) {   // one component: 16 x 16 tiles on the matrix cores
-            const dim3 grid((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + 15) / 16));
-            hipLaunchKernelGGL(k_rb_rows_2d_mfma, grid, dim3(64 * kRbMfmaWaves), 0, s, gu, P.rbw_h[0], P.rbw_h[1],
-                               P.d_rbw2, rp, cc.i4pir[0], cc.i4pirsq[0], in_rb, sub, out_rb);
-            return;
-        }
+    if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma) {   // one component: 16 x 16 tiles on the matrix cores
+        const T *Wr = std::is_same<T, float>::value ? (const T *)P.f_rbw2 : (const T *)P.d_rbw2;
+        const dim3 grid((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_u.P[1] - 2 * P.pad_u.h[1] + 15) / 16));
+        hipLaunchKernelGGL(k_rb_rows_2d_mfma<T>, grid, dim3(64 * kRbMfmaWaves), 0, s, gu, P.rbw_h[0], P.rbw_h[1], Wr, rp,
+                           cc.i4pir[0], cc.i4pirsq[0], in_rb, sub, out_rb);
+        return;
     }
     switch (P.p.ndim) {
     case 1: hipLaunchKernelGGL((k_rb_rows<T, 1>), dim3(grid_for(P.N)), dim3(kBlock), 0, s, gu, ra, rp, cc, in_rb, sub, out_rb); break;
@@ -1921,14 +1923,12 @@ void el_rows(Problem &P, bool linear, const double *x_rho, const double *in_rho,
                            P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
         return;
     }
-    if constexpr (std::is_same<T, double>::value) {
-        if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma && P.rbw_el) {   // one component: 16 x 16 tiles on the matrix cores
-            const dim3 g2((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_f.P[1] - 2 * P.pad_f.h[1] + 15) / 16));
-            hipLaunchKernelGGL(k_el_rows_2d_mfma, g2, dim3(64 * kRbMfmaWaves), 0, s, gf, P.rbw_h[0], P.rbw_h[1], P.d_rbw2,
-                               (const double *)adj, cc, lin, P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero,
-                               add, out_el);
-            return;
-        }
+    if (P.p.ndim == 2 && P.d_rbw2 && P.rb2d_mfma && P.rbw_el) {   // one component: 16 x 16 tiles on the matrix cores
+        const T *Wr = std::is_same<T, float>::value ? (const T *)P.f_rbw2 : (const T *)P.d_rbw2;
+        const dim3 g2((unsigned)((P.p.nx[0] + 15) / 16), (unsigned)((P.pad_f.P[1] - 2 * P.pad_f.h[1] + 15) / 16));
+        hipLaunchKernelGGL(k_el_rows_2d_mfma<T>, g2, dim3(64 * kRbMfmaWaves), 0, s, gf, P.rbw_h[0], P.rbw_h[1], Wr, adj, cc,
+                           lin, P.p.nwall, x_rho, in_rho, in_mf, P.d_vext, P.d_elzero, add, out_el);
+        return;
     }
     dim3 grid(grid_for(P.N), P.p.ncomp);
     switch (P.p.ndim) {
